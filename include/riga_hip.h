@@ -1,0 +1,169 @@
+/*
+ * riga_hip.h -- C ABI of the MI355X (gfx950) Phoenix path-simulation backend.
+ *
+ * This is the drop-in boundary for the reference engine's pricing path.  Each
+ * entry point names the reference interface it replaces:
+ *
+ *   riga_hip_init / riga_hip_device_name / riga_hip_arch
+ *       gpu.rs:490-512 ensure_available(), device_name(), arch()
+ *       (process-global device initialised once; a build with this backend
+ *        fails loudly with code "gpu_unavailable", never falls back: lib.rs:188-202)
+ *   riga_price_put_leg
+ *       lib.rs:183-186 price_put_leg(&PriceRequest) -> Result<PriceResult, RigaError>
+ *       (note.rs:38-373 price_full_phoenix on the QMC lanes)
+ *   riga_price_batch
+ *       riga-risk/src/lib.rs:117-186 price_central_greeks and
+ *       riga-risk/src/lib.rs:573-626 price_risk_batch: many requests priced in
+ *       one launch; requests that share seed, schedule and correlation become
+ *       CRN scenarios of one item and share their Sobol points
+ *   riga_replay_paths
+ *       explain.rs:90-316 run_paths: per-path records, written to device memory
+ *       only for this query
+ *
+ * Conventions (lib.rs, riga-core/src/lib.rs):
+ *   - every function returns 0 on success or a negative status and fills
+ *     riga_error {code, message} (RigaError, riga-core/src/lib.rs:5-19);
+ *   - inputs are caller-owned and only borrowed for the call; outputs are
+ *     caller-allocated;
+ *   - Option<f64> fields are expressed with has_* flags; None maps to the
+ *     reference defaults (KO/coupon barrier +inf, coupon rate 0, 9 periods,
+ *     call start step 0; note.rs:43-53).
+ *   - the context is thread-safe: calls are serialised on the context's stream.
+ */
+#ifndef RIGA_HIP_H
+#define RIGA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RIGA_TRACE_CAP 64
+
+typedef struct riga_error {
+  char code[48];
+  char message[464];
+} riga_error;
+
+typedef struct riga_ctx riga_ctx;
+
+enum riga_precision { RIGA_FP64 = 0, RIGA_FP32 = 1 };
+enum riga_ladder { RIGA_VR_NONE = 0, RIGA_VR_HALTON = 1, RIGA_VR_SOBOL = 2 };
+enum riga_trace { RIGA_TRACE_OFF = 0, RIGA_TRACE_AGGREGATED = 1, RIGA_TRACE_SAMPLED = 2,
+                  RIGA_TRACE_FULL = 3 };
+
+/* ImpliedVolGrid (riga-market/src/lib.rs:84-91); vols row-major [n_maturities][n_strikes]. */
+typedef struct riga_implied_grid {
+  int32_t n_maturities, n_strikes;
+  const double *maturities, *strikes, *vols;
+} riga_implied_grid;
+
+/* A calibrated DupireLocalVol surface (riga-market/src/lib.rs:119-123): v[n_t][n_x]. */
+typedef struct riga_lv_grid {
+  int32_t n_t, n_x;
+  const double *t_axis, *log_s, *v;
+} riga_lv_grid;
+
+/* PriceRequest (riga-engine/src/lib.rs:36-46) with F = factors underlyings. */
+typedef struct riga_request {
+  int32_t factors;
+  /* MarketSnapshot (riga-core/src/lib.rs:37-45) */
+  const double *spots, *vols, *dividend_yields;
+  const double *correlation; /* F x F row-major */
+  double risk_free_rate;
+  /* PhoenixTrade (riga-core/src/lib.rs:57-75) */
+  const double *initial_spots;
+  double maturity_years;
+  int64_t steps;
+  double strike, knock_in_barrier;
+  int32_t has_knock_out;   double knock_out_barrier;
+  int32_t has_coupon_barrier; double coupon_barrier;
+  int32_t has_coupon_rate; double coupon_rate;
+  int32_t memory_coupon;
+  int64_t coupon_periods;  /* < 0: None */
+  int32_t fixed_first_coupon;
+  int64_t call_start_step; /* < 0: None */
+  /* SimulationConfig (riga-core/src/lib.rs:77-86) */
+  uint64_t paths, path_block_size, seed;
+  int32_t trace_level;     /* riga_trace */
+  int32_t ladder;          /* riga_ladder: VarianceReduction kind */
+  int32_t ladder_dimension;/* VarianceReduction dimension */
+  int32_t max_dimension;   /* ladder cap: 512 reproduces qmc.rs:9; larger lifts the wrap */
+  /* LocalVolSpec (riga-engine/src/lib.rs:31-34): one per factor, or NULL for flat vols.
+   * Either implied grids (calibrated here with grid_x log-spot points, as
+   * build_local_vol, lib.rs:326-343) or already-calibrated grids. */
+  const riga_implied_grid *implied;
+  const riga_lv_grid *calibrated;
+  int32_t grid_x;          /* 96 = GRID_X (riga-market/src/lib.rs:28) */
+} riga_request;
+
+/* PriceResult (riga-core/src/lib.rs:243-255) in flat form.  Strings (ledger
+ * conditions, evidence, the snapshot id) are formed by the host-language side
+ * from these numbers, as note.rs:272-354 does. */
+typedef struct riga_result {
+  double pv, mc_error, ki_probability;
+  /* LegAttribution: coupon and memory_coupon on the x10 quote scale */
+  double leg_coupon, leg_memory_coupon, leg_put, leg_funding, leg_total;
+  uint64_t paths, ko_count, ki_count, put_eligible_count, coupon_path_count;
+  /* TransitionStats: daily_global_ko, final_fixing_eki, memory_coupon */
+  double ko_mean, ko_variance, ki_mean, ki_variance, coupon_mean, coupon_variance;
+  int32_t n_trace;
+  int64_t trace_path[RIGA_TRACE_CAP];
+  int64_t trace_step[RIGA_TRACE_CAP]; /* KO at time (step+1)*dt */
+  double device_ms;                   /* kernel time of the batch this result came from */
+} riga_result;
+
+/* explain.rs PathRecord subset, one per path. */
+typedef struct riga_path_record {
+  int32_t ko_step; /* -1: survived */
+  int32_t hit_ki;
+  double final_worst, funding, put, coupon_pv, memory_pv, pv;
+} riga_path_record;
+
+/* Tuning knobs chosen by the autotuner (riga_autotune) or by hand. */
+typedef struct riga_launch_config {
+  int32_t paths_per_thread; /* 1, 2, 4 */
+  int32_t reserved[3];
+} riga_launch_config;
+
+/* ---- device (gpu.rs:490-512) ---- */
+int riga_hip_init(int device, riga_ctx **out, riga_error *err);
+void riga_hip_destroy(riga_ctx *ctx);
+int riga_hip_device_name(riga_ctx *ctx, char *buf, size_t len, riga_error *err);
+int riga_hip_arch(riga_ctx *ctx, char *buf, size_t len, riga_error *err);
+
+/* ---- validation (lib.rs:209-321, F-factor form); no device work ---- */
+int riga_validate(const riga_request *req, riga_error *err);
+
+/* ---- pricing (lib.rs:183-186) ---- */
+int riga_price_put_leg(riga_ctx *ctx, const riga_request *req, int precision, riga_result *out,
+                       riga_error *err);
+
+/* ---- batched pricing: n requests in one launch; results[i] for reqs[i] ---- */
+int riga_price_batch(riga_ctx *ctx, const riga_request *reqs, int n, int precision,
+                     const riga_launch_config *cfg, riga_result *results, riga_error *err);
+
+/* ---- replay (explain.rs:90-316): per-path records for paths [0, req->paths) ---- */
+int riga_replay_paths(riga_ctx *ctx, const riga_request *req, int precision,
+                      riga_path_record *records, riga_result *out, riga_error *err);
+
+/* ---- local-vol calibration (riga-market/src/lib.rs:210-299 DupireLocalVol::calibrate)
+ * Outputs: t_axis[n_maturities], log_s[grid_x], v[n_maturities * grid_x]. */
+int riga_lv_calibrate(const riga_implied_grid *grid, double spot, double rate, int32_t grid_x,
+                      double *t_axis, double *log_s, double *v, riga_error *err);
+/* DupireLocalVol::sigma (riga-market/src/lib.rs:305-343) on a calibrated grid. */
+double riga_lv_sigma(const riga_lv_grid *grid, double t, double s);
+
+/* ---- diagnostics: Sobol/Owen integers and normals as the device computes them.
+ * out_scrambled[p*n_dims + d] for points n = first_point + p (1-based Sobol index). */
+int riga_debug_sobol(riga_ctx *ctx, uint64_t seed, int32_t ladder_dims, uint64_t first_point,
+                     int32_t n_points, int32_t first_dim, int32_t n_dims,
+                     uint32_t *out_unscrambled, uint32_t *out_scrambled, double *out_normal,
+                     riga_error *err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

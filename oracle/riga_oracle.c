@@ -1,0 +1,1077 @@
+/*
+ * riga_oracle.c -- CPU FP64 oracle (TEST INFRASTRUCTURE ONLY; see riga_oracle.h).
+ *
+ * Plain-C restatement of the reference engine's path-simulation and Phoenix
+ * payoff.  Each function cites the reference file:line it follows.  Where the
+ * reference hard-wires two factors (note.rs:82,89-100,134,143) this oracle uses
+ * F factors with QMC dimension step*F + f, which is identical for F = 2.
+ *
+ * Parity pins (tests/test_oracle_*.py) -- every function below is checked
+ * against at least one of: the reference's own unit-test assertions, textbook
+ * tables, closed forms, scipy, or brute force:
+ *   primitive polys      textbook degree 2..5 list; count = phi(2^d-1)/d
+ *   direction numbers    Sobol (0,m,2)-net property for dims 0/1; dim0 = bit reversal
+ *   owen/mix64           splitmix64 finaliser known outputs; per-level bijection
+ *   sobol uniform        reference test qmc.rs:479-500; stratification of aligned blocks
+ *   probit               scipy.special.ndtri within Acklam's 1.15e-9; qmc.rs:512-517
+ *   halton               qmc.rs:502-510 radical inverse; permutation is a bijection
+ *   bridge               linear-algebra identity W = B y; variance test qmc.rs:436-477
+ *   cholesky/psd         correlation.rs:266-330; L L^T = C
+ *   dupire               flat smile -> constant vol (lib.rs:400-412); termsheet points
+ *                        fixture.rs:507-527
+ *   pricer               Black-Scholes put closed form (F=1, no KO/coupon); funding
+ *                        = exp(-rT) with KO off; KO-invariance of KI (note.rs:381-401);
+ *                        brute-force straight-line restatement on tiny inputs.
+ */
+#include "riga_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------- */
+/* probit: Acklam's rational approximation, qmc.rs:37-91                      */
+/* ------------------------------------------------------------------------- */
+static const double PA[6] = {-3.969683028665376e+01, 2.209460984245205e+02,
+                             -2.759285104469687e+02, 1.383577518672690e+02,
+                             -3.066479806614716e+01, 2.506628277459239e+00};
+static const double PB[5] = {-5.447609879822406e+01, 1.615858368580409e+02,
+                             -1.556989798598866e+02, 6.680131188771972e+01,
+                             -1.328068155288572e+01};
+static const double PC[6] = {-7.784894002430293e-03, -3.223964580411365e-01,
+                             -2.400758277161838e+00, -2.549732539343734e+00,
+                             4.374664141464968e+00, 2.938163982698783e+00};
+static const double PD[4] = {7.784695709041462e-03, 3.224671290700398e-01,
+                             2.445134137142996e+00, 3.754408661907416e+00};
+
+double ro_probit(double p) {
+  if (p <= 0.0) return -INFINITY;
+  if (p >= 1.0) return INFINITY;
+  const double plow = 0.02425;
+  const double phigh = 1.0 - plow;
+  if (p < plow) {
+    double q = sqrt(-2.0 * log(p));
+    return (((((PC[0] * q + PC[1]) * q + PC[2]) * q + PC[3]) * q + PC[4]) * q + PC[5]) /
+           ((((PD[0] * q + PD[1]) * q + PD[2]) * q + PD[3]) * q + 1.0);
+  } else if (p <= phigh) {
+    double q = p - 0.5;
+    double r = q * q;
+    return (((((PA[0] * r + PA[1]) * r + PA[2]) * r + PA[3]) * r + PA[4]) * r + PA[5]) * q /
+           (((((PB[0] * r + PB[1]) * r + PB[2]) * r + PB[3]) * r + PB[4]) * r + 1.0);
+  } else {
+    double q = sqrt(-2.0 * log(1.0 - p));
+    return -(((((PC[0] * q + PC[1]) * q + PC[2]) * q + PC[3]) * q + PC[4]) * q + PC[5]) /
+           ((((PD[0] * q + PD[1]) * q + PD[2]) * q + PD[3]) * q + 1.0);
+  }
+}
+
+/* qmc.rs:25-35 */
+double ro_radical_inverse(uint64_t index, uint32_t base) {
+  double inv_base = 1.0 / (double)base;
+  double factor = inv_base;
+  double result = 0.0;
+  while (index > 0) {
+    result += factor * (double)(uint32_t)(index % (uint64_t)base);
+    index /= (uint64_t)base;
+    factor *= inv_base;
+  }
+  return result;
+}
+
+/* ------------------------------------------------------------------------- */
+/* GF(2) primitive polynomials, qmc.rs:345-430                               */
+/* ------------------------------------------------------------------------- */
+static int lz32(uint32_t x) { return x ? __builtin_clz(x) : 32; }
+
+static uint32_t gf_mulmod(uint32_t a32, uint32_t b32, uint32_t mask) { /* qmc.rs:365-382 */
+  int s = 31 - lz32(mask);
+  uint64_t r = 0, a = a32, b = b32;
+  while (b > 0) {
+    if (b & 1) r ^= a;
+    a <<= 1;
+    b >>= 1;
+  }
+  for (int bit = 63; bit >= s; --bit)
+    if ((r >> bit) & 1) r ^= (uint64_t)mask << (bit - s);
+  return (uint32_t)r;
+}
+
+static uint32_t gf_powmod(uint32_t base, uint64_t exp, uint32_t mask) { /* qmc.rs:384-395 */
+  uint32_t result = 1, b = base;
+  while (exp > 0) {
+    if (exp & 1) result = gf_mulmod(result, b, mask);
+    b = gf_mulmod(b, b, mask);
+    exp >>= 1;
+  }
+  return result;
+}
+
+static int prime_factors(uint64_t n, uint64_t *out) { /* qmc.rs:414-430 */
+  int k = 0;
+  for (uint64_t d = 2; d * d <= n; ++d) {
+    if (n % d == 0) {
+      out[k++] = d;
+      while (n % d == 0) n /= d;
+    }
+  }
+  if (n > 1) out[k++] = n;
+  return k;
+}
+
+static int is_primitive(uint32_t mask, int degree) { /* qmc.rs:397-412 */
+  if (degree < 2) return 1;
+  uint64_t order = ((uint64_t)1 << degree) - 1;
+  if (gf_powmod(2, order, mask) != 1) return 0;
+  uint64_t f[64];
+  int nf = prime_factors(order, f);
+  for (int i = 0; i < nf; ++i)
+    if (gf_powmod(2, order / f[i], mask) == 1) return 0;
+  return 1;
+}
+
+int ro_primitive_polynomials(int count, uint32_t *out) { /* qmc.rs:347-363 */
+  int n = 0;
+  int degree = 2;
+  while (n < count) {
+    if (degree > 30) break;
+    uint32_t lo = 1u << degree, hi = 1u << (degree + 1);
+    uint32_t mask = lo | 1u;
+    while (mask < hi && n < count) {
+      if (is_primitive(mask, degree)) out[n++] = mask;
+      mask += 2;
+    }
+    degree += 1;
+  }
+  return n;
+}
+
+void ro_direction_numbers(uint32_t mask, uint32_t out[32]) { /* qmc.rs:319-343 */
+  int s = 31 - lz32(mask);
+  uint32_t m[33];
+  memset(m, 0, sizeof m);
+  for (int i = 1; i <= s; ++i) m[i] = 1;
+  for (int k = s + 1; k <= 32; ++k) {
+    uint32_t value = ((1u << s) * m[k - s]) ^ m[k - s];
+    for (int i = 1; i < s; ++i) {
+      uint32_t a = (mask >> (s - i)) & 1u; /* a_i = coefficient of x^{s-i} */
+      if (a == 1) value ^= (1u << i) * m[k - i];
+    }
+    m[k] = value;
+  }
+  for (int k = 0; k < 32; ++k) out[k] = m[k + 1] << (31 - k);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Owen scramble, qmc.rs:292-314                                             */
+/* ------------------------------------------------------------------------- */
+uint64_t ro_mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+uint32_t ro_owen_scramble(uint32_t x, uint64_t dim, uint64_t seed) {
+  uint32_t result = 0, prefix = 0;
+  for (int bit = 31; bit >= 0; --bit) {
+    uint32_t b = (x >> bit) & 1u;
+    uint64_t h = ro_mix64(seed ^ (dim * 0x9E3779B97F4A7C15ull) ^
+                          ((uint64_t)bit * 0xD6E8FEB86659FD93ull) ^
+                          ((uint64_t)prefix * 0xA24BAED4963EE407ull));
+    uint32_t scrambled = b ^ (uint32_t)(h >> 63);
+    result |= scrambled << bit;
+    prefix = (prefix << 1) | scrambled;
+  }
+  return result;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Sobol ladder, qmc.rs:239-290                                              */
+/* ------------------------------------------------------------------------- */
+int ro_sobol_init(ro_sobol *s, int dimension, uint64_t seed, int max_dim) {
+  if (max_dim < 1) max_dim = RO_REFERENCE_MAX_DIM;
+  if (dimension < 1) dimension = 1;
+  if (dimension > max_dim) dimension = max_dim; /* clamp(1, MAX_DIM), qmc.rs:249 */
+  s->dimension = dimension;
+  s->seed = seed;
+  s->v = (uint32_t *)calloc((size_t)dimension * 32, sizeof(uint32_t));
+  if (!s->v) return -1;
+  for (int k = 0; k < 32; ++k) s->v[k] = 1u << (31 - k); /* dimension 1: van der Corput */
+  int need = dimension - 1;
+  uint32_t *polys = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(need > 0 ? need : 1));
+  int got = need > 0 ? ro_primitive_polynomials(need, polys) : 0;
+  for (int d = 1; d < dimension; ++d) {
+    if (d - 1 < got)
+      ro_direction_numbers(polys[d - 1], s->v + (size_t)d * 32);
+    else
+      for (int k = 0; k < 32; ++k) s->v[(size_t)d * 32 + k] = 1u << (31 - k);
+  }
+  free(polys);
+  return 0;
+}
+
+void ro_sobol_free(ro_sobol *s) {
+  free(s->v);
+  s->v = NULL;
+}
+
+uint32_t ro_sobol_unscrambled(const ro_sobol *s, uint64_t n, int dim) { /* qmc.rs:269-279 */
+  const uint32_t *row = s->v + (size_t)(dim % s->dimension) * 32;
+  uint32_t x = 0;
+  for (int k = 0; k < 32; ++k) {
+    uint64_t count = (n >> k) - (n >> (k + 1)); /* wrapping_sub on u64 */
+    if (count & 1) x ^= row[k];
+  }
+  return x;
+}
+
+uint32_t ro_sobol_scrambled(const ro_sobol *s, uint64_t n, int dim) {
+  return ro_owen_scramble(ro_sobol_unscrambled(s, n, dim), (uint64_t)dim, s->seed);
+}
+
+double ro_sobol_uniform(const ro_sobol *s, uint64_t n, int dim) { /* qmc.rs:280-281 */
+  return ((double)ro_sobol_scrambled(s, n, dim) + 0.5) / 4294967296.0;
+}
+
+static double clamp_unit(double u) {
+  if (u < 1.0e-12) return 1.0e-12;
+  if (u > 1.0 - 1.0e-12) return 1.0 - 1.0e-12;
+  return u;
+}
+
+double ro_sobol_normal(const ro_sobol *s, uint64_t point, int dim) { /* qmc.rs:284-289 */
+  return ro_probit(clamp_unit(ro_sobol_uniform(s, point + 1, dim)));
+}
+
+/* ------------------------------------------------------------------------- */
+/* Halton ladder, qmc.rs:93-153                                              */
+/* ------------------------------------------------------------------------- */
+static void first_primes(int count, uint32_t *out) { /* qmc.rs:12-22 */
+  int n = 0;
+  uint32_t cand = 2;
+  while (n < count) {
+    int ok = 1;
+    for (int i = 0; i < n; ++i)
+      if (cand % out[i] == 0) {
+        ok = 0;
+        break;
+      }
+    if (ok) out[n++] = cand;
+    cand += 1;
+  }
+}
+
+static void permutation(uint32_t base, uint64_t seed, uint32_t *perm) { /* qmc.rs:129-140 */
+  for (uint32_t i = 0; i < base; ++i) perm[i] = i;
+  uint64_t state = seed ^ ((uint64_t)base * 0x9E3779B97F4A7C15ull);
+  for (uint64_t i = base - 1; i >= 1; --i) {
+    state = state * 6364136223846793005ull + 1442695040888963407ull;
+    uint64_t j = (state >> 33) % (i + 1);
+    uint32_t t = perm[i];
+    perm[i] = perm[j];
+    perm[j] = t;
+  }
+}
+
+int ro_halton_init(ro_halton *h, int dimension, uint64_t seed, int max_dim) {
+  if (max_dim < 1) max_dim = RO_REFERENCE_MAX_DIM;
+  if (dimension < 1) dimension = 1;
+  if (dimension > max_dim) dimension = max_dim;
+  h->dimension = dimension;
+  h->bases = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)dimension);
+  h->perms = (uint32_t **)malloc(sizeof(uint32_t *) * (size_t)dimension);
+  if (!h->bases || !h->perms) return -1;
+  first_primes(dimension, h->bases);
+  for (int d = 0; d < dimension; ++d) {
+    h->perms[d] = (uint32_t *)malloc(sizeof(uint32_t) * h->bases[d]);
+    permutation(h->bases[d], seed, h->perms[d]);
+  }
+  return 0;
+}
+
+void ro_halton_free(ro_halton *h) {
+  for (int d = 0; d < h->dimension; ++d) free(h->perms[d]);
+  free(h->perms);
+  free(h->bases);
+  h->perms = NULL;
+  h->bases = NULL;
+}
+
+uint32_t ro_halton_digit_perm(const ro_halton *h, int dim_index, uint32_t digit) {
+  return h->perms[dim_index][digit];
+}
+
+double ro_halton_uniform(const ro_halton *h, uint64_t n, int dim) { /* qmc.rs:143-153 */
+  int index = dim % h->dimension;
+  uint32_t base = h->bases[index];
+  const uint32_t *perm = h->perms[index];
+  double inv_base = 1.0 / (double)base;
+  double factor = inv_base;
+  double result = 0.0;
+  while (n > 0) {
+    result += factor * (double)perm[n % (uint64_t)base];
+    n /= (uint64_t)base;
+    factor *= inv_base;
+  }
+  return result;
+}
+
+double ro_halton_normal(const ro_halton *h, uint64_t point, int dim) { /* qmc.rs:119-125 */
+  return ro_probit(clamp_unit(ro_halton_uniform(h, point + 1, dim)));
+}
+
+/* ------------------------------------------------------------------------- */
+/* Brownian bridge, qmc.rs:155-191                                           */
+/* ------------------------------------------------------------------------- */
+static void bridge_fill(double *path, int lo, int hi, const double *normals, int n, int *index,
+                        double dt) {
+  if (hi <= lo + 1 || *index >= n) return;
+  int mid = (lo + hi) / 2;
+  double mean = 0.5 * (path[lo] + path[hi]);
+  double variance = ((double)(hi - mid)) * ((double)(mid - lo)) / ((double)(hi - lo)) * dt;
+  path[mid] = mean + normals[*index] * sqrt(variance);
+  *index += 1;
+  bridge_fill(path, lo, mid, normals, n, index, dt);
+  bridge_fill(path, mid, hi, normals, n, index, dt);
+}
+
+void ro_brownian_increments(const double *normals, int n, double dt, double *out) {
+  if (n == 0) return;
+  double *path = (double *)calloc((size_t)n + 1, sizeof(double));
+  path[n] = normals[0] * sqrt((double)n * dt);
+  int index = 1;
+  bridge_fill(path, 0, n, normals, n, &index, dt);
+  for (int t = 0; t < n; ++t) out[t] = path[t + 1] - path[t];
+  free(path);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Correlation, correlation.rs:74-260                                        */
+/* ------------------------------------------------------------------------- */
+int ro_corr_validate(int n, const double *c, char *err, size_t errlen) { /* :74-93 */
+  if (n <= 0) {
+    snprintf(err, errlen, "correlation matrix must have at least one factor");
+    return -1;
+  }
+  for (int i = 0; i < n; ++i) {
+    if (fabs(c[i * n + i] - 1.0) > 1.0e-9) {
+      snprintf(err, errlen, "correlation diagonal [%d] must be exactly one", i);
+      return -1;
+    }
+    for (int j = 0; j < n; ++j) {
+      double v = c[i * n + j];
+      if (!isfinite(v)) {
+        snprintf(err, errlen, "correlation entries must be finite");
+        return -1;
+      }
+      if (fabs(v) > 1.0 + 1.0e-9) {
+        snprintf(err, errlen, "correlation [%d,%d]=%g is outside [-1, 1]", i, j, v);
+        return -1;
+      }
+      if (fabs(v - c[j * n + i]) > 1.0e-9) {
+        snprintf(err, errlen, "correlation [%d,%d] is not symmetric", i, j);
+        return -1;
+      }
+    }
+  }
+  return 0;
+}
+
+static void jacobi_eigenvalues(double *a, int n, double *eig) { /* :218-260 */
+  if (n == 1) {
+    eig[0] = a[0];
+    return;
+  }
+  for (int it = 0; it < 100; ++it) {
+    int p = 0, q = 0;
+    double max = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) {
+        double v = fabs(a[i * n + j]);
+        if (v > max) {
+          max = v;
+          p = i;
+          q = j;
+        }
+      }
+    if (max < 1.0e-14) break;
+    double app = a[p * n + p], aqq = a[q * n + q], apq = a[p * n + q];
+    double theta = 0.5 * (aqq - app) / apq;
+    /* f64::signum: 1.0 for +0.0 and positive, -1.0 for -0.0 and negative */
+    double sg = signbit(theta) ? -1.0 : 1.0;
+    double t = sg / (fabs(theta) + sqrt(theta * theta + 1.0));
+    double cc = 1.0 / sqrt(t * t + 1.0);
+    double s = t * cc;
+    for (int k = 0; k < n; ++k) {
+      double akp = a[k * n + p], akq = a[k * n + q];
+      a[k * n + p] = cc * akp - s * akq;
+      a[k * n + q] = s * akp + cc * akq;
+    }
+    for (int k = 0; k < n; ++k) {
+      double apk = a[p * n + k], aqk = a[q * n + k];
+      a[p * n + k] = cc * apk - s * aqk;
+      a[q * n + k] = s * apk + cc * aqk;
+    }
+  }
+  for (int i = 0; i < n; ++i) eig[i] = a[i * n + i];
+}
+
+double ro_min_eigenvalue(int n, const double *c) { /* :96-101 */
+  double *a = (double *)malloc(sizeof(double) * (size_t)n * n);
+  double *e = (double *)malloc(sizeof(double) * (size_t)n);
+  memcpy(a, c, sizeof(double) * (size_t)n * n);
+  jacobi_eigenvalues(a, n, e);
+  double m = INFINITY;
+  for (int i = 0; i < n; ++i) m = fmin(m, e[i]); /* fold(f64::INFINITY, f64::min) */
+  free(a);
+  free(e);
+  return m;
+}
+
+void ro_psd_clamp(int n, const double *c, double floor_, double *out) { /* :113-136 */
+  double min = ro_min_eigenvalue(n, c);
+  memcpy(out, c, sizeof(double) * (size_t)n * n);
+  if (min >= floor_) return;
+  double shift = floor_ - min;
+  for (int i = 0; i < n; ++i) out[i * n + i] += shift;
+  double *inv_sqrt = (double *)malloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; ++i) inv_sqrt[i] = 1.0 / sqrt(out[i * n + i]);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) out[i * n + j] *= inv_sqrt[i] * inv_sqrt[j];
+  free(inv_sqrt);
+}
+
+int ro_cholesky(int n, const double *c, double *l, char *err, size_t errlen) { /* :142-168 */
+  if (ro_corr_validate(n, c, err, errlen)) return -1;
+  double *src = (double *)malloc(sizeof(double) * (size_t)n * n);
+  if (ro_min_eigenvalue(n, c) >= -(-1.0e-10)) /* is_positive_semidefinite(-1e-10) */
+    memcpy(src, c, sizeof(double) * (size_t)n * n);
+  else
+    ro_psd_clamp(n, c, 1.0e-8, src);
+  memset(l, 0, sizeof(double) * (size_t)n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double sum = src[i * n + j];
+      for (int k = 0; k < j; ++k) sum -= l[i * n + k] * l[j * n + k];
+      if (i == j) {
+        l[i * n + j] = sqrt(fmax(sum, 0.0)); /* sum.max(0.0).sqrt() */
+      } else {
+        double pivot = l[j * n + j];
+        l[i * n + j] = pivot > 0.0 ? sum / pivot : 0.0;
+      }
+    }
+  free(src);
+  return 0;
+}
+
+void ro_chol_apply(int n, const double *l, const double *z, double *y) { /* :196-206 */
+  for (int i = 0; i < n; ++i) {
+    double acc = 0.0;
+    for (int j = 0; j <= i; ++j) acc += l[i * n + j] * z[j];
+    y[i] = acc;
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Dupire local vol, riga-market/src/lib.rs:28-380                           */
+/* ------------------------------------------------------------------------- */
+#define LV_CLAMP_LO 0.10
+#define LV_CLAMP_HI 3.00
+
+double ro_interp_linear(int n, const double *xs, const double *ys, double x) { /* :35-57 */
+  if (n == 0) return 0.0;
+  if (x <= xs[0]) return ys[0];
+  if (x >= xs[n - 1]) return ys[n - 1];
+  for (int i = 0; i < n - 1; ++i) {
+    if (x <= xs[i + 1]) {
+      double dx = xs[i + 1] - xs[i];
+      if (dx <= 0.0) return ys[i];
+      double w = (x - xs[i]) / dx;
+      return ys[i] + w * (ys[i + 1] - ys[i]);
+    }
+  }
+  return ys[n - 1];
+}
+
+static void natural_second_derivatives(int n, const double *x, const double *y,
+                                       double *y2) { /* :130-176 */
+  for (int i = 0; i < n; ++i) y2[i] = 0.0;
+  if (n < 3) return;
+  double *h = (double *)malloc(sizeof(double) * (size_t)(n - 1));
+  for (int i = 0; i < n - 1; ++i) h[i] = x[i + 1] - x[i];
+  int bad = 0;
+  for (int i = 0; i < n - 1; ++i)
+    if (h[i] <= 0.0) bad = 1;
+  for (int i = 0; i < n; ++i)
+    if (!isfinite(y[i])) bad = 1;
+  if (bad) {
+    free(h);
+    return;
+  }
+  double *a = calloc((size_t)n, sizeof(double)), *b = calloc((size_t)n, sizeof(double));
+  double *c = calloc((size_t)n, sizeof(double)), *d = calloc((size_t)n, sizeof(double));
+  double *cp = calloc((size_t)n, sizeof(double)), *dp = calloc((size_t)n, sizeof(double));
+  b[0] = 1.0;
+  b[n - 1] = 1.0;
+  for (int i = 1; i < n - 1; ++i) b[i] = 2.0 * (h[i - 1] + h[i]);
+  for (int i = 1; i < n; ++i) a[i] = (i < n - 1) ? h[i] : 0.0;
+  for (int i = 0; i < n - 1; ++i) c[i] = (i + 1 < n - 1) ? h[i + 1] : 0.0;
+  for (int i = 1; i < n - 1; ++i)
+    d[i] = 6.0 * ((y[i + 1] - y[i]) / h[i] - (y[i] - y[i - 1]) / h[i - 1]);
+  cp[0] = c[0] / b[0];
+  dp[0] = d[0] / b[0];
+  for (int i = 1; i < n; ++i) {
+    double m = b[i] - a[i] * cp[i - 1];
+    if (fabs(m) < 1e-300) m = 1e-300;
+    cp[i] = c[i] / m;
+    dp[i] = (d[i] - a[i] * dp[i - 1]) / m;
+  }
+  y2[n - 1] = dp[n - 1];
+  for (int i = n - 2; i >= 0; --i) y2[i] = dp[i] - cp[i] * y2[i + 1];
+  free(h);
+  free(a);
+  free(b);
+  free(c);
+  free(d);
+  free(cp);
+  free(dp);
+}
+
+static double powi3(double v) { return v * v * v; } /* Rust powi(3): v*v*v via repeated mul */
+
+static void spline_eval(int np, const double *xp, int n, const double *x, const double *y,
+                        const double *y2, double *f, double *f1, double *f2) { /* :179-203 */
+  for (int k = 0; k < np; ++k) {
+    double p = xp[k];
+    int seg = 0;
+    while (seg + 1 < n - 1 && p > x[seg + 1]) seg += 1;
+    double h = x[seg + 1] - x[seg];
+    double t = (p - x[seg]) / h;
+    if (t < 0.0) t = 0.0;
+    if (t > 1.0) t = 1.0;
+    f[k] = (1.0 - t) * y[seg] + t * y[seg + 1] +
+           (h * h / 6.0) * ((t * t * t - t) * y2[seg] + (powi3(1.0 - t) - (1.0 - t)) * y2[seg + 1]);
+    f1[k] = (y[seg + 1] - y[seg] +
+             (h * h / 6.0) * ((3.0 * t * t - 1.0) * y2[seg] -
+                              (3.0 * (1.0 - t) * (1.0 - t) - 1.0) * y2[seg + 1])) /
+            h;
+    f2[k] = (1.0 - t) * y2[seg + 1] + t * y2[seg];
+  }
+}
+
+static int grid_validate(const ro_implied_grid *g, char *err, size_t errlen) { /* :94-111 */
+  if (g->n_maturities <= 0 || g->n_strikes <= 0) {
+    snprintf(err, errlen, "implied vol grid has an empty axis");
+    return -1;
+  }
+  for (int i = 0; i + 1 < g->n_maturities; ++i)
+    if (!(g->maturities[i] < g->maturities[i + 1])) {
+      snprintf(err, errlen, "implied vol maturities must be strictly increasing");
+      return -1;
+    }
+  for (int i = 0; i + 1 < g->n_strikes; ++i)
+    if (!(g->strikes[i] < g->strikes[i + 1])) {
+      snprintf(err, errlen, "implied vol strikes must be strictly increasing");
+      return -1;
+    }
+  return 0;
+}
+
+int ro_lv_calibrate(const ro_implied_grid *g, double spot, double rate, int grid_x, ro_lv *out,
+                    char *err, size_t errlen) { /* :210-299 */
+  if (grid_validate(g, err, errlen)) return -1;
+  if (!isfinite(spot) || spot <= 0.0) {
+    snprintf(err, errlen, "local-vol calibration needs a positive finite spot");
+    return -1;
+  }
+  const int n_t = g->n_maturities, ns = g->n_strikes, GX = grid_x;
+  const double *strikes = g->strikes;
+  double s_min = strikes[0] * 0.55;
+  double s_max = strikes[ns - 1] * 1.05;
+  double log_lo = log(fmax(s_min, 1e-6));
+  double log_hi = log(fmax(s_max, 1e-6));
+  out->n_t = n_t;
+  out->n_x = GX;
+  out->t_axis = malloc(sizeof(double) * (size_t)n_t);
+  out->log_s = malloc(sizeof(double) * (size_t)GX);
+  out->v = malloc(sizeof(double) * (size_t)n_t * GX);
+  for (int i = 0; i < GX; ++i)
+    out->log_s[i] = log_lo + (log_hi - log_lo) * (double)i / (double)(GX - 1);
+  double *s_axis = malloc(sizeof(double) * (size_t)GX);
+  for (int i = 0; i < GX; ++i) s_axis[i] = exp(out->log_s[i]);
+  double *w = malloc(sizeof(double) * (size_t)n_t * GX), *wx = malloc(sizeof(double) * (size_t)n_t * GX);
+  double *wxx = malloc(sizeof(double) * (size_t)n_t * GX), *xa = malloc(sizeof(double) * (size_t)n_t * GX);
+  double *xj = malloc(sizeof(double) * (size_t)ns), *y2 = malloc(sizeof(double) * (size_t)ns);
+  double *xd = malloc(sizeof(double) * (size_t)GX);
+  double *sg = malloc(sizeof(double) * (size_t)GX), *sg1 = malloc(sizeof(double) * (size_t)GX);
+  double *sg2 = malloc(sizeof(double) * (size_t)GX);
+  for (int k = 0; k < n_t; ++k) {
+    double tk = g->maturities[k];
+    double forward = spot * exp(rate * tk);
+    for (int j = 0; j < ns; ++j) xj[j] = log(strikes[j] / forward);
+    const double *y = g->vols + (size_t)k * ns;
+    natural_second_derivatives(ns, xj, y, y2);
+    double x_lo = xj[0], x_hi = xj[ns - 1];
+    for (int i = 0; i < GX; ++i) {
+      double v = log(s_axis[i] / forward);
+      /* f64::clamp(x_lo, x_hi) */
+      if (v < x_lo) v = x_lo;
+      if (v > x_hi) v = x_hi;
+      xd[i] = v;
+    }
+    spline_eval(GX, xd, ns, xj, y, y2, sg, sg1, sg2);
+    for (int i = 0; i < GX; ++i) {
+      w[k * GX + i] = sg[i] * sg[i] * tk;
+      wx[k * GX + i] = 2.0 * sg[i] * tk * sg1[i];
+      wxx[k * GX + i] = 2.0 * tk * (sg1[i] * sg1[i] + sg[i] * sg2[i]);
+      xa[k * GX + i] = xd[i];
+    }
+  }
+  const double *T = g->maturities;
+  for (int k = 0; k < n_t; ++k) {
+    for (int i = 0; i < GX; ++i) {
+      double dwdt;
+      if (n_t == 1)
+        dwdt = w[k * GX + i] / fmax(T[k], 1e-9);
+      else if (k == 0)
+        dwdt = (w[1 * GX + i] - w[0 * GX + i]) / fmax(T[1] - T[0], 1e-9);
+      else if (k == n_t - 1)
+        dwdt = (w[(n_t - 1) * GX + i] - w[(n_t - 2) * GX + i]) / fmax(T[n_t - 1] - T[n_t - 2], 1e-9);
+      else
+        dwdt = (w[(k + 1) * GX + i] - w[(k - 1) * GX + i]) / fmax(T[k + 1] - T[k - 1], 1e-9);
+      double ww = fmax(w[k * GX + i], 1e-12);
+      double wxv = wx[k * GX + i], wxxv = wxx[k * GX + i];
+      double denom = 1.0 - (xa[k * GX + i] / ww) * wxv + 0.25 * (-0.25 - 1.0 / ww) * wxv * wxv +
+                     0.5 * wxxv;
+      double lv2 = (isfinite(denom) && fabs(denom) > 1e-12) ? dwdt / denom
+                                                             : LV_CLAMP_LO * LV_CLAMP_LO;
+      double lo = LV_CLAMP_LO * LV_CLAMP_LO, hi = LV_CLAMP_HI * LV_CLAMP_HI;
+      /* f64::clamp propagates NaN; a NaN lv2 stays NaN */
+      if (lv2 < lo) lv2 = lo;
+      if (lv2 > hi) lv2 = hi;
+      out->v[k * GX + i] = sqrt(lv2);
+    }
+    out->t_axis[k] = T[k];
+  }
+  free(s_axis);
+  free(w);
+  free(wx);
+  free(wxx);
+  free(xa);
+  free(xj);
+  free(y2);
+  free(xd);
+  free(sg);
+  free(sg1);
+  free(sg2);
+  return 0;
+}
+
+int ro_lv_copy(int n_t, int n_x, const double *t_axis, const double *log_s, const double *v,
+               ro_lv *out) {
+  out->n_t = n_t;
+  out->n_x = n_x;
+  out->t_axis = malloc(sizeof(double) * (size_t)n_t);
+  out->log_s = malloc(sizeof(double) * (size_t)n_x);
+  out->v = malloc(sizeof(double) * (size_t)n_t * n_x);
+  if (!out->t_axis || !out->log_s || !out->v) return -1;
+  memcpy(out->t_axis, t_axis, sizeof(double) * (size_t)n_t);
+  memcpy(out->log_s, log_s, sizeof(double) * (size_t)n_x);
+  memcpy(out->v, v, sizeof(double) * (size_t)n_t * n_x);
+  return 0;
+}
+
+void ro_lv_free(ro_lv *lv) {
+  free(lv->t_axis);
+  free(lv->log_s);
+  free(lv->v);
+  lv->t_axis = lv->log_s = lv->v = NULL;
+}
+
+static void t_bracket(const ro_lv *lv, double t, int *i0, int *i1, double *wt) {
+  /* lib.rs:324-339 */
+  if (lv->n_t == 1) {
+    *i0 = 0;
+    *i1 = 0;
+    *wt = 0.0;
+    return;
+  }
+  double tc = t;
+  if (tc < lv->t_axis[0]) tc = lv->t_axis[0];
+  if (tc > lv->t_axis[lv->n_t - 1]) tc = lv->t_axis[lv->n_t - 1];
+  int ii = 0;
+  while (ii + 1 < lv->n_t - 1 && tc > lv->t_axis[ii + 1]) ii += 1;
+  double span = lv->t_axis[ii + 1] - lv->t_axis[ii];
+  *wt = span > 0.0 ? (tc - lv->t_axis[ii]) / span : 0.0;
+  *i0 = ii;
+  *i1 = ii + 1;
+}
+
+double ro_lv_sigma(const ro_lv *lv, double t, double s) { /* lib.rs:305-343 */
+  double x = log(fmax(s, 1e-12));
+  int n = lv->n_x, lo, hi;
+  if (x <= lv->log_s[0]) {
+    lo = hi = 0;
+  } else if (x >= lv->log_s[n - 1]) {
+    lo = hi = n - 1;
+  } else {
+    int j = 0;
+    while (j + 1 < n && x > lv->log_s[j + 1]) j += 1;
+    lo = j;
+    hi = j + 1;
+  }
+  double wx = (hi == lo) ? 0.0 : (x - lv->log_s[lo]) / (lv->log_s[hi] - lv->log_s[lo]);
+  int i, i1;
+  double wt;
+  t_bracket(lv, t, &i, &i1, &wt);
+  const double *vi = lv->v + (size_t)i * n, *vi1 = lv->v + (size_t)i1 * n;
+  double v0 = (1.0 - wx) * vi[lo] + wx * vi[hi];
+  double v1 = (1.0 - wx) * vi1[lo] + wx * vi1[hi];
+  return (1.0 - wt) * v0 + wt * v1;
+}
+
+void ro_lv_row(const ro_lv *lv, double t, double *out) { /* lib.rs:346-364 */
+  int n = lv->n_x;
+  if (lv->n_t == 1) {
+    memcpy(out, lv->v, sizeof(double) * (size_t)n);
+    return;
+  }
+  double tc = t;
+  if (tc < lv->t_axis[0]) tc = lv->t_axis[0];
+  if (tc > lv->t_axis[lv->n_t - 1]) tc = lv->t_axis[lv->n_t - 1];
+  int i = 0;
+  while (i + 1 < lv->n_t - 1 && tc > lv->t_axis[i + 1]) i += 1;
+  double span = lv->t_axis[i + 1] - lv->t_axis[i];
+  double w = span > 0.0 ? (tc - lv->t_axis[i]) / span : 0.0;
+  for (int j = 0; j < n; ++j)
+    out[j] = (1.0 - w) * lv->v[(size_t)i * n + j] + w * lv->v[(size_t)(i + 1) * n + j];
+}
+
+int ro_lv_resample(const ro_lv *src, int n_times, const double *times, ro_lv *out) {
+  out->n_t = n_times;
+  out->n_x = src->n_x;
+  out->t_axis = malloc(sizeof(double) * (size_t)n_times);
+  out->log_s = malloc(sizeof(double) * (size_t)src->n_x);
+  out->v = malloc(sizeof(double) * (size_t)n_times * src->n_x);
+  memcpy(out->t_axis, times, sizeof(double) * (size_t)n_times);
+  memcpy(out->log_s, src->log_s, sizeof(double) * (size_t)src->n_x);
+  for (int k = 0; k < n_times; ++k) ro_lv_row(src, times[k], out->v + (size_t)k * src->n_x);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Request validation, lib.rs:209-321 (F-factor form)                        */
+/* ------------------------------------------------------------------------- */
+#define FAIL(c, ...)                                  \
+  do {                                                \
+    snprintf(code, codelen, "%s", c);                 \
+    snprintf(msg, msglen, __VA_ARGS__);               \
+    return -1;                                        \
+  } while (0)
+
+int ro_validate(const ro_request *r, char *code, size_t codelen, char *msg, size_t msglen) {
+  int F = r->factors;
+  if (r->paths == 0 || r->path_block_size == 0)
+    FAIL("invalid_paths", "paths and path_block_size must be positive");
+  if (r->steps <= 0 || !isfinite(r->maturity_years) || r->maturity_years <= 0.0)
+    FAIL("invalid_schedule", "steps and maturity_years must be positive and finite");
+  if (F < 1) FAIL("invalid_market_shape", "at least one asset is required");
+  for (int f = 0; f < F; ++f) {
+    double vals[5] = {r->spots[f], r->vols[f], r->dividend_yields[f], r->initial_spots[f], 0.0};
+    for (int k = 0; k < 4; ++k)
+      if (!isfinite(vals[k])) FAIL("invalid_market", "market inputs must be finite");
+    for (int g = 0; g < F; ++g)
+      if (!isfinite(r->correlation[f * F + g])) FAIL("invalid_market", "rho must be finite");
+  }
+  if (!isfinite(r->rate)) FAIL("invalid_market", "risk_free_rate must be finite");
+  if (!isfinite(r->strike)) FAIL("invalid_market", "strike must be finite");
+  if (!isfinite(r->knock_in_barrier)) FAIL("invalid_market", "KI barrier must be finite");
+  for (int f = 0; f < F; ++f) {
+    int bad = r->spots[f] <= 0.0 || r->initial_spots[f] <= 0.0 || r->vols[f] < 0.0;
+    for (int g = 0; g < F; ++g) {
+      double v = r->correlation[f * F + g];
+      if (f == g && fabs(v - 1.0) > 1.0e-12) bad = 1;
+      if (f != g && (fabs(v) > 1.0 || fabs(v - r->correlation[g * F + f]) > 1.0e-12)) bad = 1;
+    }
+    if (bad)
+      FAIL("invalid_market", "spots must be positive, vols non-negative, and correlation in [-1,1]");
+  }
+  if (r->has_ko && (!isfinite(r->knock_out_barrier) || r->knock_out_barrier <= 0.0))
+    FAIL("invalid_trade", "knock_out_barrier must be finite and positive when present");
+  if (r->has_cb && (!isfinite(r->coupon_barrier) || r->coupon_barrier <= 0.0))
+    FAIL("invalid_trade", "coupon_barrier must be finite and positive when present");
+  if (r->has_cr && (!isfinite(r->coupon_rate) || r->coupon_rate < 0.0))
+    FAIL("invalid_trade", "coupon_rate must be finite and non-negative when present");
+  if (r->strike <= 0.0 || r->knock_in_barrier <= 0.0)
+    FAIL("invalid_trade", "strike must be positive and barriers finite");
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Pricer, note.rs:38-373                                                     */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  uint64_t count;
+  double mean, m2;
+} moments;
+
+static void observe(moments *m, double value) { /* note.rs:17-23 */
+  m->count += 1;
+  double delta = value - m->mean;
+  m->mean += delta / (double)m->count;
+  m->m2 += delta * (value - m->mean);
+}
+
+static double sample_variance(const moments *m) { /* note.rs:25-31 */
+  return m->count > 1 ? m->m2 / (double)(m->count - 1) : 0.0;
+}
+
+typedef struct {
+  int kind;
+  ro_sobol sobol;
+  ro_halton halton;
+} ladder;
+
+static double ladder_normal(const ladder *l, uint64_t point, int dim) { /* qmc.rs:203-208 */
+  return l->kind == RO_VR_SOBOL ? ro_sobol_normal(&l->sobol, point, dim)
+                                : ro_halton_normal(&l->halton, point, dim);
+}
+
+/* note.rs:83-106 for one path: F correlated series, per-factor bridge, 1/sqrt(dt). */
+static void qmc_path_shocks(const ladder *lad, const double *chol, int F, int64_t steps, double dt,
+                            uint64_t path_id, double *normals, double *incr) {
+  double inv_sqrt_dt = 1.0 / sqrt(dt);
+  double z[64], y[64];
+  for (int64_t step = 0; step < steps; ++step) {
+    for (int f = 0; f < F; ++f) z[f] = ladder_normal(lad, path_id, (int)(step * F + f));
+    ro_chol_apply(F, chol, z, y);
+    for (int f = 0; f < F; ++f) normals[f * steps + step] = y[f];
+  }
+  for (int f = 0; f < F; ++f) {
+    ro_brownian_increments(normals + f * steps, (int)steps, dt, incr + f * steps);
+    for (int64_t t = 0; t < steps; ++t) incr[f * steps + t] *= inv_sqrt_dt;
+  }
+}
+
+static int ladder_init(ladder *l, const ro_request *r) {
+  l->kind = r->vr_kind;
+  if (r->vr_kind == RO_VR_SOBOL) return ro_sobol_init(&l->sobol, r->vr_dimension, r->seed, r->max_dim);
+  if (r->vr_kind == RO_VR_HALTON)
+    return ro_halton_init(&l->halton, r->vr_dimension, r->seed, r->max_dim);
+  return -1;
+}
+
+static void ladder_free(ladder *l) {
+  if (l->kind == RO_VR_SOBOL) ro_sobol_free(&l->sobol);
+  if (l->kind == RO_VR_HALTON) ro_halton_free(&l->halton);
+}
+
+int ro_path_shocks(const ro_request *r, uint64_t path_id, double *out) {
+  int F = r->factors;
+  double *L = malloc(sizeof(double) * (size_t)F * F);
+  char err[256];
+  if (ro_cholesky(F, r->correlation, L, err, sizeof err)) {
+    free(L);
+    return -1;
+  }
+  ladder lad;
+  if (ladder_init(&lad, r)) {
+    free(L);
+    return -1;
+  }
+  double dt = r->maturity_years / (double)r->steps;
+  double *normals = malloc(sizeof(double) * (size_t)F * r->steps);
+  qmc_path_shocks(&lad, L, F, r->steps, dt, path_id, normals, out);
+  free(normals);
+  ladder_free(&lad);
+  free(L);
+  return 0;
+}
+
+int ro_price(const ro_request *r, uint64_t path_begin, uint64_t path_end, ro_result *out,
+             ro_path_record *records, char *code, size_t codelen, char *msg, size_t msglen) {
+  if (ro_validate(r, code, codelen, msg, msglen)) return -1;
+  const int F = r->factors;
+  if (F > 64) FAIL("invalid_market_shape", "the oracle supports at most 64 factors");
+  if (r->vr_kind != RO_VR_SOBOL && r->vr_kind != RO_VR_HALTON)
+    FAIL("unsupported", "the oracle restates the QMC (Sobol/Halton) lanes only");
+  const int64_t steps = r->steps;
+  const int64_t periods = r->coupon_periods < 0 ? 9 : (r->coupon_periods < 1 ? 1 : r->coupon_periods);
+  const int64_t call_start_step = r->call_start_step < 0 ? 0 : r->call_start_step;
+  const double dt = r->maturity_years / (double)steps;
+  double *L = malloc(sizeof(double) * (size_t)F * F);
+  char err[256];
+  if (ro_cholesky(F, r->correlation, L, err, sizeof err)) {
+    free(L);
+    FAIL("invalid_market", "%s", err);
+  }
+  const double rate = r->rate;
+  const double call_barrier = r->has_ko ? r->knock_out_barrier : INFINITY;
+  const double coupon_barrier = r->has_cb ? r->coupon_barrier : INFINITY;
+  const double coupon_rate = r->has_cr ? r->coupon_rate : 0.0;
+  ladder lad;
+  ladder_init(&lad, r);
+  double *normals = malloc(sizeof(double) * (size_t)F * steps);
+  double *incr = malloc(sizeof(double) * (size_t)F * steps);
+
+  moments pv_m = {0, 0, 0}, ko_m = {0, 0, 0}, ki_m = {0, 0, 0}, cp_m = {0, 0, 0};
+  double put_sum = 0.0, funding_sum = 0.0, coupon_sum = 0.0, coupon_memory_sum = 0.0;
+  uint64_t ko_count = 0, ki_count = 0, put_eligible_count = 0, coupon_path_count = 0;
+  memset(out, 0, sizeof *out);
+  double spots[64];
+
+  for (uint64_t path_id = path_begin; path_id < path_end; ++path_id) {
+    for (int f = 0; f < F; ++f) spots[f] = r->spots[f];
+    qmc_path_shocks(&lad, L, F, steps, dt, path_id, normals, incr);
+    int knocked_out = 0;
+    int64_t ko_step = -1;
+    double redemption_time = r->maturity_years;
+    int64_t current_period = 0;
+    double accrual_this_period = 0.0, unpaid_memory = 0.0;
+    double path_coupon_pv = 0.0, path_memory_coupon_pv = 0.0;
+    int path_received_coupon = 0;
+    double final_worst = 1.0;
+    for (int64_t step = 0; step < steps; ++step) {
+      int note_is_active = !knocked_out;
+      double time = (double)(step + 1) * dt;
+      for (int f = 0; f < F; ++f) {
+        double vol = r->lv ? ro_lv_sigma(&r->lv[f], (double)step * dt, spots[f]) : r->vols[f];
+        double drift = (rate - r->dividend_yields[f] - 0.5 * vol * vol) * dt;
+        spots[f] *= exp(drift + vol * sqrt(dt) * incr[f * steps + step]);
+      }
+      double worst = spots[0] / r->initial_spots[0];
+      for (int f = 1; f < F; ++f) worst = fmin(worst, spots[f] / r->initial_spots[f]);
+      final_worst = worst;
+      if (!note_is_active) continue;
+      int64_t period = (step * periods) / steps;
+      if (period > periods - 1) period = periods - 1;
+      if (period != current_period) {
+        current_period = period;
+        accrual_this_period = 0.0;
+      }
+      if (period == 0 && r->fixed_first_coupon) {
+        /* fixed first coupon: paid at the first period end */
+      } else if (worst >= coupon_barrier) {
+        int64_t a = steps * (period + 1) / periods, b = steps * period / periods;
+        int64_t period_steps = a > b ? a - b : 0; /* saturating_sub */
+        if (period_steps < 1) period_steps = 1;
+        accrual_this_period += coupon_rate / (double)period_steps;
+      }
+      int call_active = step >= call_start_step;
+      int all_above = 1;
+      for (int f = 0; f < F; ++f)
+        if (!(spots[f] / r->initial_spots[f] >= call_barrier)) {
+          all_above = 0;
+          break;
+        }
+      if (call_active && all_above) {
+        knocked_out = 1;
+        ko_step = step;
+        redemption_time = time;
+        ko_count += 1;
+        if (r->trace_level != RO_TRACE_OFF && out->n_trace < RO_TRACE_CAP) {
+          out->trace_path[out->n_trace] = (int64_t)path_id;
+          out->trace_step[out->n_trace] = step;
+          out->n_trace += 1;
+        }
+        double due = (period == 0 && r->fixed_first_coupon) ? coupon_rate : accrual_this_period;
+        double carry_paid = due > 0.0 ? unpaid_memory : 0.0;
+        double coupon_amount = due + carry_paid;
+        if (coupon_amount > 0.0) {
+          double df = exp(-rate * time);
+          path_coupon_pv += due * df;
+          path_memory_coupon_pv += carry_paid * df;
+          path_received_coupon = 1;
+        }
+        continue;
+      }
+      int64_t next_period;
+      if (step + 1 == steps) {
+        next_period = periods;
+      } else {
+        next_period = ((step + 1) * periods) / steps;
+        if (next_period > periods - 1) next_period = periods - 1;
+      }
+      int is_period_end = (step + 1 == steps) || next_period != period;
+      if (is_period_end) {
+        double due = (period == 0 && r->fixed_first_coupon) ? coupon_rate : accrual_this_period;
+        if (due > 0.0) {
+          double df = exp(-rate * time);
+          path_coupon_pv += due * df;
+          path_memory_coupon_pv += unpaid_memory * df;
+          path_received_coupon = 1;
+          unpaid_memory = 0.0;
+        } else if (r->memory_coupon) {
+          unpaid_memory += coupon_rate;
+        }
+      }
+    }
+    double funding = exp(-rate * redemption_time);
+    if (knocked_out) observe(&ko_m, funding);
+    int hit_ki = final_worst <= r->knock_in_barrier;
+    if (hit_ki) ki_count += 1;
+    int put_eligible = hit_ki && !knocked_out;
+    if (put_eligible) put_eligible_count += 1;
+    double put = put_eligible ? fmax(r->strike - final_worst, 0.0) * exp(-rate * r->maturity_years)
+                              : 0.0;
+    if (hit_ki) observe(&ki_m, put);
+    double coupon_nominal = path_coupon_pv + path_memory_coupon_pv;
+    if (path_received_coupon) {
+      coupon_path_count += 1;
+      observe(&cp_m, coupon_nominal);
+    }
+    double coupon_quote = coupon_nominal * 10.0;
+    double memory_quote = path_memory_coupon_pv * 10.0;
+    double pv = funding + coupon_nominal - put;
+    funding_sum += funding;
+    put_sum += put;
+    coupon_sum += coupon_quote - memory_quote;
+    coupon_memory_sum += memory_quote;
+    observe(&pv_m, pv);
+    if (records) {
+      ro_path_record *rec = &records[path_id - path_begin];
+      rec->ko_step = (int32_t)ko_step;
+      rec->hit_ki = hit_ki;
+      rec->final_worst = final_worst;
+      rec->funding = funding;
+      rec->put = put;
+      rec->coupon_pv = path_coupon_pv;
+      rec->memory_pv = path_memory_coupon_pv;
+      rec->pv = pv;
+    }
+  }
+  /* aggregates, note.rs:262-271 */
+  double n = (double)(path_end - path_begin);
+  double variance = sample_variance(&pv_m);
+  out->paths = path_end - path_begin;
+  out->mc_error = sqrt(variance / n);
+  out->ki_probability = (double)ki_count / n;
+  double mean_put = put_sum / n, mean_funding = funding_sum / n;
+  double mean_coupon_quote = coupon_sum / n, mean_memory_quote = coupon_memory_sum / n;
+  double total = mean_funding + (mean_coupon_quote + mean_memory_quote) / 10.0 - mean_put;
+  out->pv = total;
+  out->leg_coupon = mean_coupon_quote;
+  out->leg_memory_coupon = mean_memory_quote;
+  out->leg_put = mean_put;
+  out->leg_funding = mean_funding;
+  out->leg_total = total;
+  out->ko_count = ko_count;
+  out->ki_count = ki_count;
+  out->put_eligible_count = put_eligible_count;
+  out->coupon_path_count = coupon_path_count;
+  out->ko_mean = ko_m.mean;
+  out->ko_var = sample_variance(&ko_m);
+  out->ki_mean = ki_m.mean;
+  out->ki_var = sample_variance(&ki_m);
+  out->cpn_mean = cp_m.mean;
+  out->cpn_var = sample_variance(&cp_m);
+  out->funding_sum = funding_sum;
+  out->put_sum = put_sum;
+  out->coupon_sum = coupon_sum;
+  out->memory_sum = coupon_memory_sum;
+  free(normals);
+  free(incr);
+  free(L);
+  ladder_free(&lad);
+  return 0;
+}

@@ -1,0 +1,191 @@
+"""ctypes binding of ``libriga_hip.so`` (the C ABI declared in include/riga_hip.h).
+
+The library is built in-tree by ``riga_flow_amd/csrc/Makefile``.  There is no
+CPU fallback: when the library or the device is missing every pricing call
+raises ``RigaError("gpu_unavailable", ...)``, as the reference's GPU build
+does (crates/riga-engine/src/lib.rs:188-202, gpu.rs:490-497).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import threading
+
+import numpy as np
+
+from .types import RigaError
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libriga_hip.so")
+TRACE_CAP = 64
+
+FP64 = 0
+FP32 = 1
+VR_NONE, VR_HALTON, VR_SOBOL = 0, 1, 2
+TRACE = {"off": 0, "aggregated": 1, "sampled": 2, "full": 3}
+
+_D = C.POINTER(C.c_double)
+
+
+class Error(C.Structure):
+    _fields_ = [("code", C.c_char * 48), ("message", C.c_char * 464)]
+
+
+class ImpliedGrid(C.Structure):
+    _fields_ = [("n_maturities", C.c_int32), ("n_strikes", C.c_int32),
+                ("maturities", _D), ("strikes", _D), ("vols", _D)]
+
+
+class LvGrid(C.Structure):
+    _fields_ = [("n_t", C.c_int32), ("n_x", C.c_int32), ("t_axis", _D), ("log_s", _D), ("v", _D)]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("factors", C.c_int32),
+        ("spots", _D), ("vols", _D), ("dividend_yields", _D), ("correlation", _D),
+        ("risk_free_rate", C.c_double),
+        ("initial_spots", _D),
+        ("maturity_years", C.c_double),
+        ("steps", C.c_int64),
+        ("strike", C.c_double), ("knock_in_barrier", C.c_double),
+        ("has_knock_out", C.c_int32), ("knock_out_barrier", C.c_double),
+        ("has_coupon_barrier", C.c_int32), ("coupon_barrier", C.c_double),
+        ("has_coupon_rate", C.c_int32), ("coupon_rate", C.c_double),
+        ("memory_coupon", C.c_int32),
+        ("coupon_periods", C.c_int64),
+        ("fixed_first_coupon", C.c_int32),
+        ("call_start_step", C.c_int64),
+        ("paths", C.c_uint64), ("path_block_size", C.c_uint64), ("seed", C.c_uint64),
+        ("trace_level", C.c_int32), ("ladder", C.c_int32), ("ladder_dimension", C.c_int32),
+        ("max_dimension", C.c_int32),
+        ("implied", C.POINTER(ImpliedGrid)),
+        ("calibrated", C.POINTER(LvGrid)),
+        ("grid_x", C.c_int32),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("pv", C.c_double), ("mc_error", C.c_double), ("ki_probability", C.c_double),
+        ("leg_coupon", C.c_double), ("leg_memory_coupon", C.c_double), ("leg_put", C.c_double),
+        ("leg_funding", C.c_double), ("leg_total", C.c_double),
+        ("paths", C.c_uint64), ("ko_count", C.c_uint64), ("ki_count", C.c_uint64),
+        ("put_eligible_count", C.c_uint64), ("coupon_path_count", C.c_uint64),
+        ("ko_mean", C.c_double), ("ko_variance", C.c_double), ("ki_mean", C.c_double),
+        ("ki_variance", C.c_double), ("coupon_mean", C.c_double), ("coupon_variance", C.c_double),
+        ("n_trace", C.c_int32),
+        ("trace_path", C.c_int64 * TRACE_CAP),
+        ("trace_step", C.c_int64 * TRACE_CAP),
+        ("device_ms", C.c_double),
+    ]
+
+
+class PathRecord(C.Structure):
+    _fields_ = [("ko_step", C.c_int32), ("hit_ki", C.c_int32), ("final_worst", C.c_double),
+                ("funding", C.c_double), ("put", C.c_double), ("coupon_pv", C.c_double),
+                ("memory_pv", C.c_double), ("pv", C.c_double)]
+
+
+RECORD_DTYPE = np.dtype([("ko_step", "<i4"), ("hit_ki", "<i4"), ("final_worst", "<f8"),
+                         ("funding", "<f8"), ("put", "<f8"), ("coupon_pv", "<f8"),
+                         ("memory_pv", "<f8"), ("pv", "<f8")])
+
+
+class LaunchConfig(C.Structure):
+    _fields_ = [("paths_per_thread", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
+EXPORTS = ("riga_hip_init", "riga_hip_destroy", "riga_hip_device_name", "riga_hip_arch",
+           "riga_validate", "riga_price_put_leg", "riga_price_batch", "riga_replay_paths",
+           "riga_lv_calibrate", "riga_lv_sigma", "riga_debug_sobol")
+
+_lib = None
+_lock = threading.Lock()
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "csrc")], check=True)
+    return LIB_PATH
+
+
+def lib():
+    """Load the in-tree library (no implicit rebuild, no fallback)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RigaError("gpu_unavailable",
+                                f"{LIB_PATH} is not built; run riga_flow_amd/csrc/Makefile")
+            L = C.CDLL(LIB_PATH)
+            L.riga_hip_init.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.POINTER(Error)]
+            L.riga_hip_destroy.argtypes = [C.c_void_p]
+            L.riga_hip_destroy.restype = None
+            L.riga_hip_device_name.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t, C.POINTER(Error)]
+            L.riga_hip_arch.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t, C.POINTER(Error)]
+            L.riga_validate.argtypes = [C.POINTER(Request), C.POINTER(Error)]
+            L.riga_price_put_leg.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int,
+                                             C.POINTER(Result), C.POINTER(Error)]
+            L.riga_price_batch.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int, C.c_int,
+                                           C.POINTER(LaunchConfig), C.POINTER(Result),
+                                           C.POINTER(Error)]
+            L.riga_replay_paths.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int,
+                                            C.POINTER(PathRecord), C.POINTER(Result),
+                                            C.POINTER(Error)]
+            L.riga_lv_calibrate.argtypes = [C.POINTER(ImpliedGrid), C.c_double, C.c_double,
+                                            C.c_int32, _D, _D, _D, C.POINTER(Error)]
+            L.riga_lv_sigma.argtypes = [C.POINTER(LvGrid), C.c_double, C.c_double]
+            L.riga_lv_sigma.restype = C.c_double
+            L.riga_debug_sobol.argtypes = [C.c_void_p, C.c_uint64, C.c_int32, C.c_uint64,
+                                           C.c_int32, C.c_int32, C.c_int32,
+                                           C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _D,
+                                           C.POINTER(Error)]
+            _lib = L
+    return _lib
+
+
+def check(rc: int, err: Error) -> None:
+    if rc != 0:
+        raise RigaError(err.code.decode() or "gpu_error", err.message.decode())
+
+
+class Device:
+    """The process-global device (``gpu.rs`` ``OnceLock<Gpu>``)."""
+
+    _instance = None
+    _inst_lock = threading.Lock()
+
+    def __init__(self, index: int = 0):
+        L = lib()
+        h = C.c_void_p()
+        err = Error()
+        check(L.riga_hip_init(index, C.byref(h), C.byref(err)), err)
+        self.handle = h
+
+    @classmethod
+    def get(cls) -> "Device":
+        with cls._inst_lock:
+            if cls._instance is None:
+                cls._instance = Device(0)
+            return cls._instance
+
+    def name(self) -> str:
+        buf = C.create_string_buffer(256)
+        err = Error()
+        check(lib().riga_hip_device_name(self.handle, buf, 256, C.byref(err)), err)
+        return buf.value.decode()
+
+    def arch(self) -> str:
+        buf = C.create_string_buffer(64)
+        err = Error()
+        check(lib().riga_hip_arch(self.handle, buf, 64, C.byref(err)), err)
+        return buf.value.decode()
+
+
+def darr(values) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(values, dtype=np.float64))
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(_D)

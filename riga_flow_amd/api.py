@@ -1,0 +1,380 @@
+"""The pricing API of the gfx950 backend, mirroring riga-engine's surface.
+
+    gpu_enabled()                 crates/riga-engine/src/lib.rs:205-207
+    gpu_backend()                 crates/riga-engine/src/lib.rs:193-202 (fails loudly)
+    device_name(), arch()         crates/riga-engine/src/gpu.rs:504-512
+    validate(request)             crates/riga-engine/src/lib.rs:209-321 (F factors)
+    price_put_leg(request)        crates/riga-engine/src/lib.rs:183-186
+    price_batch(requests)         many requests in one launch (CRN items)
+    replay_paths(request)         crates/riga-engine/src/explain.rs:90-316
+    build_local_vol(request)      crates/riga-engine/src/lib.rs:326-343
+
+``PriceResult`` is assembled exactly as note.rs:262-372 does (stats, ledger,
+evidence strings, trace events, determinism envelope with the market
+snapshot id of riga-core/src/lib.rs:47-55).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import math
+from decimal import Decimal
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _native as N
+from .types import (DeterminismEnvelope, LedgerEntry, LegAttribution, LocalVolGrid, PriceRequest,
+                    PriceResult, RigaError, TransitionStats)
+
+PRECISIONS = {"fp64": N.FP64, "fp32": N.FP32}
+
+
+def gpu_enabled() -> bool:
+    """This build always advertises the HIP backend."""
+    return True
+
+
+def gpu_backend() -> None:
+    """Initialise the process-global device or raise ``gpu_unavailable``."""
+    N.Device.get()
+
+
+def device_name() -> str:
+    return N.Device.get().name()
+
+
+def arch() -> str:
+    return N.Device.get().arch()
+
+
+# ------------------------------------------------------------------ packing
+class _Packed:
+    """A ``riga_request`` plus the numpy buffers it points into."""
+
+    def __init__(self, request: PriceRequest, max_dimension: int = 512):
+        t, m, c = request.trade, request.market, request.config
+        F = len(m.spots)
+        keep = []
+
+        def arr(v):
+            a = N.darr(v)
+            keep.append(a)
+            return N.dptr(a)
+
+        r = N.Request()
+        r.factors = F
+        r.spots = arr(m.spots)
+        r.vols = arr(m.vols)
+        r.dividend_yields = arr(m.dividend_yields)
+        r.correlation = arr(np.asarray(m.correlation, dtype=np.float64).reshape(-1))
+        r.risk_free_rate = m.risk_free_rate
+        r.initial_spots = arr(t.initial_spots)
+        r.maturity_years = t.maturity_years
+        r.steps = t.steps
+        r.strike = t.strike
+        r.knock_in_barrier = t.knock_in_barrier
+        r.has_knock_out = t.knock_out_barrier is not None
+        r.knock_out_barrier = t.knock_out_barrier if t.knock_out_barrier is not None else 0.0
+        r.has_coupon_barrier = t.coupon_barrier is not None
+        r.coupon_barrier = t.coupon_barrier if t.coupon_barrier is not None else 0.0
+        r.has_coupon_rate = t.coupon_rate is not None
+        r.coupon_rate = t.coupon_rate if t.coupon_rate is not None else 0.0
+        r.memory_coupon = bool(t.memory_coupon)
+        r.coupon_periods = -1 if t.coupon_periods is None else t.coupon_periods
+        r.fixed_first_coupon = bool(t.fixed_first_coupon)
+        r.call_start_step = -1 if t.call_start_step is None else t.call_start_step
+        r.paths = c.paths
+        r.path_block_size = c.path_block_size
+        r.seed = c.seed
+        r.trace_level = N.TRACE[c.trace_level]
+        r.ladder = {"none": N.VR_NONE, "halton": N.VR_HALTON, "sobol": N.VR_SOBOL}[
+            c.variance_reduction.kind]
+        r.ladder_dimension = c.variance_reduction.dimension
+        r.max_dimension = max_dimension
+        lv = request.local_vol
+        if lv is not None:
+            if lv.calibrated is not None:
+                grids = (N.LvGrid * F)()
+                for f, g in enumerate(lv.calibrated):
+                    v = np.asarray(g.v, dtype=np.float64)
+                    grids[f] = N.LvGrid(len(g.t_axis), len(g.log_s), arr(g.t_axis), arr(g.log_s),
+                                        arr(v.reshape(-1)))
+                keep.append(grids)
+                r.calibrated = C.cast(grids, C.POINTER(N.LvGrid))
+            else:
+                if len(lv.grids) != F:
+                    raise RigaError("invalid_local_vol",
+                                    "local_vol.grids must be aligned with market.spots")
+                grids = (N.ImpliedGrid * F)()
+                for f, g in enumerate(lv.grids):
+                    v = np.asarray(g.vols, dtype=np.float64)
+                    grids[f] = N.ImpliedGrid(len(g.maturities_years), len(g.strikes),
+                                             arr(g.maturities_years), arr(g.strikes),
+                                             arr(v.reshape(-1)))
+                keep.append(grids)
+                r.implied = C.cast(grids, C.POINTER(N.ImpliedGrid))
+            r.grid_x = lv.grid_x
+        self.r = r
+        self.keep = keep
+
+
+def _check_as_of(request: PriceRequest) -> None:
+    """lib.rs:213-224: as_of is required and must match the market snapshot."""
+    if not request.as_of:
+        raise RigaError("missing_as_of", "as_of is required")
+    if request.as_of != request.market.as_of:
+        raise RigaError("as_of_mismatch", "request as_of must match the market snapshot as_of")
+
+
+def validate(request: PriceRequest, max_dimension: int = 512) -> None:
+    _check_as_of(request)
+    p = _Packed(request, max_dimension)
+    err = N.Error()
+    N.check(N.lib().riga_validate(C.byref(p.r), C.byref(err)), err)
+
+
+# ------------------------------------------------------------------ JSON / snapshot id
+def _ryu(x: float) -> str:
+    """Shortest round-trip float text in the layout serde_json (ryu) prints."""
+    if x != x or x in (math.inf, -math.inf):
+        return "null"
+    if x == 0.0:
+        return "-0.0" if math.copysign(1.0, x) < 0 else "0.0"
+    sign = "-" if x < 0 else ""
+    t = Decimal(repr(abs(x))).as_tuple()  # repr is the shortest round-trip digit string
+    digits = "".join(map(str, t.digits)).lstrip("0")
+    k = t.exponent
+    stripped = digits.rstrip("0")
+    k += len(digits) - len(stripped)
+    digits = stripped
+    length = len(digits)
+    kk = length + k  # 10^(kk-1) <= |x| < 10^kk
+    if 0 <= k and kk <= 16:
+        return sign + digits + "0" * k + ".0"
+    if 0 < kk <= 16:
+        return sign + digits[:kk] + "." + digits[kk:]
+    if -5 < kk <= 0:
+        return sign + "0." + "0" * (-kk) + digits
+    if length == 1:
+        return sign + digits + "e" + str(kk - 1)
+    return sign + digits[0] + "." + digits[1:] + "e" + str(kk - 1)
+
+
+def _json(v) -> str:
+    if isinstance(v, bool):
+        return "true" if v else "false"
+    if isinstance(v, int):
+        return str(v)
+    if isinstance(v, float):
+        return _ryu(v)
+    if isinstance(v, str):
+        out = ['"']
+        for ch in v:
+            if ch == '"':
+                out.append('\\"')
+            elif ch == "\\":
+                out.append("\\\\")
+            elif ch == "\n":
+                out.append("\\n")
+            elif ch == "\r":
+                out.append("\\r")
+            elif ch == "\t":
+                out.append("\\t")
+            elif ord(ch) < 0x20:
+                out.append("\\u%04x" % ord(ch))
+            else:
+                out.append(ch)
+        out.append('"')
+        return "".join(out)
+    if isinstance(v, (list, tuple)):
+        return "[" + ",".join(_json(x) for x in v) + "]"
+    if isinstance(v, dict):
+        return "{" + ",".join(_json(k) + ":" + _json(x) for k, x in v.items()) + "}"
+    raise TypeError(type(v))
+
+
+def market_snapshot_id(market) -> str:
+    """``MarketSnapshot::snapshot_id`` (riga-core/src/lib.rs:47-55)."""
+    doc = {
+        "as_of": market.as_of,
+        "spots": [float(x) for x in market.spots],
+        "vols": [float(x) for x in market.vols],
+        "risk_free_rate": float(market.risk_free_rate),
+        "dividend_yields": [float(x) for x in market.dividend_yields],
+        "correlation": [[float(x) for x in row] for row in market.correlation],
+    }
+    return "sha256:" + hashlib.sha256(_json(doc).encode()).hexdigest()
+
+
+# ------------------------------------------------------------------ result assembly
+def _assemble(request: PriceRequest, r: N.Result) -> PriceResult:
+    """note.rs:262-372 from the device aggregates."""
+    t = request.trade
+    paths = int(r.paths)
+    steps = t.steps
+    dt = t.maturity_years / steps
+    call_barrier = t.knock_out_barrier if t.knock_out_barrier is not None else math.inf
+    coupon_barrier = t.coupon_barrier if t.coupon_barrier is not None else math.inf
+    snapshot_id = market_snapshot_id(request.market)
+    ko, ki, cp = int(r.ko_count), int(r.ki_count), int(r.coupon_path_count)
+    ko_prob = ko / paths
+    cp_prob = cp / paths
+    stats = [
+        TransitionStats("daily_global_ko", paths, ko, ko_prob, r.ko_mean, r.ko_variance),
+        TransitionStats("final_fixing_eki", paths, ki, r.ki_probability, r.ki_mean, r.ki_variance),
+        TransitionStats("memory_coupon", paths, cp, cp_prob, r.coupon_mean, r.coupon_variance),
+    ]
+    evidence = [f"trade:{t.trade_id}", f"market:{snapshot_id}", f"seed:{request.config.seed}"]
+    ledger = [
+        LedgerEntry("daily_global_ko",
+                    f"all names >= {call_barrier:.6f} during callable observations",
+                    paths, ko, r.leg_funding * ko_prob, 0.0, 0.0, list(evidence)),
+        LedgerEntry("final_fixing_eki",
+                    f"terminal worst <= {t.knock_in_barrier:.6f} and no prior autocall; "
+                    f"put strike {t.strike:.6f}",
+                    paths - ko, int(r.put_eligible_count), -r.leg_put, 0.0, 0.0, list(evidence)),
+        LedgerEntry("memory_coupon",
+                    f"daily worst-of >= coupon barrier {coupon_barrier:.6f}; memory carry "
+                    f"enabled={'true' if t.memory_coupon else 'false'}",
+                    paths, cp, (r.leg_coupon + r.leg_memory_coupon) / 10.0, 0.0, 0.0,
+                    list(evidence)),
+    ]
+    trace = []
+    if request.config.trace_level != "off":
+        for i in range(r.n_trace):
+            step = int(r.trace_step[i])
+            time = (step + 1) * dt
+            trace.append(f"daily_global_ko:path={int(r.trace_path[i])}:t={time:.6f}")
+    legs = LegAttribution(0.0, r.leg_coupon, r.leg_memory_coupon, r.leg_put, r.leg_funding, 0.0,
+                          r.leg_total)
+    res = PriceResult(r.pv, r.mc_error, r.ki_probability, legs, stats, ledger, trace,
+                      DeterminismEnvelope(request.config.seed, request.as_of, snapshot_id))
+    res.device_ms = r.device_ms
+    return res
+
+
+def result_to_json(res: PriceResult) -> dict:
+    """The serde field layout of ``PriceResult`` (riga-core/src/lib.rs:243-255)."""
+    out = {
+        "pv": res.pv, "mc_error": res.mc_error, "ki_probability": res.ki_probability,
+        "legs": dict(vars(res.legs)),
+        "stats": [dict(vars(s)) for s in res.stats],
+        "ledger": [dict(vars(l)) for l in res.ledger],
+    }
+    if res.trace_events:
+        out["trace_events"] = list(res.trace_events)
+    out["determinism"] = dict(vars(res.determinism))
+    return out
+
+
+# ------------------------------------------------------------------ pricing
+def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",
+                paths_per_thread: int = 1, max_dimension: int = 512,
+                raw: bool = False):
+    """Price many requests in one launch; CRN scenarios share their Sobol points."""
+    gpu_backend()
+    for q in requests:
+        _check_as_of(q)
+    packed = [_Packed(q, max_dimension) for q in requests]
+    n = len(packed)
+    arr = (N.Request * n)(*[p.r for p in packed])
+    res = (N.Result * n)()
+    cfg = N.LaunchConfig(paths_per_thread)
+    err = N.Error()
+    dev = N.Device.get()
+    N.check(N.lib().riga_price_batch(dev.handle, arr, n, PRECISIONS[precision], C.byref(cfg), res,
+                                     C.byref(err)), err)
+    if raw:
+        return list(res)
+    return [_assemble(q, res[i]) for i, q in enumerate(requests)]
+
+
+def price_put_leg(request: PriceRequest, precision: str = "fp64",
+                  max_dimension: int = 512) -> PriceResult:
+    """``price_put_leg`` (lib.rs:183-186) on the device."""
+    return price_batch([request], precision, max_dimension=max_dimension)[0]
+
+
+def replay_paths(request: PriceRequest, precision: str = "fp64",
+                 max_dimension: int = 512) -> Tuple[np.ndarray, PriceResult]:
+    """Per-path records (explain.rs:90-316); the only call that writes paths to HBM."""
+    gpu_backend()
+    _check_as_of(request)
+    p = _Packed(request, max_dimension)
+    recs = (N.PathRecord * request.config.paths)()
+    res = N.Result()
+    err = N.Error()
+    N.check(N.lib().riga_replay_paths(N.Device.get().handle, C.byref(p.r), PRECISIONS[precision],
+                                      recs, C.byref(res), C.byref(err)), err)
+    arr = np.frombuffer(recs, dtype=N.RECORD_DTYPE, count=request.config.paths).copy()
+    return arr, _assemble(request, res)
+
+
+def lv_calibrate(maturities, strikes, vols, spot: float, rate: float,
+                 grid_x: int = 96) -> LocalVolGrid:
+    """``DupireLocalVol::calibrate`` (riga-market/src/lib.rs:210-299) on the host."""
+    m, k = N.darr(maturities), N.darr(strikes)
+    v = N.darr(vols).reshape(-1)
+    g = N.ImpliedGrid(len(m), len(k), N.dptr(m), N.dptr(k), N.dptr(v))
+    t = np.zeros(len(m))
+    x = np.zeros(grid_x)
+    out = np.zeros(len(m) * grid_x)
+    err = N.Error()
+    N.check(N.lib().riga_lv_calibrate(C.byref(g), spot, rate, grid_x, N.dptr(t), N.dptr(x),
+                                      N.dptr(out), C.byref(err)), err)
+    return LocalVolGrid(t.tolist(), x.tolist(), out.reshape(len(m), grid_x).tolist())
+
+
+def lv_sigma(grid: LocalVolGrid, t: float, s: float) -> float:
+    ta, xa = N.darr(grid.t_axis), N.darr(grid.log_s)
+    va = N.darr(grid.v).reshape(-1)
+    g = N.LvGrid(len(ta), len(xa), N.dptr(ta), N.dptr(xa), N.dptr(va))
+    return N.lib().riga_lv_sigma(C.byref(g), t, s)
+
+
+def lv_row(grid: LocalVolGrid, t: float) -> List[float]:
+    """``DupireLocalVol::row`` (riga-market/src/lib.rs:346-364)."""
+    ta = grid.t_axis
+    if len(ta) == 1:
+        return list(grid.v[0])
+    tc = min(max(t, ta[0]), ta[-1])
+    i = 0
+    while i + 1 < len(ta) - 1 and tc > ta[i + 1]:
+        i += 1
+    span = ta[i + 1] - ta[i]
+    w = (tc - ta[i]) / span if span > 0.0 else 0.0
+    return [(1.0 - w) * a + w * b for a, b in zip(grid.v[i], grid.v[i + 1])]
+
+
+def lv_resample(grid: LocalVolGrid, times: Sequence[float]) -> LocalVolGrid:
+    """Rows of ``grid`` at ``times`` with ``row(t)`` semantics (a 64 x n_x grid for the books)."""
+    return LocalVolGrid([float(t) for t in times], list(grid.log_s), [lv_row(grid, t) for t in times])
+
+
+def build_local_vol(request: PriceRequest) -> Optional[List[LocalVolGrid]]:
+    """``build_local_vol`` (lib.rs:326-343): one calibrated surface per factor."""
+    spec = request.local_vol
+    if spec is None:
+        return None
+    if spec.calibrated is not None:
+        return spec.calibrated
+    return [lv_calibrate(g.maturities_years, g.strikes, g.vols, request.market.spots[f],
+                         request.market.risk_free_rate, spec.grid_x)
+            for f, g in enumerate(spec.grids)]
+
+
+def debug_sobol(seed: int, ladder_dims: int, first_point: int, n_points: int, first_dim: int,
+                n_dims: int):
+    """Device Sobol integers (unscrambled, Owen-scrambled) and FP64 normals."""
+    gpu_backend()
+    cells = n_points * n_dims
+    u = np.zeros(cells, dtype=np.uint32)
+    s = np.zeros(cells, dtype=np.uint32)
+    z = np.zeros(cells, dtype=np.float64)
+    err = N.Error()
+    N.check(N.lib().riga_debug_sobol(
+        N.Device.get().handle, seed, ladder_dims, first_point, n_points, first_dim, n_dims,
+        u.ctypes.data_as(C.POINTER(C.c_uint32)), s.ctypes.data_as(C.POINTER(C.c_uint32)),
+        N.dptr(z), C.byref(err)), err)
+    return (u.reshape(n_points, n_dims), s.reshape(n_points, n_dims), z.reshape(n_points, n_dims))

@@ -1,0 +1,835 @@
+// engine.cpp -- batch planner, launcher and the extern "C" entry points.
+//
+// A batch of requests is grouped into items: requests that share seed, path
+// count, ladder, schedule, payoff terms, rate and correlation differ only in
+// spots, dividend yields and vols/surfaces, i.e. they are the reference's CRN
+// scenarios (riga-risk/src/lib.rs:117-186, dispatch.rs:23-70) and one lane
+// prices all of them from one set of Sobol points.  Items are laid out in one
+// device arena, launched per (factor count, scenario bucket) longest-first,
+// merged by the finalize kernel, and turned back into PriceResult fields
+// with the aggregate formulas of note.rs:262-307.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "host.h"
+#include "kernels.h"
+
+using namespace riga;
+
+namespace {
+
+void set_err(riga_error* e, const std::string& code, const std::string& msg) {
+  if (!e) return;
+  snprintf(e->code, sizeof e->code, "%s", code.c_str());
+  snprintf(e->message, sizeof e->message, "%s", msg.c_str());
+}
+
+void clear_err(riga_error* e) {
+  if (!e) return;
+  e->code[0] = 0;
+  e->message[0] = 0;
+}
+
+constexpr int RIGA_EINVAL = -1;
+constexpr int RIGA_EGPU = -2;
+constexpr int RIGA_ENODEV = -3;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, (size_t)4096);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct HostBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, (size_t)4096);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  ~HostBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+}  // namespace
+
+struct riga_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string name, arch;
+  size_t lds_limit = 65536;
+  std::mutex mu;
+  DevBuf arena, items, block_item, partials, finals, final_base, records;
+  HostBuf staging, finals_host;
+};
+
+namespace {
+
+#define HIP_TRY(expr, what)                                                              \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess) {                                                              \
+      set_err(err, "gpu_error", std::string(what) + ": " + hipGetErrorString(_e));      \
+      return RIGA_EGPU;                                                                  \
+    }                                                                                    \
+  } while (0)
+
+// ------------------------------------------------------------ per-request prep
+struct Prep {
+  const riga_request* r = nullptr;
+  int F = 0;
+  int ladder_dims = 0;
+  std::vector<double> chol;
+  std::vector<LvGrid> lv;  // one per factor when local vol
+  std::string key;         // CRN group key
+};
+
+template <typename T>
+void put_bytes(std::string& k, const T& v) {
+  k.append(reinterpret_cast<const char*>(&v), sizeof v);
+}
+
+bool prepare(const riga_request& r, Prep* p, Error* e) {
+  if (!validate_request(r, e)) return false;
+  const int F = r.factors;
+  if (F > kMaxFactors) {
+    e->code = "invalid_market_shape";
+    e->message = "the gfx950 kernels price at most " + std::to_string(kMaxFactors) + " underlyings per trade";
+    return false;
+  }
+  if (r.steps > 1000000 || r.paths >= (1ull << 31)) {
+    e->code = "invalid_schedule";
+    e->message = "steps and paths exceed the supported range";
+    return false;
+  }
+  p->r = &r;
+  p->F = F;
+  if (!cholesky_factor(F, r.correlation, &p->chol, e)) return false;
+  const int cap = r.max_dimension > 0 ? r.max_dimension : 512;
+  p->ladder_dims = std::max(1, std::min(r.ladder_dimension, cap));  // qmc.rs:105,249 clamp
+  const bool has_lv = r.implied != nullptr || r.calibrated != nullptr;
+  if (has_lv) {
+    p->lv.resize(F);
+    for (int f = 0; f < F; ++f) {
+      if (r.calibrated) {
+        const riga_lv_grid& g = r.calibrated[f];
+        if (g.n_t < 1 || g.n_x < 2 || g.n_x > kMaxLvX) {
+          e->code = "invalid_local_vol";
+          e->message = "calibrated local-vol grid must have 1+ tenors and 2.." + std::to_string(kMaxLvX) + " knots";
+          return false;
+        }
+        p->lv[f].t_axis.assign(g.t_axis, g.t_axis + g.n_t);
+        p->lv[f].log_s.assign(g.log_s, g.log_s + g.n_x);
+        p->lv[f].v.assign(g.v, g.v + (size_t)g.n_t * g.n_x);
+      } else {
+        const int gx = r.grid_x > 0 ? r.grid_x : 96;
+        if (gx > kMaxLvX) {
+          e->code = "invalid_local_vol";
+          e->message = "grid_x exceeds " + std::to_string(kMaxLvX);
+          return false;
+        }
+        if (!dupire_calibrate(r.implied[f], r.spots[f], r.risk_free_rate, gx, &p->lv[f], e)) {
+          e->code = "invalid_local_vol";
+          return false;
+        }
+      }
+    }
+  }
+  // group key: everything the scenarios of one item must share
+  std::string& k = p->key;
+  put_bytes(k, F);
+  put_bytes(k, r.steps);
+  put_bytes(k, r.maturity_years);
+  put_bytes(k, r.strike);
+  put_bytes(k, r.knock_in_barrier);
+  put_bytes(k, r.has_knock_out);
+  put_bytes(k, r.has_knock_out ? r.knock_out_barrier : 0.0);
+  put_bytes(k, r.has_coupon_barrier);
+  put_bytes(k, r.has_coupon_barrier ? r.coupon_barrier : 0.0);
+  put_bytes(k, r.has_coupon_rate);
+  put_bytes(k, r.has_coupon_rate ? r.coupon_rate : 0.0);
+  put_bytes(k, r.memory_coupon);
+  put_bytes(k, r.coupon_periods);
+  put_bytes(k, r.fixed_first_coupon);
+  put_bytes(k, r.call_start_step);
+  put_bytes(k, r.paths);
+  put_bytes(k, r.seed);
+  put_bytes(k, r.ladder);
+  put_bytes(k, p->ladder_dims);
+  put_bytes(k, r.risk_free_rate);
+  put_bytes(k, (int)(r.trace_level != RIGA_TRACE_OFF));
+  put_bytes(k, (int)has_lv);
+  for (int f = 0; f < F; ++f) put_bytes(k, r.initial_spots[f]);
+  for (double v : p->chol) put_bytes(k, v);
+  if (has_lv)
+    for (int f = 0; f < F; ++f) {
+      put_bytes(k, (int)p->lv[f].t_axis.size());
+      for (double t : p->lv[f].t_axis) put_bytes(k, t);
+    }
+  return true;
+}
+
+// ------------------------------------------------------------ item building
+struct Item {
+  std::vector<int> members;  // request indices (scenarios in order)
+  int F = 0, smax = 1;
+  ItemDesc d{};
+  std::vector<BridgeNode> nodes;
+  std::vector<StepInfo> steps;
+  std::vector<LvBracket> lvb;
+  std::vector<ScenarioDesc> scen;
+  std::vector<const LvGrid*> surfaces;  // distinct, in first-use order
+  std::vector<int> surface_factor;
+  std::vector<uint32_t> halton;
+  int depth = 1;
+  double cost = 0.0;
+};
+
+// Streaming traversal of the reference bridge (qmc.rs:158-191): for every step,
+// the pre-order nodes whose midpoints are first needed, and the stack depth.
+void plan_bridge(int T, double dt, Item* it) {
+  struct Pending {
+    int hi, base;
+  };
+  std::vector<Pending> st;
+  st.push_back({T, 1});
+  int lo = 0, max_depth = 1;
+  it->steps.resize(T);
+  for (int t = 0; t < T; ++t) {
+    StepInfo& si = it->steps[t];
+    si.depth = (int)st.size();
+    int c = 0;
+    while (st.back().hi > lo + 1) {
+      const int hi = st.back().hi, base = st.back().base;
+      const int mid = (lo + hi) / 2;
+      BridgeNode bn{};
+      bn.node = base;
+      bn.sd = sqrt((double)(hi - mid) * (double)(mid - lo) / (double)(hi - lo) * dt);
+      it->nodes.push_back(bn);
+      st.back().base = base + (mid - lo);  // right child (mid, hi)
+      st.push_back({mid, base + 1});       // left child (lo, mid)
+      ++c;
+      max_depth = std::max(max_depth, (int)st.size());
+    }
+    si.n_desc = c;
+    st.pop_back();
+    ++lo;
+  }
+  it->depth = max_depth;
+}
+
+void plan_schedule(const riga_request& r, double dt, Item* it) {
+  const int64_t T = r.steps;
+  const int64_t P = r.coupon_periods < 0 ? 9 : std::max<int64_t>(r.coupon_periods, 1);  // note.rs:43
+  const int64_t call_start = r.call_start_step < 0 ? 0 : r.call_start_step;            // note.rs:44
+  const double coupon_rate = r.has_coupon_rate ? r.coupon_rate : 0.0;
+  const double rate = r.risk_free_rate;
+  int64_t prev = 0;
+  for (int64_t step = 0; step < T; ++step) {
+    StepInfo& si = it->steps[step];
+    const int64_t period = std::min((step * P) / T, P - 1);
+    const int64_t next = (step + 1 == T) ? P : std::min(((step + 1) * P) / T, P - 1);
+    const int64_t a = T * (period + 1) / P, b = T * period / P;
+    const int64_t period_steps = std::max<int64_t>(a > b ? a - b : 0, 1);
+    const double time = (double)(step + 1) * dt;
+    si.period = (int32_t)period;
+    si.accrual_inc = coupon_rate / (double)period_steps;
+    si.df = exp(-rate * time);
+    si.flags = ((step + 1 == T || next != period) ? 1 : 0) | (step >= call_start ? 2 : 0) |
+               (period != prev ? 4 : 0);
+    prev = period;
+  }
+}
+
+bool build_item(const std::vector<Prep>& preps, Item* it, int precision, bool records, Error* e) {
+  const Prep& p0 = preps[it->members[0]];
+  const riga_request& r = *p0.r;
+  const int F = p0.F, T = (int)r.steps;
+  const double dt = r.maturity_years / (double)r.steps;
+  it->F = F;
+  const int S = (int)it->members.size();
+  it->smax = S <= 1 ? 1 : (S <= 4 ? 4 : 10);
+  plan_bridge(T, dt, it);
+  if (it->depth > kBridgeStack) {
+    e->code = "invalid_schedule";
+    e->message = "bridge depth exceeds the supported number of steps";
+    return false;
+  }
+  plan_schedule(r, dt, it);
+  ItemDesc& d = it->d;
+  memset(&d, 0, sizeof d);
+  d.F = F;
+  d.S = S;
+  d.steps = T;
+  d.periods = (int32_t)(r.coupon_periods < 0 ? 9 : std::max<int64_t>(r.coupon_periods, 1));
+  d.call_start = (int32_t)(r.call_start_step < 0 ? 0 : r.call_start_step);
+  d.flags = (r.memory_coupon ? kMemoryCoupon : 0) | (r.fixed_first_coupon ? kFixedFirst : 0) |
+            (!p0.lv.empty() ? kLocalVol : 0) | (r.trace_level != RIGA_TRACE_OFF ? kTrace : 0) |
+            (r.ladder == RIGA_VR_HALTON ? kHalton : 0) | (records ? kRecords : 0);
+  d.ladder_dims = p0.ladder_dims;
+  d.ladder_rows = (int32_t)std::min<int64_t>(p0.ladder_dims, (int64_t)F * T);
+  d.paths = r.paths;
+  d.seed = r.seed;
+  d.path_begin = 0;
+  const uint64_t max_n = d.path_begin + d.paths;
+  d.nbits = 64 - __builtin_clzll(max_n);
+  d.dt = dt;
+  d.sqrt_dt = sqrt(dt);
+  d.inv_sqrt_dt = 1.0 / sqrt(dt);
+  d.root_sd = sqrt((double)T * dt);
+  d.strike = r.strike;
+  d.ki = r.knock_in_barrier;
+  d.ko = r.has_knock_out ? r.knock_out_barrier : INFINITY;
+  d.cb = r.has_coupon_barrier ? r.coupon_barrier : INFINITY;
+  d.coupon_rate = r.has_coupon_rate ? r.coupon_rate : 0.0;
+  d.put_df = exp(-r.risk_free_rate * r.maturity_years);
+  for (int f = 0; f < F; ++f) d.initial_spot[f] = r.initial_spots[f];
+  for (int i = 0; i < F; ++i)
+    for (int j = 0; j < F; ++j) d.chol[i * kMaxFactors + j] = p0.chol[i * F + j];
+  // scenarios
+  it->scen.resize(S);
+  for (int s = 0; s < S; ++s) {
+    const Prep& p = preps[it->members[s]];
+    const riga_request& q = *p.r;
+    ScenarioDesc& sc = it->scen[s];
+    memset(&sc, 0, sizeof sc);
+    for (int f = 0; f < F; ++f) {
+      const double vol = q.vols[f];
+      sc.spot[f] = q.spots[f];
+      sc.rq[f] = q.risk_free_rate - q.dividend_yields[f];
+      sc.drift[f] = (q.risk_free_rate - q.dividend_yields[f] - 0.5 * vol * vol) * dt;  // note.rs:139-140
+      sc.vsd[f] = vol * sqrt(dt);
+      sc.lv[f] = -1;
+      if (!p.lv.empty()) {
+        const LvGrid* g = &p.lv[f];
+        int found = -1;
+        for (size_t u = 0; u < it->surfaces.size(); ++u) {
+          const LvGrid* h = it->surfaces[u];
+          if (it->surface_factor[u] == f && h->t_axis == g->t_axis && h->log_s == g->log_s && h->v == g->v) {
+            found = (int)u;
+            break;
+          }
+        }
+        if (found < 0) {
+          found = (int)it->surfaces.size();
+          it->surfaces.push_back(g);
+          it->surface_factor.push_back(f);
+        }
+        sc.lv[f] = found;
+      }
+    }
+  }
+  if (!p0.lv.empty()) {
+    it->lvb.resize((size_t)T * F);
+    for (int step = 0; step < T; ++step)
+      for (int f = 0; f < F; ++f) {
+        LvBracket& b = it->lvb[(size_t)step * F + f];
+        lv_bracket(p0.lv[f], (double)step * dt, &b.i0, &b.i1, &b.wt);  // note.rs:136 t = step*dt
+      }
+  }
+  if (r.ladder == RIGA_VR_HALTON) it->halton = halton_tables(d.ladder_rows, r.seed);
+  it->cost = (double)T * (double)F * (double)S * (double)r.paths;
+  (void)precision;
+  return true;
+}
+
+size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+struct Arena {
+  std::vector<unsigned char> bytes;
+  template <typename T>
+  int64_t add(const T* data, size_t n) {
+    const size_t off = align16(bytes.size());
+    bytes.resize(off + n * sizeof(T));
+    if (n) memcpy(bytes.data() + off, data, n * sizeof(T));
+    return (int64_t)off;
+  }
+};
+
+// ------------------------------------------------------------ result assembly
+// Aggregates of note.rs:262-307 from the merged double-double sums.
+void fill_result(const Partial& fr, uint64_t paths, riga_result* out) {
+  auto val = [](const DD& d) { return (long double)d.hi + (long double)d.lo; };
+  const long double n = (long double)paths;
+  const long double mean_funding = val(fr.q[kQFunding]) / n;
+  const long double mean_put = val(fr.q[kQPut]) / n;
+  const long double mean_cq = val(fr.q[kQCq]) / n;
+  const long double mean_mq = val(fr.q[kQMq]) / n;
+  auto moments = [&](int k1, int k2, uint64_t c, double* mean, double* var) {
+    if (c == 0) {
+      *mean = 0.0;
+      *var = 0.0;
+      return;
+    }
+    const long double s1 = val(fr.q[k1]), s2 = val(fr.q[k2]);
+    *mean = (double)(s1 / (long double)c);
+    *var = c > 1 ? (double)((s2 - s1 * s1 / (long double)c) / (long double)(c - 1)) : 0.0;
+    if (*var < 0.0) *var = 0.0;
+  };
+  double pv_mean, pv_var;
+  moments(kQPv1, kQPv2, paths, &pv_mean, &pv_var);
+  out->paths = paths;
+  out->mc_error = sqrt(pv_var / (double)paths);
+  out->ki_probability = (double)fr.c[kCKi] / (double)paths;
+  out->leg_funding = (double)mean_funding;
+  out->leg_put = (double)mean_put;
+  out->leg_coupon = (double)mean_cq;
+  out->leg_memory_coupon = (double)mean_mq;
+  const double total = out->leg_funding + (out->leg_coupon + out->leg_memory_coupon) / 10.0 - out->leg_put;
+  out->pv = total;
+  out->leg_total = total;
+  out->ko_count = fr.c[kCKo];
+  out->ki_count = fr.c[kCKi];
+  out->put_eligible_count = fr.c[kCPutEligible];
+  out->coupon_path_count = fr.c[kCCouponPaths];
+  moments(kQKo1, kQKo2, fr.c[kCKo], &out->ko_mean, &out->ko_variance);
+  moments(kQKi1, kQKi2, fr.c[kCKi], &out->ki_mean, &out->ki_variance);
+  moments(kQCp1, kQCp2, fr.c[kCCouponPaths], &out->coupon_mean, &out->coupon_variance);
+  out->n_trace = fr.n_events;
+  for (int e = 0; e < fr.n_events && e < RIGA_TRACE_CAP; ++e) {
+    out->trace_path[e] = fr.ev_path_lo[e];
+    out->trace_step[e] = fr.ev_step[e];
+  }
+}
+
+// ------------------------------------------------------------ the batch run
+int run_batch(riga_ctx* ctx, const riga_request* reqs, int n, int precision,
+              const riga_launch_config* cfg, riga_result* results, riga_path_record* records,
+              riga_error* err) {
+  clear_err(err);
+  if (!ctx) {
+    set_err(err, "gpu_unavailable", "no device context");
+    return RIGA_ENODEV;
+  }
+  if (n <= 0) return 0;
+  if (precision != RIGA_FP64 && precision != RIGA_FP32) {
+    set_err(err, "invalid_precision", "precision must be RIGA_FP64 or RIGA_FP32");
+    return RIGA_EINVAL;
+  }
+  const int ppt = cfg && cfg->paths_per_thread > 0 ? cfg->paths_per_thread : 1;
+  if (ppt != 1 && ppt != 2 && ppt != 4 && ppt != 8) {
+    set_err(err, "invalid_launch_config", "paths_per_thread must be 1, 2, 4 or 8");
+    return RIGA_EINVAL;
+  }
+  std::vector<Prep> preps(n);
+  for (int i = 0; i < n; ++i) {
+    Error e;
+    if (!prepare(reqs[i], &preps[i], &e)) {
+      set_err(err, e.code, "request " + std::to_string(i) + ": " + e.message);
+      return RIGA_EINVAL;
+    }
+  }
+  // CRN grouping (records need one scenario per item so path order is per request)
+  std::vector<Item> items;
+  {
+    std::map<std::string, std::vector<int>> groups;
+    std::vector<std::string> order;
+    for (int i = 0; i < n; ++i) {
+      std::string key = preps[i].key;
+      if (records) put_bytes(key, i);
+      auto itg = groups.find(key);
+      if (itg == groups.end()) {
+        order.push_back(key);
+        groups[key] = {i};
+      } else {
+        itg->second.push_back(i);
+      }
+    }
+    for (const std::string& k : order) {
+      const std::vector<int>& m = groups[k];
+      for (size_t s = 0; s < m.size(); s += 10) {
+        Item it;
+        it.members.assign(m.begin() + s, m.begin() + std::min(m.size(), s + 10));
+        items.push_back(std::move(it));
+      }
+    }
+  }
+  for (Item& it : items) {
+    Error e;
+    if (!build_item(preps, &it, precision, records != nullptr, &e)) {
+      set_err(err, e.code, e.message);
+      return RIGA_EINVAL;
+    }
+  }
+  // launches: (F, smax) partitions, blocks longest-first
+  const int real_size = precision == RIGA_FP64 ? 8 : 4;
+  std::map<std::pair<int, int>, std::vector<int>> parts;
+  for (size_t i = 0; i < items.size(); ++i) parts[{items[i].F, items[i].smax}].push_back((int)i);
+
+  Arena arena;
+  int sobol_rows = 1;
+  for (Item& it : items)
+    if (!(it.d.flags & kHalton)) sobol_rows = std::max(sobol_rows, it.d.ladder_rows);
+  const std::vector<uint32_t>& dirs = sobol_directions(sobol_rows);
+  const int64_t off_dirs = arena.add(dirs.data(), (size_t)sobol_rows * 32);
+
+  struct LaunchPlan {
+    int F, smax, blocks;
+    size_t lds;
+    std::vector<int32_t> block_item;
+  };
+  std::vector<LaunchPlan> launches;
+  int64_t partial_slots = 0, record_slots = 0;
+  std::vector<int64_t> final_base(items.size());
+  int64_t final_slots = 0;
+  for (size_t i = 0; i < items.size(); ++i) {
+    final_base[i] = final_slots;
+    final_slots += items[i].d.S;
+  }
+  for (auto& kv : parts) {
+    std::vector<int>& idx = kv.second;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return items[a].cost > items[b].cost; });
+    LaunchPlan lp;
+    lp.F = kv.first.first;
+    lp.smax = kv.first.second;
+    int stack_levels = 0, lv_words = 0;
+    for (int i : idx) {
+      stack_levels = std::max(stack_levels, items[i].depth - 1);
+      int max_nx = 0;
+      for (const LvGrid* g : items[i].surfaces) max_nx = std::max(max_nx, g->n_x());
+      lv_words = std::max(lv_words, (int)items[i].surfaces.size() * 3 * max_nx);
+    }
+    lp.lds = (size_t)stack_levels * lp.F * kBlock * real_size + (size_t)lv_words * real_size;
+    if (lp.lds + (size_t)lp.smax * sizeof(BlockShared) > ctx->lds_limit) {
+      set_err(err, "invalid_local_vol",
+              "the batch needs " + std::to_string(lp.lds) + " bytes of LDS per workgroup (limit " +
+                  std::to_string(ctx->lds_limit) + "); use fewer distinct surfaces or knots");
+      return RIGA_EINVAL;
+    }
+    int blocks = 0;
+    for (int i : idx) {
+      Item& it = items[i];
+      ItemDesc& d = it.d;
+      d.stack_levels = stack_levels;
+      d.paths_per_block = kBlock * ppt;
+      d.n_blocks = (int32_t)((d.paths + d.paths_per_block - 1) / d.paths_per_block);
+      d.block_begin = blocks;
+      blocks += d.n_blocks;
+      for (int b = 0; b < d.n_blocks; ++b) lp.block_item.push_back(i);
+      d.partial_base = partial_slots;
+      partial_slots += (int64_t)d.S * d.n_blocks;
+      d.record_base = record_slots;
+      if (d.flags & kRecords) record_slots += (int64_t)d.S * (int64_t)d.paths;
+      d.off_dirs = off_dirs;
+      d.off_nodes = arena.add(it.nodes.data(), it.nodes.size());
+      d.off_steps = arena.add(it.steps.data(), it.steps.size());
+      d.off_lvb = arena.add(it.lvb.data(), it.lvb.size());
+      d.off_scen = arena.add(it.scen.data(), it.scen.size());
+      d.off_halton = arena.add(it.halton.data(), it.halton.size());
+      std::vector<LvSurface> su(it.surfaces.size());
+      int max_nx = 0;
+      for (size_t u = 0; u < it.surfaces.size(); ++u) {
+        const LvGrid& g = *it.surfaces[u];
+        su[u].off_t = arena.add(g.t_axis.data(), g.t_axis.size());
+        su[u].off_x = arena.add(g.log_s.data(), g.log_s.size());
+        su[u].off_v = arena.add(g.v.data(), g.v.size());
+        su[u].n_t = g.n_t();
+        su[u].n_x = g.n_x();
+        su[u].factor = it.surface_factor[u];
+        su[u].inv_dx = (double)(g.n_x() - 1) / (g.log_s[g.n_x() - 1] - g.log_s[0]);
+        max_nx = std::max(max_nx, g.n_x());
+      }
+      d.off_surf = arena.add(su.data(), su.size());
+      d.n_surf = (int32_t)su.size();
+      d.max_nx = max_nx;
+    }
+    lp.blocks = blocks;
+    launches.push_back(std::move(lp));
+  }
+  std::vector<ItemDesc> descs(items.size());
+  for (size_t i = 0; i < items.size(); ++i) descs[i] = items[i].d;
+  std::vector<int32_t> all_blocks;
+  std::vector<size_t> launch_block_off;
+  for (auto& lp : launches) {
+    launch_block_off.push_back(all_blocks.size());
+    all_blocks.insert(all_blocks.end(), lp.block_item.begin(), lp.block_item.end());
+  }
+
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  HIP_TRY(ctx->arena.ensure(arena.bytes.size()), "arena alloc");
+  HIP_TRY(ctx->items.ensure(descs.size() * sizeof(ItemDesc)), "items alloc");
+  HIP_TRY(ctx->block_item.ensure(all_blocks.size() * sizeof(int32_t)), "block table alloc");
+  HIP_TRY(ctx->partials.ensure((size_t)partial_slots * sizeof(Partial)), "partials alloc");
+  HIP_TRY(ctx->finals.ensure((size_t)final_slots * sizeof(Partial)), "finals alloc");
+  HIP_TRY(ctx->final_base.ensure(final_base.size() * sizeof(int64_t)), "final index alloc");
+  if (records) HIP_TRY(ctx->records.ensure((size_t)record_slots * sizeof(PathRecordDev)), "records alloc");
+  const size_t up = align16(arena.bytes.size()) + align16(descs.size() * sizeof(ItemDesc)) +
+                    align16(all_blocks.size() * 4) + align16(final_base.size() * 8);
+  HIP_TRY(ctx->staging.ensure(up), "staging alloc");
+  unsigned char* st = (unsigned char*)ctx->staging.p;
+  size_t o = 0;
+  auto stage = [&](const void* src, size_t bytes, void* dst) -> hipError_t {
+    memcpy(st + o, src, bytes);
+    hipError_t e = hipMemcpyAsync(dst, st + o, bytes, hipMemcpyHostToDevice, ctx->stream);
+    o += align16(bytes);
+    return e;
+  };
+  HIP_TRY(stage(arena.bytes.data(), arena.bytes.size(), ctx->arena.p), "arena upload");
+  HIP_TRY(stage(descs.data(), descs.size() * sizeof(ItemDesc), ctx->items.p), "item upload");
+  HIP_TRY(stage(all_blocks.data(), all_blocks.size() * 4, ctx->block_item.p), "block table upload");
+  HIP_TRY(stage(final_base.data(), final_base.size() * 8, ctx->final_base.p), "final index upload");
+
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
+  for (size_t li = 0; li < launches.size(); ++li) {
+    const LaunchPlan& lp = launches[li];
+    PhoenixLaunch L;
+    L.F = lp.F;
+    L.smax = lp.smax;
+    L.blocks = lp.blocks;
+    L.lds = lp.lds;
+    L.items = (const ItemDesc*)ctx->items.p;
+    L.block_item = (const int32_t*)ctx->block_item.p + launch_block_off[li];
+    L.arena = (const unsigned char*)ctx->arena.p;
+    L.partials = (Partial*)ctx->partials.p;
+    L.records = records ? (PathRecordDev*)ctx->records.p : nullptr;
+    hipError_t e = precision == RIGA_FP64 ? launch_phoenix_f64(L, ctx->stream) : launch_phoenix_f32(L, ctx->stream);
+    HIP_TRY(e, "phoenix kernel launch");
+  }
+  HIP_TRY(launch_finalize((const ItemDesc*)ctx->items.p, (int)items.size(), (const Partial*)ctx->partials.p,
+                          (Partial*)ctx->finals.p, (const int64_t*)ctx->final_base.p, ctx->stream),
+          "finalize launch");
+  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
+  HIP_TRY(ctx->finals_host.ensure((size_t)final_slots * sizeof(Partial)), "host finals alloc");
+  HIP_TRY(hipMemcpyAsync(ctx->finals_host.p, ctx->finals.p, (size_t)final_slots * sizeof(Partial),
+                         hipMemcpyDeviceToHost, ctx->stream),
+          "finals download");
+  std::vector<PathRecordDev> recs;
+  if (records) {
+    recs.resize((size_t)record_slots);
+    HIP_TRY(hipMemcpyAsync(recs.data(), ctx->records.p, recs.size() * sizeof(PathRecordDev),
+                           hipMemcpyDeviceToHost, ctx->stream),
+            "records download");
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "kernel execution");
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event timing");
+  const Partial* finals = (const Partial*)ctx->finals_host.p;
+  for (size_t i = 0; i < items.size(); ++i) {
+    const Item& it = items[i];
+    for (int s = 0; s < it.d.S; ++s) {
+      const int req = it.members[s];
+      riga_result* out = &results[req];
+      memset(out, 0, sizeof *out);
+      fill_result(finals[final_base[i] + s], it.d.paths, out);
+      out->device_ms = ms;
+      if (records) {
+        const PathRecordDev* src = recs.data() + it.d.record_base + (int64_t)s * (int64_t)it.d.paths;
+        for (uint64_t p = 0; p < it.d.paths; ++p) {
+          riga_path_record& r = records[p];
+          r.ko_step = src[p].ko_step;
+          r.hit_ki = src[p].hit_ki;
+          r.final_worst = src[p].final_worst;
+          r.funding = src[p].funding;
+          r.put = src[p].put;
+          r.coupon_pv = src[p].coupon_pv;
+          r.memory_pv = src[p].memory_pv;
+          r.pv = src[p].pv;
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+int riga_hip_init(int device, riga_ctx** out, riga_error* err) {
+  clear_err(err);
+  if (!out) {
+    set_err(err, "invalid_argument", "out is NULL");
+    return RIGA_EINVAL;
+  }
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= 0) {
+    set_err(err, "gpu_unavailable", std::string("no HIP device: ") + hipGetErrorString(e));
+    return RIGA_ENODEV;
+  }
+  if (device < 0 || device >= count) {
+    set_err(err, "gpu_unavailable", "device index out of range");
+    return RIGA_ENODEV;
+  }
+  hipDeviceProp_t prop;
+  if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    set_err(err, "gpu_unavailable", "cannot select the HIP device");
+    return RIGA_ENODEV;
+  }
+  std::string arch = prop.gcnArchName;
+  if (arch.rfind("gfx950", 0) != 0) {
+    set_err(err, "gpu_unavailable", "kernels are built for gfx950; device is " + arch);
+    return RIGA_ENODEV;
+  }
+  std::unique_ptr<riga_ctx> ctx(new riga_ctx());
+  ctx->device = device;
+  ctx->name = prop.name;
+  ctx->arch = arch;
+  ctx->lds_limit = prop.sharedMemPerBlock;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    set_err(err, "gpu_error", "stream/event creation failed");
+    return RIGA_EGPU;
+  }
+  *out = ctx.release();
+  return 0;
+}
+
+void riga_hip_destroy(riga_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int riga_hip_device_name(riga_ctx* ctx, char* buf, size_t len, riga_error* err) {
+  clear_err(err);
+  if (!ctx) {
+    set_err(err, "gpu_unavailable", "no device context");
+    return RIGA_ENODEV;
+  }
+  snprintf(buf, len, "%s", ctx->name.c_str());
+  return 0;
+}
+
+int riga_hip_arch(riga_ctx* ctx, char* buf, size_t len, riga_error* err) {
+  clear_err(err);
+  if (!ctx) {
+    set_err(err, "gpu_unavailable", "no device context");
+    return RIGA_ENODEV;
+  }
+  snprintf(buf, len, "%s", ctx->arch.c_str());
+  return 0;
+}
+
+int riga_validate(const riga_request* req, riga_error* err) {
+  clear_err(err);
+  Prep p;
+  Error e;
+  if (!req || !prepare(*req, &p, &e)) {
+    set_err(err, req ? e.code : "invalid_argument", req ? e.message : "request is NULL");
+    return RIGA_EINVAL;
+  }
+  return 0;
+}
+
+int riga_price_put_leg(riga_ctx* ctx, const riga_request* req, int precision, riga_result* out,
+                       riga_error* err) {
+  return run_batch(ctx, req, 1, precision, nullptr, out, nullptr, err);
+}
+
+int riga_price_batch(riga_ctx* ctx, const riga_request* reqs, int n, int precision,
+                     const riga_launch_config* cfg, riga_result* results, riga_error* err) {
+  return run_batch(ctx, reqs, n, precision, cfg, results, nullptr, err);
+}
+
+int riga_replay_paths(riga_ctx* ctx, const riga_request* req, int precision,
+                      riga_path_record* records, riga_result* out, riga_error* err) {
+  if (!records) {
+    set_err(err, "invalid_argument", "records is NULL");
+    return RIGA_EINVAL;
+  }
+  return run_batch(ctx, req, 1, precision, nullptr, out, records, err);
+}
+
+int riga_lv_calibrate(const riga_implied_grid* grid, double spot, double rate, int32_t grid_x,
+                      double* t_axis, double* log_s, double* v, riga_error* err) {
+  clear_err(err);
+  if (!grid || !t_axis || !log_s || !v) {
+    set_err(err, "invalid_argument", "NULL argument");
+    return RIGA_EINVAL;
+  }
+  LvGrid g;
+  Error e;
+  if (!dupire_calibrate(*grid, spot, rate, grid_x, &g, &e)) {
+    set_err(err, e.code, e.message);
+    return RIGA_EINVAL;
+  }
+  memcpy(t_axis, g.t_axis.data(), g.t_axis.size() * sizeof(double));
+  memcpy(log_s, g.log_s.data(), g.log_s.size() * sizeof(double));
+  memcpy(v, g.v.data(), g.v.size() * sizeof(double));
+  return 0;
+}
+
+double riga_lv_sigma(const riga_lv_grid* grid, double t, double s) {
+  LvGrid g;
+  g.t_axis.assign(grid->t_axis, grid->t_axis + grid->n_t);
+  g.log_s.assign(grid->log_s, grid->log_s + grid->n_x);
+  g.v.assign(grid->v, grid->v + (size_t)grid->n_t * grid->n_x);
+  return lv_sigma(g, t, s);
+}
+
+int riga_debug_sobol(riga_ctx* ctx, uint64_t seed, int32_t ladder_dims, uint64_t first_point,
+                     int32_t n_points, int32_t first_dim, int32_t n_dims, uint32_t* out_unscrambled,
+                     uint32_t* out_scrambled, double* out_normal, riga_error* err) {
+  clear_err(err);
+  if (!ctx) {
+    set_err(err, "gpu_unavailable", "no device context");
+    return RIGA_ENODEV;
+  }
+  if (n_points <= 0 || n_dims <= 0 || first_dim < 0 || ladder_dims <= 0 ||
+      first_point + (uint64_t)n_points >= (1ull << 32)) {
+    set_err(err, "invalid_argument", "bad debug_sobol ranges");
+    return RIGA_EINVAL;
+  }
+  const int rows = std::min<int64_t>(ladder_dims, (int64_t)first_dim + n_dims);
+  const std::vector<uint32_t>& dirs = sobol_directions(rows);
+  const uint64_t max_n = first_point + (uint64_t)n_points;
+  const int nbits = 64 - __builtin_clzll(max_n);
+  const size_t cells = (size_t)n_points * n_dims;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  DevBuf d_dirs, d_u, d_s, d_n;
+  HIP_TRY(d_dirs.ensure((size_t)rows * 32 * 4), "alloc");
+  HIP_TRY(d_u.ensure(cells * 4), "alloc");
+  HIP_TRY(d_s.ensure(cells * 4), "alloc");
+  HIP_TRY(d_n.ensure(cells * 8), "alloc");
+  HIP_TRY(hipMemcpyAsync(d_dirs.p, dirs.data(), (size_t)rows * 32 * 4, hipMemcpyHostToDevice, ctx->stream),
+          "upload");
+  HIP_TRY(launch_debug_sobol((const uint32_t*)d_dirs.p, ladder_dims, seed, first_point, n_points, first_dim,
+                             n_dims, nbits, (uint32_t*)d_u.p, (uint32_t*)d_s.p, (double*)d_n.p, ctx->stream),
+          "debug kernel");
+  HIP_TRY(hipMemcpyAsync(out_unscrambled, d_u.p, cells * 4, hipMemcpyDeviceToHost, ctx->stream), "download");
+  HIP_TRY(hipMemcpyAsync(out_scrambled, d_s.p, cells * 4, hipMemcpyDeviceToHost, ctx->stream), "download");
+  HIP_TRY(hipMemcpyAsync(out_normal, d_n.p, cells * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "debug kernel execution");
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,41 @@
+// kernels.h -- host-callable launchers for the gfx950 kernels (one translation
+// unit per precision tier: kernels_f64.hip is built with -ffp-contract=off).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "phoenix_layout.h"
+
+namespace riga {
+
+// Largest number of CRN scenarios one lane carries, by kernel variant.
+constexpr int kScenarioBuckets[3] = {1, 4, 10};
+
+struct PhoenixLaunch {
+  int F;       // factors (1..kMaxFactors)
+  int smax;    // scenario bucket (1, 4 or 10)
+  int blocks;  // grid size
+  size_t lds;  // dynamic LDS bytes (local-vol rows)
+  const ItemDesc* items;
+  const int32_t* block_item;
+  const unsigned char* arena;
+  Partial* partials;
+  PathRecordDev* records;
+};
+
+hipError_t launch_phoenix_f64(const PhoenixLaunch& l, hipStream_t stream);
+hipError_t launch_phoenix_f32(const PhoenixLaunch& l, hipStream_t stream);
+
+hipError_t launch_finalize(const ItemDesc* items, int n_items, const Partial* partials,
+                           Partial* finals, const int64_t* final_base, hipStream_t stream);
+
+// Sobol diagnostics: unscrambled and scrambled integers and FP64 normals for
+// points first_point .. first_point + n_points - 1 and dims first_dim ...
+hipError_t launch_debug_sobol(const uint32_t* dirs, int ladder_dims, uint64_t seed,
+                              uint64_t first_point, int n_points, int first_dim, int n_dims,
+                              int nbits, uint32_t* unscrambled, uint32_t* scrambled,
+                              double* normal, hipStream_t stream);
+
+// Register use of a kernel variant (for the autotuner and the report).
+int phoenix_vgprs(int precision, int F, int smax);
+
+}  // namespace riga

@@ -1,0 +1,108 @@
+// kernels_f64.hip -- FP64 parity tier.  Built with -ffp-contract=off so that no
+// multiply-add is fused: the reference (Rust) rounds every operation separately.
+#include "kernels.h"
+#include "phoenix_device.hpp"
+
+namespace riga {
+
+template <int F, int SMAX>
+static hipError_t launch_one(const PhoenixLaunch& l, hipStream_t stream) {
+  hipLaunchKernelGGL((phoenix_kernel<double, F, SMAX>), dim3(l.blocks), dim3(kBlock), l.lds,
+                     stream, l.items, l.block_item, l.arena, l.partials, l.records);
+  return hipGetLastError();
+}
+
+template <int F>
+static hipError_t launch_f(const PhoenixLaunch& l, hipStream_t stream) {
+  switch (l.smax) {
+    case 1: return launch_one<F, 1>(l, stream);
+    case 4: return launch_one<F, 4>(l, stream);
+    case 10: return launch_one<F, 10>(l, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_phoenix_f64(const PhoenixLaunch& l, hipStream_t stream) {
+  switch (l.F) {
+    case 1: return launch_f<1>(l, stream);
+    case 2: return launch_f<2>(l, stream);
+    case 3: return launch_f<3>(l, stream);
+    case 4: return launch_f<4>(l, stream);
+    case 5: return launch_f<5>(l, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// Merge the block partials of every (item, scenario) in block order.
+__global__ void finalize_kernel(const ItemDesc* __restrict__ items, int n_items,
+                                const Partial* __restrict__ partials, Partial* __restrict__ finals,
+                                const int64_t* __restrict__ final_base) {
+  const int item = blockIdx.x;
+  if (item >= n_items) return;
+  const ItemDesc& it = items[item];
+  for (int s = 0; s < it.S; ++s) {
+    const Partial* p = partials + it.partial_base + (int64_t)s * it.n_blocks;
+    Partial* out = finals + final_base[item] + s;
+    const int t = threadIdx.x;
+    if (t < kNumQ) {
+      DD a{0.0, 0.0};
+      for (int b = 0; b < it.n_blocks; ++b) a = dd_add(a, p[b].q[t]);
+      out->q[t] = a;
+    } else if (t >= 32 && t < 32 + kNumC) {
+      uint64_t a = 0;
+      for (int b = 0; b < it.n_blocks; ++b) a += p[b].c[t - 32];
+      out->c[t - 32] = a;
+    } else if (t == 64) {
+      int ne = 0;
+      for (int b = 0; b < it.n_blocks && ne < kTraceCap; ++b) {
+        for (int e = 0; e < p[b].n_events && ne < kTraceCap; ++e) {
+          out->ev_path_lo[ne] = p[b].ev_path_lo[e];
+          out->ev_step[ne] = p[b].ev_step[e];
+          ++ne;
+        }
+      }
+      out->n_events = ne;
+    }
+  }
+}
+
+
+hipError_t launch_finalize(const ItemDesc* items, int n_items, const Partial* partials,
+                           Partial* finals, const int64_t* final_base, hipStream_t stream) {
+  hipLaunchKernelGGL(finalize_kernel, dim3(n_items), dim3(128), 0, stream, items, n_items,
+                     partials, finals, final_base);
+  return hipGetLastError();
+}
+
+__global__ void debug_sobol_kernel(const uint32_t* __restrict__ dirs, int ladder_dims,
+                                   uint64_t seed, uint64_t first_point, int n_points,
+                                   int first_dim, int n_dims, int nbits,
+                                   uint32_t* __restrict__ unscrambled,
+                                   uint32_t* __restrict__ scrambled, double* __restrict__ normal) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = blockIdx.y;
+  if (p >= n_points || d >= n_dims) return;
+  const uint32_t n = (uint32_t)(first_point + (uint64_t)p);
+  const uint32_t gray = n ^ (n >> 1);
+  const uint32_t dim = (uint32_t)(first_dim + d);
+  const uint32_t* row = dirs + (size_t)(dim % (uint32_t)ladder_dims) * 32;
+  const uint32_t x = sobol_x(gray, row, nbits);
+  const uint32_t xs = owen_scramble(x, seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull));
+  const size_t o = (size_t)p * n_dims + d;
+  unscrambled[o] = x;
+  scrambled[o] = xs;
+  normal[o] = probit_f64(((double)xs + 0.5) / 4294967296.0);
+}
+
+hipError_t launch_debug_sobol(const uint32_t* dirs, int ladder_dims, uint64_t seed,
+                              uint64_t first_point, int n_points, int first_dim, int n_dims,
+                              int nbits, uint32_t* unscrambled, uint32_t* scrambled,
+                              double* normal, hipStream_t stream) {
+  dim3 grid((n_points + 255) / 256, n_dims);
+  hipLaunchKernelGGL(debug_sobol_kernel, grid, dim3(256), 0, stream, dirs, ladder_dims, seed,
+                     first_point, n_points, first_dim, n_dims, nbits, unscrambled, scrambled,
+                     normal);
+  return hipGetLastError();
+}
+
+}  // namespace riga

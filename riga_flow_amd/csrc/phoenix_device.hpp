@@ -1,0 +1,583 @@
+// phoenix_device.hpp -- the fused Phoenix path kernel for gfx950.
+//
+// One lane owns one Sobol point (= one path of the item) and carries all S
+// scenarios of the item through the whole horizon:
+//
+//   Sobol gray-code XOR -> Owen nested scramble -> inverse normal      (per bridge node, F dims)
+//   y = L z                                                            (Cholesky row sums)
+//   streaming Brownian bridge, W(t) produced in time order             (per-lane stack of F-vectors)
+//   per scenario: local-vol lookup from LDS rows, Euler log-step,
+//   worst-of, daily KO, day-in coupon accrual with memory, period ends (Phoenix scan)
+//   path end: EKI put, funding, coupon legs -> double-double wave sums -> block partial
+//
+// Nothing path-sized is written to HBM unless a replay query sets kRecords.
+// The reference computes the same quantities in crates/riga-engine/src/note.rs:79-259
+// with the ladder in crates/riga-engine/src/qmc.rs and the surface lookup in
+// crates/riga-market/src/lib.rs:305-343.
+//
+// Templates: Real = double (the FP64 parity tier, compiled with
+// -ffp-contract=off so every expression rounds exactly as the reference's) or
+// float (the fast tier: path arithmetic in FP32, payoff legs and all sums FP64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "phoenix_layout.h"
+
+namespace riga {
+
+constexpr int kWaves = kWavesPerBlock;
+
+// ---------------------------------------------------------------- ladders
+// Owen nested scramble (qmc.rs:293-314).  The reference hashes
+//   mix64(seed ^ dim*C1 ^ bit*C2 ^ prefix*C3)
+// and keeps bit 63 of the result.  `key` = seed ^ dim*C1 is wave-uniform; the
+// prefix product is carried incrementally: P(prefix<<1 | s) = 2 P(prefix) + s*C3
+// (mod 2^64), and only the top bit of the second multiply is formed, since the
+// final xorshift of mix64 leaves bit 63 unchanged.
+__device__ __forceinline__ uint32_t owen_scramble(uint32_t x, uint64_t key) {
+  constexpr uint64_t C2 = 0xD6E8FEB86659FD93ull;
+  constexpr uint64_t C3 = 0xA24BAED4963EE407ull;
+  constexpr uint64_t C4 = 0xBF58476D1CE4E5B9ull;
+  constexpr uint64_t C5 = 0x94D049BB133111EBull;
+  uint32_t prefix = 0;
+  uint64_t P = 0;
+#pragma unroll 4
+  for (int bit = 31; bit >= 0; --bit) {
+    uint64_t z = key ^ ((uint64_t)bit * C2) ^ P;
+    z ^= z >> 30;
+    z *= C4;
+    z ^= z >> 27;
+    const uint32_t top = (uint32_t)((z * C5) >> 63);
+    const uint32_t s = ((x >> bit) & 1u) ^ top;
+    prefix = (prefix << 1) | s;
+    P = (P << 1) + (s ? C3 : 0ull);
+  }
+  return prefix;
+}
+
+// Unscrambled Sobol coordinate (qmc.rs:269-279): XOR of the direction numbers
+// selected by the gray code of n.  `row` is wave-uniform.
+__device__ __forceinline__ uint32_t sobol_x(uint32_t gray, const uint32_t* __restrict__ row,
+                                            int nbits) {
+  uint32_t x = 0;
+  for (int k = 0; k < nbits; ++k) x ^= ((gray >> k) & 1u) ? row[k] : 0u;
+  return x;
+}
+
+// Digit-scrambled radical inverse (qmc.rs:143-153) in FP64, as the reference.
+__device__ __forceinline__ double halton_u(uint32_t n, uint32_t base,
+                                           const uint32_t* __restrict__ perm) {
+#pragma clang fp contract(off)
+  const double inv_base = 1.0 / (double)base;
+  double factor = inv_base;
+  double result = 0.0;
+  while (n > 0) {
+    result += factor * (double)perm[n % base];
+    n /= base;
+    factor *= inv_base;
+  }
+  return result;
+}
+
+// Acklam inverse normal (qmc.rs:37-91) with the ladder's clamp (qmc.rs:284-289).
+__device__ __forceinline__ double probit_f64(double p) {
+  if (p < 1.0e-12) p = 1.0e-12;
+  if (p > 1.0 - 1.0e-12) p = 1.0 - 1.0e-12;
+  constexpr double A0 = -3.969683028665376e+01, A1 = 2.209460984245205e+02,
+                   A2 = -2.759285104469687e+02, A3 = 1.383577518672690e+02,
+                   A4 = -3.066479806614716e+01, A5 = 2.506628277459239e+00;
+  constexpr double B0 = -5.447609879822406e+01, B1 = 1.615858368580409e+02,
+                   B2 = -1.556989798598866e+02, B3 = 6.680131188771972e+01,
+                   B4 = -1.328068155288572e+01;
+  constexpr double C0 = -7.784894002430293e-03, C1 = -3.223964580411365e-01,
+                   C2 = -2.400758277161838e+00, C3 = -2.549732539343734e+00,
+                   C4 = 4.374664141464968e+00, C5 = 2.938163982698783e+00;
+  constexpr double D0 = 7.784695709041462e-03, D1 = 3.224671290700398e-01,
+                   D2 = 2.445134137142996e+00, D3 = 3.754408661907416e+00;
+  constexpr double plow = 0.02425;
+  constexpr double phigh = 1.0 - plow;
+  if (p < plow || p > phigh) {
+    const bool lower = p < plow;
+    const double q = sqrt(-2.0 * log(lower ? p : 1.0 - p));
+    const double num = ((((C0 * q + C1) * q + C2) * q + C3) * q + C4) * q + C5;
+    const double den = (((D0 * q + D1) * q + D2) * q + D3) * q + 1.0;
+    return lower ? num / den : -num / den;
+  }
+  const double q = p - 0.5;
+  const double r = q * q;
+  return (((((A0 * r + A1) * r + A2) * r + A3) * r + A4) * r + A5) * q /
+         (((((B0 * r + B1) * r + B2) * r + B3) * r + B4) * r + 1.0);
+}
+
+// FP32 inverse normal for the fast tier.  The uniform arrives in FP64 so the
+// tail argument (p or 1-p) and the central offset p-0.5 are exact before the
+// rounding to float.
+__device__ __forceinline__ float probit_f32(double p) {
+  if (p < 1.0e-12) p = 1.0e-12;
+  if (p > 1.0 - 1.0e-12) p = 1.0 - 1.0e-12;
+  constexpr float A0 = -3.969683028665376e+01f, A1 = 2.209460984245205e+02f,
+                  A2 = -2.759285104469687e+02f, A3 = 1.383577518672690e+02f,
+                  A4 = -3.066479806614716e+01f, A5 = 2.506628277459239e+00f;
+  constexpr float B0 = -5.447609879822406e+01f, B1 = 1.615858368580409e+02f,
+                  B2 = -1.556989798598866e+02f, B3 = 6.680131188771972e+01f,
+                  B4 = -1.328068155288572e+01f;
+  constexpr float C0 = -7.784894002430293e-03f, C1 = -3.223964580411365e-01f,
+                  C2 = -2.400758277161838e+00f, C3 = -2.549732539343734e+00f,
+                  C4 = 4.374664141464968e+00f, C5 = 2.938163982698783e+00f;
+  constexpr float D0 = 7.784695709041462e-03f, D1 = 3.224671290700398e-01f,
+                  D2 = 2.445134137142996e+00f, D3 = 3.754408661907416e+00f;
+  if (p < 0.02425 || p > 1.0 - 0.02425) {
+    const bool lower = p < 0.5;
+    const float t = (float)(lower ? p : 1.0 - p);
+    const float q = __fsqrt_rn(-2.0f * __logf(t));
+    const float num = ((((C0 * q + C1) * q + C2) * q + C3) * q + C4) * q + C5;
+    const float den = (((D0 * q + D1) * q + D2) * q + D3) * q + 1.0f;
+    const float z = __fdividef(num, den);
+    return lower ? z : -z;
+  }
+  const float q = (float)(p - 0.5);
+  const float r = q * q;
+  return __fdividef((((((A0 * r + A1) * r + A2) * r + A3) * r + A4) * r + A5) * q,
+                    (((((B0 * r + B1) * r + B2) * r + B3) * r + B4) * r + 1.0f));
+}
+
+// ---------------------------------------------------------------- double-double
+__device__ __forceinline__ DD dd_from(double x) { return DD{x, 0.0}; }
+
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+#pragma clang fp contract(off)
+  const double s = a.hi + b.hi;
+  const double bb = s - a.hi;
+  const double e = (a.hi - (s - bb)) + (b.hi - bb) + (a.lo + b.lo);
+  const double hi = s + e;
+  return DD{hi, e - (hi - s)};
+}
+
+// Exact square x*x = p + e (FMA error term).
+__device__ __forceinline__ DD dd_sq(double x) {
+  const double p = x * x;
+  return DD{p, __builtin_fma(x, x, -p)};
+}
+
+__device__ __forceinline__ DD dd_shfl_xor(DD v, int m) {
+  return DD{__shfl_xor(v.hi, m, 64), __shfl_xor(v.lo, m, 64)};
+}
+
+__device__ __forceinline__ DD wave_sum(DD v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = dd_add(v, dd_shfl_xor(v, m));
+  return v;
+}
+
+// ---------------------------------------------------------------- bridge stack
+// The streaming bridge keeps the W values of pending right endpoints, one
+// F-vector per level.  The top level lives in VGPRs; the levels below it live
+// in this lane's own LDS column ([level][factor][lane], so a wave touches 64
+// consecutive words: no bank conflicts and no barrier).  The depth is uniform
+// across the block.  A step that needs c new nodes pushes c-1 of them (the
+// last one is emitted at once) and a step that needs none pops one.
+template <typename Real, int F>
+struct BridgeStack {
+  Real top[F];
+  Real* col;  // this lane's column: col[(level*F + f) * kBlock]
+  int sp;     // levels held in LDS
+
+  __device__ __forceinline__ void push(const Real (&v)[F]) {
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      col[(sp * F + f) * kBlock] = top[f];
+      top[f] = v[f];
+    }
+    ++sp;
+  }
+  __device__ __forceinline__ void pop() {
+    --sp;
+#pragma unroll
+    for (int f = 0; f < F; ++f) top[f] = col[(sp * F + f) * kBlock];
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ const T* at(const unsigned char* arena, int64_t off) {
+  return reinterpret_cast<const T*>(arena + off);
+}
+
+// Correlated normals of one bridge node: dims node*F + f (note.rs:89-98 with F
+// factors), y = L z with the reference's sequential row sums (correlation.rs:196-206).
+template <typename Real, int F>
+__device__ __forceinline__ void node_normals(const ItemDesc& it, const unsigned char* arena,
+                                             uint32_t n, uint32_t gray, int node, Real (&y)[F]) {
+  Real z[F];
+  const bool halton = (it.flags & kHalton) != 0;
+#pragma unroll
+  for (int f = 0; f < F; ++f) {
+    const uint32_t dim = (uint32_t)node * F + f;
+    const uint32_t d = dim % (uint32_t)it.ladder_dims;
+    double u;
+    if (!halton) {
+      const uint32_t* row = at<uint32_t>(arena, it.off_dirs) + (size_t)d * 32;
+      const uint32_t x = sobol_x(gray, row, it.nbits);
+      const uint64_t key = it.seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
+      const uint32_t xs = owen_scramble(x, key);
+      u = ((double)xs + 0.5) / 4294967296.0;
+    } else {
+      const uint32_t* h = at<uint32_t>(arena, it.off_halton);
+      const uint32_t base = h[d];
+      const uint32_t* perm = h + h[it.ladder_rows + d];
+      u = halton_u(n, base, perm);
+    }
+    if constexpr (sizeof(Real) == 8)
+      z[f] = probit_f64(u);
+    else
+      z[f] = probit_f32(u);
+  }
+#pragma unroll
+  for (int i = 0; i < F; ++i) {
+    Real acc = Real(0);
+#pragma unroll
+    for (int j = 0; j <= i; ++j) acc += (Real)it.chol[i * kMaxFactors + j] * z[j];
+    y[i] = acc;
+  }
+}
+
+// exp/log per tier: FP64 uses the correctly rounded-to-1ulp device libm.
+__device__ __forceinline__ double tier_exp(double x) { return exp(x); }
+__device__ __forceinline__ float tier_exp(float x) { return __expf(x); }
+__device__ __forceinline__ double tier_log(double x) { return log(x); }
+__device__ __forceinline__ float tier_log(float x) { return __logf(x); }
+__device__ __forceinline__ double tier_div(double a, double b) { return a / b; }
+__device__ __forceinline__ float tier_div(float a, float b) { return __fdividef(a, b); }
+
+// Local-vol lookup sigma(t, S) (riga-market/src/lib.rs:305-343) against a
+// surface whose knots and two bracketing tenor rows sit in LDS.
+template <typename Real>
+__device__ __forceinline__ Real lv_sigma(const Real* __restrict__ ls, int nx, Real inv_dx, Real wt,
+                                         Real s) {
+  const Real* r0 = ls + nx;
+  const Real* r1 = ls + 2 * nx;
+  const Real x = tier_log(s > Real(1e-12) ? s : Real(1e-12));
+  int lo, hi;
+  if (x <= ls[0]) {
+    lo = hi = 0;
+  } else if (x >= ls[nx - 1]) {
+    lo = hi = nx - 1;
+  } else {
+    int j = (int)((x - ls[0]) * inv_dx);
+    j = j < 0 ? 0 : (j > nx - 2 ? nx - 2 : j);
+    while (j > 0 && !(x > ls[j])) --j;
+    while (j < nx - 2 && x > ls[j + 1]) ++j;
+    lo = j;
+    hi = j + 1;
+  }
+  const Real wx = (hi == lo) ? Real(0) : tier_div(x - ls[lo], ls[hi] - ls[lo]);
+  const Real v0 = (Real(1) - wx) * r0[lo] + wx * r0[hi];
+  const Real v1 = (Real(1) - wx) * r1[lo] + wx * r1[hi];
+  return (Real(1) - wt) * v0 + wt * v1;
+}
+
+// ---------------------------------------------------------------- the kernel
+template <typename Real, int F, int SMAX>
+__global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restrict__ items,
+                                                          const int32_t* __restrict__ block_item,
+                                                          const unsigned char* __restrict__ arena,
+                                                          Partial* __restrict__ partials,
+                                                          PathRecordDev* __restrict__ records) {
+  extern __shared__ __align__(16) unsigned char lds_dyn[];
+  __shared__ BlockShared sh[SMAX];
+
+  const int item = block_item[blockIdx.x];
+  const ItemDesc& it = items[item];
+  const int S = it.S;
+  const int steps = it.steps;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int chunk = (int)blockIdx.x - it.block_begin;
+  const bool local_vol = (it.flags & kLocalVol) != 0;
+  const bool trace = (it.flags & kTrace) != 0;
+  const bool want_records = (it.flags & kRecords) != 0 && records != nullptr;
+  // dynamic LDS: [bridge stack: (depth-1) x F x kBlock][local-vol rows: n_surf x 3 x max_nx]
+  Real* lds_stack = reinterpret_cast<Real*>(lds_dyn);
+  Real* lds_w = lds_stack + (size_t)it.stack_levels * F * kBlock;
+  const Real* lds = lds_w;
+  const int nxs = it.max_nx;
+  const StepInfo* stepinfo = at<StepInfo>(arena, it.off_steps);
+  const BridgeNode* bnodes = at<BridgeNode>(arena, it.off_nodes);
+  const LvBracket* lvb = at<LvBracket>(arena, it.off_lvb);
+  const ScenarioDesc* scen = at<ScenarioDesc>(arena, it.off_scen);
+  const LvSurface* surf = at<LvSurface>(arena, it.off_surf);
+
+  for (int s = 0; s < SMAX; ++s) {
+    for (int i = tid; i < kWaves * kNumQ; i += kBlock) sh[s].acc[i / kNumQ][i % kNumQ] = DD{0.0, 0.0};
+    for (int i = tid; i < kWaves * kNumC; i += kBlock) sh[s].cnt[i / kNumC][i % kNumC] = 0;
+    if (tid == 0) sh[s].ev_n = 0;
+  }
+  __syncthreads();
+
+  const int ppb = it.paths_per_block;
+  const int batches = ppb / kBlock;
+  for (int b = 0; b < batches; ++b) {
+    const uint64_t path_local = (uint64_t)chunk * ppb + (uint64_t)b * kBlock + tid;
+    const bool valid = path_local < it.paths;
+    const uint64_t path = it.path_begin + path_local;
+    const uint32_t n = (uint32_t)(path + 1);  // Sobol/Halton point index (qmc.rs:286, 121)
+    const uint32_t gray = n ^ (n >> 1);
+
+    // ---- path state
+    BridgeStack<Real, F> bs;
+    bs.col = lds_stack + tid;
+    bs.sp = 0;
+    Real wlo[F];
+    Real spot[SMAX][F];
+    int ko_step[SMAX];
+    Real accrual[SMAX], unpaid[SMAX], cpv[SMAX], mpv[SMAX];
+    bool received[SMAX];
+    Real worst_last[SMAX];
+    {
+      Real y[F];
+      node_normals<Real, F>(it, arena, n, gray, 0, y);
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        bs.top[f] = y[f] * (Real)it.root_sd;  // W[T] = y_0 sqrt(T dt), qmc.rs:165
+        wlo[f] = Real(0);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s) {
+#pragma unroll
+      for (int f = 0; f < F; ++f) spot[s][f] = (Real)scen[s < S ? s : 0].spot[f];
+      ko_step[s] = -1;
+      accrual[s] = unpaid[s] = cpv[s] = mpv[s] = Real(0);
+      received[s] = false;
+      worst_last[s] = Real(1);
+    }
+
+    int node_ptr = 0;
+    int loaded_i0[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) loaded_i0[f] = -1;
+
+    for (int step = 0; step < steps; ++step) {
+      const StepInfo si = stepinfo[step];
+      // ---- streaming bridge: descend to the interval (step, step+1)
+      Real wnext[F], shock[F];
+      if (si.n_desc == 0) {
+#pragma unroll
+        for (int f = 0; f < F; ++f) wnext[f] = bs.top[f];
+        bs.pop();
+      } else {
+        for (int d = 0; d < si.n_desc; ++d) {
+          const BridgeNode bn = bnodes[node_ptr++];
+          Real y[F], wmid[F];
+          node_normals<Real, F>(it, arena, n, gray, bn.node, y);
+#pragma unroll
+          for (int f = 0; f < F; ++f)
+            wmid[f] = Real(0.5) * (wlo[f] + bs.top[f]) + y[f] * (Real)bn.sd;  // qmc.rs:185-187
+          if (d + 1 < si.n_desc) {
+            bs.push(wmid);
+          } else {
+#pragma unroll
+            for (int f = 0; f < F; ++f) wnext[f] = wmid[f];
+          }
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        shock[f] = (wnext[f] - wlo[f]) * (Real)it.inv_sqrt_dt;  // qmc.rs:169, note.rs:101-105
+        wlo[f] = wnext[f];
+      }
+
+      // ---- local-vol rows for this step (uniform across the block)
+      if (local_vol) {
+        bool reload = false;
+#pragma unroll
+        for (int f = 0; f < F; ++f) reload |= lvb[step * F + f].i0 != loaded_i0[f];
+        if (reload) {
+          __syncthreads();
+          for (int u = 0; u < it.n_surf; ++u) {
+            const LvSurface su = surf[u];
+            const LvBracket br = lvb[step * F + su.factor];
+            const double* xs = at<double>(arena, su.off_x);
+            const double* v0 = at<double>(arena, su.off_v) + (size_t)br.i0 * su.n_x;
+            const double* v1 = at<double>(arena, su.off_v) + (size_t)br.i1 * su.n_x;
+            Real* dst = lds_w + (size_t)u * 3 * nxs;
+            for (int i = tid; i < su.n_x; i += kBlock) {
+              dst[i] = (Real)xs[i];
+              dst[nxs + i] = (Real)v0[i];
+              dst[2 * nxs + i] = (Real)v1[i];
+            }
+          }
+#pragma unroll
+          for (int f = 0; f < F; ++f) loaded_i0[f] = lvb[step * F + f].i0;
+          __syncthreads();
+        }
+      }
+
+      const bool call_active = (si.flags & 2) != 0;
+      const bool period_end = (si.flags & 1) != 0;
+      const bool new_period = (si.flags & 4) != 0;
+      const bool fixed_now = si.period == 0 && (it.flags & kFixedFirst) != 0;
+#pragma unroll
+      for (int s = 0; s < SMAX; ++s) {
+        if (s >= S) continue;
+        const ScenarioDesc& sc = scen[s];
+        // ---- Euler step (note.rs:134-142)
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          Real incr;
+          if (local_vol) {
+            const int u = sc.lv[f];
+            const LvSurface& su = surf[u];
+            const Real vol = lv_sigma<Real>(lds + (size_t)u * 3 * nxs, su.n_x, (Real)su.inv_dx,
+                                            (Real)lvb[step * F + f].wt, spot[s][f]);
+            const Real drift = ((Real)sc.rq[f] - Real(0.5) * vol * vol) * (Real)it.dt;
+            incr = drift + vol * (Real)it.sqrt_dt * shock[f];
+          } else {
+            incr = (Real)sc.drift[f] + (Real)sc.vsd[f] * shock[f];
+          }
+          spot[s][f] *= tier_exp(incr);
+        }
+        // ---- worst-of (note.rs:143-145)
+        Real worst = spot[s][0] / (Real)it.initial_spot[0];
+#pragma unroll
+        for (int f = 1; f < F; ++f) worst = fmin(worst, spot[s][f] / (Real)it.initial_spot[f]);
+        worst_last[s] = worst;
+        if (ko_step[s] >= 0) continue;  // note.rs:146-148: cash flows stop after KO
+        // ---- coupon accrual (note.rs:151-163)
+        if (new_period) accrual[s] = Real(0);
+        if (!fixed_now && (double)worst >= it.cb) accrual[s] += (Real)si.accrual_inc;
+        // ---- daily global KO (note.rs:166-195)
+        bool all_above = call_active;
+#pragma unroll
+        for (int f = 0; f < F; ++f)
+          all_above = all_above && ((double)(spot[s][f] / (Real)it.initial_spot[f]) >= it.ko);
+        if (all_above) {
+          ko_step[s] = step;
+          const Real due = fixed_now ? (Real)it.coupon_rate : accrual[s];
+          const Real carry = due > Real(0) ? unpaid[s] : Real(0);
+          if (due + carry > Real(0)) {
+            cpv[s] += due * (Real)si.df;
+            mpv[s] += carry * (Real)si.df;
+            received[s] = true;
+          }
+          continue;
+        }
+        // ---- period end (note.rs:197-218)
+        if (period_end) {
+          const Real due = fixed_now ? (Real)it.coupon_rate : accrual[s];
+          if (due > Real(0)) {
+            cpv[s] += due * (Real)si.df;
+            mpv[s] += unpaid[s] * (Real)si.df;
+            received[s] = true;
+            unpaid[s] = Real(0);
+          } else if (it.flags & kMemoryCoupon) {
+            unpaid[s] += (Real)it.coupon_rate;
+          }
+        }
+      }
+    }
+
+    // ---- path end (note.rs:221-258) and reductions
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s) {
+      if (s >= S) continue;
+      const bool ko = ko_step[s] >= 0;
+      const double funding = ko ? stepinfo[ko_step[s]].df : it.put_df;
+      const double fw = (double)worst_last[s];
+      const bool hit_ki = fw <= it.ki;
+      const bool eligible = hit_ki && !ko;
+      const double put = eligible ? fmax(it.strike - fw, 0.0) * it.put_df : 0.0;
+      const double nominal = (double)cpv[s] + (double)mpv[s];
+      const double coupon_quote = nominal * 10.0;
+      const double memory_quote = (double)mpv[s] * 10.0;
+      const double pv = funding + nominal - put;
+      if (want_records && valid) {
+        PathRecordDev r;
+        r.ko_step = ko_step[s];
+        r.hit_ki = hit_ki;
+        r.final_worst = fw;
+        r.funding = funding;
+        r.put = put;
+        r.coupon_pv = (double)cpv[s];
+        r.memory_pv = (double)mpv[s];
+        r.pv = pv;
+        records[it.record_base + (int64_t)s * (int64_t)it.paths + (int64_t)path_local] = r;
+      }
+      // plain sums (funding, put, coupon and memory quotes) and (x, x^2)
+      // pairs for the four moment sets; invalid lanes contribute zero.
+      DD qd[kNumQ];
+      qd[kQFunding] = dd_from(valid ? funding : 0.0);
+      qd[kQPut] = dd_from(valid ? put : 0.0);
+      qd[kQCq] = dd_from(valid ? coupon_quote - memory_quote : 0.0);
+      qd[kQMq] = dd_from(valid ? memory_quote : 0.0);
+      const double mx[4] = {pv, ko ? funding : 0.0, hit_ki ? put : 0.0, received[s] ? nominal : 0.0};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double x = valid ? mx[k] : 0.0;
+        qd[kQPv1 + 2 * k] = dd_from(x);
+        qd[kQPv2 + 2 * k] = dd_sq(x);
+      }
+#pragma unroll
+      for (int k = 0; k < kNumQ; ++k) qd[k] = wave_sum(qd[k]);
+      const uint64_t b_valid = __ballot(valid);
+      const uint64_t b_ko = __ballot(valid && ko);
+      const uint64_t b_ki = __ballot(valid && hit_ki);
+      const uint64_t b_el = __ballot(valid && eligible);
+      const uint64_t b_cp = __ballot(valid && received[s]);
+      if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < kNumQ; ++k) sh[s].acc[wave][k] = dd_add(sh[s].acc[wave][k], qd[k]);
+        sh[s].cnt[wave][kCN] += __popcll(b_valid);
+        sh[s].cnt[wave][kCKo] += __popcll(b_ko);
+        sh[s].cnt[wave][kCKi] += __popcll(b_ki);
+        sh[s].cnt[wave][kCPutEligible] += __popcll(b_el);
+        sh[s].cnt[wave][kCCouponPaths] += __popcll(b_cp);
+      }
+      if (trace) {
+        // first 64 KO events of this block in path order (note.rs:174-180)
+        if (lane == 0) sh[s].wave_ko[wave] = __popcll(b_ko);
+        __syncthreads();
+        int base = sh[s].ev_n;
+        for (int w = 0; w < wave; ++w) base += sh[s].wave_ko[w];
+        const int rank = base + (int)__popcll(b_ko & ((1ull << lane) - 1ull));
+        if (valid && ko && rank < kTraceCap) {
+          sh[s].ev_path[rank] = (int32_t)path_local;
+          sh[s].ev_step[rank] = ko_step[s];
+        }
+        __syncthreads();
+        if (tid == 0) {
+          int tot = sh[s].ev_n;
+          for (int w = 0; w < kWaves; ++w) tot += sh[s].wave_ko[w];
+          sh[s].ev_n = tot < kTraceCap ? tot : kTraceCap;
+        }
+        __syncthreads();
+      }
+    }
+  }
+
+  __syncthreads();
+  // ---- block partials, waves merged in order
+  for (int s = 0; s < S && s < SMAX; ++s) {
+    Partial* out = partials + it.partial_base + (int64_t)s * it.n_blocks + chunk;
+    if (tid < kNumQ) {
+      DD a = sh[s].acc[0][tid];
+      for (int w = 1; w < kWaves; ++w) a = dd_add(a, sh[s].acc[w][tid]);
+      out->q[tid] = a;
+    } else if (tid >= 32 && tid < 32 + kNumC) {
+      const int c = tid - 32;
+      uint64_t a = 0;
+      for (int w = 0; w < kWaves; ++w) a += sh[s].cnt[w][c];
+      out->c[c] = a;
+    } else if (tid == 64) {
+      const int ne = trace ? sh[s].ev_n : 0;
+      out->n_events = ne;
+      for (int e = 0; e < ne; ++e) {
+        out->ev_path_lo[e] = sh[s].ev_path[e];
+        out->ev_step[e] = sh[s].ev_step[e];
+      }
+    }
+  }
+}
+
+}  // namespace riga

@@ -1,0 +1,146 @@
+// phoenix_layout.h -- plain-old-data records shared by the host planner and the
+// fused Phoenix kernel.  Everything here lives in one device arena per batch.
+//
+// Vocabulary follows the reference engine: an *item* is one trade priced under
+// S scenarios that share its seed, schedule, correlation and Sobol points (the
+// reference's CRN scenario set, crates/riga-risk/src/lib.rs:117-186 and
+// crates/riga-engine/src/dispatch.rs:23-70); a *path-step* is one (trade,
+// scenario, path) advanced one observation date across all F factors.
+#pragma once
+#include <stdint.h>
+
+namespace riga {
+
+constexpr int kMaxFactors = 5;      // per-trade underlyings handled by the kernels
+constexpr int kBridgeStack = 14;    // pending bridge intervals (host-checked per trade)
+constexpr int kTraceCap = 64;       // note.rs:57 TRACE_EVENT_CAP
+constexpr int kMaxLvX = 256;        // log-spot points per local-vol row staged in LDS
+constexpr int kBlock = 256;         // threads per workgroup (4 waves of 64)
+
+enum ItemFlags : uint32_t {
+  kMemoryCoupon = 1u << 0,   // PhoenixTrade::memory_coupon
+  kFixedFirst = 1u << 1,     // PhoenixTrade::fixed_first_coupon
+  kLocalVol = 1u << 2,       // PriceRequest::local_vol present
+  kTrace = 1u << 3,          // trace_level != Off
+  kHalton = 1u << 4,         // VarianceReduction::Halton ladder (else Sobol)
+  kRecords = 1u << 5,        // write one PathRecord per (scenario, path)
+};
+
+// Per observation date, uniform across all paths of an item.
+struct StepInfo {
+  double accrual_inc;  // coupon_rate / period_steps(period), note.rs:159-162
+  double df;           // exp(-r * (step+1) * dt), note.rs:189,210
+  int32_t period;      // min(step*P/T, P-1), note.rs:151
+  int32_t flags;       // bit0 period end (note.rs:197-202), bit1 call active (note.rs:166),
+                       // bit2 new period: accrual restarts (note.rs:152-155)
+  int32_t n_desc;      // bridge nodes first needed at this step (streaming order)
+  int32_t depth;       // bridge stack depth before this step's descents (host check)
+};
+
+// One streaming-bridge node (qmc.rs:173-191): pre-order index and its
+// conditional standard deviation sqrt((hi-mid)(mid-lo)/(hi-lo) * dt).
+struct BridgeNode {
+  double sd;
+  int32_t node;
+  int32_t pad;
+};
+
+// Local-vol time bracket per (step, factor): riga-market/src/lib.rs:324-339
+struct LvBracket {
+  double wt;
+  int32_t i0, i1;
+};
+
+// One local-vol surface (DupireLocalVol fields, riga-market/src/lib.rs:119-123)
+// stored as byte offsets into the batch arena.
+struct LvSurface {
+  int64_t off_t;     // n_t tenors
+  int64_t off_x;     // n_x log-spot knots
+  int64_t off_v;     // n_t * n_x vols
+  int32_t n_t, n_x;
+  int32_t factor;    // which factor of the item this surface drives
+  int32_t pad;
+  double inv_dx;     // (n_x - 1) / (log_s[n_x-1] - log_s[0]): bracket guess only
+};
+
+struct ScenarioDesc {
+  double spot[kMaxFactors];   // MarketSnapshot::spots (bumped per scenario)
+  double rq[kMaxFactors];     // rate - dividend_yield[f]
+  double drift[kMaxFactors];  // flat vol: (rate - q - 0.5 vol^2) dt
+  double vsd[kMaxFactors];    // flat vol: vol * sqrt(dt)
+  int32_t lv[kMaxFactors];    // surface index per factor (local vol)
+};
+
+struct ItemDesc {
+  int32_t F, S, steps, periods;
+  int32_t call_start, flags, ladder_dims, nbits;
+  uint64_t paths, seed;
+  uint64_t path_begin;        // first path id (for sharded/partial runs)
+  double dt, sqrt_dt, inv_sqrt_dt, root_sd;
+  double strike, ki, ko, cb, coupon_rate, put_df;
+  double initial_spot[kMaxFactors];
+  double chol[kMaxFactors * kMaxFactors];  // lower triangular, row-major F x F
+  int64_t off_nodes;     // BridgeNode[steps - 1] in streaming order
+  int64_t off_steps;     // StepInfo[steps]
+  int64_t off_lvb;       // LvBracket[steps * F]
+  int64_t off_dirs;      // uint32: Sobol direction rows, ladder_dims x 32
+                         // (all off_* are byte offsets into the batch arena)
+  int64_t off_scen;      // ScenarioDesc[S]
+  int64_t off_halton;    // uint32: Halton [ladder_dims] bases, [ladder_dims] perm offsets, perms
+  int64_t off_surf;      // LvSurface[n_surf] distinct surfaces used by the scenarios
+  int32_t n_surf;
+  int32_t max_nx;
+  int32_t stack_levels;  // bridge levels below the top held in LDS (launch-wide)
+  int32_t pad2;
+  int64_t partial_base;  // first Partial slot: [scenario][block_in_item]
+  int64_t record_base;   // first PathRecord slot: [scenario][path]
+  int32_t block_begin;   // first block of this item in the launch
+  int32_t n_blocks;
+  int32_t paths_per_block;
+  int32_t ladder_rows;   // ladder rows uploaded (min(ladder_dims, F*steps))
+};
+
+// Double-double accumulator (hi + lo).
+struct DD {
+  double hi, lo;
+};
+
+// Reduction record: one per (item, scenario, block) as a partial, and one per
+// (item, scenario) once merged in block order by the finalize kernel.
+enum Quantity {
+  kQFunding = 0, kQPut, kQCq, kQMq,    // note.rs:254-257 sums (coupon_quote - memory_quote, memory_quote)
+  kQPv1, kQPv2,                        // pv moments: sum x, sum x^2   (note.rs:258)
+  kQKo1, kQKo2,                        // KO funding moments           (note.rs:223-225)
+  kQKi1, kQKi2,                        // KI put moments               (note.rs:242-244)
+  kQCp1, kQCp2,                        // coupon nominal moments       (note.rs:246-249)
+  kNumQ
+};
+enum Count { kCN = 0, kCKo, kCKi, kCPutEligible, kCCouponPaths, kNumC };
+
+struct Partial {
+  DD q[kNumQ];
+  uint64_t c[kNumC];
+  int32_t n_events, pad;
+  int32_t ev_path_lo[kTraceCap];      // path index within the item, first 64 KO in path order
+  int32_t ev_step[kTraceCap];
+};
+
+// Static LDS of the kernel, one per scenario the lane carries.
+constexpr int kWavesPerBlock = kBlock / 64;
+struct BlockShared {
+  DD acc[kWavesPerBlock][kNumQ];     // per-wave running sums (lane 0 of each wave writes its slot)
+  uint64_t cnt[kWavesPerBlock][kNumC];
+  int32_t ev_n;                      // KO events collected so far in this block
+  int32_t wave_ko[kWavesPerBlock];
+  int32_t ev_path[kTraceCap];
+  int32_t ev_step[kTraceCap];
+};
+
+// Per-path record (explain.rs:58-74 subset) written only for replay queries.
+struct PathRecordDev {
+  int32_t ko_step;
+  int32_t hit_ki;
+  double final_worst, funding, put, coupon_pv, memory_pv, pv;
+};
+
+}  // namespace riga

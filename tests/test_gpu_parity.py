@@ -1,0 +1,115 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bars (stated per test): Sobol/Halton integers and KO/KI event indices are
+bit-exact; FP64 prices within 1e-10 relative; FP32-tier prices within 1e-4
+relative of the FP64 oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from parity import FP32_REL, assert_fp64_parity, rel
+from riga_flow_amd import api, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_is_gfx950_and_named():
+    api.gpu_backend()
+    assert api.arch().startswith("gfx950")
+    assert api.device_name()
+
+
+@pytest.mark.parametrize("seed,ladder_dims,first_point,n_points,first_dim,n_dims", [
+    (42, 8, 1, 64, 0, 8),            # the reference test's point/dim range (qmc.rs:479-500)
+    (1, 756, 1, 1024, 0, 24),
+    (1, 512, 60000, 256, 500, 40),   # reference cap: dims >= 512 wrap their direction row
+    (7, 2268, 262100, 44, 2240, 28),
+])
+def test_sobol_integers_bit_exact(seed, ladder_dims, first_point, n_points, first_dim, n_dims):
+    u, s, z = api.debug_sobol(seed, ladder_dims, first_point, n_points, first_dim, n_dims)
+    sob = O.Sobol(ladder_dims, seed, max_dim=max(512, ladder_dims))
+    for p in range(n_points):
+        n = first_point + p
+        for d in range(n_dims):
+            dim = first_dim + d
+            assert int(u[p, d]) == sob.unscrambled(n, dim)
+            assert int(s[p, d]) == sob.scrambled(n, dim)
+            assert z[p, d] == pytest.approx(sob.normal(n - 1, dim), rel=1e-15, abs=1e-15)
+
+
+def _flat(paths, steps=252, **kw):
+    req = W.flat_book(paths=paths, steps=steps)
+    for k, v in kw.items():
+        setattr(req.trade, k, v)
+    return req
+
+
+def test_flat_book_fp64_parity():
+    req = _flat(4096)
+    md = W.max_dimension_for(req)
+    res = api.price_put_leg(req, "fp64", max_dimension=md)
+    ora = O.price(req, max_dim=md)
+    assert_fp64_parity(res, ora)
+    # trace events: first 64 KO (path, step) in path order, note.rs:174-180
+    dt = req.trade.maturity_years / req.trade.steps
+    want = [f"daily_global_ko:path={p}:t={(s + 1) * dt:.6f}" for p, s in ora.trace]
+    assert res.trace_events == want
+
+
+def test_flat_book_event_indices_bit_exact():
+    req = _flat(2048)
+    md = W.max_dimension_for(req)
+    recs, res = api.replay_paths(req, "fp64", max_dimension=md)
+    ora, orecs = O.price(req, max_dim=md, records=True)
+    assert np.array_equal(recs["ko_step"], orecs["ko_step"])
+    assert np.array_equal(recs["hit_ki"], orecs["hit_ki"])
+    for f in ("final_worst", "funding", "put", "coupon_pv", "memory_pv", "pv"):
+        np.testing.assert_allclose(recs[f], orecs[f], rtol=1e-12, atol=1e-15)
+    assert_fp64_parity(res, ora)
+
+
+def test_lv_book_fp64_parity():
+    req = W.lv_book(paths=512, steps=756)
+    md = W.max_dimension_for(req)
+    res = api.price_put_leg(req, "fp64", max_dimension=md)
+    ora = O.price(req, max_dim=md)
+    assert_fp64_parity(res, ora)
+
+
+def test_lv_book_fp32_tier_within_1e4():
+    req = W.lv_book(paths=2048, steps=756)
+    md = W.max_dimension_for(req)
+    res = api.price_put_leg(req, "fp32", max_dimension=md)
+    ora = O.price(req, max_dim=md)
+    assert rel(res.pv, ora.pv) <= FP32_REL
+    assert rel(res.legs.funding, ora.leg_funding) <= FP32_REL
+
+
+def test_crn_scenarios_alone_equal_in_mix():
+    names, reqs = W.crn_scenarios(paths=1024, steps=126, years=0.5)
+    md = max(W.max_dimension_for(r) for r in reqs)
+    mix = api.price_batch(reqs, "fp64", max_dimension=md)
+    for i in (0, 1, 7):
+        alone = api.price_put_leg(reqs[i], "fp64", max_dimension=md)
+        assert alone.pv == mix[i].pv
+        assert alone.legs.down_and_in_put == mix[i].legs.down_and_in_put
+    ora = O.price(reqs[3], max_dim=md)
+    assert_fp64_parity(mix[3], ora)
+
+
+def test_halton_lane_parity():
+    req = _flat(1024, steps=64)
+    req.config.variance_reduction = W.VarianceReduction("halton", 64 * 3)
+    res = api.price_put_leg(req, "fp64")
+    ora = O.price(req)
+    assert_fp64_parity(res, ora)
+
+
+def test_reference_dimension_cap_wraps_like_the_reference():
+    req = _flat(1024, steps=252)  # 756 dims requested, 512 cap of qmc.rs:9
+    res = api.price_put_leg(req, "fp64", max_dimension=512)
+    ora = O.price(req, max_dim=512)
+    assert_fp64_parity(res, ora)
