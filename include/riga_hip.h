@@ -122,11 +122,25 @@ typedef struct riga_path_record {
   double final_worst, funding, put, coupon_pv, memory_pv, pv;
 } riga_path_record;
 
-/* Tuning knobs chosen by the autotuner (riga_autotune) or by hand. */
+/* Launch shape (chosen by the autotuner in riga_flow_amd/autotune.py or by hand). */
 typedef struct riga_launch_config {
-  int32_t paths_per_thread; /* 1, 2, 4 */
-  int32_t reserved[3];
+  int32_t paths_per_thread; /* 1, 2, 4 or 8 paths per lane */
+  int32_t use_graph;        /* 1: capture the launch sequence in a hipGraph at plan creation */
+  int32_t reserved[2];
 } riga_launch_config;
+
+/* A batch laid out in device memory once (surfaces, factors, schedules, Sobol
+ * tables) and launched any number of times; each plan owns a stream, so plans
+ * launched back to back overlap on the device. */
+typedef struct riga_plan riga_plan;
+
+typedef struct riga_plan_info {
+  int32_t n_requests, n_items, n_launches, blocks;
+  double path_steps;    /* sum over requests of paths * steps */
+  int64_t lds_bytes;    /* dynamic LDS of the largest launch */
+  int64_t device_bytes; /* device memory held by the plan */
+  int32_t graph, reserved;
+} riga_plan_info;
 
 /* ---- device (gpu.rs:490-512) ---- */
 int riga_hip_init(int device, riga_ctx **out, riga_error *err);
@@ -144,6 +158,15 @@ int riga_price_put_leg(riga_ctx *ctx, const riga_request *req, int precision, ri
 /* ---- batched pricing: n requests in one launch; results[i] for reqs[i] ---- */
 int riga_price_batch(riga_ctx *ctx, const riga_request *reqs, int n, int precision,
                      const riga_launch_config *cfg, riga_result *results, riga_error *err);
+
+/* ---- resident plans: create once, launch many times (no device allocation per launch) ---- */
+int riga_plan_create(riga_ctx *ctx, const riga_request *reqs, int n, int precision,
+                     const riga_launch_config *cfg, riga_plan **out, riga_error *err);
+int riga_plan_launch(riga_plan *plan, riga_error *err);  /* asynchronous */
+int riga_plan_wait(riga_plan *plan, riga_result *results, riga_error *err);
+int riga_plan_run(riga_plan *plan, riga_result *results, riga_error *err); /* launch + wait */
+int riga_plan_query(const riga_plan *plan, riga_plan_info *info);
+void riga_plan_destroy(riga_plan *plan);
 
 /* ---- replay (explain.rs:90-316): per-path records for paths [0, req->paths) ---- */
 int riga_replay_paths(riga_ctx *ctx, const riga_request *req, int precision,

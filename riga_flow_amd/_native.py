@@ -94,11 +94,20 @@ RECORD_DTYPE = np.dtype([("ko_step", "<i4"), ("hit_ki", "<i4"), ("final_worst", 
 
 
 class LaunchConfig(C.Structure):
-    _fields_ = [("paths_per_thread", C.c_int32), ("reserved", C.c_int32 * 3)]
+    _fields_ = [("paths_per_thread", C.c_int32), ("use_graph", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("n_requests", C.c_int32), ("n_items", C.c_int32), ("n_launches", C.c_int32),
+                ("blocks", C.c_int32), ("path_steps", C.c_double), ("lds_bytes", C.c_int64),
+                ("device_bytes", C.c_int64), ("graph", C.c_int32), ("reserved", C.c_int32)]
 
 
 EXPORTS = ("riga_hip_init", "riga_hip_destroy", "riga_hip_device_name", "riga_hip_arch",
            "riga_validate", "riga_price_put_leg", "riga_price_batch", "riga_replay_paths",
+           "riga_plan_create", "riga_plan_launch", "riga_plan_wait", "riga_plan_run",
+           "riga_plan_query", "riga_plan_destroy",
            "riga_lv_calibrate", "riga_lv_sigma", "riga_debug_sobol")
 
 _lib = None
@@ -133,6 +142,15 @@ def lib():
             L.riga_replay_paths.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int,
                                             C.POINTER(PathRecord), C.POINTER(Result),
                                             C.POINTER(Error)]
+            L.riga_plan_create.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int, C.c_int,
+                                           C.POINTER(LaunchConfig), C.POINTER(C.c_void_p),
+                                           C.POINTER(Error)]
+            L.riga_plan_launch.argtypes = [C.c_void_p, C.POINTER(Error)]
+            L.riga_plan_wait.argtypes = [C.c_void_p, C.POINTER(Result), C.POINTER(Error)]
+            L.riga_plan_run.argtypes = [C.c_void_p, C.POINTER(Result), C.POINTER(Error)]
+            L.riga_plan_query.argtypes = [C.c_void_p, C.POINTER(PlanInfo)]
+            L.riga_plan_destroy.argtypes = [C.c_void_p]
+            L.riga_plan_destroy.restype = None
             L.riga_lv_calibrate.argtypes = [C.POINTER(ImpliedGrid), C.c_double, C.c_double,
                                             C.c_int32, _D, _D, _D, C.POINTER(Error)]
             L.riga_lv_sigma.argtypes = [C.POINTER(LvGrid), C.c_double, C.c_double]

@@ -290,6 +290,60 @@ def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",
     return [_assemble(q, res[i]) for i, q in enumerate(requests)]
 
 
+class Plan:
+    """A batch resident in device memory, launched any number of times."""
+
+    def __init__(self, requests: Sequence[PriceRequest], precision: str = "fp64",
+                 paths_per_thread: int = 1, use_graph: bool = False, max_dimension: int = 512):
+        gpu_backend()
+        for q in requests:
+            _check_as_of(q)
+        self.requests = list(requests)
+        packed = [_Packed(q, max_dimension) for q in requests]
+        n = len(packed)
+        arr = (N.Request * n)(*[p.r for p in packed])
+        cfg = N.LaunchConfig(paths_per_thread, 1 if use_graph else 0)
+        err = N.Error()
+        h = C.c_void_p()
+        N.check(N.lib().riga_plan_create(N.Device.get().handle, arr, n, PRECISIONS[precision],
+                                         C.byref(cfg), C.byref(h), C.byref(err)), err)
+        self.handle = h
+        self._res = (N.Result * n)()
+        info = N.PlanInfo()
+        N.lib().riga_plan_query(h, C.byref(info))
+        self.info = info
+
+    @property
+    def path_steps(self) -> float:
+        return self.info.path_steps
+
+    def launch(self) -> None:
+        err = N.Error()
+        N.check(N.lib().riga_plan_launch(self.handle, C.byref(err)), err)
+
+    def wait(self, raw: bool = False):
+        err = N.Error()
+        N.check(N.lib().riga_plan_wait(self.handle, self._res, C.byref(err)), err)
+        if raw:
+            return list(self._res)
+        return [_assemble(q, self._res[i]) for i, q in enumerate(self.requests)]
+
+    def run(self, raw: bool = False):
+        self.launch()
+        return self.wait(raw)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            N.lib().riga_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def price_put_leg(request: PriceRequest, precision: str = "fp64",
                   max_dimension: int = 512) -> PriceResult:
     """``price_put_leg`` (lib.rs:183-186) on the device."""

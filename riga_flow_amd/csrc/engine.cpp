@@ -422,16 +422,85 @@ void fill_result(const Partial& fr, uint64_t paths, riga_result* out) {
   }
 }
 
-// ------------------------------------------------------------ the batch run
-int run_batch(riga_ctx* ctx, const riga_request* reqs, int n, int precision,
-              const riga_launch_config* cfg, riga_result* results, riga_path_record* records,
-              riga_error* err) {
+// ------------------------------------------------------------ plans
+struct LaunchPlan {
+  int F = 0, smax = 1, blocks = 0;
+  size_t lds = 0;
+  size_t block_off = 0;
+};
+
+}  // namespace
+
+// A batch laid out in device memory once and launched any number of times.
+struct riga_plan {
+  riga_ctx* ctx = nullptr;
+  int precision = RIGA_FP64;
+  bool records = false;
+  bool graph = false;
+  int n_requests = 0;
+  std::vector<Item> items;
+  std::vector<int64_t> final_base;
+  int64_t final_slots = 0, record_slots = 0, partial_slots = 0;
+  std::vector<LaunchPlan> launches;
+  double path_steps = 0.0;
+  int blocks = 0;
+  size_t lds_max = 0;
+  DevBuf arena, items_d, block_item, partials, finals, final_base_d, records_d;
+  HostBuf finals_host;
+  std::vector<PathRecordDev> recs_host;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  ~riga_plan() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (g) (void)hipGraphDestroy(g);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Enqueue the kernels, the finalize merge and the download of the merged records.
+hipError_t enqueue(riga_plan* pl, bool timed) {
+  hipError_t e;
+  if (timed && (e = hipEventRecord(pl->e0, pl->stream)) != hipSuccess) return e;
+  for (const LaunchPlan& lp : pl->launches) {
+    PhoenixLaunch L;
+    L.F = lp.F;
+    L.smax = lp.smax;
+    L.blocks = lp.blocks;
+    L.lds = lp.lds;
+    L.items = (const ItemDesc*)pl->items_d.p;
+    L.block_item = (const int32_t*)pl->block_item.p + lp.block_off;
+    L.arena = (const unsigned char*)pl->arena.p;
+    L.partials = (Partial*)pl->partials.p;
+    L.records = pl->records ? (PathRecordDev*)pl->records_d.p : nullptr;
+    e = pl->precision == RIGA_FP64 ? launch_phoenix_f64(L, pl->stream) : launch_phoenix_f32(L, pl->stream);
+    if (e != hipSuccess) return e;
+  }
+  if (timed && (e = hipEventRecord(pl->e1, pl->stream)) != hipSuccess) return e;
+  e = launch_finalize((const ItemDesc*)pl->items_d.p, (int)pl->items.size(), (const Partial*)pl->partials.p,
+                      (Partial*)pl->finals.p, (const int64_t*)pl->final_base_d.p, pl->stream);
+  if (e != hipSuccess) return e;
+  return hipMemcpyAsync(pl->finals_host.p, pl->finals.p, (size_t)pl->final_slots * sizeof(Partial),
+                        hipMemcpyDeviceToHost, pl->stream);
+}
+
+int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, const riga_launch_config* cfg,
+                bool records, riga_plan** out, riga_error* err) {
   clear_err(err);
+  *out = nullptr;
   if (!ctx) {
     set_err(err, "gpu_unavailable", "no device context");
     return RIGA_ENODEV;
   }
-  if (n <= 0) return 0;
+  if (n <= 0 || !reqs) {
+    set_err(err, "invalid_argument", "a plan needs at least one request");
+    return RIGA_EINVAL;
+  }
   if (precision != RIGA_FP64 && precision != RIGA_FP32) {
     set_err(err, "invalid_precision", "precision must be RIGA_FP64 or RIGA_FP32");
     return RIGA_EINVAL;
@@ -441,6 +510,12 @@ int run_batch(riga_ctx* ctx, const riga_request* reqs, int n, int precision,
     set_err(err, "invalid_launch_config", "paths_per_thread must be 1, 2, 4 or 8");
     return RIGA_EINVAL;
   }
+  std::unique_ptr<riga_plan> pl(new riga_plan());
+  pl->ctx = ctx;
+  pl->precision = precision;
+  pl->records = records;
+  pl->graph = cfg && cfg->use_graph != 0 && !records;
+  pl->n_requests = n;
   std::vector<Prep> preps(n);
   for (int i = 0; i < n; ++i) {
     Error e;
@@ -449,8 +524,8 @@ int run_batch(riga_ctx* ctx, const riga_request* reqs, int n, int precision,
       return RIGA_EINVAL;
     }
   }
-  // CRN grouping (records need one scenario per item so path order is per request)
-  std::vector<Item> items;
+  // CRN grouping (records keep one scenario per item so path order is per request)
+  std::vector<Item>& items = pl->items;
   {
     std::map<std::string, std::vector<int>> groups;
     std::vector<std::string> order;
@@ -476,42 +551,35 @@ int run_batch(riga_ctx* ctx, const riga_request* reqs, int n, int precision,
   }
   for (Item& it : items) {
     Error e;
-    if (!build_item(preps, &it, precision, records != nullptr, &e)) {
+    if (!build_item(preps, &it, precision, records, &e)) {
       set_err(err, e.code, e.message);
       return RIGA_EINVAL;
     }
+    pl->path_steps += (double)it.d.S * (double)it.d.paths * (double)it.d.steps;
   }
-  // launches: (F, smax) partitions, blocks longest-first
+  // launches: one per (factors, scenario bucket), blocks longest-first
   const int real_size = precision == RIGA_FP64 ? 8 : 4;
   std::map<std::pair<int, int>, std::vector<int>> parts;
   for (size_t i = 0; i < items.size(); ++i) parts[{items[i].F, items[i].smax}].push_back((int)i);
-
   Arena arena;
   int sobol_rows = 1;
   for (Item& it : items)
     if (!(it.d.flags & kHalton)) sobol_rows = std::max(sobol_rows, it.d.ladder_rows);
   const std::vector<uint32_t>& dirs = sobol_directions(sobol_rows);
   const int64_t off_dirs = arena.add(dirs.data(), (size_t)sobol_rows * 32);
-
-  struct LaunchPlan {
-    int F, smax, blocks;
-    size_t lds;
-    std::vector<int32_t> block_item;
-  };
-  std::vector<LaunchPlan> launches;
-  int64_t partial_slots = 0, record_slots = 0;
-  std::vector<int64_t> final_base(items.size());
-  int64_t final_slots = 0;
+  pl->final_base.resize(items.size());
   for (size_t i = 0; i < items.size(); ++i) {
-    final_base[i] = final_slots;
-    final_slots += items[i].d.S;
+    pl->final_base[i] = pl->final_slots;
+    pl->final_slots += items[i].d.S;
   }
+  std::vector<int32_t> all_blocks;
   for (auto& kv : parts) {
     std::vector<int>& idx = kv.second;
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return items[a].cost > items[b].cost; });
     LaunchPlan lp;
     lp.F = kv.first.first;
     lp.smax = kv.first.second;
+    lp.block_off = all_blocks.size();
     int stack_levels = 0, lv_words = 0;
     for (int i : idx) {
       stack_levels = std::max(stack_levels, items[i].depth - 1);
@@ -526,6 +594,7 @@ int run_batch(riga_ctx* ctx, const riga_request* reqs, int n, int precision,
                   std::to_string(ctx->lds_limit) + "); use fewer distinct surfaces or knots");
       return RIGA_EINVAL;
     }
+    pl->lds_max = std::max(pl->lds_max, lp.lds);
     int blocks = 0;
     for (int i : idx) {
       Item& it = items[i];
@@ -535,11 +604,11 @@ int run_batch(riga_ctx* ctx, const riga_request* reqs, int n, int precision,
       d.n_blocks = (int32_t)((d.paths + d.paths_per_block - 1) / d.paths_per_block);
       d.block_begin = blocks;
       blocks += d.n_blocks;
-      for (int b = 0; b < d.n_blocks; ++b) lp.block_item.push_back(i);
-      d.partial_base = partial_slots;
-      partial_slots += (int64_t)d.S * d.n_blocks;
-      d.record_base = record_slots;
-      if (d.flags & kRecords) record_slots += (int64_t)d.S * (int64_t)d.paths;
+      for (int b = 0; b < d.n_blocks; ++b) all_blocks.push_back(i);
+      d.partial_base = pl->partial_slots;
+      pl->partial_slots += (int64_t)d.S * d.n_blocks;
+      d.record_base = pl->record_slots;
+      if (d.flags & kRecords) pl->record_slots += (int64_t)d.S * (int64_t)d.paths;
       d.off_dirs = off_dirs;
       d.off_nodes = arena.add(it.nodes.data(), it.nodes.size());
       d.off_steps = arena.add(it.steps.data(), it.steps.size());
@@ -564,87 +633,92 @@ int run_batch(riga_ctx* ctx, const riga_request* reqs, int n, int precision,
       d.max_nx = max_nx;
     }
     lp.blocks = blocks;
-    launches.push_back(std::move(lp));
+    pl->blocks += blocks;
+    pl->launches.push_back(lp);
   }
   std::vector<ItemDesc> descs(items.size());
   for (size_t i = 0; i < items.size(); ++i) descs[i] = items[i].d;
-  std::vector<int32_t> all_blocks;
-  std::vector<size_t> launch_block_off;
-  for (auto& lp : launches) {
-    launch_block_off.push_back(all_blocks.size());
-    all_blocks.insert(all_blocks.end(), lp.block_item.begin(), lp.block_item.end());
-  }
 
-  std::lock_guard<std::mutex> lock(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
-  HIP_TRY(ctx->arena.ensure(arena.bytes.size()), "arena alloc");
-  HIP_TRY(ctx->items.ensure(descs.size() * sizeof(ItemDesc)), "items alloc");
-  HIP_TRY(ctx->block_item.ensure(all_blocks.size() * sizeof(int32_t)), "block table alloc");
-  HIP_TRY(ctx->partials.ensure((size_t)partial_slots * sizeof(Partial)), "partials alloc");
-  HIP_TRY(ctx->finals.ensure((size_t)final_slots * sizeof(Partial)), "finals alloc");
-  HIP_TRY(ctx->final_base.ensure(final_base.size() * sizeof(int64_t)), "final index alloc");
-  if (records) HIP_TRY(ctx->records.ensure((size_t)record_slots * sizeof(PathRecordDev)), "records alloc");
-  const size_t up = align16(arena.bytes.size()) + align16(descs.size() * sizeof(ItemDesc)) +
-                    align16(all_blocks.size() * 4) + align16(final_base.size() * 8);
-  HIP_TRY(ctx->staging.ensure(up), "staging alloc");
-  unsigned char* st = (unsigned char*)ctx->staging.p;
-  size_t o = 0;
-  auto stage = [&](const void* src, size_t bytes, void* dst) -> hipError_t {
-    memcpy(st + o, src, bytes);
-    hipError_t e = hipMemcpyAsync(dst, st + o, bytes, hipMemcpyHostToDevice, ctx->stream);
-    o += align16(bytes);
-    return e;
-  };
-  HIP_TRY(stage(arena.bytes.data(), arena.bytes.size(), ctx->arena.p), "arena upload");
-  HIP_TRY(stage(descs.data(), descs.size() * sizeof(ItemDesc), ctx->items.p), "item upload");
-  HIP_TRY(stage(all_blocks.data(), all_blocks.size() * 4, ctx->block_item.p), "block table upload");
-  HIP_TRY(stage(final_base.data(), final_base.size() * 8, ctx->final_base.p), "final index upload");
-
-  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream), "event");
-  for (size_t li = 0; li < launches.size(); ++li) {
-    const LaunchPlan& lp = launches[li];
-    PhoenixLaunch L;
-    L.F = lp.F;
-    L.smax = lp.smax;
-    L.blocks = lp.blocks;
-    L.lds = lp.lds;
-    L.items = (const ItemDesc*)ctx->items.p;
-    L.block_item = (const int32_t*)ctx->block_item.p + launch_block_off[li];
-    L.arena = (const unsigned char*)ctx->arena.p;
-    L.partials = (Partial*)ctx->partials.p;
-    L.records = records ? (PathRecordDev*)ctx->records.p : nullptr;
-    hipError_t e = precision == RIGA_FP64 ? launch_phoenix_f64(L, ctx->stream) : launch_phoenix_f32(L, ctx->stream);
-    HIP_TRY(e, "phoenix kernel launch");
-  }
-  HIP_TRY(launch_finalize((const ItemDesc*)ctx->items.p, (int)items.size(), (const Partial*)ctx->partials.p,
-                          (Partial*)ctx->finals.p, (const int64_t*)ctx->final_base.p, ctx->stream),
-          "finalize launch");
-  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream), "event");
-  HIP_TRY(ctx->finals_host.ensure((size_t)final_slots * sizeof(Partial)), "host finals alloc");
-  HIP_TRY(hipMemcpyAsync(ctx->finals_host.p, ctx->finals.p, (size_t)final_slots * sizeof(Partial),
-                         hipMemcpyDeviceToHost, ctx->stream),
-          "finals download");
-  std::vector<PathRecordDev> recs;
+  HIP_TRY(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "stream");
+  HIP_TRY(hipEventCreate(&pl->e0), "event");
+  HIP_TRY(hipEventCreate(&pl->e1), "event");
+  HIP_TRY(pl->arena.ensure(arena.bytes.size()), "arena alloc");
+  HIP_TRY(pl->items_d.ensure(descs.size() * sizeof(ItemDesc)), "items alloc");
+  HIP_TRY(pl->block_item.ensure(all_blocks.size() * sizeof(int32_t)), "block table alloc");
+  HIP_TRY(pl->partials.ensure((size_t)pl->partial_slots * sizeof(Partial)), "partials alloc");
+  HIP_TRY(pl->finals.ensure((size_t)pl->final_slots * sizeof(Partial)), "finals alloc");
+  HIP_TRY(pl->final_base_d.ensure(pl->final_base.size() * sizeof(int64_t)), "final index alloc");
+  HIP_TRY(pl->finals_host.ensure((size_t)pl->final_slots * sizeof(Partial)), "host finals alloc");
   if (records) {
-    recs.resize((size_t)record_slots);
-    HIP_TRY(hipMemcpyAsync(recs.data(), ctx->records.p, recs.size() * sizeof(PathRecordDev),
-                           hipMemcpyDeviceToHost, ctx->stream),
-            "records download");
+    HIP_TRY(pl->records_d.ensure((size_t)pl->record_slots * sizeof(PathRecordDev)), "records alloc");
+    pl->recs_host.resize((size_t)pl->record_slots);
   }
-  HIP_TRY(hipStreamSynchronize(ctx->stream), "kernel execution");
+  HIP_TRY(hipMemcpyAsync(pl->arena.p, arena.bytes.data(), arena.bytes.size(), hipMemcpyHostToDevice, pl->stream),
+          "arena upload");
+  HIP_TRY(hipMemcpyAsync(pl->items_d.p, descs.data(), descs.size() * sizeof(ItemDesc), hipMemcpyHostToDevice,
+                         pl->stream),
+          "item upload");
+  HIP_TRY(hipMemcpyAsync(pl->block_item.p, all_blocks.data(), all_blocks.size() * 4, hipMemcpyHostToDevice,
+                         pl->stream),
+          "block table upload");
+  HIP_TRY(hipMemcpyAsync(pl->final_base_d.p, pl->final_base.data(), pl->final_base.size() * 8,
+                         hipMemcpyHostToDevice, pl->stream),
+          "final index upload");
+  HIP_TRY(hipStreamSynchronize(pl->stream), "plan upload");
+  if (pl->graph) {
+    HIP_TRY(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+    hipError_t e = enqueue(pl.get(), false);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(pl->stream, &g);
+    HIP_TRY(e, "graph capture enqueue");
+    HIP_TRY(e2, "graph capture end");
+    pl->g = g;
+    HIP_TRY(hipGraphInstantiate(&pl->gexec, g, nullptr, nullptr, 0), "graph instantiate");
+  }
+  *out = pl.release();
+  return 0;
+}
+
+int plan_launch(riga_plan* pl, riga_error* err) {
+  clear_err(err);
+  if (!pl) {
+    set_err(err, "invalid_argument", "plan is NULL");
+    return RIGA_EINVAL;
+  }
+  HIP_TRY(hipSetDevice(pl->ctx->device), "hipSetDevice");
+  if (pl->graph) {
+    HIP_TRY(hipEventRecord(pl->e0, pl->stream), "event");
+    HIP_TRY(hipGraphLaunch(pl->gexec, pl->stream), "graph launch");
+    HIP_TRY(hipEventRecord(pl->e1, pl->stream), "event");
+  } else {
+    HIP_TRY(enqueue(pl, true), "phoenix launch");
+  }
+  if (pl->records)
+    HIP_TRY(hipMemcpyAsync(pl->recs_host.data(), pl->records_d.p, pl->recs_host.size() * sizeof(PathRecordDev),
+                           hipMemcpyDeviceToHost, pl->stream),
+            "records download");
+  return 0;
+}
+
+int plan_wait(riga_plan* pl, riga_result* results, riga_path_record* records, riga_error* err) {
+  clear_err(err);
+  HIP_TRY(hipStreamSynchronize(pl->stream), "kernel execution");
   float ms = 0.f;
-  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), "event timing");
-  const Partial* finals = (const Partial*)ctx->finals_host.p;
-  for (size_t i = 0; i < items.size(); ++i) {
-    const Item& it = items[i];
+  HIP_TRY(hipEventElapsedTime(&ms, pl->e0, pl->e1), "event timing");
+  const Partial* finals = (const Partial*)pl->finals_host.p;
+  for (size_t i = 0; i < pl->items.size(); ++i) {
+    const Item& it = pl->items[i];
     for (int s = 0; s < it.d.S; ++s) {
       const int req = it.members[s];
-      riga_result* out = &results[req];
-      memset(out, 0, sizeof *out);
-      fill_result(finals[final_base[i] + s], it.d.paths, out);
-      out->device_ms = ms;
-      if (records) {
-        const PathRecordDev* src = recs.data() + it.d.record_base + (int64_t)s * (int64_t)it.d.paths;
+      if (results) {
+        riga_result* out = &results[req];
+        memset(out, 0, sizeof *out);
+        fill_result(finals[pl->final_base[i] + s], it.d.paths, out);
+        out->device_ms = ms;
+      }
+      if (records && pl->records) {
+        const PathRecordDev* src = pl->recs_host.data() + it.d.record_base + (int64_t)s * (int64_t)it.d.paths;
         for (uint64_t p = 0; p < it.d.paths; ++p) {
           riga_path_record& r = records[p];
           r.ko_step = src[p].ko_step;
@@ -660,6 +734,17 @@ int run_batch(riga_ctx* ctx, const riga_request* reqs, int n, int precision,
     }
   }
   return 0;
+}
+
+int run_batch(riga_ctx* ctx, const riga_request* reqs, int n, int precision, const riga_launch_config* cfg,
+              riga_result* results, riga_path_record* records, riga_error* err) {
+  riga_plan* pl = nullptr;
+  int rc = plan_create(ctx, reqs, n, precision, cfg, records != nullptr, &pl, err);
+  if (rc) return rc;
+  rc = plan_launch(pl, err);
+  if (!rc) rc = plan_wait(pl, results, records, err);
+  delete pl;
+  return rc;
 }
 
 }  // namespace
@@ -747,6 +832,52 @@ int riga_validate(const riga_request* req, riga_error* err) {
     return RIGA_EINVAL;
   }
   return 0;
+}
+
+int riga_plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision,
+                     const riga_launch_config* cfg, riga_plan** out, riga_error* err) {
+  if (!out) {
+    set_err(err, "invalid_argument", "out is NULL");
+    return RIGA_EINVAL;
+  }
+  return plan_create(ctx, reqs, n, precision, cfg, false, out, err);
+}
+
+int riga_plan_launch(riga_plan* plan, riga_error* err) { return plan_launch(plan, err); }
+
+int riga_plan_wait(riga_plan* plan, riga_result* results, riga_error* err) {
+  if (!plan) {
+    set_err(err, "invalid_argument", "plan is NULL");
+    return RIGA_EINVAL;
+  }
+  return plan_wait(plan, results, nullptr, err);
+}
+
+int riga_plan_run(riga_plan* plan, riga_result* results, riga_error* err) {
+  int rc = riga_plan_launch(plan, err);
+  return rc ? rc : riga_plan_wait(plan, results, err);
+}
+
+int riga_plan_query(const riga_plan* plan, riga_plan_info* info) {
+  if (!plan || !info) return RIGA_EINVAL;
+  memset(info, 0, sizeof *info);
+  info->n_requests = plan->n_requests;
+  info->n_items = (int32_t)plan->items.size();
+  info->n_launches = (int32_t)plan->launches.size();
+  info->blocks = plan->blocks;
+  info->path_steps = plan->path_steps;
+  info->lds_bytes = (int64_t)plan->lds_max;
+  info->device_bytes = (int64_t)(plan->arena.cap + plan->items_d.cap + plan->block_item.cap + plan->partials.cap +
+                                 plan->finals.cap + plan->final_base_d.cap + plan->records_d.cap);
+  info->graph = plan->graph ? 1 : 0;
+  return 0;
+}
+
+void riga_plan_destroy(riga_plan* plan) {
+  if (!plan) return;
+  (void)hipSetDevice(plan->ctx->device);
+  if (plan->stream) (void)hipStreamSynchronize(plan->stream);
+  delete plan;
 }
 
 int riga_price_put_leg(riga_ctx* ctx, const riga_request* req, int precision, riga_result* out,

@@ -1,0 +1,81 @@
+"""The C-ABI library: it loads, exports every entry point include/riga_hip.h
+declares, validates requests with the reference's error codes
+(crates/riga-engine/src/lib.rs:209-321) and fails loudly without a device.
+No device work happens here."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from riga_flow_amd import _native as N
+from riga_flow_amd import api, workloads as W
+from riga_flow_amd.types import RigaError
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "riga_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|double)\s+\**(riga_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = N.lib()
+    names = declared_functions()
+    assert len(names) >= 11
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(N.EXPORTS)
+
+
+def test_ctypes_struct_sizes_match_the_header_layout():
+    # riga_request/riga_result are plain C structs; keep the Python mirror in step
+    assert C.sizeof(N.Error) == 512
+    assert C.sizeof(N.Result) == 8 * 8 + 5 * 8 + 6 * 8 + 8 + 64 * 8 * 2 + 8
+    assert C.sizeof(N.PathRecord) == 56
+
+
+def _code(req, **trade):
+    for k, v in trade.items():
+        setattr(req.trade, k, v)
+    with pytest.raises(RigaError) as e:
+        api.validate(req, max_dimension=W.max_dimension_for(req))
+    return e.value.code
+
+
+def test_validation_error_codes():
+    ok = W.flat_book(paths=16)
+    api.validate(ok, max_dimension=W.max_dimension_for(ok))
+    req = W.flat_book(paths=16)
+    req.market.correlation[0][1] = 2.0
+    req.market.correlation[1][0] = 2.0
+    assert _code(req) == "invalid_market"  # lib.rs:595-607
+    assert _code(W.flat_book(paths=16), steps=0) == "invalid_schedule"
+    assert _code(W.flat_book(paths=16), knock_out_barrier=-1.0) == "invalid_trade"
+    assert _code(W.flat_book(paths=16), coupon_rate=-0.1) == "invalid_trade"
+    assert _code(W.flat_book(paths=16), strike=0.0) == "invalid_trade"
+    req = W.flat_book(paths=16)
+    req.config.paths = 0
+    assert _code(req) == "invalid_paths"
+    req = W.flat_book(paths=16)
+    req.market.spots[1] = -1.0
+    assert _code(req) == "invalid_market"
+    req = W.flat_book(paths=16)
+    req.config.variance_reduction = W.VarianceReduction("none")
+    assert _code(req) == "unsupported_variance_reduction"
+    req = W.flat_book(paths=16)
+    req.market.spots = [100.0] * 6
+    req.market.vols = [0.2] * 6
+    req.market.dividend_yields = [0.0] * 6
+    req.trade.initial_spots = [100.0] * 6
+    req.market.correlation = [[1.0 if i == j else 0.0 for j in range(6)] for i in range(6)]
+    assert _code(req) == "invalid_market_shape"
+
+
+def test_pricing_without_a_device_fails_loudly():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a device node is present")
+    with pytest.raises(RigaError) as e:
+        api.price_put_leg(W.flat_book(paths=16))
+    assert e.value.code == "gpu_unavailable"

@@ -1,0 +1,110 @@
+"""The term-sheet loader, the serde-compatible snapshot id and the PriceResult
+assembly strings, pinned against the reference's fixture, its golden result
+(fixtures/termsheet1.golden.json) and its loader test (fixture.rs:471-538).
+
+The golden PriceResult comes from the ChaCha8/ziggurat Monte Carlo lane
+(rand 0.9.5 / rand_chacha 0.9.0 / rand_distr 0.5.1, not vendored, not
+importable here): its RNG-dependent numbers are not reproducible by this
+build (parity unpinned for that lane); the RNG-independent parts are checked.
+"""
+import json
+import math
+import os
+
+import pytest
+
+from riga_flow_amd import api
+from riga_flow_amd.fixture import (civil_day_number, load_termsheet1_fixture,
+                                   remaining_business_steps)
+from riga_flow_amd.types import RigaError
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def loaded():
+    return load_termsheet1_fixture(os.path.join(GOLDEN, "termsheet1.json"))
+
+
+def test_source_market_inputs_and_trade_economics_are_mapped():
+    # fixture.rs:471-505
+    lf = loaded()
+    r = lf.request
+    assert r.market.spots == [267.885, 258.355]
+    assert abs(r.market.vols[0] - 0.45619) < 0.002
+    assert abs(r.market.vols[1] - 0.35599) < 0.002
+    assert abs(r.market.risk_free_rate - 0.038825) < 0.0005
+    assert r.market.dividend_yields == [0.0, 0.002979]
+    assert r.market.correlation[0][1] == 0.4593248180905
+    assert r.trade.initial_spots == [239.97, 260.0]
+    assert r.trade.strike == 0.78
+    assert r.trade.knock_in_barrier == 0.70
+    assert r.trade.knock_out_barrier == 1.10
+    assert r.trade.coupon_barrier == 0.1
+    assert r.trade.coupon_rate == 0.009642
+    assert r.trade.memory_coupon
+    assert r.trade.coupon_periods == 5
+    assert not r.trade.fixed_first_coupon
+    assert r.trade.call_start_step == 0
+    assert lf.market_is_pinned
+    assert lf.source_model == "dupire_local_vol"
+    assert lf.engine_model == "dupire_local_vol_calibrated_cpu"
+    g = r.local_vol.grids
+    assert len(g) == 2 and len(g[0].maturities_years) == 10 and len(g[0].strikes) == 21
+    assert len(g[0].vols) == 10 and len(g[0].vols[0]) == 21
+
+
+def test_date_conversion_counts_actual_days_and_weekdays():
+    years, weekdays = remaining_business_steps("2026-09-07", "2027-02-01")
+    assert abs(years - 147.0 / 365.0) < 1e-16
+    assert 100 < weekdays < 110
+    assert civil_day_number("2027-02-01") - civil_day_number("2026-09-07") == 147
+
+
+def test_market_snapshot_id_matches_golden():
+    golden = json.load(open(os.path.join(GOLDEN, "termsheet1.golden.json")))
+    want = golden["determinism"]["market_snapshot_id"]
+    assert api.market_snapshot_id(loaded().request.market) == want
+
+
+def test_result_strings_match_golden_layout():
+    """Ledger conditions, evidence refs and populations that do not depend on the RNG."""
+    golden = json.load(open(os.path.join(GOLDEN, "termsheet1.golden.json")))
+    req = loaded().request
+    from riga_flow_amd import _native as N
+    r = N.Result()
+    r.paths = req.config.paths
+    r.ko_count = golden["ledger"][0]["population_after"]
+    r.put_eligible_count = golden["ledger"][1]["population_after"]
+    r.coupon_path_count = golden["ledger"][2]["population_after"]
+    res = api._assemble(req, r)
+    doc = api.result_to_json(res)
+    assert list(doc.keys())[:6] == ["pv", "mc_error", "ki_probability", "legs", "stats", "ledger"]
+    for mine, gold in zip(doc["ledger"], golden["ledger"]):
+        assert mine["transition_id"] == gold["transition_id"]
+        assert mine["condition"] == gold["condition"]
+        assert mine["evidence_refs"] == gold["evidence_refs"]
+        assert mine["population_before"] == gold["population_before"]
+        assert mine["population_after"] == gold["population_after"]
+    assert doc["determinism"] == golden["determinism"]
+    assert sorted(doc["legs"].keys()) == sorted(golden["legs"].keys())
+    assert [s["transition_id"] for s in doc["stats"]] == [s["transition_id"] for s in golden["stats"]]
+
+
+def test_serde_float_layout():
+    cases = {1.0: "1.0", 267.885: "267.885", 1e-7: "1e-7", 1.5e20: "1.5e20", 1e-05: "0.00001",
+             0.0001: "0.0001", 1e16: "1e16", 1e15: "1000000000000000.0", -0.0: "-0.0"}
+    for v, s in cases.items():
+        assert api._ryu(v) == s
+
+
+def test_as_of_checks_come_first():
+    req = loaded().request
+    req.as_of = ""
+    with pytest.raises(RigaError) as e:
+        api.validate(req)
+    assert e.value.code == "missing_as_of"
+    req = loaded().request
+    req.as_of = "2026-09-08"
+    with pytest.raises(RigaError) as e:
+        api.validate(req)
+    assert e.value.code == "as_of_mismatch"

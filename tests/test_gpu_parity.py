@@ -100,6 +100,22 @@ def test_crn_scenarios_alone_equal_in_mix():
     assert_fp64_parity(mix[3], ora)
 
 
+def test_plan_graph_replays_match_eager():
+    # a hipGraph-captured plan replayed several times, with eager work between
+    # replays, gives the eager result every time
+    names, reqs = W.crn_scenarios(paths=2048, steps=126, years=0.5)
+    md = max(W.max_dimension_for(r) for r in reqs)
+    eager = api.price_batch(reqs, "fp32", max_dimension=md)
+    plan = api.Plan(reqs, "fp32", use_graph=True, max_dimension=md)
+    assert plan.info.graph == 1
+    for _ in range(3):
+        got = plan.run()
+        for a, b in zip(got, eager):
+            assert a.pv == b.pv and a.legs.down_and_in_put == b.legs.down_and_in_put
+        api.price_put_leg(reqs[0], "fp64", max_dimension=md)
+    plan.close()
+
+
 def test_halton_lane_parity():
     req = _flat(1024, steps=64)
     req.config.variance_reduction = W.VarianceReduction("halton", 64 * 3)
