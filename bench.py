@@ -1,0 +1,198 @@
+"""Benchmark of the Phoenix path-simulation hot path on one MI355X.
+
+Metric: path-steps/s of the FP32 batched form (10 CRN scenarios of the 3-asset
+local-vol trade sharing 1,048,576 Sobol paths x 756 daily steps, one hipGraph
+replay per step) beside the plain FP64 form (the same trade, one scenario,
+262,144 paths), with the FP32 price/Greek error against FP64 on identical
+paths, the roofline of the dominant kernel and the CPU oracle timed on the
+host's cores.  Prints ONE JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import threading
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from riga_flow_amd import api, workloads as W  # noqa: E402
+
+# ---------------------------------------------------------------- roofline model
+# Algorithmic work in 32-bit lane operations (FP64 operations count one each:
+# MI355X issues FP64 at the full vector rate).  Per generated coordinate:
+OWEN_ROUND = 27     # hash input xor 2, xorshift 4, 64-bit multiply 5, xorshift 4,
+                    # top word of the 2nd multiply 4, bit logic 3, prefix 1, prefix*C3 4
+PROBIT = 22         # rational (12) + divide (8) + tails (5% of log + sqrt + rational)
+UNIFORM = 4         # int->float, +0.5, scale, clamp
+LV_LOOKUP = 41      # log 15, knot bracket 6, divide 8, three lerps 12
+EULER = 22          # drift 4, increment 2, exp 15, multiply 1
+VALU_PEAK = 256 * 4 * 16 * 2.4e9  # CUs x SIMDs x lanes x clock = 3.93e13 lane-ops/s
+HBM_PEAK = 8.0e12
+
+
+def ops_per_path_step(F: int, S: int, nbits: int, local_vol: bool) -> float:
+    coord = 32 * OWEN_ROUND + 2 * nbits + UNIFORM + PROBIT
+    node = F * coord + F * (F + 1) + 3 * F + 2 * F  # coordinates, L z, bridge node, shock
+    scen = F * ((LV_LOOKUP if local_vol else 0) + EULER) + 9 * F + 5  # + worst/KO/coupon scan
+    return node + S * scen
+
+
+# ---------------------------------------------------------------- CPU baseline
+def cpu_baseline(seconds_target: float = 12.0):
+    """The oracle (single thread, then all host threads) on a bounded sample of the
+    3-asset local-vol trade; path-steps/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker, timed here as the reported CPU baseline
+    req = W.lv_book(paths=1024, steps=756)
+    md = W.max_dimension_for(req)
+    lvs = O.local_vols_for(req)
+    t0 = time.perf_counter()
+    O.price(req, max_dim=md, lvs=lvs, path_range=(0, 256))
+    per_path = (time.perf_counter() - t0) / 256
+    n1 = max(256, min(65536, int(seconds_target / per_path)))
+    req1 = W.lv_book(paths=n1, steps=756)
+    t0 = time.perf_counter()
+    O.price(req1, max_dim=md, lvs=lvs)
+    t1 = time.perf_counter() - t0
+    single = n1 * 756 / t1
+    threads = min(16, os.cpu_count() or 1)
+    nper = max(64, int(seconds_target / per_path / 2))
+    reqm = W.lv_book(paths=nper * threads, steps=756)
+
+    def work(i):
+        O.price(reqm, max_dim=md, lvs=lvs, path_range=(i * nper, (i + 1) * nper))
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    t0 = time.perf_counter()
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    tm = time.perf_counter() - t0
+    multi = nper * threads * 756 / tm
+    return {
+        "value": single, "unit": "path-steps/s", "cores": 1, "kind": "port",
+        "sample": f"oracle FP64 on the 3-asset local-vol trade, {n1} paths x 756 steps, "
+                  f"1 thread ({t1:.1f} s)",
+        "multi_thread": {"value": multi, "cores": threads,
+                         "sample": f"{nper * threads} paths x 756 steps over {threads} threads "
+                                   f"({tm:.1f} s)"},
+    }
+
+
+# ---------------------------------------------------------------- GPU legs
+def timed(plan, steps: int, warmup: int):
+    for _ in range(warmup):
+        plan.run(raw=True)
+    dev = []
+    t0 = time.perf_counter()
+    last = None
+    for _ in range(steps):
+        last = plan.run(raw=True)
+        dev.append(last[0].device_ms)
+    wall = time.perf_counter() - t0
+    return wall, dev, last
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="smaller path counts (smoke)")
+    ap.add_argument("--ppt", type=int, default=0, help="paths per thread (0 = tuned default)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the hot path shards by path range with a single "
+              f"partial merge (see DESIGN.md); --gpus {args.gpus} is not run here.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    api.gpu_backend()
+    big = 1_048_576 if not args.quick else 65_536
+    plain_paths = 262_144 if not args.quick else 16_384
+    names, crn = W.crn_scenarios(paths=big, steps=756, years=3.0)
+    md = max(W.max_dimension_for(r) for r in crn)
+    ppt = args.ppt or 1
+    fp32 = api.Plan(crn, "fp32", paths_per_thread=ppt, use_graph=True, max_dimension=md)
+    plain = W.lv_book(paths=plain_paths, steps=756)
+    fp64 = api.Plan([plain], "fp64", paths_per_thread=ppt, use_graph=True, max_dimension=md)
+
+    wall32, dev32, res32 = timed(fp32, args.steps, args.warmup)
+    wall64, dev64, res64 = timed(fp64, args.steps, args.warmup)
+    ps32 = fp32.path_steps
+    ps64 = fp64.path_steps
+    value = ps32 * args.steps / wall32
+    fp64_rate = ps64 * args.steps / wall64
+
+    # FP32 error against FP64 on identical paths and scenarios (the FP64 GPU
+    # path is the oracle's arithmetic; tests/test_gpu_parity.py pins it).
+    ref = api.Plan(crn, "fp64", max_dimension=md).run()
+    got = fp32.run()
+    rel_pv = max(abs(a.pv - b.pv) / abs(b.pv) for a, b in zip(got, ref))
+    h = [0.01 * s for s in crn[0].market.spots]
+
+    def greeks(rs):
+        out = []
+        for f in range(3):
+            up, dn, base, vg = rs[1 + 2 * f], rs[2 + 2 * f], rs[0], rs[7 + f]
+            out.append(((up.pv - dn.pv) / (2 * h[f]),
+                        (up.pv - 2 * base.pv + dn.pv) / (h[f] * h[f]),
+                        (vg.pv - base.pv) / 0.01))
+        return out
+
+    g32, g64 = greeks(got), greeks(ref)
+    gerr = {k: max(abs(a[i] - b[i]) for a, b in zip(g32, g64)) for i, k in
+            enumerate(("delta_abs", "gamma_abs", "vega_abs"))}
+
+    nbits = (big).bit_length()
+    ops = ops_per_path_step(3, 10, nbits, True) / 10.0  # per (scenario) path-step
+    dev_s = sum(dev32) / len(dev32) / 1e3
+    achieved = ops * ps32 / dev_s
+    out = {
+        "metric": "path-steps/s FP32-batched vs FP64-plain at 1e-4 price err; p50/p99 latency",
+        "value": value,
+        "unit": "path-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall32 / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "Dupire local-vol worst-of Phoenix, 3 underlyings, 3y x 756 daily steps; "
+                        "FP32 batched = 10 CRN scenarios (base, spot +-1% x3, vega +1pt x3) on "
+                        f"{big} shared Sobol paths; FP64 plain = 1 scenario, {plain_paths} paths",
+            "paths": big, "scenarios": 10, "steps": 756, "underlyings": 3,
+            "lv_grid": "64 tenors x 128 log-spot knots", "hip_graph": True,
+            "paths_per_thread": ppt,
+        },
+        "fp64_plain": {"value": fp64_rate, "unit": "path-steps/s", "paths": plain_paths,
+                       "ms_per_step": wall64 / args.steps * 1e3,
+                       "device_ms": sum(dev64) / len(dev64)},
+        "fp32_batched_over_fp64_plain": value / fp64_rate,
+        "fp32_vs_fp64": {"max_rel_pv_err": rel_pv, **gerr,
+                         "base_pv_fp32": got[0].pv, "base_pv_fp64": ref[0].pv},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
+                     "unit": "Tops/s", "frac": achieved / VALU_PEAK, "traffic": None,
+                     "kernel": "phoenix_kernel<float,3,10>", "device_ms": dev_s * 1e3,
+                     "ops_per_path_step": ops},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

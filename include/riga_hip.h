@@ -126,7 +126,9 @@ typedef struct riga_path_record {
 typedef struct riga_launch_config {
   int32_t paths_per_thread; /* 1, 2, 4 or 8 paths per lane */
   int32_t use_graph;        /* 1: capture the launch sequence in a hipGraph at plan creation */
-  int32_t reserved[2];
+  int32_t owen_table;       /* 0: tabulate the top 8 Owen levels per dimension; -1: hash all 32 */
+  int32_t normal_table;     /* 0: auto (share one [dim][path] normal table among same-seed trades
+                               when they need 4x more normals than it holds); 1: force; -1: off */
 } riga_launch_config;
 
 /* A batch laid out in device memory once (surfaces, factors, schedules, Sobol
@@ -139,7 +141,8 @@ typedef struct riga_plan_info {
   double path_steps;    /* sum over requests of paths * steps */
   int64_t lds_bytes;    /* dynamic LDS of the largest launch */
   int64_t device_bytes; /* device memory held by the plan */
-  int32_t graph, reserved;
+  int32_t graph;
+  int32_t normal_tables; /* shared [dim][path] normal tables the plan fills per launch */
 } riga_plan_info;
 
 /* ---- device (gpu.rs:490-512) ---- */

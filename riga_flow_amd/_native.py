@@ -95,13 +95,13 @@ RECORD_DTYPE = np.dtype([("ko_step", "<i4"), ("hit_ki", "<i4"), ("final_worst", 
 
 class LaunchConfig(C.Structure):
     _fields_ = [("paths_per_thread", C.c_int32), ("use_graph", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("owen_table", C.c_int32), ("normal_table", C.c_int32)]
 
 
 class PlanInfo(C.Structure):
     _fields_ = [("n_requests", C.c_int32), ("n_items", C.c_int32), ("n_launches", C.c_int32),
                 ("blocks", C.c_int32), ("path_steps", C.c_double), ("lds_bytes", C.c_int64),
-                ("device_bytes", C.c_int64), ("graph", C.c_int32), ("reserved", C.c_int32)]
+                ("device_bytes", C.c_int64), ("graph", C.c_int32), ("normal_tables", C.c_int32)]
 
 
 EXPORTS = ("riga_hip_init", "riga_hip_destroy", "riga_hip_device_name", "riga_hip_arch",

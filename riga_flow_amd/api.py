@@ -271,7 +271,7 @@ def result_to_json(res: PriceResult) -> dict:
 # ------------------------------------------------------------------ pricing
 def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",
                 paths_per_thread: int = 1, max_dimension: int = 512,
-                raw: bool = False):
+                raw: bool = False, owen_table: bool = True, normal_table: int = 0):
     """Price many requests in one launch; CRN scenarios share their Sobol points."""
     gpu_backend()
     for q in requests:
@@ -280,7 +280,7 @@ def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",
     n = len(packed)
     arr = (N.Request * n)(*[p.r for p in packed])
     res = (N.Result * n)()
-    cfg = N.LaunchConfig(paths_per_thread)
+    cfg = N.LaunchConfig(paths_per_thread, 0, 0 if owen_table else -1, normal_table)
     err = N.Error()
     dev = N.Device.get()
     N.check(N.lib().riga_price_batch(dev.handle, arr, n, PRECISIONS[precision], C.byref(cfg), res,
@@ -294,7 +294,8 @@ class Plan:
     """A batch resident in device memory, launched any number of times."""
 
     def __init__(self, requests: Sequence[PriceRequest], precision: str = "fp64",
-                 paths_per_thread: int = 1, use_graph: bool = False, max_dimension: int = 512):
+                 paths_per_thread: int = 1, use_graph: bool = False, max_dimension: int = 512,
+                 owen_table: bool = True, normal_table: int = 0):
         gpu_backend()
         for q in requests:
             _check_as_of(q)
@@ -302,7 +303,8 @@ class Plan:
         packed = [_Packed(q, max_dimension) for q in requests]
         n = len(packed)
         arr = (N.Request * n)(*[p.r for p in packed])
-        cfg = N.LaunchConfig(paths_per_thread, 1 if use_graph else 0)
+        cfg = N.LaunchConfig(paths_per_thread, 1 if use_graph else 0, 0 if owen_table else -1,
+                             normal_table)
         err = N.Error()
         h = C.c_void_p()
         N.check(N.lib().riga_plan_create(N.Device.get().handle, arr, n, PRECISIONS[precision],

@@ -448,6 +448,14 @@ struct riga_plan {
   DevBuf arena, items_d, block_item, partials, finals, final_base_d, records_d;
   HostBuf finals_host;
   std::vector<PathRecordDev> recs_host;
+  struct NormalTable {
+    uint64_t seed = 0;
+    int dims = 0, nbits = 0;
+    int64_t paths = 0, owen_off = -1;
+    DevBuf buf;
+  };
+  std::vector<std::unique_ptr<NormalTable>> ztables;
+  int64_t off_dirs = 0;
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipGraph_t g = nullptr;
@@ -467,6 +475,17 @@ namespace {
 hipError_t enqueue(riga_plan* pl, bool timed) {
   hipError_t e;
   if (timed && (e = hipEventRecord(pl->e0, pl->stream)) != hipSuccess) return e;
+  for (const auto& zt : pl->ztables) {
+    const unsigned char* a = (const unsigned char*)pl->arena.p;
+    const uint32_t* dirs = (const uint32_t*)(a + pl->off_dirs);
+    const uint8_t* ot = zt->owen_off >= 0 ? (const uint8_t*)(a + zt->owen_off) : nullptr;
+    e = pl->precision == RIGA_FP64
+            ? launch_normal_table_f64(dirs, ot, zt->seed, zt->dims, (int)zt->paths, zt->nbits, (double*)zt->buf.p,
+                                      pl->stream)
+            : launch_normal_table_f32(dirs, ot, zt->seed, zt->dims, (int)zt->paths, zt->nbits, (float*)zt->buf.p,
+                                      pl->stream);
+    if (e != hipSuccess) return e;
+  }
   for (const LaunchPlan& lp : pl->launches) {
     PhoenixLaunch L;
     L.F = lp.F;
@@ -567,6 +586,21 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     if (!(it.d.flags & kHalton)) sobol_rows = std::max(sobol_rows, it.d.ladder_rows);
   const std::vector<uint32_t>& dirs = sobol_directions(sobol_rows);
   const int64_t off_dirs = arena.add(dirs.data(), (size_t)sobol_rows * 32);
+  // one top-byte Owen table per seed, covering the largest raw dimension used
+  const bool owen_table = !(cfg && cfg->owen_table < 0);
+  std::map<uint64_t, int> owen_dims;
+  std::map<uint64_t, int64_t> owen_off;
+  if (owen_table) {
+    for (Item& it : items)
+      if (!(it.d.flags & kHalton)) {
+        int& m = owen_dims[it.d.seed];
+        m = std::max(m, it.d.F * it.d.steps);
+      }
+    for (auto& kv : owen_dims) {
+      const std::vector<uint8_t>* tab = owen_top8_table(kv.first, kv.second);
+      owen_off[kv.first] = arena.add(tab->data(), tab->size());
+    }
+  }
   pl->final_base.resize(items.size());
   for (size_t i = 0; i < items.size(); ++i) {
     pl->final_base[i] = pl->final_slots;
@@ -610,6 +644,10 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       d.record_base = pl->record_slots;
       if (d.flags & kRecords) pl->record_slots += (int64_t)d.S * (int64_t)d.paths;
       d.off_dirs = off_dirs;
+      if (owen_table && !(d.flags & kHalton)) {
+        d.flags |= kOwenTable;
+        d.off_owen = owen_off[d.seed];
+      }
       d.off_nodes = arena.add(it.nodes.data(), it.nodes.size());
       d.off_steps = arena.add(it.steps.data(), it.steps.size());
       d.off_lvb = arena.add(it.lvb.data(), it.lvb.size());
@@ -636,10 +674,51 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     pl->blocks += blocks;
     pl->launches.push_back(lp);
   }
+  pl->off_dirs = off_dirs;
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  // shared normal tables: same-seed Sobol trades without wrapped direction rows
+  // read their normals from one [dim][path] table when they need many more
+  // normals than it holds (an uneven book), instead of hashing them per trade
+  const int zmode = cfg ? cfg->normal_table : 0;
+  if (zmode >= 0 && !records) {
+    std::map<uint64_t, std::vector<int>> by_seed;
+    for (size_t i = 0; i < items.size(); ++i) {
+      const ItemDesc& d = items[i].d;
+      if (!(d.flags & kHalton) && d.ladder_dims >= d.F * d.steps && d.path_begin == 0)
+        by_seed[d.seed].push_back((int)i);
+    }
+    const size_t real = precision == RIGA_FP64 ? 8 : 4;
+    for (auto& kv : by_seed) {
+      int dims = 0;
+      uint64_t paths = 0;
+      double demand = 0.0;
+      for (int i : kv.second) {
+        const ItemDesc& d = items[i].d;
+        dims = std::max(dims, d.F * d.steps);
+        paths = std::max<uint64_t>(paths, d.paths);
+        demand += (double)d.paths * d.F * d.steps;
+      }
+      const double size = (double)dims * (double)paths;
+      const double bytes = size * real;
+      if (bytes > 32e9 || (zmode == 0 && demand < 4.0 * size)) continue;
+      std::unique_ptr<riga_plan::NormalTable> zt(new riga_plan::NormalTable());
+      zt->seed = kv.first;
+      zt->dims = dims;
+      zt->paths = (int64_t)paths;
+      zt->nbits = 64 - __builtin_clzll(paths);
+      zt->owen_off = owen_table ? owen_off[kv.first] : -1;
+      HIP_TRY(zt->buf.ensure((size_t)bytes), "normal table alloc");
+      for (int i : kv.second) {
+        items[i].d.flags |= kNormalTable;
+        items[i].d.ztable = (uint64_t)(uintptr_t)zt->buf.p;
+        items[i].d.zt_stride = zt->paths;
+      }
+      pl->ztables.push_back(std::move(zt));
+    }
+  }
   std::vector<ItemDesc> descs(items.size());
   for (size_t i = 0; i < items.size(); ++i) descs[i] = items[i].d;
 
-  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
   HIP_TRY(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "stream");
   HIP_TRY(hipEventCreate(&pl->e0), "event");
   HIP_TRY(hipEventCreate(&pl->e1), "event");
@@ -782,6 +861,11 @@ int riga_hip_init(int device, riga_ctx** out, riga_error* err) {
   std::unique_ptr<riga_ctx> ctx(new riga_ctx());
   ctx->device = device;
   ctx->name = prop.name;
+  if (ctx->name.empty()) {
+    char buf[256] = {0};
+    if (hipDeviceGetName(buf, sizeof buf, device) == hipSuccess) ctx->name = buf;
+  }
+  if (ctx->name.empty()) ctx->name = "AMD Instinct (" + arch + ")";
   ctx->arch = arch;
   ctx->lds_limit = prop.sharedMemPerBlock;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -869,6 +953,8 @@ int riga_plan_query(const riga_plan* plan, riga_plan_info* info) {
   info->lds_bytes = (int64_t)plan->lds_max;
   info->device_bytes = (int64_t)(plan->arena.cap + plan->items_d.cap + plan->block_item.cap + plan->partials.cap +
                                  plan->finals.cap + plan->final_base_d.cap + plan->records_d.cap);
+  for (const auto& zt : plan->ztables) info->device_bytes += (int64_t)zt->buf.cap;
+  info->normal_tables = (int32_t)plan->ztables.size();
   info->graph = plan->graph ? 1 : 0;
   return 0;
 }

@@ -105,4 +105,12 @@ hipError_t launch_debug_sobol(const uint32_t* dirs, int ladder_dims, uint64_t se
   return hipGetLastError();
 }
 
+hipError_t launch_normal_table_f64(const uint32_t* dirs, const uint8_t* owen_tab, uint64_t seed,
+                                   int dims, int n_paths, int nbits, double* out, hipStream_t stream) {
+  dim3 grid((n_paths + 255) / 256, dims);
+  hipLaunchKernelGGL((normal_table_kernel<double>), grid, dim3(256), 0, stream, dirs, owen_tab, seed,
+                     n_paths, nbits, out);
+  return hipGetLastError();
+}
+
 }  // namespace riga

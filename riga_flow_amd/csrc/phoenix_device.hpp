@@ -56,6 +56,31 @@ __device__ __forceinline__ uint32_t owen_scramble(uint32_t x, uint64_t key) {
   return prefix;
 }
 
+// The same scramble with its first 8 levels read from a table: the top byte of
+// the output depends only on the top byte of x (each level hashes the already
+// scrambled prefix), so tab[x >> 24] = owen(x)[31:24] for the dimension, and the
+// remaining 24 levels continue from prefix = that byte, P = prefix * C3.
+__device__ __forceinline__ uint32_t owen_scramble_top8(uint32_t x, uint64_t key, uint32_t top) {
+  constexpr uint64_t C2 = 0xD6E8FEB86659FD93ull;
+  constexpr uint64_t C3 = 0xA24BAED4963EE407ull;
+  constexpr uint64_t C4 = 0xBF58476D1CE4E5B9ull;
+  constexpr uint64_t C5 = 0x94D049BB133111EBull;
+  uint32_t prefix = top;
+  uint64_t P = (uint64_t)top * C3;
+#pragma unroll 4
+  for (int bit = 23; bit >= 0; --bit) {
+    uint64_t z = key ^ ((uint64_t)bit * C2) ^ P;
+    z ^= z >> 30;
+    z *= C4;
+    z ^= z >> 27;
+    const uint32_t topbit = (uint32_t)((z * C5) >> 63);
+    const uint32_t s = ((x >> bit) & 1u) ^ topbit;
+    prefix = (prefix << 1) | s;
+    P = (P << 1) + (s ? C3 : 0ull);
+  }
+  return prefix;
+}
+
 // Unscrambled Sobol coordinate (qmc.rs:269-279): XOR of the direction numbers
 // selected by the gray code of n.  `row` is wave-uniform.
 __device__ __forceinline__ uint32_t sobol_x(uint32_t gray, const uint32_t* __restrict__ row,
@@ -209,6 +234,20 @@ template <typename Real, int F>
 __device__ __forceinline__ void node_normals(const ItemDesc& it, const unsigned char* arena,
                                              uint32_t n, uint32_t gray, int node, Real (&y)[F]) {
   Real z[F];
+  if (it.flags & kNormalTable) {
+    // the batch's shared table holds exactly these normals (same ladder, same point)
+    const Real* zt = reinterpret_cast<const Real*>(it.ztable);
+#pragma unroll
+    for (int f = 0; f < F; ++f) z[f] = zt[(size_t)((uint32_t)node * F + f) * (size_t)it.zt_stride + (n - 1)];
+#pragma unroll
+    for (int i = 0; i < F; ++i) {
+      Real acc = Real(0);
+#pragma unroll
+      for (int j = 0; j <= i; ++j) acc += (Real)it.chol[i * kMaxFactors + j] * z[j];
+      y[i] = acc;
+    }
+    return;
+  }
   const bool halton = (it.flags & kHalton) != 0;
 #pragma unroll
   for (int f = 0; f < F; ++f) {
@@ -219,7 +258,13 @@ __device__ __forceinline__ void node_normals(const ItemDesc& it, const unsigned 
       const uint32_t* row = at<uint32_t>(arena, it.off_dirs) + (size_t)d * 32;
       const uint32_t x = sobol_x(gray, row, it.nbits);
       const uint64_t key = it.seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
-      const uint32_t xs = owen_scramble(x, key);
+      uint32_t xs;
+      if (it.flags & kOwenTable) {
+        const uint8_t* tab = at<uint8_t>(arena, it.off_owen) + (size_t)dim * 256;
+        xs = owen_scramble_top8(x, key, tab[x >> 24]);
+      } else {
+        xs = owen_scramble(x, key);
+      }
       u = ((double)xs + 0.5) / 4294967296.0;
     } else {
       const uint32_t* h = at<uint32_t>(arena, it.off_halton);
@@ -274,6 +319,33 @@ __device__ __forceinline__ Real lv_sigma(const Real* __restrict__ ls, int nx, Re
   const Real v0 = (Real(1) - wx) * r0[lo] + wx * r0[hi];
   const Real v1 = (Real(1) - wx) * r1[lo] + wx * r1[hi];
   return (Real(1) - wt) * v0 + wt * v1;
+}
+
+// ---------------------------------------------------------------- shared normal table
+// Ladder normals z[dim][path] for every raw dimension a batch of same-seed
+// trades uses (dims without a wrapped direction row); blockIdx.y is the
+// dimension, so the direction row, the key and the Owen table row are uniform.
+template <typename Real>
+__global__ __launch_bounds__(256) void normal_table_kernel(const uint32_t* __restrict__ dirs,
+                                                           const uint8_t* __restrict__ owen_tab,
+                                                           uint64_t seed, int n_paths, int nbits,
+                                                           Real* __restrict__ out) {
+  const int path = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t dim = blockIdx.y;
+  if (path >= n_paths) return;
+  const uint32_t n = (uint32_t)path + 1;
+  const uint32_t gray = n ^ (n >> 1);
+  const uint32_t x = sobol_x(gray, dirs + (size_t)dim * 32, nbits);
+  const uint64_t key = seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
+  const uint32_t xs = owen_tab ? owen_scramble_top8(x, key, owen_tab[(size_t)dim * 256 + (x >> 24)])
+                               : owen_scramble(x, key);
+  const double u = ((double)xs + 0.5) / 4294967296.0;
+  Real z;
+  if constexpr (sizeof(Real) == 8)
+    z = probit_f64(u);
+  else
+    z = probit_f32(u);
+  out[(size_t)dim * n_paths + path] = z;
 }
 
 // ---------------------------------------------------------------- the kernel

@@ -24,6 +24,8 @@ enum ItemFlags : uint32_t {
   kTrace = 1u << 3,          // trace_level != Off
   kHalton = 1u << 4,         // VarianceReduction::Halton ladder (else Sobol)
   kRecords = 1u << 5,        // write one PathRecord per (scenario, path)
+  kOwenTable = 1u << 6,      // top 8 Owen levels read from a per-dimension table
+  kNormalTable = 1u << 7,    // ladder normals read from the batch's shared [dim][path] table
 };
 
 // Per observation date, uniform across all paths of an item.
@@ -98,6 +100,9 @@ struct ItemDesc {
   int32_t n_blocks;
   int32_t paths_per_block;
   int32_t ladder_rows;   // ladder rows uploaded (min(ladder_dims, F*steps))
+  int64_t off_owen;      // uint8: [F*steps][256] scrambled top byte per raw dimension
+  uint64_t ztable;       // device address of the shared normal table (Real[dims][zt_stride])
+  int64_t zt_stride;     // paths per table row
 };
 
 // Double-double accumulator (hi + lo).

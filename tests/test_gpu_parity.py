@@ -129,3 +129,45 @@ def test_reference_dimension_cap_wraps_like_the_reference():
     res = api.price_put_leg(req, "fp64", max_dimension=512)
     ora = O.price(req, max_dim=512)
     assert_fp64_parity(res, ora)
+
+
+def test_owen_table_and_full_hash_agree():
+    # the tabulated top byte and the 32-level hash give identical integers
+    req = _flat(2048, steps=252)
+    md = W.max_dimension_for(req)
+    a = api.price_batch([req], "fp64", max_dimension=md, owen_table=True)[0]
+    b = api.price_batch([req], "fp64", max_dimension=md, owen_table=False)[0]
+    assert a.pv == b.pv and a.legs.down_and_in_put == b.legs.down_and_in_put
+    assert a.stats[0].output_count == b.stats[0].output_count
+
+
+@pytest.mark.parametrize("ppt", [2, 4])
+def test_paths_per_thread_changes_nothing_but_the_shape(ppt):
+    req = W.lv_book(paths=3000, steps=126, years=0.5)
+    md = W.max_dimension_for(req)
+    a = api.price_batch([req], "fp64", max_dimension=md)[0]
+    b = api.price_batch([req], "fp64", max_dimension=md, paths_per_thread=ppt)[0]
+    assert b.stats[0].output_count == a.stats[0].output_count
+    assert rel(b.pv, a.pv) <= 1e-14
+
+
+def test_uneven_book_shared_normal_table_is_exact():
+    # trades of 1-5 underlyings and different maturities in one launch; the
+    # shared [dim][path] normal table gives the same integers and normals as
+    # per-trade generation, and every trade matches the oracle
+    book = W.uneven_book(n_trades=12, paths=1024, seed=3)
+    for q in book:  # keep the oracle fast: shorten the horizons
+        q.trade.steps = min(q.trade.steps, 160)
+        q.trade.maturity_years = q.trade.steps / 252.0
+        q.trade.coupon_periods = max(1, round(12 * q.trade.maturity_years))
+        q.config.variance_reduction = W.VarianceReduction("sobol", q.trade.steps * q.factors)
+    md = 5 * 160
+    on = api.price_batch(book, "fp64", max_dimension=md, normal_table=1)
+    off = api.price_batch(book, "fp64", max_dimension=md, normal_table=-1)
+    for a, b in zip(on, off):
+        assert a.pv == b.pv and a.stats[0].output_count == b.stats[0].output_count
+    for i in (0, 5, 11):
+        assert_fp64_parity(on[i], O.price(book[i], max_dim=md))
+    f32 = api.price_batch(book, "fp32", max_dimension=md, normal_table=1)
+    for a, b in zip(f32, on):
+        assert rel(a.pv, b.pv) <= FP32_REL
