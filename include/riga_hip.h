@@ -129,6 +129,8 @@ typedef struct riga_launch_config {
   int32_t owen_table;       /* 0: tabulate the top 8 Owen levels per dimension; -1: hash all 32 */
   int32_t normal_table;     /* 0: auto (share one [dim][path] normal table among same-seed trades
                                when they need 4x more normals than it holds); 1: force; -1: off */
+  int32_t stream_ordered_alloc; /* 1: plan memory from the stream-ordered pool (serving) */
+  int32_t reserved[3];
 } riga_launch_config;
 
 /* A batch laid out in device memory once (surfaces, factors, schedules, Sobol
@@ -165,11 +167,61 @@ int riga_price_batch(riga_ctx *ctx, const riga_request *reqs, int n, int precisi
 /* ---- resident plans: create once, launch many times (no device allocation per launch) ---- */
 int riga_plan_create(riga_ctx *ctx, const riga_request *reqs, int n, int precision,
                      const riga_launch_config *cfg, riga_plan **out, riga_error *err);
+/* Device bytes a plan for these requests would hold, without allocating. */
+int riga_plan_estimate(riga_ctx *ctx, const riga_request *reqs, int n, int precision,
+                       const riga_launch_config *cfg, int64_t *bytes, riga_error *err);
 int riga_plan_launch(riga_plan *plan, riga_error *err);  /* asynchronous */
+int riga_plan_ready(riga_plan *plan);  /* 1: finished, 0: running, < 0: device error */
 int riga_plan_wait(riga_plan *plan, riga_result *results, riga_error *err);
 int riga_plan_run(riga_plan *plan, riga_result *results, riga_error *err); /* launch + wait */
 int riga_plan_query(const riga_plan *plan, riga_plan_info *info);
 void riga_plan_destroy(riga_plan *plan);
+
+/* ---- serving: continuous batching on one device ----
+ * Replaces the reference's per-call blocking pricing in its MCP tool handlers
+ * (riga-mcp/src/main.rs:1821-1831, 2005-2035; tasks 1005-1042) with one
+ * device-owning scheduler: tickets are admitted in arrival order while the
+ * device-memory budget and in-flight limit allow, batched into one plan, run on
+ * their own stream (overlapping earlier plans), and shed after the queue limit. */
+typedef struct riga_server riga_server;
+
+typedef struct riga_server_config {
+  int64_t hbm_budget_bytes;  /* device memory the in-flight plans may hold (0: 64 GiB) */
+  int32_t max_inflight;      /* plans running at once (0: 2) */
+  int32_t max_batch_requests;/* requests per plan (0: 4096) */
+  double max_queue_seconds;  /* shed tickets queued longer (0: never) */
+  int32_t paths_per_thread;  /* launch shape for every plan (0: 1) */
+  int32_t normal_table;      /* as riga_launch_config.normal_table */
+} riga_server_config;
+
+typedef struct riga_ticket_info {
+  int32_t state;             /* 2 done, 3 shed, 4 failed */
+  int32_t n_requests;
+  int32_t batch_tickets;     /* tickets that shared its plan */
+  int32_t reserved;
+  double submit_s, start_s, end_s, latency_s; /* seconds since the server started */
+  int64_t device_bytes;
+  double path_steps;
+} riga_ticket_info;
+
+typedef struct riga_server_stats {
+  int64_t submitted_tickets, completed_tickets, completed_requests, shed_tickets, failed_tickets;
+  int64_t batches, peak_bytes, bytes_in_use;
+  double path_steps;
+  int32_t queued_tickets, running_plans, max_concurrent, reserved;
+} riga_server_stats;
+
+int riga_server_create(riga_ctx *ctx, const riga_server_config *cfg, riga_server **out,
+                       riga_error *err);
+/* Copies the requests; returns at once with a ticket id. */
+int riga_server_submit(riga_server *server, const riga_request *reqs, int n, int precision,
+                       uint64_t *ticket, riga_error *err);
+/* Blocks up to timeout_s (< 0: forever).  0: done (results[n] filled); 1: still pending;
+ * 2: shed; < 0: failed (err filled).  A finished ticket is released by this call. */
+int riga_server_wait(riga_server *server, uint64_t ticket, double timeout_s, riga_result *results,
+                     riga_ticket_info *info, riga_error *err);
+int riga_server_stats_get(riga_server *server, riga_server_stats *out);
+void riga_server_destroy(riga_server *server);
 
 /* ---- replay (explain.rs:90-316): per-path records for paths [0, req->paths) ---- */
 int riga_replay_paths(riga_ctx *ctx, const riga_request *req, int precision,

@@ -95,7 +95,8 @@ RECORD_DTYPE = np.dtype([("ko_step", "<i4"), ("hit_ki", "<i4"), ("final_worst", 
 
 class LaunchConfig(C.Structure):
     _fields_ = [("paths_per_thread", C.c_int32), ("use_graph", C.c_int32),
-                ("owen_table", C.c_int32), ("normal_table", C.c_int32)]
+                ("owen_table", C.c_int32), ("normal_table", C.c_int32),
+                ("stream_ordered_alloc", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class PlanInfo(C.Structure):

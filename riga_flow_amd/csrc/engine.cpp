@@ -44,22 +44,29 @@ constexpr int RIGA_EINVAL = -1;
 constexpr int RIGA_EGPU = -2;
 constexpr int RIGA_ENODEV = -3;
 
+// Device buffer.  With a stream set, memory comes from the device's
+// stream-ordered pool (hipMallocAsync / hipFreeAsync): no device-wide
+// synchronisation, so a serving loop can create and retire plans while other
+// plans run.
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  hipStream_t stream = nullptr;
   hipError_t ensure(size_t bytes) {
     if (bytes <= cap) return hipSuccess;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
+    release();
     size_t want = std::max(bytes, (size_t)4096);
-    hipError_t e = hipMalloc(&p, want);
+    hipError_t e = stream ? hipMallocAsync(&p, want, stream) : hipMalloc(&p, want);
     if (e == hipSuccess) cap = want;
+    else p = nullptr;
     return e;
   }
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
+  void release() {
+    if (p) (void)(stream ? hipFreeAsync(p, stream) : hipFree(p));
+    p = nullptr;
+    cap = 0;
   }
+  ~DevBuf() { release(); }
 };
 
 struct HostBuf {
@@ -465,7 +472,13 @@ struct riga_plan {
     if (g) (void)hipGraphDestroy(g);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
+    // buffers go back to the pool on this stream before it is destroyed
+    for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d}) b->release();
+    for (auto& zt : ztables) zt->buf.release();
     if (stream) (void)hipStreamDestroy(stream);
+  }
+  void bind_stream() {
+    for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d}) b->stream = stream;
   }
 };
 
@@ -508,8 +521,10 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
                         hipMemcpyDeviceToHost, pl->stream);
 }
 
+// With `estimate` set, only the device bytes the plan would hold are computed
+// (no stream, no allocation): the serving loop admits requests with it.
 int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, const riga_launch_config* cfg,
-                bool records, riga_plan** out, riga_error* err) {
+                bool records, riga_plan** out, riga_error* err, int64_t* estimate = nullptr) {
   clear_err(err);
   *out = nullptr;
   if (!ctx) {
@@ -676,6 +691,10 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   }
   pl->off_dirs = off_dirs;
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  if (!estimate) HIP_TRY(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "stream");
+  const bool stream_ordered = cfg && cfg->stream_ordered_alloc != 0;
+  if (stream_ordered) pl->bind_stream();
+  double ztable_bytes = 0.0;
   // shared normal tables: same-seed Sobol trades without wrapped direction rows
   // read their normals from one [dim][path] table when they need many more
   // normals than it holds (an uneven book), instead of hashing them per trade
@@ -707,7 +726,9 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       zt->paths = (int64_t)paths;
       zt->nbits = 64 - __builtin_clzll(paths);
       zt->owen_off = owen_table ? owen_off[kv.first] : -1;
-      HIP_TRY(zt->buf.ensure((size_t)bytes), "normal table alloc");
+      if (stream_ordered) zt->buf.stream = pl->stream;
+      ztable_bytes += bytes;
+      if (!estimate) HIP_TRY(zt->buf.ensure((size_t)bytes), "normal table alloc");
       for (int i : kv.second) {
         items[i].d.flags |= kNormalTable;
         items[i].d.ztable = (uint64_t)(uintptr_t)zt->buf.p;
@@ -718,8 +739,15 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   }
   std::vector<ItemDesc> descs(items.size());
   for (size_t i = 0; i < items.size(); ++i) descs[i] = items[i].d;
+  if (estimate) {
+    *estimate = (int64_t)(ztable_bytes + (double)arena.bytes.size() + descs.size() * sizeof(ItemDesc) +
+                          all_blocks.size() * 4.0 + (double)pl->partial_slots * sizeof(Partial) +
+                          (double)pl->final_slots * sizeof(Partial) + pl->final_base.size() * 8.0 +
+                          (double)pl->record_slots * sizeof(PathRecordDev) + 8 * 4096.0);
+    for (auto& zt : pl->ztables) zt->buf.stream = nullptr;
+    return 0;
+  }
 
-  HIP_TRY(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "stream");
   HIP_TRY(hipEventCreate(&pl->e0), "event");
   HIP_TRY(hipEventCreate(&pl->e1), "event");
   HIP_TRY(pl->arena.ensure(arena.bytes.size()), "arena alloc");
@@ -927,7 +955,23 @@ int riga_plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precisi
   return plan_create(ctx, reqs, n, precision, cfg, false, out, err);
 }
 
+int riga_plan_estimate(riga_ctx* ctx, const riga_request* reqs, int n, int precision,
+                       const riga_launch_config* cfg, int64_t* bytes, riga_error* err) {
+  if (!bytes) {
+    set_err(err, "invalid_argument", "bytes is NULL");
+    return RIGA_EINVAL;
+  }
+  riga_plan* unused = nullptr;
+  return plan_create(ctx, reqs, n, precision, cfg, false, &unused, err, bytes);
+}
+
 int riga_plan_launch(riga_plan* plan, riga_error* err) { return plan_launch(plan, err); }
+
+int riga_plan_ready(riga_plan* plan) {
+  if (!plan) return RIGA_EINVAL;
+  const hipError_t e = hipStreamQuery(plan->stream);
+  return e == hipSuccess ? 1 : (e == hipErrorNotReady ? 0 : RIGA_EGPU);
+}
 
 int riga_plan_wait(riga_plan* plan, riga_result* results, riga_error* err) {
   if (!plan) {
