@@ -105,10 +105,34 @@ class PlanInfo(C.Structure):
                 ("device_bytes", C.c_int64), ("graph", C.c_int32), ("normal_tables", C.c_int32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_inflight", C.c_int32),
+                ("max_batch_requests", C.c_int32), ("max_queue_seconds", C.c_double),
+                ("paths_per_thread", C.c_int32), ("normal_table", C.c_int32)]
+
+
+class TicketInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("n_requests", C.c_int32), ("batch_tickets", C.c_int32),
+                ("reserved", C.c_int32), ("submit_s", C.c_double), ("start_s", C.c_double),
+                ("end_s", C.c_double), ("latency_s", C.c_double), ("device_bytes", C.c_int64),
+                ("path_steps", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted_tickets", C.c_int64), ("completed_tickets", C.c_int64),
+                ("completed_requests", C.c_int64), ("shed_tickets", C.c_int64),
+                ("failed_tickets", C.c_int64), ("batches", C.c_int64), ("peak_bytes", C.c_int64),
+                ("bytes_in_use", C.c_int64), ("path_steps", C.c_double),
+                ("queued_tickets", C.c_int32), ("running_plans", C.c_int32),
+                ("max_concurrent", C.c_int32), ("reserved", C.c_int32)]
+
+
 EXPORTS = ("riga_hip_init", "riga_hip_destroy", "riga_hip_device_name", "riga_hip_arch",
            "riga_validate", "riga_price_put_leg", "riga_price_batch", "riga_replay_paths",
-           "riga_plan_create", "riga_plan_launch", "riga_plan_wait", "riga_plan_run",
-           "riga_plan_query", "riga_plan_destroy",
+           "riga_plan_create", "riga_plan_estimate", "riga_plan_launch", "riga_plan_ready",
+           "riga_plan_wait", "riga_plan_run", "riga_plan_query", "riga_plan_destroy",
+           "riga_server_create", "riga_server_submit", "riga_server_wait",
+           "riga_server_stats_get", "riga_server_destroy",
            "riga_lv_calibrate", "riga_lv_sigma", "riga_debug_sobol")
 
 _lib = None
@@ -146,6 +170,19 @@ def lib():
             L.riga_plan_create.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int, C.c_int,
                                            C.POINTER(LaunchConfig), C.POINTER(C.c_void_p),
                                            C.POINTER(Error)]
+            L.riga_plan_estimate.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int, C.c_int,
+                                             C.POINTER(LaunchConfig), C.POINTER(C.c_int64),
+                                             C.POINTER(Error)]
+            L.riga_plan_ready.argtypes = [C.c_void_p]
+            L.riga_server_create.argtypes = [C.c_void_p, C.POINTER(ServerConfig),
+                                             C.POINTER(C.c_void_p), C.POINTER(Error)]
+            L.riga_server_submit.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int, C.c_int,
+                                             C.POINTER(C.c_uint64), C.POINTER(Error)]
+            L.riga_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(Result),
+                                           C.POINTER(TicketInfo), C.POINTER(Error)]
+            L.riga_server_stats_get.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
+            L.riga_server_destroy.argtypes = [C.c_void_p]
+            L.riga_server_destroy.restype = None
             L.riga_plan_launch.argtypes = [C.c_void_p, C.POINTER(Error)]
             L.riga_plan_wait.argtypes = [C.c_void_p, C.POINTER(Result), C.POINTER(Error)]
             L.riga_plan_run.argtypes = [C.c_void_p, C.POINTER(Result), C.POINTER(Error)]
