@@ -320,7 +320,12 @@ bool build_item(const std::vector<Prep>& preps, Item* it, int precision, bool re
   d.cb = r.has_coupon_barrier ? r.coupon_barrier : INFINITY;
   d.coupon_rate = r.has_coupon_rate ? r.coupon_rate : 0.0;
   d.put_df = exp(-r.risk_free_rate * r.maturity_years);
-  for (int f = 0; f < F; ++f) d.initial_spot[f] = r.initial_spots[f];
+  for (int f = 0; f < F; ++f) {
+    d.initial_spot[f] = r.initial_spots[f];
+    d.ln_s0[f] = log(r.initial_spots[f]);
+  }
+  d.ln_ko = log(d.ko);  // +inf when absent
+  d.ln_cb = log(d.cb);
   for (int i = 0; i < F; ++i)
     for (int j = 0; j < F; ++j) d.chol[i * kMaxFactors + j] = p0.chol[i * F + j];
   // scenarios
@@ -333,6 +338,7 @@ bool build_item(const std::vector<Prep>& preps, Item* it, int precision, bool re
     for (int f = 0; f < F; ++f) {
       const double vol = q.vols[f];
       sc.spot[f] = q.spots[f];
+      sc.ln_spot[f] = log(q.spots[f]);
       sc.rq[f] = q.risk_free_rate - q.dividend_yields[f];
       sc.drift[f] = (q.risk_free_rate - q.dividend_yields[f] - 0.5 * vol * vol) * dt;  // note.rs:139-140
       sc.vsd[f] = vol * sqrt(dt);

@@ -37,39 +37,64 @@ hipError_t launch_phoenix_f64(const PhoenixLaunch& l, hipStream_t stream) {
 __global__ void finalize_kernel(const ItemDesc* __restrict__ items, int n_items,
                                 const Partial* __restrict__ partials, Partial* __restrict__ finals,
                                 const int64_t* __restrict__ final_base) {
+  // one workgroup per (item, scenario): 256 lanes each sum a fixed strided
+  // subset of the block partials, then a fixed-shape tree in LDS; the order
+  // never depends on timing, so the merge is deterministic.
+  __shared__ DD sq[kNumQ][256];
+  __shared__ uint64_t sc[kNumC][256];
   const int item = blockIdx.x;
+  const int s = blockIdx.y;
   if (item >= n_items) return;
   const ItemDesc& it = items[item];
-  for (int s = 0; s < it.S; ++s) {
-    const Partial* p = partials + it.partial_base + (int64_t)s * it.n_blocks;
-    Partial* out = finals + final_base[item] + s;
-    const int t = threadIdx.x;
-    if (t < kNumQ) {
-      DD a{0.0, 0.0};
-      for (int b = 0; b < it.n_blocks; ++b) a = dd_add(a, p[b].q[t]);
-      out->q[t] = a;
-    } else if (t >= 32 && t < 32 + kNumC) {
-      uint64_t a = 0;
-      for (int b = 0; b < it.n_blocks; ++b) a += p[b].c[t - 32];
-      out->c[t - 32] = a;
-    } else if (t == 64) {
-      int ne = 0;
-      for (int b = 0; b < it.n_blocks && ne < kTraceCap; ++b) {
-        for (int e = 0; e < p[b].n_events && ne < kTraceCap; ++e) {
-          out->ev_path_lo[ne] = p[b].ev_path_lo[e];
-          out->ev_step[ne] = p[b].ev_step[e];
-          ++ne;
-        }
-      }
-      out->n_events = ne;
+  if (s >= it.S) return;
+  const Partial* p = partials + it.partial_base + (int64_t)s * it.n_blocks;
+  Partial* out = finals + final_base[item] + s;
+  const int t = threadIdx.x;
+  DD a[kNumQ];
+  uint64_t c[kNumC];
+#pragma unroll
+  for (int k = 0; k < kNumQ; ++k) a[k] = DD{0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < kNumC; ++k) c[k] = 0;
+  for (int b = t; b < it.n_blocks; b += 256) {
+#pragma unroll
+    for (int k = 0; k < kNumQ; ++k) a[k] = dd_add(a[k], p[b].q[k]);
+#pragma unroll
+    for (int k = 0; k < kNumC; ++k) c[k] += p[b].c[k];
+  }
+#pragma unroll
+  for (int k = 0; k < kNumQ; ++k) sq[k][t] = a[k];
+#pragma unroll
+  for (int k = 0; k < kNumC; ++k) sc[k][t] = c[k];
+  __syncthreads();
+  for (int half = 128; half >= 1; half >>= 1) {
+    if (t < half) {
+#pragma unroll
+      for (int k = 0; k < kNumQ; ++k) sq[k][t] = dd_add(sq[k][t], sq[k][t + half]);
+#pragma unroll
+      for (int k = 0; k < kNumC; ++k) sc[k][t] += sc[k][t + half];
     }
+    __syncthreads();
+  }
+  if (t < kNumQ) out->q[t] = sq[t][0];
+  if (t >= 32 && t < 32 + kNumC) out->c[t - 32] = sc[t - 32][0];
+  if (t == 64) {  // first 64 KO events in path order: blocks in order
+    int ne = 0;
+    for (int b = 0; b < it.n_blocks && ne < kTraceCap; ++b) {
+      for (int e = 0; e < p[b].n_events && ne < kTraceCap; ++e) {
+        out->ev_path_lo[ne] = p[b].ev_path_lo[e];
+        out->ev_step[ne] = p[b].ev_step[e];
+        ++ne;
+      }
+    }
+    out->n_events = ne;
   }
 }
 
 
 hipError_t launch_finalize(const ItemDesc* items, int n_items, const Partial* partials,
                            Partial* finals, const int64_t* final_base, hipStream_t stream) {
-  hipLaunchKernelGGL(finalize_kernel, dim3(n_items), dim3(128), 0, stream, items, n_items,
+  hipLaunchKernelGGL(finalize_kernel, dim3(n_items, 10), dim3(256), 0, stream, items, n_items,
                      partials, finals, final_base);
   return hipGetLastError();
 }

@@ -296,12 +296,18 @@ __device__ __forceinline__ float tier_div(float a, float b) { return __fdividef(
 
 // Local-vol lookup sigma(t, S) (riga-market/src/lib.rs:305-343) against a
 // surface whose knots and two bracketing tenor rows sit in LDS.
-template <typename Real>
+// `arg` is the spot S (FP64 tier: x = ln max(S, 1e-12), as the reference) or,
+// in the FP32 log-state tier, already x = ln S (clamped at ln 1e-12).
+template <typename Real, bool kLogArg = false>
 __device__ __forceinline__ Real lv_sigma(const Real* __restrict__ ls, int nx, Real inv_dx, Real wt,
-                                         Real s) {
+                                         Real arg) {
   const Real* r0 = ls + nx;
   const Real* r1 = ls + 2 * nx;
-  const Real x = tier_log(s > Real(1e-12) ? s : Real(1e-12));
+  Real x;
+  if constexpr (kLogArg)
+    x = arg > Real(-27.631021115928547) ? arg : Real(-27.631021115928547);
+  else
+    x = tier_log(arg > Real(1e-12) ? arg : Real(1e-12));
   int lo, hi;
   if (x <= ls[0]) {
     lo = hi = 0;
@@ -369,6 +375,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   const bool local_vol = (it.flags & kLocalVol) != 0;
   const bool trace = (it.flags & kTrace) != 0;
   const bool want_records = (it.flags & kRecords) != 0 && records != nullptr;
+  constexpr bool kLogState = sizeof(Real) == 4;  // FP32 tier carries X = ln S
   // dynamic LDS: [bridge stack: (depth-1) x F x kBlock][local-vol rows: n_surf x 3 x max_nx]
   Real* lds_stack = reinterpret_cast<Real*>(lds_dyn);
   Real* lds_w = lds_stack + (size_t)it.stack_levels * F * kBlock;
@@ -418,11 +425,12 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
 #pragma unroll
     for (int s = 0; s < SMAX; ++s) {
 #pragma unroll
-      for (int f = 0; f < F; ++f) spot[s][f] = (Real)scen[s < S ? s : 0].spot[f];
+      for (int f = 0; f < F; ++f)
+        spot[s][f] = kLogState ? (Real)scen[s < S ? s : 0].ln_spot[f] : (Real)scen[s < S ? s : 0].spot[f];
       ko_step[s] = -1;
       accrual[s] = unpaid[s] = cpv[s] = mpv[s] = Real(0);
       received[s] = false;
-      worst_last[s] = Real(1);
+      worst_last[s] = kLogState ? Real(0) : Real(1);  // final_worst starts at 1.0 (note.rs:115)
     }
 
     int node_ptr = 0;
@@ -494,36 +502,54 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
       for (int s = 0; s < SMAX; ++s) {
         if (s >= S) continue;
         const ScenarioDesc& sc = scen[s];
-        // ---- Euler step (note.rs:134-142)
+        // ---- Euler step (note.rs:134-142).  FP64: S *= exp(incr), as the
+        // reference rounds it.  FP32: the state is X = ln S and X += incr, the
+        // same step without an exp, a log and two divides per factor.
 #pragma unroll
         for (int f = 0; f < F; ++f) {
           Real incr;
           if (local_vol) {
             const int u = sc.lv[f];
             const LvSurface& su = surf[u];
-            const Real vol = lv_sigma<Real>(lds + (size_t)u * 3 * nxs, su.n_x, (Real)su.inv_dx,
-                                            (Real)lvb[step * F + f].wt, spot[s][f]);
+            const Real vol = lv_sigma<Real, kLogState>(lds + (size_t)u * 3 * nxs, su.n_x, (Real)su.inv_dx,
+                                                       (Real)lvb[step * F + f].wt, spot[s][f]);
             const Real drift = ((Real)sc.rq[f] - Real(0.5) * vol * vol) * (Real)it.dt;
             incr = drift + vol * (Real)it.sqrt_dt * shock[f];
           } else {
             incr = (Real)sc.drift[f] + (Real)sc.vsd[f] * shock[f];
           }
-          spot[s][f] *= tier_exp(incr);
+          if constexpr (kLogState)
+            spot[s][f] += incr;
+          else
+            spot[s][f] *= tier_exp(incr);
         }
-        // ---- worst-of (note.rs:143-145)
-        Real worst = spot[s][0] / (Real)it.initial_spot[0];
+        // ---- worst-of (note.rs:143-145); the FP32 tier keeps ln(worst)
+        Real worst;
+        if constexpr (kLogState) {
+          worst = spot[s][0] - (Real)it.ln_s0[0];
 #pragma unroll
-        for (int f = 1; f < F; ++f) worst = fmin(worst, spot[s][f] / (Real)it.initial_spot[f]);
+          for (int f = 1; f < F; ++f) worst = fminf(worst, spot[s][f] - (Real)it.ln_s0[f]);
+        } else {
+          worst = spot[s][0] / (Real)it.initial_spot[0];
+#pragma unroll
+          for (int f = 1; f < F; ++f) worst = fmin(worst, spot[s][f] / (Real)it.initial_spot[f]);
+        }
         worst_last[s] = worst;
         if (ko_step[s] >= 0) continue;  // note.rs:146-148: cash flows stop after KO
         // ---- coupon accrual (note.rs:151-163)
         if (new_period) accrual[s] = Real(0);
-        if (!fixed_now && (double)worst >= it.cb) accrual[s] += (Real)si.accrual_inc;
-        // ---- daily global KO (note.rs:166-195)
+        const bool above_coupon = kLogState ? worst >= (Real)it.ln_cb : (double)worst >= it.cb;
+        if (!fixed_now && above_coupon) accrual[s] += (Real)si.accrual_inc;
+        // ---- daily global KO (note.rs:166-195): every name at or above the
+        // barrier, i.e. the worst-of at or above it
         bool all_above = call_active;
+        if constexpr (kLogState) {
+          all_above = all_above && worst >= (Real)it.ln_ko;
+        } else {
 #pragma unroll
-        for (int f = 0; f < F; ++f)
-          all_above = all_above && ((double)(spot[s][f] / (Real)it.initial_spot[f]) >= it.ko);
+          for (int f = 0; f < F; ++f)
+            all_above = all_above && ((double)(spot[s][f] / (Real)it.initial_spot[f]) >= it.ko);
+        }
         if (all_above) {
           ko_step[s] = step;
           const Real due = fixed_now ? (Real)it.coupon_rate : accrual[s];
@@ -556,7 +582,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
       if (s >= S) continue;
       const bool ko = ko_step[s] >= 0;
       const double funding = ko ? stepinfo[ko_step[s]].df : it.put_df;
-      const double fw = (double)worst_last[s];
+      const double fw = kLogState ? exp((double)worst_last[s]) : (double)worst_last[s];
       const bool hit_ki = fw <= it.ki;
       const bool eligible = hit_ki && !ko;
       const double put = eligible ? fmax(it.strike - fw, 0.0) * it.put_df : 0.0;

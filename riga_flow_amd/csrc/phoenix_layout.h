@@ -70,6 +70,7 @@ struct ScenarioDesc {
   double rq[kMaxFactors];     // rate - dividend_yield[f]
   double drift[kMaxFactors];  // flat vol: (rate - q - 0.5 vol^2) dt
   double vsd[kMaxFactors];    // flat vol: vol * sqrt(dt)
+  double ln_spot[kMaxFactors];// FP32 log-state tier: ln(spot)
   int32_t lv[kMaxFactors];    // surface index per factor (local vol)
 };
 
@@ -81,6 +82,8 @@ struct ItemDesc {
   double dt, sqrt_dt, inv_sqrt_dt, root_sd;
   double strike, ki, ko, cb, coupon_rate, put_df;
   double initial_spot[kMaxFactors];
+  double ln_s0[kMaxFactors];  // FP32 log-state tier: ln(initial_spot), ln(KO), ln(coupon barrier)
+  double ln_ko, ln_cb;
   double chol[kMaxFactors * kMaxFactors];  // lower triangular, row-major F x F
   int64_t off_nodes;     // BridgeNode[steps - 1] in streaming order
   int64_t off_steps;     // StepInfo[steps]

@@ -1,0 +1,70 @@
+"""The continuous-batching server on the device: each ticket's result is the
+same alone and in the mix (bit for bit, both tiers), plans overlap, stale
+tickets are shed and the memory budget is enforced."""
+import time
+
+import pytest
+
+from parity import rel
+from riga_flow_amd import api, serving, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+MD = 7000
+
+
+def small_tickets():
+    names, crn = W.crn_scenarios(paths=2048, steps=126, years=0.5)
+    book = W.uneven_book(n_trades=4, paths=1024, seed=9)
+    for q in book:
+        q.trade.steps = min(q.trade.steps, 200)
+        q.trade.maturity_years = q.trade.steps / 252.0
+        q.trade.coupon_periods = max(1, round(12 * q.trade.maturity_years))
+        q.config.variance_reduction = W.VarianceReduction("sobol", q.trade.steps * q.factors)
+    flat = [W.flat_book(paths=4096, steps=252)]
+    return [(crn, "fp64"), (book, "fp64"), (flat, "fp32"), (crn[:3], "fp32")]
+
+
+def test_results_alone_equal_results_in_the_mix():
+    tickets = small_tickets()
+    srv = serving.Server(max_inflight=2, max_batch_requests=64)
+    ids = [srv.submit(reqs, tier) for reqs, tier in tickets]
+    got = [srv.wait(t) for t in ids]
+    stats = srv.stats()
+    srv.close()
+    assert all(g.state == serving.DONE for g in got)
+    assert stats["completed_tickets"] == len(tickets)
+    for (reqs, tier), g in zip(tickets, got):
+        alone = api.price_batch(reqs, tier, max_dimension=MD)
+        assert len(g.results) == len(alone)
+        for a, b in zip(g.results, alone):
+            assert a.pv == b.pv and a.legs.down_and_in_put == b.legs.down_and_in_put
+            assert a.stats[0].output_count == b.stats[0].output_count
+
+
+def test_plans_overlap_on_the_device():
+    big = [W.lv_book(paths=262_144, steps=756)]
+    srv = serving.Server(max_inflight=3, max_batch_requests=1)
+    ids = [srv.submit(big, "fp64") for _ in range(3)]
+    done = [srv.wait(t) for t in ids]
+    stats = srv.stats()
+    srv.close()
+    assert all(d.state == serving.DONE for d in done)
+    assert stats["max_concurrent"] >= 2
+    assert rel(done[0].results[0].pv, done[2].results[0].pv) == 0.0
+
+
+def test_stale_tickets_are_shed_and_budget_is_enforced():
+    big = [W.lv_book(paths=262_144, steps=756)]
+    srv = serving.Server(max_inflight=1, max_batch_requests=1, max_queue_seconds=0.002)
+    first = srv.submit(big, "fp64")
+    time.sleep(0.05)
+    later = [srv.submit(big, "fp64") for _ in range(3)]
+    res = [srv.wait(t) for t in [first] + later]
+    srv.close()
+    assert res[0].state == serving.DONE
+    assert any(r.state == serving.SHED for r in res[1:])
+    tiny = serving.Server(hbm_budget_gib=1e-4)
+    r = tiny.wait(tiny.submit(big, "fp64"))
+    tiny.close()
+    assert r.state == serving.FAILED and r.error.code == "memory_budget"
