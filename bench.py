@@ -87,6 +87,10 @@ def cpu_baseline(seconds_target: float = 12.0):
     }
 
 
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 # ---------------------------------------------------------------- GPU legs
 def timed(plan, steps: int, warmup: int):
     for _ in range(warmup):
@@ -107,6 +111,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-book", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--quick", action="store_true", help="smaller path counts (smoke)")
     ap.add_argument("--ppt", type=int, default=0, help="paths per thread (0 = tuned default)")
     args = ap.parse_args()
@@ -127,7 +133,9 @@ def main():
     fp64 = api.Plan([plain], "fp64", paths_per_thread=ppt, use_graph=True, max_dimension=md)
 
     wall32, dev32, res32 = timed(fp32, args.steps, args.warmup)
+    log(f"fp32 batched: {wall32 / args.steps * 1e3:.1f} ms/step")
     wall64, dev64, res64 = timed(fp64, args.steps, args.warmup)
+    log(f"fp64 plain: {wall64 / args.steps * 1e3:.1f} ms/step")
     ps32 = fp32.path_steps
     ps64 = fp64.path_steps
     value = ps32 * args.steps / wall32
@@ -150,8 +158,11 @@ def main():
         return out
 
     g32, g64 = greeks(got), greeks(ref)
-    gerr = {k: max(abs(a[i] - b[i]) for a, b in zip(g32, g64)) for i, k in
-            enumerate(("delta_abs", "gamma_abs", "vega_abs"))}
+    gerr = {}
+    for i, k in enumerate(("delta", "gamma", "vega")):
+        gerr[k + "_abs_err"] = max(abs(a[i] - b[i]) for a, b in zip(g32, g64))
+        gerr[k + "_rel_err"] = max(abs(a[i] - b[i]) / max(abs(b[i]), 1e-300) for a, b in zip(g32, g64))
+        gerr[k + "_fp64"] = [b[i] for b in g64]
 
     nbits = (big).bit_length()
     ops = ops_per_path_step(3, 10, nbits, True) / 10.0  # per (scenario) path-step
@@ -189,9 +200,68 @@ def main():
                      "kernel": "phoenix_kernel<float,3,10>", "device_ms": dev_s * 1e3,
                      "ops_per_path_step": ops},
     }
+    fp32.close()
+    fp64.close()
+    log("fp32 error check done")
+    if not args.no_book:
+        out["uneven_book"] = book_leg(args.quick)
+    if not args.no_serving:
+        out["serving"] = serving_leg(args.quick)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+        log("cpu baseline done")
     print(json.dumps(out), flush=True)
+
+
+def book_leg(quick: bool) -> dict:
+    """2,048 uneven Phoenix trades (1-5 of 20 assets, 0.5-5y daily, 65,536 paths
+    each) in one hipGraph: a shared normal table, then one launch per factor count."""
+    n = 2048 if not quick else 128
+    book = W.uneven_book(n_trades=n, paths=65_536)
+    out = {"trades": n, "paths": 65_536}
+    for tier in ("fp32", "fp64"):
+        t0 = time.perf_counter()
+        plan = api.Plan(book, tier, use_graph=True, max_dimension=7000)
+        setup = time.perf_counter() - t0
+        plan.run(raw=True)
+        t0 = time.perf_counter()
+        reps = 2
+        for _ in range(reps):
+            res = plan.run(raw=True)
+        wall = (time.perf_counter() - t0) / reps
+        out[tier] = {"path_steps": plan.path_steps, "ms_per_book": wall * 1e3,
+                     "path_steps_per_s": plan.path_steps / wall,
+                     "normal_tables": plan.info.normal_tables, "launches": plan.info.n_launches,
+                     "plan_setup_s": setup, "device_gb": plan.info.device_bytes / 1e9}
+        log(f"book {tier}: {wall * 1e3:.1f} ms/book")
+        plan.close()
+    return out
+
+
+def serving_leg(quick: bool) -> dict:
+    """Open-loop Poisson arrivals over a resident 20-asset local-vol market."""
+    import numpy as np
+    from riga_flow_amd import serving as SV
+    u = SV.Universe()
+    rng = np.random.default_rng(42)
+    pool = [SV.sample_request(u, rng) for _ in range(24 if not quick else 6)]
+    tiers = ["fp32" if rng.random() < 0.5 else "fp64" for _ in pool]
+    runs = []
+    for qps in (50.0, 200.0, 800.0):
+        srv = SV.Server(hbm_budget_gib=96.0, max_inflight=3, max_batch_requests=4096,
+                        max_queue_seconds=3.0, paths_per_thread=1, max_batch_path_steps=1e11)
+        r = SV.poisson_load(srv, pool, tiers, qps, duration_s=5.0 if not quick else 1.0,
+                            seed=int(qps))
+        r["max_concurrent_plans"] = srv.stats()["max_concurrent"]
+        srv.close()
+        log(f"serving qps={qps:g}: completed {r['completed']}, shed {r['shed']}, "
+            f"p50 {r['p50_s']}, p99 {r['p99_s']}, wall {r['wall_s']:.1f} s")
+        runs.append(r)
+    sizes = [sum(q.config.paths * q.trade.steps for q in req) for req in pool]
+    return {"runs": runs, "mean_request_path_steps": float(np.mean(sizes)),
+            "requests_in_pool": len(pool),
+            "note": "offered load exceeds one GPU by orders of magnitude; tickets queued "
+                    "longer than 3 s are shed, p50/p99 are over completed tickets"}
 
 
 if __name__ == "__main__":

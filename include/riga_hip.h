@@ -192,6 +192,7 @@ typedef struct riga_server_config {
   double max_queue_seconds;  /* shed tickets queued longer (0: never) */
   int32_t paths_per_thread;  /* launch shape for every plan (0: 1) */
   int32_t normal_table;      /* as riga_launch_config.normal_table */
+  double max_batch_path_steps; /* path-steps one plan may hold beyond its first ticket (0: no cap) */
 } riga_server_config;
 
 typedef struct riga_ticket_info {

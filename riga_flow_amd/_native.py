@@ -108,7 +108,8 @@ class PlanInfo(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_inflight", C.c_int32),
                 ("max_batch_requests", C.c_int32), ("max_queue_seconds", C.c_double),
-                ("paths_per_thread", C.c_int32), ("normal_table", C.c_int32)]
+                ("paths_per_thread", C.c_int32), ("normal_table", C.c_int32),
+                ("max_batch_path_steps", C.c_double)]
 
 
 class TicketInfo(C.Structure):

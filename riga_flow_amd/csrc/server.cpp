@@ -204,18 +204,23 @@ struct riga_server {
     std::vector<Ticket*> take;
     std::vector<riga_request> views;
     int64_t need = 0;
+    double work = 0;
     const riga_launch_config lc = launch_config();
     while (!queue.empty()) {
       Ticket* tk = queue.front();
       if (tk->precision != precision) break;
       if (!take.empty() && (int)(views.size() + tk->reqs.size()) > max_requests) break;
-      std::vector<riga_request> trial = views;
-      for (auto& o : tk->reqs) trial.push_back(o->r);
-      int64_t bytes = 0;
-      riga_error err{};
-      if (riga_plan_estimate(ctx, trial.data(), (int)trial.size(), precision, &lc, &bytes, &err) != 0) {
-        if (take.empty()) {  // this ticket alone is invalid: fail it
-          queue.pop_front();
+      if (!take.empty() && cfg.max_batch_path_steps > 0 &&
+          work + tk->path_steps > cfg.max_batch_path_steps)
+        break;  // keep one plan's run time bounded
+      if (tk->bytes == 0) {  // the ticket's own plan size, estimated once (an upper bound
+                             // for its share of a joint plan)
+        std::vector<riga_request> own;
+        for (auto& o : tk->reqs) own.push_back(o->r);
+        riga_error err{};
+        int64_t bytes = 0;
+        if (riga_plan_estimate(ctx, own.data(), (int)own.size(), precision, &lc, &bytes, &err) != 0) {
+          queue.pop_front();  // invalid request: fail the ticket
           tk->err = err;
           tk->state = kFailed;
           tk->t_end = now_s();
@@ -223,27 +228,25 @@ struct riga_server {
           cv_done.notify_all();
           continue;
         }
-        break;
+        tk->bytes = bytes;
       }
-      if (bytes > budget) {
-        if (take.empty()) {  // can never fit
-          queue.pop_front();
-          snprintf(tk->err.code, sizeof tk->err.code, "memory_budget");
-          snprintf(tk->err.message, sizeof tk->err.message, "request needs %lld bytes, budget %lld",
-                   (long long)bytes, (long long)budget);
-          tk->state = kFailed;
-          tk->t_end = now_s();
-          stats.failed_tickets += 1;
-          cv_done.notify_all();
-          continue;
-        }
-        break;
+      if (tk->bytes > budget) {  // can never fit
+        queue.pop_front();
+        snprintf(tk->err.code, sizeof tk->err.code, "memory_budget");
+        snprintf(tk->err.message, sizeof tk->err.message, "request needs %lld bytes, budget %lld",
+                 (long long)tk->bytes, (long long)budget);
+        tk->state = kFailed;
+        tk->t_end = now_s();
+        stats.failed_tickets += 1;
+        cv_done.notify_all();
+        continue;
       }
-      if (bytes_in_use + bytes > budget) break;  // wait for memory to free up
+      if (bytes_in_use + need + tk->bytes > budget) break;  // wait for memory to free up
       queue.pop_front();
       take.push_back(tk);
-      views.swap(trial);
-      need = bytes;
+      for (auto& o : tk->reqs) views.push_back(o->r);
+      need += tk->bytes;
+      work += tk->path_steps;
     }
     if (take.empty()) return false;
     InFlight f;

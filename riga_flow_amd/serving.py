@@ -48,10 +48,12 @@ class TicketResult:
 class Server:
     def __init__(self, hbm_budget_gib: float = 64.0, max_inflight: int = 2,
                  max_batch_requests: int = 4096, max_queue_seconds: float = 0.0,
-                 paths_per_thread: int = 1, normal_table: int = 0, max_dimension: int = 7000):
+                 paths_per_thread: int = 1, normal_table: int = 0, max_dimension: int = 7000,
+                 max_batch_path_steps: float = 0.0):
         api.gpu_backend()
         cfg = N.ServerConfig(int(hbm_budget_gib * (1 << 30)), max_inflight, max_batch_requests,
-                             max_queue_seconds, paths_per_thread, normal_table)
+                             max_queue_seconds, paths_per_thread, normal_table,
+                             float(max_batch_path_steps))
         h = C.c_void_p()
         err = N.Error()
         N.check(N.lib().riga_server_create(N.Device.get().handle, C.byref(cfg), C.byref(h),
@@ -121,6 +123,14 @@ class Universe:
         self.horizon = horizon
         self.grids = [W.lv_grid_for(W.svi_implied(self.sigmas[a]), 100.0, self.r, horizon)
                       for a in range(n_assets)]
+        # bumped surfaces are part of the resident market: calibrated once
+        self.spot_grids = {}
+        for a in range(n_assets):
+            for rel in (0.01, -0.01):
+                self.spot_grids[(a, rel)] = W.lv_grid_for(W.svi_implied(self.sigmas[a]),
+                                                          100.0 + rel * 100.0, self.r, horizon)
+        self.vega_grids = [W.lv_grid_for(W.svi_implied(self.sigmas[a], bump=0.01), 100.0, self.r,
+                                         horizon) for a in range(n_assets)]
 
     def request(self, assets: List[int], years: float, paths: int, spot_bump=None,
                 vega_asset: int = -1, trade_rng=None, trade_id: str = "t") -> PriceRequest:
@@ -131,13 +141,11 @@ class Universe:
         if spot_bump is not None:
             f, rel = spot_bump
             spots[f] = 100.0 + rel * 100.0
-            grids[f] = W.lv_grid_for(W.svi_implied(self.sigmas[assets[f]]), spots[f], self.r,
-                                     self.horizon)
+            grids[f] = self.spot_grids[(assets[f], rel)]
         vols = [float(self.vols[a]) for a in assets]
         if vega_asset >= 0:
             vols[vega_asset] += 0.01
-            grids[vega_asset] = W.lv_grid_for(W.svi_implied(self.sigmas[assets[vega_asset]],
-                                                            bump=0.01), 100.0, self.r, self.horizon)
+            grids[vega_asset] = self.vega_grids[assets[vega_asset]]
         rng = trade_rng or np.random.default_rng(0)
         market = MarketSnapshot(W.AS_OF, spots, vols, self.r, [float(self.q[a]) for a in assets],
                                 [[float(self.corr[a][b]) for b in assets] for a in assets])
