@@ -243,9 +243,11 @@ def serving_leg(quick: bool) -> dict:
     import numpy as np
     from riga_flow_amd import serving as SV
     u = SV.Universe()
+    u.make_resident()  # the market's surfaces live in device memory for the whole run
     rng = np.random.default_rng(42)
     pool = [SV.sample_request(u, rng) for _ in range(24 if not quick else 6)]
     tiers = ["fp32" if rng.random() < 0.5 else "fp64" for _ in pool]
+    log(f"serving pool ready: {len(pool)} requests, {sum(len(r) for r in pool)} trade-scenarios")
     runs = []
     for qps in (50.0, 200.0, 800.0):
         srv = SV.Server(hbm_budget_gib=96.0, max_inflight=3, max_batch_requests=4096,

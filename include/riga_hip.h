@@ -48,6 +48,7 @@ typedef struct riga_error {
 } riga_error;
 
 typedef struct riga_ctx riga_ctx;
+typedef struct riga_surface riga_surface;
 
 enum riga_precision { RIGA_FP64 = 0, RIGA_FP32 = 1 };
 enum riga_ladder { RIGA_VR_NONE = 0, RIGA_VR_HALTON = 1, RIGA_VR_SOBOL = 2 };
@@ -97,6 +98,10 @@ typedef struct riga_request {
   const riga_implied_grid *implied;
   const riga_lv_grid *calibrated;
   int32_t grid_x;          /* 96 = GRID_X (riga-market/src/lib.rs:28) */
+  /* Resident calibrated grids, one per factor (riga_surface_create); when set
+   * they take the place of implied/calibrated and nothing grid-sized is copied
+   * per call.  The surfaces must outlive every plan and ticket that uses them. */
+  const struct riga_surface *const *surfaces;
 } riga_request;
 
 /* PriceResult (riga-core/src/lib.rs:243-255) in flat form.  Strings (ledger
@@ -227,6 +232,13 @@ void riga_server_destroy(riga_server *server);
 /* ---- replay (explain.rs:90-316): per-path records for paths [0, req->paths) ---- */
 int riga_replay_paths(riga_ctx *ctx, const riga_request *req, int precision,
                       riga_path_record *records, riga_result *out, riga_error *err);
+
+/* ---- resident market surfaces (gpu.rs:280-327 resident_lv keeps the local-vol
+ * rows and log-strike axes on the device across runs).  A calibrated grid is
+ * copied to device memory once; requests then name it in riga_request.surfaces.
+ * Destroy a surface only after every plan and ticket using it has finished. */
+int riga_surface_create(riga_ctx *ctx, const riga_lv_grid *grid, riga_surface **out, riga_error *err);
+void riga_surface_destroy(riga_surface *surface);
 
 /* ---- local-vol calibration (riga-market/src/lib.rs:210-299 DupireLocalVol::calibrate)
  * Outputs: t_axis[n_maturities], log_s[grid_x], v[n_maturities * grid_x]. */

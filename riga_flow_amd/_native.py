@@ -63,6 +63,7 @@ class Request(C.Structure):
         ("implied", C.POINTER(ImpliedGrid)),
         ("calibrated", C.POINTER(LvGrid)),
         ("grid_x", C.c_int32),
+        ("surfaces", C.POINTER(C.c_void_p)),
     ]
 
 
@@ -134,6 +135,7 @@ EXPORTS = ("riga_hip_init", "riga_hip_destroy", "riga_hip_device_name", "riga_hi
            "riga_plan_wait", "riga_plan_run", "riga_plan_query", "riga_plan_destroy",
            "riga_server_create", "riga_server_submit", "riga_server_wait",
            "riga_server_stats_get", "riga_server_destroy",
+           "riga_surface_create", "riga_surface_destroy",
            "riga_lv_calibrate", "riga_lv_sigma", "riga_debug_sobol")
 
 _lib = None
@@ -192,6 +194,10 @@ def lib():
             L.riga_plan_destroy.restype = None
             L.riga_lv_calibrate.argtypes = [C.POINTER(ImpliedGrid), C.c_double, C.c_double,
                                             C.c_int32, _D, _D, _D, C.POINTER(Error)]
+            L.riga_surface_create.argtypes = [C.c_void_p, C.POINTER(LvGrid), C.POINTER(C.c_void_p),
+                                              C.POINTER(Error)]
+            L.riga_surface_destroy.argtypes = [C.c_void_p]
+            L.riga_surface_destroy.restype = None
             L.riga_lv_sigma.argtypes = [C.POINTER(LvGrid), C.c_double, C.c_double]
             L.riga_lv_sigma.restype = C.c_double
             L.riga_debug_sobol.argtypes = [C.c_void_p, C.c_uint64, C.c_int32, C.c_uint64,

@@ -94,7 +94,14 @@ class _Packed:
         r.max_dimension = max_dimension
         lv = request.local_vol
         if lv is not None:
-            if lv.calibrated is not None:
+            if lv.resident is not None:
+                if len(lv.resident) != F:
+                    raise RigaError("invalid_local_vol",
+                                    "local_vol.resident must be aligned with market.spots")
+                hs = (C.c_void_p * F)(*[s.handle.value for s in lv.resident])
+                keep.append(hs)
+                r.surfaces = C.cast(hs, C.POINTER(C.c_void_p))
+            elif lv.calibrated is not None:
                 grids = (N.LvGrid * F)()
                 for f, g in enumerate(lv.calibrated):
                     v = np.asarray(g.v, dtype=np.float64)
@@ -365,6 +372,52 @@ def replay_paths(request: PriceRequest, precision: str = "fp64",
                                       recs, C.byref(res), C.byref(err)), err)
     arr = np.frombuffer(recs, dtype=N.RECORD_DTYPE, count=request.config.paths).copy()
     return arr, _assemble(request, res)
+
+
+class Surface:
+    """A calibrated local-vol grid copied to device memory once (riga_surface_create),
+    so requests name it instead of sending the grid (the reference keeps its LV
+    rows resident the same way, crates/riga-engine/src/gpu.rs:280-327).  Plans and
+    tickets that use it hold a reference through their requests."""
+
+    def __init__(self, grid: LocalVolGrid):
+        dev = N.Device.get()
+        v = np.ascontiguousarray(np.asarray(grid.v, dtype=np.float64).reshape(-1))
+        t = N.darr(grid.t_axis)
+        x = N.darr(grid.log_s)
+        g = N.LvGrid(len(grid.t_axis), len(grid.log_s), N.dptr(t), N.dptr(x), N.dptr(v))
+        h = C.c_void_p()
+        err = N.Error()
+        N.check(N.lib().riga_surface_create(dev.handle, C.byref(g), C.byref(h), C.byref(err)), err)
+        self.handle = h
+        self.grid = grid
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) and self.handle.value:
+            N.lib().riga_surface_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def resident(request: PriceRequest, cache: Optional[dict] = None) -> PriceRequest:
+    """The request with its calibrated grids made device-resident (one Surface per
+    distinct grid object, shared through ``cache``)."""
+    import dataclasses
+    lv = request.local_vol
+    if lv is None or lv.calibrated is None:
+        return request
+    cache = {} if cache is None else cache
+    surfs = []
+    for g in lv.calibrated:
+        if id(g) not in cache:
+            cache[id(g)] = (g, Surface(g))
+        surfs.append(cache[id(g)][1])
+    return dataclasses.replace(request, local_vol=dataclasses.replace(lv, resident=surfs))
 
 
 def lv_calibrate(maturities, strikes, vols, spot: float, rate: float,

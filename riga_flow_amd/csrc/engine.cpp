@@ -100,6 +100,14 @@ struct riga_ctx {
   HostBuf staging, finals_host;
 };
 
+// A calibrated grid kept in device memory: [log_s (n_x)][v (n_t x n_x)] doubles.
+// The host copy serves the tenor brackets and the LDS sizing of each plan.
+struct riga_surface {
+  riga_ctx* ctx = nullptr;
+  LvGrid grid;
+  void* dev = nullptr;
+};
+
 namespace {
 
 #define HIP_TRY(expr, what)                                                              \
@@ -117,8 +125,10 @@ struct Prep {
   int F = 0;
   int ladder_dims = 0;
   std::vector<double> chol;
-  std::vector<LvGrid> lv;  // one per factor when local vol
-  std::string key;         // CRN group key
+  std::vector<LvGrid> lv_own;              // grids copied or calibrated for this request
+  std::vector<const LvGrid*> lv;           // one per factor when local vol
+  std::vector<const riga_surface*> res;    // per factor: the resident surface, or null
+  std::string key;                         // CRN group key
 };
 
 template <typename T>
@@ -144,10 +154,23 @@ bool prepare(const riga_request& r, Prep* p, Error* e) {
   if (!cholesky_factor(F, r.correlation, &p->chol, e)) return false;
   const int cap = r.max_dimension > 0 ? r.max_dimension : 512;
   p->ladder_dims = std::max(1, std::min(r.ladder_dimension, cap));  // qmc.rs:105,249 clamp
-  const bool has_lv = r.implied != nullptr || r.calibrated != nullptr;
+  const bool has_lv = r.surfaces != nullptr || r.implied != nullptr || r.calibrated != nullptr;
   if (has_lv) {
-    p->lv.resize(F);
+    p->lv_own.resize(r.surfaces ? 0 : F);
+    p->lv.assign(F, nullptr);
+    p->res.assign(F, nullptr);
     for (int f = 0; f < F; ++f) {
+      if (r.surfaces) {
+        if (!r.surfaces[f]) {
+          e->code = "invalid_local_vol";
+          e->message = "resident surface missing for factor " + std::to_string(f);
+          return false;
+        }
+        p->res[f] = r.surfaces[f];
+        p->lv[f] = &r.surfaces[f]->grid;
+        continue;
+      }
+      p->lv[f] = &p->lv_own[f];
       if (r.calibrated) {
         const riga_lv_grid& g = r.calibrated[f];
         if (g.n_t < 1 || g.n_x < 2 || g.n_x > kMaxLvX) {
@@ -155,9 +178,9 @@ bool prepare(const riga_request& r, Prep* p, Error* e) {
           e->message = "calibrated local-vol grid must have 1+ tenors and 2.." + std::to_string(kMaxLvX) + " knots";
           return false;
         }
-        p->lv[f].t_axis.assign(g.t_axis, g.t_axis + g.n_t);
-        p->lv[f].log_s.assign(g.log_s, g.log_s + g.n_x);
-        p->lv[f].v.assign(g.v, g.v + (size_t)g.n_t * g.n_x);
+        p->lv_own[f].t_axis.assign(g.t_axis, g.t_axis + g.n_t);
+        p->lv_own[f].log_s.assign(g.log_s, g.log_s + g.n_x);
+        p->lv_own[f].v.assign(g.v, g.v + (size_t)g.n_t * g.n_x);
       } else {
         const int gx = r.grid_x > 0 ? r.grid_x : 96;
         if (gx > kMaxLvX) {
@@ -165,7 +188,7 @@ bool prepare(const riga_request& r, Prep* p, Error* e) {
           e->message = "grid_x exceeds " + std::to_string(kMaxLvX);
           return false;
         }
-        if (!dupire_calibrate(r.implied[f], r.spots[f], r.risk_free_rate, gx, &p->lv[f], e)) {
+        if (!dupire_calibrate(r.implied[f], r.spots[f], r.risk_free_rate, gx, &p->lv_own[f], e)) {
           e->code = "invalid_local_vol";
           return false;
         }
@@ -200,8 +223,8 @@ bool prepare(const riga_request& r, Prep* p, Error* e) {
   for (double v : p->chol) put_bytes(k, v);
   if (has_lv)
     for (int f = 0; f < F; ++f) {
-      put_bytes(k, (int)p->lv[f].t_axis.size());
-      for (double t : p->lv[f].t_axis) put_bytes(k, t);
+      put_bytes(k, (int)p->lv[f]->t_axis.size());
+      for (double t : p->lv[f]->t_axis) put_bytes(k, t);
     }
   return true;
 }
@@ -216,6 +239,7 @@ struct Item {
   std::vector<LvBracket> lvb;
   std::vector<ScenarioDesc> scen;
   std::vector<const LvGrid*> surfaces;  // distinct, in first-use order
+  std::vector<const riga_surface*> surface_res;  // resident copy of each, or null
   std::vector<int> surface_factor;
   std::vector<uint32_t> halton;
   int depth = 1;
@@ -344,11 +368,12 @@ bool build_item(const std::vector<Prep>& preps, Item* it, int precision, bool re
       sc.vsd[f] = vol * sqrt(dt);
       sc.lv[f] = -1;
       if (!p.lv.empty()) {
-        const LvGrid* g = &p.lv[f];
+        const LvGrid* g = p.lv[f];
         int found = -1;
         for (size_t u = 0; u < it->surfaces.size(); ++u) {
           const LvGrid* h = it->surfaces[u];
-          if (it->surface_factor[u] == f && h->t_axis == g->t_axis && h->log_s == g->log_s && h->v == g->v) {
+          if (it->surface_factor[u] == f &&
+              (h == g || (h->t_axis == g->t_axis && h->log_s == g->log_s && h->v == g->v))) {
             found = (int)u;
             break;
           }
@@ -356,6 +381,7 @@ bool build_item(const std::vector<Prep>& preps, Item* it, int precision, bool re
         if (found < 0) {
           found = (int)it->surfaces.size();
           it->surfaces.push_back(g);
+          it->surface_res.push_back(p.res[f]);
           it->surface_factor.push_back(f);
         }
         sc.lv[f] = found;
@@ -367,7 +393,7 @@ bool build_item(const std::vector<Prep>& preps, Item* it, int precision, bool re
     for (int step = 0; step < T; ++step)
       for (int f = 0; f < F; ++f) {
         LvBracket& b = it->lvb[(size_t)step * F + f];
-        lv_bracket(p0.lv[f], (double)step * dt, &b.i0, &b.i1, &b.wt);  // note.rs:136 t = step*dt
+        lv_bracket(*p0.lv[f], (double)step * dt, &b.i0, &b.i1, &b.wt);  // note.rs:136 t = step*dt
       }
   }
   if (r.ladder == RIGA_VR_HALTON) it->halton = halton_tables(d.ladder_rows, r.seed);
@@ -489,6 +515,18 @@ struct riga_plan {
 };
 
 namespace {
+
+// Knots at x0 + i*dx to within 1e-9 of the spacing (linspace grids, as the
+// calibration writes them): the FP32 tier then locates a knot by one multiply.
+bool evenly_spaced(const std::vector<double>& x) {
+  const size_t n = x.size();
+  if (n < 2) return false;
+  const double dx = (x[n - 1] - x[0]) / (double)(n - 1);
+  if (!(dx > 0)) return false;
+  for (size_t i = 0; i < n; ++i)
+    if (std::fabs(x[i] - (x[0] + dx * (double)i)) > 1e-9 * dx) return false;
+  return true;
+}
 
 // Enqueue the kernels, the finalize merge and the download of the merged records.
 hipError_t enqueue(riga_plan* pl, bool timed) {
@@ -640,7 +678,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       stack_levels = std::max(stack_levels, items[i].depth - 1);
       int max_nx = 0;
       for (const LvGrid* g : items[i].surfaces) max_nx = std::max(max_nx, g->n_x());
-      lv_words = std::max(lv_words, (int)items[i].surfaces.size() * 3 * max_nx);
+      lv_words = std::max(lv_words, (int)items[i].surfaces.size() * (real_size == 4 ? 4 : 3) * max_nx);
     }
     lp.lds = (size_t)stack_levels * lp.F * kBlock * real_size + (size_t)lv_words * real_size;
     if (lp.lds + (size_t)lp.smax * sizeof(BlockShared) > ctx->lds_limit) {
@@ -676,17 +714,28 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       d.off_halton = arena.add(it.halton.data(), it.halton.size());
       std::vector<LvSurface> su(it.surfaces.size());
       int max_nx = 0;
+      bool uniform = real_size == 4 && !it.surfaces.empty();
       for (size_t u = 0; u < it.surfaces.size(); ++u) {
         const LvGrid& g = *it.surfaces[u];
-        su[u].off_t = arena.add(g.t_axis.data(), g.t_axis.size());
-        su[u].off_x = arena.add(g.log_s.data(), g.log_s.size());
-        su[u].off_v = arena.add(g.v.data(), g.v.size());
+        if (const riga_surface* rs = it.surface_res[u]) {  // resident: no copy in the arena
+          su[u].off_t = su[u].off_x = su[u].off_v = -1;
+          su[u].dev_x = (const double*)rs->dev;
+          su[u].dev_v = (const double*)rs->dev + g.n_x();
+        } else {
+          su[u].off_t = arena.add(g.t_axis.data(), g.t_axis.size());
+          su[u].off_x = arena.add(g.log_s.data(), g.log_s.size());
+          su[u].off_v = arena.add(g.v.data(), g.v.size());
+          su[u].dev_x = su[u].dev_v = nullptr;
+        }
         su[u].n_t = g.n_t();
         su[u].n_x = g.n_x();
         su[u].factor = it.surface_factor[u];
         su[u].inv_dx = (double)(g.n_x() - 1) / (g.log_s[g.n_x() - 1] - g.log_s[0]);
+        su[u].x0 = g.log_s[0];
+        uniform = uniform && evenly_spaced(g.log_s);
         max_nx = std::max(max_nx, g.n_x());
       }
+      if (uniform) d.flags |= kLvUniform;
       d.off_surf = arena.add(su.data(), su.size());
       d.n_surf = (int32_t)su.size();
       d.max_nx = max_nx;
@@ -1033,6 +1082,44 @@ int riga_replay_paths(riga_ctx* ctx, const riga_request* req, int precision,
     return RIGA_EINVAL;
   }
   return run_batch(ctx, req, 1, precision, nullptr, out, records, err);
+}
+
+int riga_surface_create(riga_ctx* ctx, const riga_lv_grid* grid, riga_surface** out, riga_error* err) {
+  if (!ctx || !grid || !out) {
+    set_err(err, "invalid_argument", "context, grid and output are required");
+    return RIGA_EINVAL;
+  }
+  if (grid->n_t < 1 || grid->n_x < 2 || grid->n_x > kMaxLvX || !grid->t_axis || !grid->log_s || !grid->v) {
+    set_err(err, "invalid_local_vol",
+            "calibrated local-vol grid must have 1+ tenors and 2.." + std::to_string(kMaxLvX) + " knots");
+    return RIGA_EINVAL;
+  }
+  std::unique_ptr<riga_surface> s(new riga_surface());
+  s->ctx = ctx;
+  s->grid.t_axis.assign(grid->t_axis, grid->t_axis + grid->n_t);
+  s->grid.log_s.assign(grid->log_s, grid->log_s + grid->n_x);
+  s->grid.v.assign(grid->v, grid->v + (size_t)grid->n_t * grid->n_x);
+  std::vector<double> packed(s->grid.log_s);
+  packed.insert(packed.end(), s->grid.v.begin(), s->grid.v.end());
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  HIP_TRY(hipMalloc(&s->dev, packed.size() * sizeof(double)), "surface alloc");
+  const hipError_t e = hipMemcpy(s->dev, packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(s->dev);
+    set_err(err, "gpu_error", std::string("surface upload: ") + hipGetErrorString(e));
+    return RIGA_EGPU;
+  }
+  *out = s.release();
+  return 0;
+}
+
+void riga_surface_destroy(riga_surface* s) {
+  if (!s) return;
+  if (s->dev) {
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipFree(s->dev);
+  }
+  delete s;
 }
 
 int riga_lv_calibrate(const riga_implied_grid* grid, double spot, double rate, int32_t grid_x,

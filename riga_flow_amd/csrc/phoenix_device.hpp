@@ -327,6 +327,25 @@ __device__ __forceinline__ Real lv_sigma(const Real* __restrict__ ls, int nx, Re
   return (Real(1) - wt) * v0 + wt * v1;
 }
 
+// FP32 lookup against a surface with evenly spaced knots.  LDS holds, per knot
+// j, {r0[j], r0[j+1]-r0[j], r1[j], r1[j+1]-r1[j]} of the two bracketing tenor
+// rows, so one 16-byte read and three FMAs give the bilinear value; the knot
+// position comes from t = (x - x0)/dx clamped to [0, n_x-1] (below the first
+// knot and above the last the edge value holds, as in the general lookup).
+__device__ __forceinline__ float lv_sigma_uniform(const float4* __restrict__ rows, int nx,
+                                                  float inv_dx, float neg_x0_inv_dx, float wt,
+                                                  float x) {
+  float t = fmaf(x, inv_dx, neg_x0_inv_dx);
+  t = fminf(fmaxf(t, 0.0f), (float)(nx - 1));
+  int j = (int)t;
+  j = j < nx - 2 ? j : nx - 2;
+  const float wx = t - (float)j;
+  const float4 e = rows[j];
+  const float v0 = fmaf(wx, e.y, e.x);
+  const float v1 = fmaf(wx, e.w, e.z);
+  return fmaf(wt, v1 - v0, v0);
+}
+
 // ---------------------------------------------------------------- shared normal table
 // Ladder normals z[dim][path] for every raw dimension a batch of same-seed
 // trades uses (dims without a wrapped direction row); blockIdx.y is the
@@ -376,7 +395,9 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   const bool trace = (it.flags & kTrace) != 0;
   const bool want_records = (it.flags & kRecords) != 0 && records != nullptr;
   constexpr bool kLogState = sizeof(Real) == 4;  // FP32 tier carries X = ln S
-  // dynamic LDS: [bridge stack: (depth-1) x F x kBlock][local-vol rows: n_surf x 3 x max_nx]
+  constexpr int kSurfWords = kLogState ? 4 : 3;   // LDS words per knot and surface
+  const bool lv_uniform = kLogState && (it.flags & kLvUniform) != 0;
+  // dynamic LDS: [bridge stack: (depth-1) x F x kBlock][local-vol rows: n_surf x kSurfWords x max_nx]
   Real* lds_stack = reinterpret_cast<Real*>(lds_dyn);
   Real* lds_w = lds_stack + (size_t)it.stack_levels * F * kBlock;
   const Real* lds = lds_w;
@@ -478,14 +499,24 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
           for (int u = 0; u < it.n_surf; ++u) {
             const LvSurface su = surf[u];
             const LvBracket br = lvb[step * F + su.factor];
-            const double* xs = at<double>(arena, su.off_x);
-            const double* v0 = at<double>(arena, su.off_v) + (size_t)br.i0 * su.n_x;
-            const double* v1 = at<double>(arena, su.off_v) + (size_t)br.i1 * su.n_x;
-            Real* dst = lds_w + (size_t)u * 3 * nxs;
-            for (int i = tid; i < su.n_x; i += kBlock) {
-              dst[i] = (Real)xs[i];
-              dst[nxs + i] = (Real)v0[i];
-              dst[2 * nxs + i] = (Real)v1[i];
+            const double* xs = su.dev_x ? su.dev_x : at<double>(arena, su.off_x);
+            const double* vv = su.dev_v ? su.dev_v : at<double>(arena, su.off_v);
+            const double* v0 = vv + (size_t)br.i0 * su.n_x;
+            const double* v1 = vv + (size_t)br.i1 * su.n_x;
+            Real* dst = lds_w + (size_t)u * kSurfWords * nxs;
+            if (lv_uniform) {
+              float4* d4 = reinterpret_cast<float4*>(dst);
+              for (int i = tid; i < su.n_x; i += kBlock) {
+                const int k = i + 1 < su.n_x ? i + 1 : i;
+                d4[i] = make_float4((float)v0[i], (float)(v0[k] - v0[i]), (float)v1[i],
+                                    (float)(v1[k] - v1[i]));
+              }
+            } else {
+              for (int i = tid; i < su.n_x; i += kBlock) {
+                dst[i] = (Real)xs[i];
+                dst[nxs + i] = (Real)v0[i];
+                dst[2 * nxs + i] = (Real)v1[i];
+              }
             }
           }
 #pragma unroll
@@ -511,8 +542,19 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
           if (local_vol) {
             const int u = sc.lv[f];
             const LvSurface& su = surf[u];
-            const Real vol = lv_sigma<Real, kLogState>(lds + (size_t)u * 3 * nxs, su.n_x, (Real)su.inv_dx,
-                                                       (Real)lvb[step * F + f].wt, spot[s][f]);
+            Real vol;
+            if constexpr (kLogState) {
+              if (lv_uniform)
+                vol = lv_sigma_uniform(reinterpret_cast<const float4*>(lds + (size_t)u * kSurfWords * nxs),
+                                       su.n_x, (float)su.inv_dx, (float)(-su.x0 * su.inv_dx),
+                                       (float)lvb[step * F + f].wt, spot[s][f]);
+              else
+                vol = lv_sigma<Real, true>(lds + (size_t)u * kSurfWords * nxs, su.n_x, (Real)su.inv_dx,
+                                           (Real)lvb[step * F + f].wt, spot[s][f]);
+            } else {
+              vol = lv_sigma<Real, false>(lds + (size_t)u * kSurfWords * nxs, su.n_x, (Real)su.inv_dx,
+                                          (Real)lvb[step * F + f].wt, spot[s][f]);
+            }
             const Real drift = ((Real)sc.rq[f] - Real(0.5) * vol * vol) * (Real)it.dt;
             incr = drift + vol * (Real)it.sqrt_dt * shock[f];
           } else {

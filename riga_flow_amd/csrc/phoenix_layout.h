@@ -26,6 +26,7 @@ enum ItemFlags : uint32_t {
   kRecords = 1u << 5,        // write one PathRecord per (scenario, path)
   kOwenTable = 1u << 6,      // top 8 Owen levels read from a per-dimension table
   kNormalTable = 1u << 7,    // ladder normals read from the batch's shared [dim][path] table
+  kLvUniform = 1u << 8,      // FP32 tier: every surface has evenly spaced knots (slope rows)
 };
 
 // Per observation date, uniform across all paths of an item.
@@ -62,7 +63,11 @@ struct LvSurface {
   int32_t n_t, n_x;
   int32_t factor;    // which factor of the item this surface drives
   int32_t pad;
-  double inv_dx;     // (n_x - 1) / (log_s[n_x-1] - log_s[0]): bracket guess only
+  double inv_dx;     // (n_x - 1) / (log_s[n_x-1] - log_s[0]): bracket guess (exact knot
+                     // position when the item has kLvUniform)
+  double x0;         // log_s[0]
+  const double* dev_x;  // resident surface: knots and vols in device memory
+  const double* dev_v;  // (null: at off_x / off_v in the arena)
 };
 
 struct ScenarioDesc {

@@ -23,6 +23,7 @@
 
 #include <atomic>
 #include <chrono>
+#include <algorithm>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -48,6 +49,7 @@ struct OwnedRequest {
   std::vector<std::vector<double>> grid_a, grid_b, grid_c;
   std::vector<riga_implied_grid> implied;
   std::vector<riga_lv_grid> calibrated;
+  std::vector<const riga_surface*> surfaces;  // resident: the handles only
 
   void copy_from(const riga_request& in) {
     r = in;
@@ -65,7 +67,13 @@ struct OwnedRequest {
     grid_a.clear();
     grid_b.clear();
     grid_c.clear();
-    if (in.calibrated) {
+    r.surfaces = nullptr;
+    if (in.surfaces) {
+      surfaces.assign(in.surfaces, in.surfaces + F);
+      r.surfaces = surfaces.data();
+      r.implied = nullptr;
+      r.calibrated = nullptr;
+    } else if (in.calibrated) {
       calibrated.resize(F);
       grid_a.resize(F);
       grid_b.resize(F);
@@ -194,12 +202,66 @@ struct riga_server {
     }
   }
 
-  // admit from the queue head into one new plan; called with mu held
-  bool admit() {
+  void fail_queued(Ticket* tk, const riga_error& err) {
+    auto it = std::find(queue.begin(), queue.end(), tk);
+    if (it != queue.end()) queue.erase(it);
+    tk->err = err;
+    tk->state = kFailed;
+    tk->t_end = now_s();
+    stats.failed_tickets += 1;
+    cv_done.notify_all();
+  }
+
+  // Device bytes of each new ticket's own plan (an upper bound for its share
+  // of a joint plan), estimated once and outside the lock so that submitters
+  // never wait on plan building.  Only this thread removes queued tickets, so
+  // the pointers stay valid while unlocked.
+  bool estimate_new(std::unique_lock<std::mutex>& lock) {
+    std::vector<Ticket*> todo;
+    for (Ticket* tk : queue) {
+      if (tk->bytes == 0) todo.push_back(tk);
+      if (todo.size() >= 32) break;
+    }
+    if (todo.empty()) return false;
+    const riga_launch_config lc = launch_config();
+    std::vector<int64_t> bytes(todo.size(), 0);
+    std::vector<riga_error> errs(todo.size());
+    std::vector<int> rcs(todo.size(), 0);
+    lock.unlock();
+    for (size_t i = 0; i < todo.size(); ++i) {
+      std::vector<riga_request> own;
+      for (auto& o : todo[i]->reqs) own.push_back(o->r);
+      errs[i] = riga_error{};
+      rcs[i] = riga_plan_estimate(ctx, own.data(), (int)own.size(), todo[i]->precision, &lc, &bytes[i], &errs[i]);
+    }
+    lock.lock();
+    const int64_t budget = budget_bytes();
+    for (size_t i = 0; i < todo.size(); ++i) {
+      Ticket* tk = todo[i];
+      if (rcs[i] != 0) {
+        fail_queued(tk, errs[i]);  // invalid request
+      } else if (bytes[i] > budget) {
+        riga_error e{};
+        snprintf(e.code, sizeof e.code, "memory_budget");
+        snprintf(e.message, sizeof e.message, "request needs %lld bytes, budget %lld", (long long)bytes[i],
+                 (long long)budget);
+        fail_queued(tk, e);  // can never fit
+      } else {
+        tk->bytes = bytes[i] > 0 ? bytes[i] : 1;
+      }
+    }
+    return true;
+  }
+
+  int64_t budget_bytes() const { return cfg.hbm_budget_bytes > 0 ? cfg.hbm_budget_bytes : (int64_t)64 << 30; }
+
+  // Admit estimated tickets from the queue head into one new plan.  Called
+  // with mu held; the plan is built and launched with mu released.
+  bool admit(std::unique_lock<std::mutex>& lock) {
     const int max_inflight = cfg.max_inflight > 0 ? cfg.max_inflight : 2;
     if (queue.empty() || (int)inflight.size() >= max_inflight) return false;
     const int max_requests = cfg.max_batch_requests > 0 ? cfg.max_batch_requests : 4096;
-    const int64_t budget = cfg.hbm_budget_bytes > 0 ? cfg.hbm_budget_bytes : (int64_t)64 << 30;
+    const int64_t budget = budget_bytes();
     const int precision = queue.front()->precision;
     std::vector<Ticket*> take;
     std::vector<riga_request> views;
@@ -208,39 +270,11 @@ struct riga_server {
     const riga_launch_config lc = launch_config();
     while (!queue.empty()) {
       Ticket* tk = queue.front();
+      if (tk->bytes == 0) break;  // not estimated yet
       if (tk->precision != precision) break;
       if (!take.empty() && (int)(views.size() + tk->reqs.size()) > max_requests) break;
-      if (!take.empty() && cfg.max_batch_path_steps > 0 &&
-          work + tk->path_steps > cfg.max_batch_path_steps)
+      if (!take.empty() && cfg.max_batch_path_steps > 0 && work + tk->path_steps > cfg.max_batch_path_steps)
         break;  // keep one plan's run time bounded
-      if (tk->bytes == 0) {  // the ticket's own plan size, estimated once (an upper bound
-                             // for its share of a joint plan)
-        std::vector<riga_request> own;
-        for (auto& o : tk->reqs) own.push_back(o->r);
-        riga_error err{};
-        int64_t bytes = 0;
-        if (riga_plan_estimate(ctx, own.data(), (int)own.size(), precision, &lc, &bytes, &err) != 0) {
-          queue.pop_front();  // invalid request: fail the ticket
-          tk->err = err;
-          tk->state = kFailed;
-          tk->t_end = now_s();
-          stats.failed_tickets += 1;
-          cv_done.notify_all();
-          continue;
-        }
-        tk->bytes = bytes;
-      }
-      if (tk->bytes > budget) {  // can never fit
-        queue.pop_front();
-        snprintf(tk->err.code, sizeof tk->err.code, "memory_budget");
-        snprintf(tk->err.message, sizeof tk->err.message, "request needs %lld bytes, budget %lld",
-                 (long long)tk->bytes, (long long)budget);
-        tk->state = kFailed;
-        tk->t_end = now_s();
-        stats.failed_tickets += 1;
-        cv_done.notify_all();
-        continue;
-      }
       if (bytes_in_use + need + tk->bytes > budget) break;  // wait for memory to free up
       queue.pop_front();
       take.push_back(tk);
@@ -249,6 +283,7 @@ struct riga_server {
       work += tk->path_steps;
     }
     if (take.empty()) return false;
+    bytes_in_use += need;  // reserved while the plan is built
     InFlight f;
     f.tickets = take;
     int off = 0;
@@ -258,8 +293,13 @@ struct riga_server {
     }
     f.results.resize(views.size());
     riga_error err{};
+    lock.unlock();
     int rc = riga_plan_create(ctx, views.data(), (int)views.size(), precision, &lc, &f.plan, &err);
     if (rc == 0) rc = riga_plan_launch(f.plan, &err);
+    riga_plan_info info{};
+    if (rc == 0) riga_plan_query(f.plan, &info);
+    lock.lock();
+    bytes_in_use -= need;
     const double t = now_s();
     if (rc != 0) {
       if (f.plan) riga_plan_destroy(f.plan);
@@ -272,8 +312,6 @@ struct riga_server {
       cv_done.notify_all();
       return true;
     }
-    riga_plan_info info{};
-    riga_plan_query(f.plan, &info);
     f.bytes = std::max<int64_t>(need, info.device_bytes);
     bytes_in_use += f.bytes;
     stats.peak_bytes = std::max(stats.peak_bytes, bytes_in_use);
@@ -294,8 +332,10 @@ struct riga_server {
     while (!stop) {
       reap();
       shed_stale(now_s());
-      bool progressed = false;
-      while (admit()) progressed = true;
+      bool progressed = estimate_new(lock);
+      if (stop) break;
+      reap();
+      while (admit(lock)) progressed = true;
       if (!progressed) {
         // poll running plans every 100 us; sleep until work arrives otherwise
         if (!inflight.empty())

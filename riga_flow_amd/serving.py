@@ -62,14 +62,17 @@ class Server:
         self.max_dimension = max_dimension
         self._pending = {}
 
-    def submit(self, requests: Sequence[PriceRequest], precision: str = "fp64") -> int:
-        packed = [api._Packed(q, self.max_dimension) for q in requests]
-        arr = (N.Request * len(packed))(*[p.r for p in packed])
+    def pack(self, requests: Sequence[PriceRequest]) -> "PackedTicket":
+        """Serialise a ticket's requests into C structs (the client's side of the wire)."""
+        return PackedTicket(list(requests), [api._Packed(q, self.max_dimension) for q in requests])
+
+    def submit(self, requests, precision: str = "fp64") -> int:
+        pt = requests if isinstance(requests, PackedTicket) else self.pack(requests)
         t = C.c_uint64()
         err = N.Error()
-        N.check(N.lib().riga_server_submit(self.handle, arr, len(packed), api.PRECISIONS[precision],
-                                           C.byref(t), C.byref(err)), err)
-        self._pending[t.value] = list(requests)
+        N.check(N.lib().riga_server_submit(self.handle, pt.array, len(pt.requests),
+                                           api.PRECISIONS[precision], C.byref(t), C.byref(err)), err)
+        self._pending[t.value] = pt.requests
         return t.value
 
     def wait(self, ticket: int, timeout: float = -1.0, assemble: bool = True) -> Optional[TicketResult]:
@@ -107,6 +110,16 @@ class Server:
             pass
 
 
+class PackedTicket:
+    """A ticket's requests with their C structs; submit copies them, so one packed
+    ticket can be submitted many times."""
+
+    def __init__(self, requests: List[PriceRequest], packed: list):
+        self.requests = requests
+        self._packed = packed
+        self.array = (N.Request * len(packed))(*[p.r for p in packed])
+
+
 # ---------------------------------------------------------------- resident market
 class Universe:
     """A 20-asset local-vol market: spots, vols, dividend yields, one
@@ -131,6 +144,13 @@ class Universe:
                                                           100.0 + rel * 100.0, self.r, horizon)
         self.vega_grids = [W.lv_grid_for(W.svi_implied(self.sigmas[a], bump=0.01), 100.0, self.r,
                                          horizon) for a in range(n_assets)]
+        self.surfaces = None
+
+    def make_resident(self) -> None:
+        """Copy every surface of the market (base and bumped) to device memory once;
+        requests built afterwards name them instead of carrying the grids."""
+        grids = list(self.grids) + list(self.spot_grids.values()) + list(self.vega_grids)
+        self.surfaces = {id(g): api.Surface(g) for g in grids}
 
     def request(self, assets: List[int], years: float, paths: int, spot_bump=None,
                 vega_asset: int = -1, trade_rng=None, trade_id: str = "t") -> PriceRequest:
@@ -153,8 +173,9 @@ class Universe:
                                 coupon_barrier=float(rng.uniform(0.60, 0.80)),
                                 eki=float(rng.uniform(0.50, 0.65)),
                                 coupon=float(rng.uniform(0.005, 0.012)))
+        res = [self.surfaces[id(g)] for g in grids] if self.surfaces else None
         return PriceRequest(trade, market, W.sobol_config(paths, steps, F, seed=1), W.AS_OF,
-                            LocalVolSpec(calibrated=grids))
+                            LocalVolSpec(calibrated=grids, resident=res))
 
 
 def sample_request(u: Universe, rng, path_choices=(16_384, 65_536, 262_144, 1_048_576),
@@ -188,7 +209,9 @@ def sample_request(u: Universe, rng, path_choices=(16_384, 65_536, 262_144, 1_04
 
 def poisson_load(server: Server, requests: List[List[PriceRequest]], tiers: List[str], qps: float,
                  duration_s: float, seed: int = 0) -> dict:
-    """Open-loop Poisson arrivals at ``qps`` for ``duration_s``; requests are taken in turn."""
+    """Open-loop Poisson arrivals at ``qps`` for ``duration_s``; requests are taken in
+    turn, serialised before the clock starts (as a client would send them)."""
+    requests = [r if isinstance(r, PackedTicket) else server.pack(r) for r in requests]
     rng = np.random.default_rng(seed)
     tickets = []
     t0 = time.perf_counter()

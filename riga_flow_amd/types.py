@@ -122,6 +122,9 @@ class LocalVolSpec:
     grids: List[ImpliedVolGrid] = field(default_factory=list)
     calibrated: Optional[List[LocalVolGrid]] = None
     grid_x: int = 96
+    # device-resident copies of ``calibrated`` (``api.Surface``, one per factor);
+    # when present the device path names them instead of sending the grids
+    resident: Optional[list] = None
 
 
 @dataclass
@@ -167,6 +170,7 @@ class PriceRequest:
                     for c in self.local_vol.calibrated
                 ],
                 grid_x=self.local_vol.grid_x,
+                resident=None if self.local_vol.resident is None else list(self.local_vol.resident),
             ),
         )
 

@@ -88,6 +88,28 @@ def test_lv_book_fp32_tier_within_1e4():
     assert rel(res.legs.funding, ora.leg_funding) <= FP32_REL
 
 
+def test_lv_fp32_tier_uneven_knots_within_1e4():
+    # knots moved off the even spacing: the FP32 tier falls back to the
+    # bracketing search instead of the one-multiply knot position
+    import dataclasses
+    req = W.lv_book(paths=2048, steps=252, years=1.0)
+    grids = []
+    for g in req.local_vol.calibrated:
+        n = len(g.log_s)
+        xs = [g.log_s[i] + (0.3 * (g.log_s[1] - g.log_s[0]) * math.sin(1.7 * i) if 0 < i < n - 1 else 0.0)
+              for i in range(n)]
+        assert all(b > a for a, b in zip(xs, xs[1:]))
+        grids.append(dataclasses.replace(g, log_s=xs))
+    req = dataclasses.replace(req, local_vol=dataclasses.replace(req.local_vol, calibrated=grids))
+    md = W.max_dimension_for(req)
+    ora = O.price(req, max_dim=md)
+    res = api.price_put_leg(req, "fp32", max_dimension=md)
+    assert rel(res.pv, ora.pv) <= FP32_REL
+    assert rel(res.legs.down_and_in_put, ora.leg_put) <= 1e-3
+    f64 = api.price_put_leg(req, "fp64", max_dimension=md)
+    assert_fp64_parity(f64, ora)
+
+
 def test_crn_scenarios_alone_equal_in_mix():
     names, reqs = W.crn_scenarios(paths=1024, steps=126, years=0.5)
     md = max(W.max_dimension_for(r) for r in reqs)

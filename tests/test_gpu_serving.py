@@ -68,3 +68,38 @@ def test_stale_tickets_are_shed_and_budget_is_enforced():
     r = tiny.wait(tiny.submit(big, "fp64"))
     tiny.close()
     assert r.state == serving.FAILED and r.error.code == "memory_budget"
+
+
+def test_resident_surfaces_price_as_sent_grids():
+    # a surface copied to device memory once prices exactly as the grid sent per call
+    req = W.lv_book(paths=4096, steps=252, years=1.0)
+    res_req = api.resident(req)
+    for tier in ("fp64", "fp32"):
+        a = api.price_put_leg(req, tier, max_dimension=MD)
+        b = api.price_put_leg(res_req, tier, max_dimension=MD)
+        assert a.pv == b.pv and a.legs.down_and_in_put == b.legs.down_and_in_put
+        assert a.trace_events == b.trace_events
+
+
+def test_resident_market_tickets_alone_equal_in_the_mix():
+    import numpy as np
+    u = serving.Universe(n_assets=6, horizon=2.0)
+    u.make_resident()
+    rng = np.random.default_rng(3)
+    pool = [serving.sample_request(u, rng, path_choices=(1024, 2048), max_trades=3, max_bumps=4)
+            for _ in range(4)]
+    for reqs in pool:
+        for q in reqs:  # keep the horizons within the market's 2y tenors and short
+            q.trade.steps = min(q.trade.steps, 126)
+            q.trade.maturity_years = q.trade.steps / 252.0
+            q.config.variance_reduction = W.VarianceReduction("sobol", q.trade.steps * q.factors)
+    tiers = ["fp64", "fp32", "fp64", "fp32"]
+    srv = serving.Server(max_inflight=2, max_batch_requests=64)
+    ids = [srv.submit(srv.pack(reqs), tier) for reqs, tier in zip(pool, tiers)]
+    got = [srv.wait(t) for t in ids]
+    srv.close()
+    assert all(g.state == serving.DONE for g in got)
+    for reqs, tier, g in zip(pool, tiers, got):
+        alone = api.price_batch(reqs, tier, max_dimension=MD)
+        for a, b in zip(g.results, alone):
+            assert a.pv == b.pv and a.legs.down_and_in_put == b.legs.down_and_in_put
