@@ -25,22 +25,29 @@ sys.path.insert(0, ROOT)
 from riga_flow_amd import api, workloads as W  # noqa: E402
 
 # ---------------------------------------------------------------- roofline model
-# Algorithmic work in 32-bit lane operations (FP64 operations count one each:
-# MI355X issues FP64 at the full vector rate).  Per generated coordinate:
+# Algorithmic work of the code path each tier runs, in 32-bit lane operations
+# (an FP64 operation counts one: MI355X issues FP64 at the full vector rate).
+# Per generated coordinate (both tiers, integers identical):
 OWEN_ROUND = 27     # hash input xor 2, xorshift 4, 64-bit multiply 5, xorshift 4,
                     # top word of the 2nd multiply 4, bit logic 3, prefix 1, prefix*C3 4
+OWEN_ROUNDS = 24    # the top 8 of the 32 levels come from the per-dimension byte table
+OWEN_TABLE = 3      # top-byte table address and load
 PROBIT = 22         # rational (12) + divide (8) + tails (5% of log + sqrt + rational)
 UNIFORM = 4         # int->float, +0.5, scale, clamp
-LV_LOOKUP = 41      # log 15, knot bracket 6, divide 8, three lerps 12
-EULER = 22          # drift 4, increment 2, exp 15, multiply 1
+# Per scenario and factor:
+LV_LOOKUP = {"fp64": 41,  # log 15, knot bracket 6, divide 8, three lerps 12
+             "fp32": 12}  # knot position 2, clamp 2, index 3, address 1, slope-row lerps 4
+EULER = {"fp64": 22,      # drift 4, increment 2, exp 15, multiply 1
+         "fp32": 6}       # drift 3, increment 2, X += incr (log state: no exp)
 VALU_PEAK = 256 * 4 * 16 * 2.4e9  # CUs x SIMDs x lanes x clock = 3.93e13 lane-ops/s
 HBM_PEAK = 8.0e12
 
 
-def ops_per_path_step(F: int, S: int, nbits: int, local_vol: bool) -> float:
-    coord = 32 * OWEN_ROUND + 2 * nbits + UNIFORM + PROBIT
+def ops_per_path_step(F: int, S: int, nbits: int, local_vol: bool, tier: str) -> float:
+    """Lane operations of one path advanced one step for all S scenarios."""
+    coord = OWEN_ROUNDS * OWEN_ROUND + OWEN_TABLE + 2 * nbits + UNIFORM + PROBIT
     node = F * coord + F * (F + 1) + 3 * F + 2 * F  # coordinates, L z, bridge node, shock
-    scen = F * ((LV_LOOKUP if local_vol else 0) + EULER) + 9 * F + 5  # + worst/KO/coupon scan
+    scen = F * ((LV_LOOKUP[tier] if local_vol else 0) + EULER[tier]) + 9 * F + 5  # + worst/KO/coupon
     return node + S * scen
 
 
@@ -114,7 +121,9 @@ def main():
     ap.add_argument("--no-book", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--quick", action="store_true", help="smaller path counts (smoke)")
-    ap.add_argument("--ppt", type=int, default=0, help="paths per thread (0 = tuned default)")
+    ap.add_argument("--ppt", type=int, default=0, help="paths per thread (0 = autotuned)")
+    ap.add_argument("--waves", type=int, default=0, help="waves per workgroup (0 = autotuned)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (shape 1 x 4 waves)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the hot path shards by path range with a single "
@@ -127,10 +136,24 @@ def main():
     plain_paths = 262_144 if not args.quick else 16_384
     names, crn = W.crn_scenarios(paths=big, steps=756, years=3.0)
     md = max(W.max_dimension_for(r) for r in crn)
-    ppt = args.ppt or 1
-    fp32 = api.Plan(crn, "fp32", paths_per_thread=ppt, use_graph=True, max_dimension=md)
+    tuning = {}
+    if not args.no_tune and not (args.ppt and args.waves):
+        # the autotuner times every launch shape on these batches (outside the
+        # timed region) and the plans below pick up its choice
+        tuning["fp32_batched"] = api.autotune(crn, "fp32", use_graph=True, max_dimension=md, reps=1)
+        log(f"autotune fp32: {tuning['fp32_batched']['paths_per_thread']} paths/lane x "
+            f"{tuning['fp32_batched']['waves_per_block']} waves")
+    ppt, waves = args.ppt, args.waves
+    fp32 = api.Plan(crn, "fp32", paths_per_thread=ppt, waves_per_block=waves, use_graph=True,
+                    max_dimension=md)
     plain = W.lv_book(paths=plain_paths, steps=756)
-    fp64 = api.Plan([plain], "fp64", paths_per_thread=ppt, use_graph=True, max_dimension=md)
+    if not args.no_tune and not (args.ppt and args.waves):
+        tuning["fp64_plain"] = api.autotune([plain], "fp64", use_graph=True, max_dimension=md,
+                                            reps=1)
+        log(f"autotune fp64: {tuning['fp64_plain']['paths_per_thread']} paths/lane x "
+            f"{tuning['fp64_plain']['waves_per_block']} waves")
+    fp64 = api.Plan([plain], "fp64", paths_per_thread=ppt, waves_per_block=waves, use_graph=True,
+                    max_dimension=md)
 
     wall32, dev32, res32 = timed(fp32, args.steps, args.warmup)
     log(f"fp32 batched: {wall32 / args.steps * 1e3:.1f} ms/step")
@@ -165,9 +188,12 @@ def main():
         gerr[k + "_fp64"] = [b[i] for b in g64]
 
     nbits = (big).bit_length()
-    ops = ops_per_path_step(3, 10, nbits, True) / 10.0  # per (scenario) path-step
+    ops = ops_per_path_step(3, 10, nbits, True, "fp32") / 10.0  # per (scenario) path-step
     dev_s = sum(dev32) / len(dev32) / 1e3
     achieved = ops * ps32 / dev_s
+    ops64 = ops_per_path_step(3, 1, plain_paths.bit_length(), True, "fp64")
+    dev64_s = sum(dev64) / len(dev64) / 1e3
+    achieved64 = ops64 * ps64 / dev64_s
     out = {
         "metric": "path-steps/s FP32-batched vs FP64-plain at 1e-4 price err; p50/p99 latency",
         "value": value,
@@ -187,11 +213,19 @@ def main():
                         f"{big} shared Sobol paths; FP64 plain = 1 scenario, {plain_paths} paths",
             "paths": big, "scenarios": 10, "steps": 756, "underlyings": 3,
             "lv_grid": "64 tenors x 128 log-spot knots", "hip_graph": True,
-            "paths_per_thread": ppt,
+            "launch_shape": {"paths_per_thread": fp32.info.paths_per_thread,
+                             "waves_per_block": fp32.info.waves_per_block},
         },
         "fp64_plain": {"value": fp64_rate, "unit": "path-steps/s", "paths": plain_paths,
                        "ms_per_step": wall64 / args.steps * 1e3,
-                       "device_ms": sum(dev64) / len(dev64)},
+                       "device_ms": sum(dev64) / len(dev64),
+                       "launch_shape": {"paths_per_thread": fp64.info.paths_per_thread,
+                                        "waves_per_block": fp64.info.waves_per_block},
+                       "roofline": {"bound": "valu", "achieved": achieved64 / 1e12,
+                                    "peak": VALU_PEAK / 1e12, "unit": "Tops/s",
+                                    "frac": achieved64 / VALU_PEAK,
+                                    "kernel": "phoenix_kernel<double,3,1>",
+                                    "ops_per_path_step": ops64}},
         "fp32_batched_over_fp64_plain": value / fp64_rate,
         "fp32_vs_fp64": {"max_rel_pv_err": rel_pv, **gerr,
                          "base_pv_fp32": got[0].pv, "base_pv_fp64": ref[0].pv},
@@ -199,6 +233,7 @@ def main():
                      "unit": "Tops/s", "frac": achieved / VALU_PEAK, "traffic": None,
                      "kernel": "phoenix_kernel<float,3,10>", "device_ms": dev_s * 1e3,
                      "ops_per_path_step": ops},
+        "autotune": tuning,
     }
     fp32.close()
     fp64.close()

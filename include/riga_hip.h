@@ -129,13 +129,15 @@ typedef struct riga_path_record {
 
 /* Launch shape (chosen by the autotuner in riga_flow_amd/autotune.py or by hand). */
 typedef struct riga_launch_config {
-  int32_t paths_per_thread; /* 1, 2, 4 or 8 paths per lane */
+  int32_t paths_per_thread; /* 1, 2, 4 or 8 paths per lane; 0: the autotuned choice (riga_autotune)
+                               for the batch's shape, else 1 */
   int32_t use_graph;        /* 1: capture the launch sequence in a hipGraph at plan creation */
   int32_t owen_table;       /* 0: tabulate the top 8 Owen levels per dimension; -1: hash all 32 */
   int32_t normal_table;     /* 0: auto (share one [dim][path] normal table among same-seed trades
                                when they need 4x more normals than it holds); 1: force; -1: off */
   int32_t stream_ordered_alloc; /* 1: plan memory from the stream-ordered pool (serving) */
-  int32_t reserved[3];
+  int32_t waves_per_block;  /* 1, 2 or 4 waves of 64 lanes per workgroup; 0: autotuned, else 4 */
+  int32_t reserved[2];
 } riga_launch_config;
 
 /* A batch laid out in device memory once (surfaces, factors, schedules, Sobol
@@ -150,7 +152,32 @@ typedef struct riga_plan_info {
   int64_t device_bytes; /* device memory held by the plan */
   int32_t graph;
   int32_t normal_tables; /* shared [dim][path] normal tables the plan fills per launch */
+  int32_t paths_per_thread, waves_per_block; /* the launch shape in use */
 } riga_plan_info;
+
+/* One autotuner trial: a launch shape and its best device time (< 0: the shape
+ * does not fit, e.g. its LDS exceeds the workgroup limit). */
+typedef struct riga_tune_trial {
+  int32_t paths_per_thread, waves_per_block;
+  double device_ms;
+} riga_tune_trial;
+
+#define RIGA_TUNE_MAX_TRIALS 12
+typedef struct riga_tune_report {
+  int32_t n_trials;
+  int32_t best;          /* index of the fastest trial */
+  riga_tune_trial trials[RIGA_TUNE_MAX_TRIALS];
+} riga_tune_report;
+
+/* Autotuner (no reference counterpart: the reference launches a fixed 256-lane
+ * shape, gpu.rs:240-271).  Runs the batch once per launch shape -- paths per lane
+ * 1/2/4/8 x waves per workgroup 4/2/1 -- timing the path kernels with device
+ * events (best of `reps` after one warm-up), and keeps the fastest for the
+ * batch's shape signature in the context: plans created afterwards with
+ * paths_per_thread = 0 and waves_per_block = 0 use it.  cfg gives the other
+ * launch options the plans will use.  Results are the same for every shape. */
+int riga_autotune(riga_ctx *ctx, const riga_request *reqs, int n, int precision,
+                  const riga_launch_config *cfg, int reps, riga_tune_report *report, riga_error *err);
 
 /* ---- device (gpu.rs:490-512) ---- */
 int riga_hip_init(int device, riga_ctx **out, riga_error *err);
@@ -195,7 +222,7 @@ typedef struct riga_server_config {
   int32_t max_inflight;      /* plans running at once (0: 2) */
   int32_t max_batch_requests;/* requests per plan (0: 4096) */
   double max_queue_seconds;  /* shed tickets queued longer (0: never) */
-  int32_t paths_per_thread;  /* launch shape for every plan (0: 1) */
+  int32_t paths_per_thread;  /* launch shape for every plan (0: the autotuned one, else 1) */
   int32_t normal_table;      /* as riga_launch_config.normal_table */
   double max_batch_path_steps; /* path-steps one plan may hold beyond its first ticket (0: no cap) */
 } riga_server_config;

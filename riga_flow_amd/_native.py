@@ -97,13 +97,24 @@ RECORD_DTYPE = np.dtype([("ko_step", "<i4"), ("hit_ki", "<i4"), ("final_worst", 
 class LaunchConfig(C.Structure):
     _fields_ = [("paths_per_thread", C.c_int32), ("use_graph", C.c_int32),
                 ("owen_table", C.c_int32), ("normal_table", C.c_int32),
-                ("stream_ordered_alloc", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("stream_ordered_alloc", C.c_int32), ("waves_per_block", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class PlanInfo(C.Structure):
     _fields_ = [("n_requests", C.c_int32), ("n_items", C.c_int32), ("n_launches", C.c_int32),
                 ("blocks", C.c_int32), ("path_steps", C.c_double), ("lds_bytes", C.c_int64),
-                ("device_bytes", C.c_int64), ("graph", C.c_int32), ("normal_tables", C.c_int32)]
+                ("device_bytes", C.c_int64), ("graph", C.c_int32), ("normal_tables", C.c_int32),
+                ("paths_per_thread", C.c_int32), ("waves_per_block", C.c_int32)]
+
+
+class TuneTrial(C.Structure):
+    _fields_ = [("paths_per_thread", C.c_int32), ("waves_per_block", C.c_int32),
+                ("device_ms", C.c_double)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("n_trials", C.c_int32), ("best", C.c_int32), ("trials", TuneTrial * 12)]
 
 
 class ServerConfig(C.Structure):
@@ -135,7 +146,7 @@ EXPORTS = ("riga_hip_init", "riga_hip_destroy", "riga_hip_device_name", "riga_hi
            "riga_plan_wait", "riga_plan_run", "riga_plan_query", "riga_plan_destroy",
            "riga_server_create", "riga_server_submit", "riga_server_wait",
            "riga_server_stats_get", "riga_server_destroy",
-           "riga_surface_create", "riga_surface_destroy",
+           "riga_surface_create", "riga_surface_destroy", "riga_autotune",
            "riga_lv_calibrate", "riga_lv_sigma", "riga_debug_sobol")
 
 _lib = None
@@ -196,6 +207,9 @@ def lib():
                                             C.c_int32, _D, _D, _D, C.POINTER(Error)]
             L.riga_surface_create.argtypes = [C.c_void_p, C.POINTER(LvGrid), C.POINTER(C.c_void_p),
                                               C.POINTER(Error)]
+            L.riga_autotune.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int, C.c_int,
+                                        C.POINTER(LaunchConfig), C.c_int, C.POINTER(TuneReport),
+                                        C.POINTER(Error)]
             L.riga_surface_destroy.argtypes = [C.c_void_p]
             L.riga_surface_destroy.restype = None
             L.riga_lv_sigma.argtypes = [C.POINTER(LvGrid), C.c_double, C.c_double]

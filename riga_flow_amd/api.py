@@ -277,8 +277,9 @@ def result_to_json(res: PriceResult) -> dict:
 
 # ------------------------------------------------------------------ pricing
 def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",
-                paths_per_thread: int = 1, max_dimension: int = 512,
-                raw: bool = False, owen_table: bool = True, normal_table: int = 0):
+                paths_per_thread: int = 0, max_dimension: int = 512,
+                raw: bool = False, owen_table: bool = True, normal_table: int = 0,
+                waves_per_block: int = 0):
     """Price many requests in one launch; CRN scenarios share their Sobol points."""
     gpu_backend()
     for q in requests:
@@ -287,7 +288,8 @@ def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",
     n = len(packed)
     arr = (N.Request * n)(*[p.r for p in packed])
     res = (N.Result * n)()
-    cfg = N.LaunchConfig(paths_per_thread, 0, 0 if owen_table else -1, normal_table)
+    cfg = N.LaunchConfig(paths_per_thread, 0, 0 if owen_table else -1, normal_table, 0,
+                         waves_per_block)
     err = N.Error()
     dev = N.Device.get()
     N.check(N.lib().riga_price_batch(dev.handle, arr, n, PRECISIONS[precision], C.byref(cfg), res,
@@ -297,12 +299,34 @@ def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",
     return [_assemble(q, res[i]) for i, q in enumerate(requests)]
 
 
+def autotune(requests: Sequence[PriceRequest], precision: str = "fp64", use_graph: bool = True,
+             max_dimension: int = 512, owen_table: bool = True, normal_table: int = 0,
+             reps: int = 2) -> dict:
+    """Time the batch at every launch shape (paths per lane x waves per workgroup) and
+    keep the fastest for batches of its shape: plans and batches created afterwards
+    with the default (0) shape use it.  Returns the trials and the choice."""
+    gpu_backend()
+    packed = [_Packed(q, max_dimension) for q in requests]
+    n = len(packed)
+    arr = (N.Request * n)(*[p.r for p in packed])
+    cfg = N.LaunchConfig(0, 1 if use_graph else 0, 0 if owen_table else -1, normal_table)
+    rep = N.TuneReport()
+    err = N.Error()
+    N.check(N.lib().riga_autotune(N.Device.get().handle, arr, n, PRECISIONS[precision],
+                                  C.byref(cfg), reps, C.byref(rep), C.byref(err)), err)
+    trials = [(t.paths_per_thread, t.waves_per_block, t.device_ms)
+              for t in rep.trials[:rep.n_trials]]
+    b = trials[rep.best]
+    return {"paths_per_thread": b[0], "waves_per_block": b[1], "device_ms": b[2],
+            "trials": trials}
+
+
 class Plan:
     """A batch resident in device memory, launched any number of times."""
 
     def __init__(self, requests: Sequence[PriceRequest], precision: str = "fp64",
-                 paths_per_thread: int = 1, use_graph: bool = False, max_dimension: int = 512,
-                 owen_table: bool = True, normal_table: int = 0):
+                 paths_per_thread: int = 0, use_graph: bool = False, max_dimension: int = 512,
+                 owen_table: bool = True, normal_table: int = 0, waves_per_block: int = 0):
         gpu_backend()
         for q in requests:
             _check_as_of(q)
@@ -311,7 +335,7 @@ class Plan:
         n = len(packed)
         arr = (N.Request * n)(*[p.r for p in packed])
         cfg = N.LaunchConfig(paths_per_thread, 1 if use_graph else 0, 0 if owen_table else -1,
-                             normal_table)
+                             normal_table, 0, waves_per_block)
         err = N.Error()
         h = C.c_void_p()
         N.check(N.lib().riga_plan_create(N.Device.get().handle, arr, n, PRECISIONS[precision],

@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -69,22 +70,62 @@ struct DevBuf {
   ~DevBuf() { release(); }
 };
 
+// Pinned host blocks recycled across plans: hipHostFree waits for the whole
+// device, so a serving loop that freed each plan's download buffer would stall
+// behind every plan still running.
+struct PinnedPool {
+  std::mutex mu;
+  std::multimap<size_t, void*> idle;  // capacity -> block
+  hipError_t get(size_t bytes, void** p, size_t* cap) {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      auto it = idle.lower_bound(bytes);
+      if (it != idle.end() && it->first <= 4 * bytes + 65536) {
+        *p = it->second;
+        *cap = it->first;
+        idle.erase(it);
+        return hipSuccess;
+      }
+    }
+    size_t want = 4096;
+    while (want < bytes) want <<= 1;
+    hipError_t e = hipHostMalloc(p, want, hipHostMallocDefault);
+    *cap = e == hipSuccess ? want : 0;
+    return e;
+  }
+  void put(void* p, size_t cap) {
+    std::lock_guard<std::mutex> g(mu);
+    idle.emplace(cap, p);
+  }
+  ~PinnedPool() {
+    for (auto& kv : idle) (void)hipHostFree(kv.second);
+  }
+};
+
 struct HostBuf {
   void* p = nullptr;
   size_t cap = 0;
+  PinnedPool* pool = nullptr;  // plans borrow from the context's pool
   hipError_t ensure(size_t bytes) {
     if (bytes <= cap) return hipSuccess;
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    cap = 0;
+    drop();
+    if (pool) return pool->get(std::max(bytes, (size_t)4096), &p, &cap);
     size_t want = std::max(bytes, (size_t)4096);
     hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
     if (e == hipSuccess) cap = want;
     return e;
   }
-  ~HostBuf() {
-    if (p) (void)hipHostFree(p);
+  void drop() {
+    if (p) {
+      if (pool)
+        pool->put(p, cap);
+      else
+        (void)hipHostFree(p);
+    }
+    p = nullptr;
+    cap = 0;
   }
+  ~HostBuf() { drop(); }
 };
 
 }  // namespace
@@ -96,8 +137,11 @@ struct riga_ctx {
   std::string name, arch;
   size_t lds_limit = 65536;
   std::mutex mu;
+  std::mutex tune_mu;
+  std::map<std::string, std::pair<int, int>> tuned;  // shape signature -> (paths/thread, waves)
   DevBuf arena, items, block_item, partials, finals, final_base, records;
   HostBuf staging, finals_host;
+  PinnedPool pinned;  // plans' download buffers (plans end before their context)
 };
 
 // A calibrated grid kept in device memory: [log_s (n_x)][v (n_t x n_x)] doubles.
@@ -464,9 +508,31 @@ void fill_result(const Partial& fr, uint64_t paths, riga_result* out) {
 // ------------------------------------------------------------ plans
 struct LaunchPlan {
   int F = 0, smax = 1, blocks = 0;
+  int threads = kBlock;  // lanes per workgroup
   size_t lds = 0;
   size_t block_off = 0;
 };
+
+// Launch-shape signature of a batch: precision and, per (factors, scenario
+// bucket) launch, the item count and the log2 sizes of its paths and steps.
+// The autotuner's choices are kept per signature.
+std::string shape_signature(int precision, const std::map<std::pair<int, int>, std::vector<int>>& parts,
+                            const std::vector<Item>& items) {
+  std::string k = precision == RIGA_FP64 ? "d" : "f";
+  for (const auto& kv : parts) {
+    double paths = 0, steps = 0;
+    for (int i : kv.second) {
+      paths += (double)items[i].d.paths * items[i].d.S;
+      steps += items[i].d.steps;
+    }
+    steps /= (double)kv.second.size();
+    char buf[96];
+    snprintf(buf, sizeof buf, "|F%d.S%d.n%zu.p%d.t%d", kv.first.first, kv.first.second, kv.second.size(),
+             (int)std::ceil(std::log2(paths + 1.0)), (int)std::lround(std::log2(steps + 1.0)));
+    k += buf;
+  }
+  return k;
+}
 
 }  // namespace
 
@@ -477,6 +543,8 @@ struct riga_plan {
   bool records = false;
   bool graph = false;
   int n_requests = 0;
+  int ppt = 1, waves = 4;  // launch shape
+  std::string signature;
   std::vector<Item> items;
   std::vector<int64_t> final_base;
   int64_t final_slots = 0, record_slots = 0, partial_slots = 0;
@@ -549,6 +617,7 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
     L.smax = lp.smax;
     L.blocks = lp.blocks;
     L.lds = lp.lds;
+    L.threads = lp.threads;
     L.items = (const ItemDesc*)pl->items_d.p;
     L.block_item = (const int32_t*)pl->block_item.p + lp.block_off;
     L.arena = (const unsigned char*)pl->arena.p;
@@ -583,13 +652,19 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     set_err(err, "invalid_precision", "precision must be RIGA_FP64 or RIGA_FP32");
     return RIGA_EINVAL;
   }
-  const int ppt = cfg && cfg->paths_per_thread > 0 ? cfg->paths_per_thread : 1;
-  if (ppt != 1 && ppt != 2 && ppt != 4 && ppt != 8) {
+  int ppt = cfg && cfg->paths_per_thread > 0 ? cfg->paths_per_thread : 0;
+  int waves = cfg && cfg->waves_per_block > 0 ? cfg->waves_per_block : 0;
+  if (ppt != 0 && ppt != 1 && ppt != 2 && ppt != 4 && ppt != 8) {
     set_err(err, "invalid_launch_config", "paths_per_thread must be 1, 2, 4 or 8");
+    return RIGA_EINVAL;
+  }
+  if (waves != 0 && waves != 1 && waves != 2 && waves != 4) {
+    set_err(err, "invalid_launch_config", "waves_per_block must be 1, 2 or 4");
     return RIGA_EINVAL;
   }
   std::unique_ptr<riga_plan> pl(new riga_plan());
   pl->ctx = ctx;
+  pl->finals_host.pool = &ctx->pinned;
   pl->precision = precision;
   pl->records = records;
   pl->graph = cfg && cfg->use_graph != 0 && !records;
@@ -639,6 +714,23 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   const int real_size = precision == RIGA_FP64 ? 8 : 4;
   std::map<std::pair<int, int>, std::vector<int>> parts;
   for (size_t i = 0; i < items.size(); ++i) parts[{items[i].F, items[i].smax}].push_back((int)i);
+  pl->signature = shape_signature(precision, parts, items);
+  if (ppt == 0 || waves == 0) {  // the autotuned shape for this signature, else 1 path x 4 waves
+    int tp = 1, tw = 4;
+    {
+      std::lock_guard<std::mutex> g(ctx->tune_mu);
+      auto f = ctx->tuned.find(pl->signature);
+      if (f != ctx->tuned.end()) {
+        tp = f->second.first;
+        tw = f->second.second;
+      }
+    }
+    if (ppt == 0) ppt = tp;
+    if (waves == 0) waves = tw;
+  }
+  pl->ppt = ppt;
+  pl->waves = waves;
+  const int threads = 64 * waves;
   Arena arena;
   int sobol_rows = 1;
   for (Item& it : items)
@@ -680,7 +772,8 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       for (const LvGrid* g : items[i].surfaces) max_nx = std::max(max_nx, g->n_x());
       lv_words = std::max(lv_words, (int)items[i].surfaces.size() * (real_size == 4 ? 4 : 3) * max_nx);
     }
-    lp.lds = (size_t)stack_levels * lp.F * kBlock * real_size + (size_t)lv_words * real_size;
+    lp.threads = threads;
+    lp.lds = (size_t)stack_levels * lp.F * threads * real_size + (size_t)lv_words * real_size;
     if (lp.lds + (size_t)lp.smax * sizeof(BlockShared) > ctx->lds_limit) {
       set_err(err, "invalid_local_vol",
               "the batch needs " + std::to_string(lp.lds) + " bytes of LDS per workgroup (limit " +
@@ -693,7 +786,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       Item& it = items[i];
       ItemDesc& d = it.d;
       d.stack_levels = stack_levels;
-      d.paths_per_block = kBlock * ppt;
+      d.paths_per_block = threads * ppt;
       d.n_blocks = (int32_t)((d.paths + d.paths_per_block - 1) / d.paths_per_block);
       d.block_begin = blocks;
       blocks += d.n_blocks;
@@ -1055,6 +1148,8 @@ int riga_plan_query(const riga_plan* plan, riga_plan_info* info) {
   for (const auto& zt : plan->ztables) info->device_bytes += (int64_t)zt->buf.cap;
   info->normal_tables = (int32_t)plan->ztables.size();
   info->graph = plan->graph ? 1 : 0;
+  info->paths_per_thread = plan->ppt;
+  info->waves_per_block = plan->waves;
   return 0;
 }
 
@@ -1082,6 +1177,67 @@ int riga_replay_paths(riga_ctx* ctx, const riga_request* req, int precision,
     return RIGA_EINVAL;
   }
   return run_batch(ctx, req, 1, precision, nullptr, out, records, err);
+}
+
+int riga_autotune(riga_ctx* ctx, const riga_request* reqs, int n, int precision, const riga_launch_config* cfg,
+                  int reps, riga_tune_report* report, riga_error* err) {
+  clear_err(err);
+  if (!ctx || !reqs || n <= 0) {
+    set_err(err, "invalid_argument", "context and at least one request are required");
+    return RIGA_EINVAL;
+  }
+  reps = std::max(1, std::min(reps, 20));
+  riga_launch_config base = cfg ? *cfg : riga_launch_config{};
+  riga_tune_report rep{};
+  std::vector<riga_result> res((size_t)n);
+  std::string signature;
+  static const int kPpt[4] = {1, 2, 4, 8};
+  static const int kWv[3] = {4, 2, 1};
+  double best = 0.0;
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 3; ++b) {
+      riga_tune_trial& t = rep.trials[rep.n_trials++];
+      t.paths_per_thread = kPpt[a];
+      t.waves_per_block = kWv[b];
+      t.device_ms = -1.0;
+      riga_launch_config c = base;
+      c.paths_per_thread = kPpt[a];
+      c.waves_per_block = kWv[b];
+      riga_plan* pl = nullptr;
+      riga_error e{};
+      int rc = plan_create(ctx, reqs, n, precision, &c, false, &pl, &e);
+      if (rc != 0) {
+        if (rc == RIGA_EINVAL && strstr(e.message, "LDS")) continue;  // shape does not fit
+        if (err) *err = e;
+        return rc;
+      }
+      signature = pl->signature;
+      double ms = 1e300;
+      for (int r = 0; r <= reps && rc == 0; ++r) {
+        rc = riga_plan_run(pl, res.data(), &e);
+        if (r > 0 && rc == 0) ms = std::min(ms, res[0].device_ms);
+      }
+      riga_plan_destroy(pl);
+      if (rc != 0) {
+        if (err) *err = e;
+        return rc;
+      }
+      t.device_ms = ms;
+      if (best == 0.0 || ms < best) {
+        best = ms;
+        rep.best = rep.n_trials - 1;
+      }
+    }
+  if (best == 0.0) {
+    set_err(err, "invalid_launch_config", "no launch shape fits this batch");
+    return RIGA_EINVAL;
+  }
+  {
+    std::lock_guard<std::mutex> g(ctx->tune_mu);
+    ctx->tuned[signature] = {rep.trials[rep.best].paths_per_thread, rep.trials[rep.best].waves_per_block};
+  }
+  if (report) *report = rep;
+  return 0;
 }
 
 int riga_surface_create(riga_ctx* ctx, const riga_lv_grid* grid, riga_surface** out, riga_error* err) {

@@ -14,7 +14,8 @@ struct PhoenixLaunch {
   int F;       // factors (1..kMaxFactors)
   int smax;    // scenario bucket (1, 4 or 10)
   int blocks;  // grid size
-  size_t lds;  // dynamic LDS bytes (local-vol rows)
+  int threads; // lanes per workgroup: 64, 128 or 256
+  size_t lds;  // dynamic LDS bytes (bridge stack and local-vol rows)
   const ItemDesc* items;
   const int32_t* block_item;
   const unsigned char* arena;

@@ -205,13 +205,14 @@ __device__ __forceinline__ DD wave_sum(DD v) {
 template <typename Real, int F>
 struct BridgeStack {
   Real top[F];
-  Real* col;  // this lane's column: col[(level*F + f) * kBlock]
-  int sp;     // levels held in LDS
+  Real* col;   // this lane's column: col[(level*F + f) * stride]
+  int stride;  // lanes per workgroup
+  int sp;      // levels held in LDS
 
   __device__ __forceinline__ void push(const Real (&v)[F]) {
 #pragma unroll
     for (int f = 0; f < F; ++f) {
-      col[(sp * F + f) * kBlock] = top[f];
+      col[(sp * F + f) * stride] = top[f];
       top[f] = v[f];
     }
     ++sp;
@@ -219,7 +220,7 @@ struct BridgeStack {
   __device__ __forceinline__ void pop() {
     --sp;
 #pragma unroll
-    for (int f = 0; f < F; ++f) top[f] = col[(sp * F + f) * kBlock];
+    for (int f = 0; f < F; ++f) top[f] = col[(sp * F + f) * stride];
   }
 };
 
@@ -388,6 +389,8 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   const int S = it.S;
   const int steps = it.steps;
   const int tid = threadIdx.x;
+  const int nthreads = blockDim.x;  // 64, 128 or 256: the launch shape the autotuner picked
+  const int nwaves = nthreads >> 6;
   const int wave = tid >> 6;
   const int lane = tid & 63;
   const int chunk = (int)blockIdx.x - it.block_begin;
@@ -397,9 +400,9 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   constexpr bool kLogState = sizeof(Real) == 4;  // FP32 tier carries X = ln S
   constexpr int kSurfWords = kLogState ? 4 : 3;   // LDS words per knot and surface
   const bool lv_uniform = kLogState && (it.flags & kLvUniform) != 0;
-  // dynamic LDS: [bridge stack: (depth-1) x F x kBlock][local-vol rows: n_surf x kSurfWords x max_nx]
+  // dynamic LDS: [bridge stack: (depth-1) x F x lanes][local-vol rows: n_surf x kSurfWords x max_nx]
   Real* lds_stack = reinterpret_cast<Real*>(lds_dyn);
-  Real* lds_w = lds_stack + (size_t)it.stack_levels * F * kBlock;
+  Real* lds_w = lds_stack + (size_t)it.stack_levels * F * nthreads;
   const Real* lds = lds_w;
   const int nxs = it.max_nx;
   const StepInfo* stepinfo = at<StepInfo>(arena, it.off_steps);
@@ -409,16 +412,17 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   const LvSurface* surf = at<LvSurface>(arena, it.off_surf);
 
   for (int s = 0; s < SMAX; ++s) {
-    for (int i = tid; i < kWaves * kNumQ; i += kBlock) sh[s].acc[i / kNumQ][i % kNumQ] = DD{0.0, 0.0};
-    for (int i = tid; i < kWaves * kNumC; i += kBlock) sh[s].cnt[i / kNumC][i % kNumC] = 0;
+    for (int i = tid; i < kWaves * kNumQ; i += nthreads) sh[s].acc[i / kNumQ][i % kNumQ] = DD{0.0, 0.0};
+    for (int i = tid; i < kWaves * kNumC; i += nthreads) sh[s].cnt[i / kNumC][i % kNumC] = 0;
+    for (int i = tid; i < kWaves; i += nthreads) sh[s].wave_ko[i] = 0;
     if (tid == 0) sh[s].ev_n = 0;
   }
   __syncthreads();
 
   const int ppb = it.paths_per_block;
-  const int batches = ppb / kBlock;
+  const int batches = ppb / nthreads;
   for (int b = 0; b < batches; ++b) {
-    const uint64_t path_local = (uint64_t)chunk * ppb + (uint64_t)b * kBlock + tid;
+    const uint64_t path_local = (uint64_t)chunk * ppb + (uint64_t)b * nthreads + tid;
     const bool valid = path_local < it.paths;
     const uint64_t path = it.path_begin + path_local;
     const uint32_t n = (uint32_t)(path + 1);  // Sobol/Halton point index (qmc.rs:286, 121)
@@ -427,6 +431,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     // ---- path state
     BridgeStack<Real, F> bs;
     bs.col = lds_stack + tid;
+    bs.stride = nthreads;
     bs.sp = 0;
     Real wlo[F];
     Real spot[SMAX][F];
@@ -506,13 +511,13 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
             Real* dst = lds_w + (size_t)u * kSurfWords * nxs;
             if (lv_uniform) {
               float4* d4 = reinterpret_cast<float4*>(dst);
-              for (int i = tid; i < su.n_x; i += kBlock) {
+              for (int i = tid; i < su.n_x; i += nthreads) {
                 const int k = i + 1 < su.n_x ? i + 1 : i;
                 d4[i] = make_float4((float)v0[i], (float)(v0[k] - v0[i]), (float)v1[i],
                                     (float)(v1[k] - v1[i]));
               }
             } else {
-              for (int i = tid; i < su.n_x; i += kBlock) {
+              for (int i = tid; i < su.n_x; i += nthreads) {
                 dst[i] = (Real)xs[i];
                 dst[nxs + i] = (Real)v0[i];
                 dst[2 * nxs + i] = (Real)v1[i];
@@ -688,7 +693,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         __syncthreads();
         if (tid == 0) {
           int tot = sh[s].ev_n;
-          for (int w = 0; w < kWaves; ++w) tot += sh[s].wave_ko[w];
+          for (int w = 0; w < nwaves; ++w) tot += sh[s].wave_ko[w];
           sh[s].ev_n = tot < kTraceCap ? tot : kTraceCap;
         }
         __syncthreads();
@@ -702,14 +707,14 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     Partial* out = partials + it.partial_base + (int64_t)s * it.n_blocks + chunk;
     if (tid < kNumQ) {
       DD a = sh[s].acc[0][tid];
-      for (int w = 1; w < kWaves; ++w) a = dd_add(a, sh[s].acc[w][tid]);
+      for (int w = 1; w < nwaves; ++w) a = dd_add(a, sh[s].acc[w][tid]);
       out->q[tid] = a;
     } else if (tid >= 32 && tid < 32 + kNumC) {
       const int c = tid - 32;
       uint64_t a = 0;
-      for (int w = 0; w < kWaves; ++w) a += sh[s].cnt[w][c];
+      for (int w = 0; w < nwaves; ++w) a += sh[s].cnt[w][c];
       out->c[c] = a;
-    } else if (tid == 64) {
+    } else if (tid == 48) {  // within the first wave: every launch shape has it
       const int ne = trace ? sh[s].ev_n : 0;
       out->n_events = ne;
       for (int e = 0; e < ne; ++e) {

@@ -148,13 +148,15 @@ struct riga_server {
 
   riga_launch_config launch_config() const {
     riga_launch_config lc{};
-    lc.paths_per_thread = cfg.paths_per_thread > 0 ? cfg.paths_per_thread : 1;
+    lc.paths_per_thread = cfg.paths_per_thread;  // 0: the autotuned shape, if any
     lc.normal_table = cfg.normal_table;
     lc.stream_ordered_alloc = 1;
     return lc;
   }
 
-  // reap finished plans; called with mu held
+  // reap finished plans; called with mu held.  The plans themselves are
+  // destroyed by the caller with mu released.
+  std::vector<riga_plan*> done_plans;
   void reap() {
     for (size_t i = 0; i < inflight.size();) {
       InFlight& f = inflight[i];
@@ -181,7 +183,7 @@ struct riga_server {
           stats.failed_tickets += 1;
         }
       }
-      riga_plan_destroy(f.plan);
+      done_plans.push_back(f.plan);
       bytes_in_use -= f.bytes;
       inflight.erase(inflight.begin() + (long)i);
       cv_done.notify_all();
@@ -327,10 +329,20 @@ struct riga_server {
     return true;
   }
 
+  void destroy_done(std::unique_lock<std::mutex>& lock) {
+    if (done_plans.empty()) return;
+    std::vector<riga_plan*> d;
+    d.swap(done_plans);
+    lock.unlock();
+    for (riga_plan* p : d) riga_plan_destroy(p);
+    lock.lock();
+  }
+
   void loop() {
     std::unique_lock<std::mutex> lock(mu);
     while (!stop) {
       reap();
+      destroy_done(lock);
       shed_stale(now_s());
       bool progressed = estimate_new(lock);
       if (stop) break;
@@ -345,8 +357,9 @@ struct riga_server {
       }
     }
     // drain
-    while (!inflight.empty()) {
+    while (!inflight.empty() || !done_plans.empty()) {
       reap();
+      destroy_done(lock);
       if (!inflight.empty()) {
         lock.unlock();
         std::this_thread::sleep_for(std::chrono::microseconds(200));

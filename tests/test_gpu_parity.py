@@ -163,14 +163,32 @@ def test_owen_table_and_full_hash_agree():
     assert a.stats[0].output_count == b.stats[0].output_count
 
 
-@pytest.mark.parametrize("ppt", [2, 4])
-def test_paths_per_thread_changes_nothing_but_the_shape(ppt):
+@pytest.mark.parametrize("ppt,waves", [(2, 4), (4, 4), (1, 1), (8, 2)])
+def test_launch_shape_changes_nothing_but_the_shape(ppt, waves):
     req = W.lv_book(paths=3000, steps=126, years=0.5)
     md = W.max_dimension_for(req)
-    a = api.price_batch([req], "fp64", max_dimension=md)[0]
-    b = api.price_batch([req], "fp64", max_dimension=md, paths_per_thread=ppt)[0]
+    a = api.price_batch([req], "fp64", max_dimension=md, paths_per_thread=1, waves_per_block=4)[0]
+    b = api.price_batch([req], "fp64", max_dimension=md, paths_per_thread=ppt,
+                        waves_per_block=waves)[0]
     assert b.stats[0].output_count == a.stats[0].output_count
+    assert b.trace_events == a.trace_events  # first KO events in path order
     assert rel(b.pv, a.pv) <= 1e-14
+
+
+def test_autotuner_picks_a_shape_that_plans_then_use():
+    names, reqs = W.crn_scenarios(paths=8192, steps=126, years=0.5)
+    md = max(W.max_dimension_for(r) for r in reqs)
+    rep = api.autotune(reqs, "fp32", max_dimension=md, reps=1)
+    assert len(rep["trials"]) == 12 and rep["device_ms"] > 0
+    assert rep["device_ms"] == min(t[2] for t in rep["trials"] if t[2] > 0)
+    plan = api.Plan(reqs, "fp32", use_graph=True, max_dimension=md)
+    assert (plan.info.paths_per_thread, plan.info.waves_per_block) == \
+        (rep["paths_per_thread"], rep["waves_per_block"])
+    tuned = plan.run()
+    plan.close()
+    ref = api.price_batch(reqs, "fp32", max_dimension=md, paths_per_thread=1, waves_per_block=4)
+    for a, b in zip(tuned, ref):
+        assert rel(a.pv, b.pv) <= 1e-14
 
 
 def test_uneven_book_shared_normal_table_is_exact():
