@@ -284,6 +284,7 @@ struct Item {
   std::vector<ScenarioDesc> scen;
   std::vector<const LvGrid*> surfaces;  // distinct, in first-use order
   std::vector<const riga_surface*> surface_res;  // resident copy of each, or null
+  std::vector<float> wt32;                       // FP32 tier tenor weights [step][factor]
   std::vector<int> surface_factor;
   std::vector<uint32_t> halton;
   int depth = 1;
@@ -310,6 +311,7 @@ void plan_bridge(int T, double dt, Item* it) {
       BridgeNode bn{};
       bn.node = base;
       bn.sd = sqrt((double)(hi - mid) * (double)(mid - lo) / (double)(hi - lo) * dt);
+      bn.sd32 = (float)bn.sd;
       it->nodes.push_back(bn);
       st.back().base = base + (mid - lo);  // right child (mid, hi)
       st.push_back({mid, base + 1});       // left child (lo, mid)
@@ -395,7 +397,14 @@ bool build_item(const std::vector<Prep>& preps, Item* it, int precision, bool re
   d.ln_ko = log(d.ko);  // +inf when absent
   d.ln_cb = log(d.cb);
   for (int i = 0; i < F; ++i)
-    for (int j = 0; j < F; ++j) d.chol[i * kMaxFactors + j] = p0.chol[i * F + j];
+    for (int j = 0; j < F; ++j) {
+      d.chol[i * kMaxFactors + j] = p0.chol[i * F + j];
+      d.chol32[i * kMaxFactors + j] = (float)p0.chol[i * F + j];
+    }
+  d.neg_half_dt = (float)(-0.5 * dt);
+  d.ln_ko32 = (float)d.ln_ko;
+  d.ln_cb32 = (float)d.ln_cb;
+  d.root_sd32 = (float)d.root_sd;
   // scenarios
   it->scen.resize(S);
   for (int s = 0; s < S; ++s) {
@@ -406,7 +415,7 @@ bool build_item(const std::vector<Prep>& preps, Item* it, int precision, bool re
     for (int f = 0; f < F; ++f) {
       const double vol = q.vols[f];
       sc.spot[f] = q.spots[f];
-      sc.ln_spot[f] = log(q.spots[f]);
+      sc.ln_rel[f] = log(q.spots[f]) - log(r.initial_spots[f]);
       sc.rq[f] = q.risk_free_rate - q.dividend_yields[f];
       sc.drift[f] = (q.risk_free_rate - q.dividend_yields[f] - 0.5 * vol * vol) * dt;  // note.rs:139-140
       sc.vsd[f] = vol * sqrt(dt);
@@ -434,10 +443,12 @@ bool build_item(const std::vector<Prep>& preps, Item* it, int precision, bool re
   }
   if (!p0.lv.empty()) {
     it->lvb.resize((size_t)T * F);
+    it->wt32.resize((size_t)T * F);
     for (int step = 0; step < T; ++step)
       for (int f = 0; f < F; ++f) {
         LvBracket& b = it->lvb[(size_t)step * F + f];
         lv_bracket(*p0.lv[f], (double)step * dt, &b.i0, &b.i1, &b.wt);  // note.rs:136 t = step*dt
+        it->wt32[(size_t)step * F + f] = (float)b.wt;
       }
   }
   if (r.ladder == RIGA_VR_HALTON) it->halton = halton_tables(d.ladder_rows, r.seed);
@@ -825,10 +836,36 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
         su[u].factor = it.surface_factor[u];
         su[u].inv_dx = (double)(g.n_x() - 1) / (g.log_s[g.n_x() - 1] - g.log_s[0]);
         su[u].x0 = g.log_s[0];
+        su[u].rel_off = (d.ln_s0[su[u].factor] - g.log_s[0]) * su[u].inv_dx;
         uniform = uniform && evenly_spaced(g.log_s);
         max_nx = std::max(max_nx, g.n_x());
       }
       if (uniform) d.flags |= kLvUniform;
+      // FP32 tier Euler constants per (scenario, factor)
+      std::vector<SfF32> sf((size_t)d.S * d.F);
+      for (int sc = 0; sc < d.S; ++sc) {
+        const riga_request& q = *preps[it.members[sc]].r;
+        for (int f = 0; f < d.F; ++f) {
+          SfF32& c = sf[(size_t)sc * d.F + f];
+          memset(&c, 0, sizeof c);
+          const double rq = q.risk_free_rate - q.dividend_yields[f];
+          const int u = it.scen[sc].lv[f];
+          if (u >= 0) {
+            const LvSurface& g = su[u];
+            c.a = (float)g.inv_dx;
+            c.b = (float)g.rel_off;
+            c.c = (float)(rq * d.dt);
+            c.tmax = std::nextafter((float)(g.n_x - 1), 0.0f);
+            c.lds_off = u * 4 * max_nx * 4;
+          } else {
+            const double vol = q.vols[f];
+            c.a = (float)vol;
+            c.c = (float)((rq - 0.5 * vol * vol) * d.dt);
+          }
+        }
+      }
+      d.off_sf32 = arena.add(sf.data(), sf.size());
+      d.off_wt32 = arena.add(it.wt32.data(), it.wt32.size());
       d.off_surf = arena.add(su.data(), su.size());
       d.n_surf = (int32_t)su.size();
       d.max_nx = max_nx;

@@ -229,6 +229,15 @@ __device__ __forceinline__ const T* at(const unsigned char* arena, int64_t off) 
   return reinterpret_cast<const T*>(arena + off);
 }
 
+// Tier constants: the FP64 tier reads the host's doubles, the FP32 tier their
+// pre-rounded float copies (no conversion inside the path loop).
+__device__ __forceinline__ double chol_of(const ItemDesc& it, int k, double) { return it.chol[k]; }
+__device__ __forceinline__ float chol_of(const ItemDesc& it, int k, float) { return it.chol32[k]; }
+__device__ __forceinline__ double sd_of(const BridgeNode& b, double) { return b.sd; }
+__device__ __forceinline__ float sd_of(const BridgeNode& b, float) { return b.sd32; }
+__device__ __forceinline__ double root_sd_of(const ItemDesc& it, double) { return it.root_sd; }
+__device__ __forceinline__ float root_sd_of(const ItemDesc& it, float) { return it.root_sd32; }
+
 // Correlated normals of one bridge node: dims node*F + f (note.rs:89-98 with F
 // factors), y = L z with the reference's sequential row sums (correlation.rs:196-206).
 template <typename Real, int F>
@@ -244,7 +253,7 @@ __device__ __forceinline__ void node_normals(const ItemDesc& it, const unsigned 
     for (int i = 0; i < F; ++i) {
       Real acc = Real(0);
 #pragma unroll
-      for (int j = 0; j <= i; ++j) acc += (Real)it.chol[i * kMaxFactors + j] * z[j];
+      for (int j = 0; j <= i; ++j) acc += chol_of(it, i * kMaxFactors + j, Real(0)) * z[j];
       y[i] = acc;
     }
     return;
@@ -282,7 +291,7 @@ __device__ __forceinline__ void node_normals(const ItemDesc& it, const unsigned 
   for (int i = 0; i < F; ++i) {
     Real acc = Real(0);
 #pragma unroll
-    for (int j = 0; j <= i; ++j) acc += (Real)it.chol[i * kMaxFactors + j] * z[j];
+    for (int j = 0; j <= i; ++j) acc += chol_of(it, i * kMaxFactors + j, Real(0)) * z[j];
     y[i] = acc;
   }
 }
@@ -330,13 +339,14 @@ __device__ __forceinline__ Real lv_sigma(const Real* __restrict__ ls, int nx, Re
 
 // FP32 lookup against a surface with evenly spaced knots.  LDS holds, per knot
 // j, {r0[j], r0[j+1]-r0[j], r1[j], r1[j+1]-r1[j]} of the two bracketing tenor
-// rows, so one 16-byte read and three FMAs give the bilinear value; the knot
-// position comes from t = (x - x0)/dx clamped to [0, n_x-1] (below the first
-// knot and above the last the edge value holds, as in the general lookup).
+// rows, so one 16-byte read and three FMAs give the bilinear value.  The state
+// is xr = ln(S/S0); its knot position t = (xr + ln S0 - x0)/dx = xr/dx + rel_off
+// is clamped to [0, n_x-1] (below the first knot and above the last the edge
+// value holds, as in the general lookup).
 __device__ __forceinline__ float lv_sigma_uniform(const float4* __restrict__ rows, int nx,
-                                                  float inv_dx, float neg_x0_inv_dx, float wt,
-                                                  float x) {
-  float t = fmaf(x, inv_dx, neg_x0_inv_dx);
+                                                  float inv_dx, float rel_off, float wt,
+                                                  float xr) {
+  float t = fmaf(xr, inv_dx, rel_off);
   t = fminf(fmaxf(t, 0.0f), (float)(nx - 1));
   int j = (int)t;
   j = j < nx - 2 ? j : nx - 2;
@@ -383,6 +393,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
                                                           PathRecordDev* __restrict__ records) {
   extern __shared__ __align__(16) unsigned char lds_dyn[];
   __shared__ BlockShared sh[SMAX];
+  __shared__ SfF32 sf_lds[sizeof(Real) == 4 ? SMAX * F : 1];  // FP32 Euler constants, read per step
 
   const int item = block_item[blockIdx.x];
   const ItemDesc& it = items[item];
@@ -397,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   const bool local_vol = (it.flags & kLocalVol) != 0;
   const bool trace = (it.flags & kTrace) != 0;
   const bool want_records = (it.flags & kRecords) != 0 && records != nullptr;
-  constexpr bool kLogState = sizeof(Real) == 4;  // FP32 tier carries X = ln S
+  constexpr bool kLogState = sizeof(Real) == 4;  // FP32 tier carries X = ln(S / S0)
   constexpr int kSurfWords = kLogState ? 4 : 3;   // LDS words per knot and surface
   const bool lv_uniform = kLogState && (it.flags & kLvUniform) != 0;
   // dynamic LDS: [bridge stack: (depth-1) x F x lanes][local-vol rows: n_surf x kSurfWords x max_nx]
@@ -416,6 +427,10 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     for (int i = tid; i < kWaves * kNumC; i += nthreads) sh[s].cnt[i / kNumC][i % kNumC] = 0;
     for (int i = tid; i < kWaves; i += nthreads) sh[s].wave_ko[i] = 0;
     if (tid == 0) sh[s].ev_n = 0;
+  }
+  if constexpr (sizeof(Real) == 4) {
+    const SfF32* sfg = at<SfF32>(arena, it.off_sf32);
+    for (int i = tid; i < S * F; i += nthreads) sf_lds[i] = sfg[i];
   }
   __syncthreads();
 
@@ -444,7 +459,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
       node_normals<Real, F>(it, arena, n, gray, 0, y);
 #pragma unroll
       for (int f = 0; f < F; ++f) {
-        bs.top[f] = y[f] * (Real)it.root_sd;  // W[T] = y_0 sqrt(T dt), qmc.rs:165
+        bs.top[f] = y[f] * root_sd_of(it, Real(0));  // W[T] = y_0 sqrt(T dt), qmc.rs:165
         wlo[f] = Real(0);
       }
     }
@@ -452,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     for (int s = 0; s < SMAX; ++s) {
 #pragma unroll
       for (int f = 0; f < F; ++f)
-        spot[s][f] = kLogState ? (Real)scen[s < S ? s : 0].ln_spot[f] : (Real)scen[s < S ? s : 0].spot[f];
+        spot[s][f] = kLogState ? (Real)scen[s < S ? s : 0].ln_rel[f] : (Real)scen[s < S ? s : 0].spot[f];
       ko_step[s] = -1;
       accrual[s] = unpaid[s] = cpv[s] = mpv[s] = Real(0);
       received[s] = false;
@@ -479,7 +494,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
           node_normals<Real, F>(it, arena, n, gray, bn.node, y);
 #pragma unroll
           for (int f = 0; f < F; ++f)
-            wmid[f] = Real(0.5) * (wlo[f] + bs.top[f]) + y[f] * (Real)bn.sd;  // qmc.rs:185-187
+            wmid[f] = Real(0.5) * (wlo[f] + bs.top[f]) + y[f] * sd_of(bn, Real(0));  // qmc.rs:185-187
           if (d + 1 < si.n_desc) {
             bs.push(wmid);
           } else {
@@ -488,9 +503,11 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
           }
         }
       }
+      Real dw[F];  // Brownian increment W[t+1] - W[t]
 #pragma unroll
       for (int f = 0; f < F; ++f) {
-        shock[f] = (wnext[f] - wlo[f]) * (Real)it.inv_sqrt_dt;  // qmc.rs:169, note.rs:101-105
+        dw[f] = wnext[f] - wlo[f];
+        shock[f] = dw[f] * (Real)it.inv_sqrt_dt;  // qmc.rs:169, note.rs:101-105
         wlo[f] = wnext[f];
       }
 
@@ -534,6 +551,84 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
       const bool period_end = (si.flags & 1) != 0;
       const bool new_period = (si.flags & 4) != 0;
       const bool fixed_now = si.period == 0 && (it.flags & kFixedFirst) != 0;
+      if constexpr (kLogState) {
+        // ---- FP32 tier: X = ln(S/S0) per (scenario, factor), constants from
+        // SfF32, the local-vol knot found by one FMA on evenly spaced knots.
+        // The Phoenix scan below is note.rs:143-218 with the KO'd-path skip
+        // turned into masks (values of a KO'd scenario are never read again).
+        // staged in LDS and read per use; the opaque zero keeps the compiler from
+        // hoisting these reads out of the step loop (which would pin S*F records
+        // in registers for the whole path)
+        int opaque = 0;
+        asm volatile("" : "+s"(opaque));
+        const SfF32* sf = sf_lds + opaque;
+        float wt[F];
+        if (local_vol) {
+#pragma unroll
+          for (int f = 0; f < F; ++f) wt[f] = at<float>(arena, it.off_wt32)[step * F + f];
+        }
+        const float inc = fixed_now ? 0.0f : (float)si.accrual_inc;
+        const unsigned char* lv_rows = reinterpret_cast<const unsigned char*>(lds);
+#pragma unroll
+        for (int s = 0; s < SMAX; ++s) {
+          if (s >= S) continue;
+#pragma unroll
+          for (int f = 0; f < F; ++f) {
+            const SfF32 c = sf[s * F + f];
+            float incr;
+            if (lv_uniform) {
+              float t = fmaf(spot[s][f], c.a, c.b);
+              t = fminf(fmaxf(t, 0.0f), c.tmax);
+              const int j = (int)t;
+              const float wx = t - (float)j;
+              const float4 e = *reinterpret_cast<const float4*>(lv_rows + c.lds_off + j * 16);
+              const float v0 = fmaf(wx, e.y, e.x);
+              const float v1 = fmaf(wx, e.w, e.z);
+              const float vol = fmaf(wt[f], v1 - v0, v0);
+              incr = fmaf(vol, dw[f], fmaf(vol * vol, it.neg_half_dt, c.c));
+            } else if (local_vol) {  // knots not evenly spaced: bracketing search
+              const int u = scen[s].lv[f];
+              const LvSurface& su = surf[u];
+              const float vol = lv_sigma<float, true>(lds + (size_t)u * kSurfWords * nxs, su.n_x, (float)su.inv_dx,
+                                                      wt[f], spot[s][f] + (float)it.ln_s0[f]);
+              incr = fmaf(vol, dw[f], fmaf(vol * vol, it.neg_half_dt, c.c));
+            } else {
+              incr = fmaf(c.a, dw[f], c.c);
+            }
+            spot[s][f] += incr;
+          }
+          float w = spot[s][0];
+#pragma unroll
+          for (int f = 1; f < F; ++f) w = fminf(w, spot[s][f]);
+          worst_last[s] = w;
+          const bool alive = ko_step[s] < 0;
+          if (new_period) accrual[s] = 0.0f;
+          accrual[s] += w >= it.ln_cb32 ? inc : 0.0f;
+          const bool ko_now = alive && call_active && w >= it.ln_ko32;
+          if (ko_now) {  // note.rs:166-195
+            ko_step[s] = step;
+            const float due = fixed_now ? (float)it.coupon_rate : accrual[s];
+            const float carry = due > 0.0f ? unpaid[s] : 0.0f;
+            if (due + carry > 0.0f) {
+              cpv[s] += due * (float)si.df;
+              mpv[s] += carry * (float)si.df;
+              received[s] = true;
+            }
+          }
+          if (period_end && alive && !ko_now) {  // note.rs:197-218
+            const float due = fixed_now ? (float)it.coupon_rate : accrual[s];
+            if (due > 0.0f) {
+              cpv[s] += due * (float)si.df;
+              mpv[s] += unpaid[s] * (float)si.df;
+              received[s] = true;
+              unpaid[s] = 0.0f;
+            } else if (it.flags & kMemoryCoupon) {
+              unpaid[s] += (float)it.coupon_rate;
+            }
+          }
+        }
+        continue;
+      }
 #pragma unroll
       for (int s = 0; s < SMAX; ++s) {
         if (s >= S) continue;
@@ -551,11 +646,11 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
             if constexpr (kLogState) {
               if (lv_uniform)
                 vol = lv_sigma_uniform(reinterpret_cast<const float4*>(lds + (size_t)u * kSurfWords * nxs),
-                                       su.n_x, (float)su.inv_dx, (float)(-su.x0 * su.inv_dx),
+                                       su.n_x, (float)su.inv_dx, (float)su.rel_off,
                                        (float)lvb[step * F + f].wt, spot[s][f]);
               else
                 vol = lv_sigma<Real, true>(lds + (size_t)u * kSurfWords * nxs, su.n_x, (Real)su.inv_dx,
-                                           (Real)lvb[step * F + f].wt, spot[s][f]);
+                                           (Real)lvb[step * F + f].wt, spot[s][f] + (Real)it.ln_s0[f]);
             } else {
               vol = lv_sigma<Real, false>(lds + (size_t)u * kSurfWords * nxs, su.n_x, (Real)su.inv_dx,
                                           (Real)lvb[step * F + f].wt, spot[s][f]);
@@ -573,9 +668,9 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         // ---- worst-of (note.rs:143-145); the FP32 tier keeps ln(worst)
         Real worst;
         if constexpr (kLogState) {
-          worst = spot[s][0] - (Real)it.ln_s0[0];
+          worst = spot[s][0];
 #pragma unroll
-          for (int f = 1; f < F; ++f) worst = fminf(worst, spot[s][f] - (Real)it.ln_s0[f]);
+          for (int f = 1; f < F; ++f) worst = fminf(worst, spot[s][f]);
         } else {
           worst = spot[s][0] / (Real)it.initial_spot[0];
 #pragma unroll

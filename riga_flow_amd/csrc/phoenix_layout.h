@@ -45,7 +45,7 @@ struct StepInfo {
 struct BridgeNode {
   double sd;
   int32_t node;
-  int32_t pad;
+  float sd32;  // FP32 tier copy of sd
 };
 
 // Local-vol time bracket per (step, factor): riga-market/src/lib.rs:324-339
@@ -66,8 +66,19 @@ struct LvSurface {
   double inv_dx;     // (n_x - 1) / (log_s[n_x-1] - log_s[0]): bracket guess (exact knot
                      // position when the item has kLvUniform)
   double x0;         // log_s[0]
+  double rel_off;    // (ln initial_spot[factor] - x0) * inv_dx: knot position of ln(S/S0) = 0
   const double* dev_x;  // resident surface: knots and vols in device memory
   const double* dev_v;  // (null: at off_x / off_v in the arena)
+};
+
+// FP32 tier, per (scenario, factor): the Euler step's constants.
+struct SfF32 {
+  float a;          // local vol: 1/dx of the surface's evenly spaced knots; flat vol: vol
+  float b;          // local vol: knot position of ln(S/S0) = 0
+  float c;          // local vol: (rate - q) dt; flat vol: (rate - q - vol^2/2) dt
+  float tmax;       // largest float below n_x - 1 (knot positions are clamped to [0, tmax])
+  int32_t lds_off;  // byte offset of the surface's slope rows in the local-vol LDS area
+  int32_t pad[3];
 };
 
 struct ScenarioDesc {
@@ -75,7 +86,7 @@ struct ScenarioDesc {
   double rq[kMaxFactors];     // rate - dividend_yield[f]
   double drift[kMaxFactors];  // flat vol: (rate - q - 0.5 vol^2) dt
   double vsd[kMaxFactors];    // flat vol: vol * sqrt(dt)
-  double ln_spot[kMaxFactors];// FP32 log-state tier: ln(spot)
+  double ln_rel[kMaxFactors]; // FP32 log-state tier: ln(spot / initial_spot)
   int32_t lv[kMaxFactors];    // surface index per factor (local vol)
 };
 
@@ -109,6 +120,11 @@ struct ItemDesc {
   int32_t paths_per_block;
   int32_t ladder_rows;   // ladder rows uploaded (min(ladder_dims, F*steps))
   int64_t off_owen;      // uint8: [F*steps][256] scrambled top byte per raw dimension
+  // FP32 tier: constants pre-rounded on the host, so the path loop converts nothing
+  float chol32[kMaxFactors * kMaxFactors];
+  float neg_half_dt, ln_ko32, ln_cb32, root_sd32;
+  int64_t off_sf32;      // SfF32[S * F]
+  int64_t off_wt32;      // float[steps * F]: local-vol tenor weight per (step, factor)
   uint64_t ztable;       // device address of the shared normal table (Real[dims][zt_stride])
   int64_t zt_stride;     // paths per table row
 };
