@@ -30,8 +30,8 @@ from riga_flow_amd import api, workloads as W  # noqa: E402
 # Per generated coordinate (both tiers, integers identical):
 OWEN_ROUND = 27     # hash input xor 2, xorshift 4, 64-bit multiply 5, xorshift 4,
                     # top word of the 2nd multiply 4, bit logic 3, prefix 1, prefix*C3 4
-OWEN_ROUNDS = 24    # the top 8 of the 32 levels come from the per-dimension byte table
-OWEN_TABLE = 3      # top-byte table address and load
+OWEN_ROUNDS = 20    # the top 12 of the 32 levels come from the per-dimension prefix table
+OWEN_TABLE = 3      # prefix table address and load
 PROBIT = 22         # rational (12) + divide (8) + tails (5% of log + sqrt + rational)
 UNIFORM = 4         # int->float, +0.5, scale, clamp
 # Per scenario and factor:

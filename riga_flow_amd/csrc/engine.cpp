@@ -363,6 +363,12 @@ bool build_item(const std::vector<Prep>& preps, Item* it, int precision, bool re
     return false;
   }
   plan_schedule(r, dt, it);
+  const int64_t periods = r.coupon_periods < 0 ? 9 : std::max<int64_t>(r.coupon_periods, 1);
+  if (precision == RIGA_FP32 && periods > 2047) {  // the FP32 state word counts unpaid coupons in 11 bits
+    e->code = "invalid_schedule";
+    e->message = "the FP32 tier supports at most 2047 coupon periods";
+    return false;
+  }
   ItemDesc& d = it->d;
   memset(&d, 0, sizeof d);
   d.F = F;
@@ -453,7 +459,6 @@ bool build_item(const std::vector<Prep>& preps, Item* it, int precision, bool re
   }
   if (r.ladder == RIGA_VR_HALTON) it->halton = halton_tables(d.ladder_rows, r.seed);
   it->cost = (double)T * (double)F * (double)S * (double)r.paths;
-  (void)precision;
   return true;
 }
 
@@ -569,10 +574,17 @@ struct riga_plan {
   struct NormalTable {
     uint64_t seed = 0;
     int dims = 0, nbits = 0;
-    int64_t paths = 0, owen_off = -1;
+    int64_t paths = 0;
+    const uint16_t* owen = nullptr;
     DevBuf buf;
   };
   std::vector<std::unique_ptr<NormalTable>> ztables;
+  struct OwenTable {
+    uint64_t seed = 0;
+    int dims = 0;
+    DevBuf buf;
+  };
+  std::vector<std::unique_ptr<OwenTable>> owen_tabs;
   int64_t off_dirs = 0;
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -586,6 +598,7 @@ struct riga_plan {
     // buffers go back to the pool on this stream before it is destroyed
     for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d}) b->release();
     for (auto& zt : ztables) zt->buf.release();
+    for (auto& ot : owen_tabs) ot->buf.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
   void bind_stream() {
@@ -614,7 +627,7 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
   for (const auto& zt : pl->ztables) {
     const unsigned char* a = (const unsigned char*)pl->arena.p;
     const uint32_t* dirs = (const uint32_t*)(a + pl->off_dirs);
-    const uint8_t* ot = zt->owen_off >= 0 ? (const uint8_t*)(a + zt->owen_off) : nullptr;
+    const uint16_t* ot = zt->owen;
     e = pl->precision == RIGA_FP64
             ? launch_normal_table_f64(dirs, ot, zt->seed, zt->dims, (int)zt->paths, zt->nbits, (double*)zt->buf.p,
                                       pl->stream)
@@ -748,20 +761,16 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     if (!(it.d.flags & kHalton)) sobol_rows = std::max(sobol_rows, it.d.ladder_rows);
   const std::vector<uint32_t>& dirs = sobol_directions(sobol_rows);
   const int64_t off_dirs = arena.add(dirs.data(), (size_t)sobol_rows * 32);
-  // one top-byte Owen table per seed, covering the largest raw dimension used
+  // one Owen prefix table per seed (built on the device at creation), covering
+  // the largest raw dimension used
   const bool owen_table = !(cfg && cfg->owen_table < 0);
   std::map<uint64_t, int> owen_dims;
-  std::map<uint64_t, int64_t> owen_off;
   if (owen_table) {
     for (Item& it : items)
       if (!(it.d.flags & kHalton)) {
         int& m = owen_dims[it.d.seed];
         m = std::max(m, it.d.F * it.d.steps);
       }
-    for (auto& kv : owen_dims) {
-      const std::vector<uint8_t>* tab = owen_top8_table(kv.first, kv.second);
-      owen_off[kv.first] = arena.add(tab->data(), tab->size());
-    }
   }
   pl->final_base.resize(items.size());
   for (size_t i = 0; i < items.size(); ++i) {
@@ -807,10 +816,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       d.record_base = pl->record_slots;
       if (d.flags & kRecords) pl->record_slots += (int64_t)d.S * (int64_t)d.paths;
       d.off_dirs = off_dirs;
-      if (owen_table && !(d.flags & kHalton)) {
-        d.flags |= kOwenTable;
-        d.off_owen = owen_off[d.seed];
-      }
+      if (owen_table && !(d.flags & kHalton)) d.flags |= kOwenTable;  // table address set below
       d.off_nodes = arena.add(it.nodes.data(), it.nodes.size());
       d.off_steps = arena.add(it.steps.data(), it.steps.size());
       d.off_lvb = arena.add(it.lvb.data(), it.lvb.size());
@@ -880,6 +886,23 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   const bool stream_ordered = cfg && cfg->stream_ordered_alloc != 0;
   if (stream_ordered) pl->bind_stream();
   double ztable_bytes = 0.0;
+  std::map<uint64_t, const uint16_t*> owen_ptr;
+  for (auto& kv : owen_dims) {
+    std::unique_ptr<riga_plan::OwenTable> ot(new riga_plan::OwenTable());
+    ot->seed = kv.first;
+    ot->dims = kv.second;
+    if (stream_ordered) ot->buf.stream = pl->stream;
+    const size_t bytes = (size_t)kv.second << 13;  // 4096 x u16 per dimension
+    ztable_bytes += (double)bytes;
+    if (!estimate) {
+      HIP_TRY(ot->buf.ensure(bytes), "owen table alloc");
+      HIP_TRY(launch_owen_table(kv.first, kv.second, (uint16_t*)ot->buf.p, pl->stream), "owen table build");
+    }
+    owen_ptr[kv.first] = (const uint16_t*)ot->buf.p;
+    pl->owen_tabs.push_back(std::move(ot));
+  }
+  for (Item& it : items)
+    if (it.d.flags & kOwenTable) it.d.owen_tab = owen_ptr[it.d.seed];
   // shared normal tables: same-seed Sobol trades without wrapped direction rows
   // read their normals from one [dim][path] table when they need many more
   // normals than it holds (an uneven book), instead of hashing them per trade
@@ -910,7 +933,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       zt->dims = dims;
       zt->paths = (int64_t)paths;
       zt->nbits = 64 - __builtin_clzll(paths);
-      zt->owen_off = owen_table ? owen_off[kv.first] : -1;
+      zt->owen = owen_table ? owen_ptr[kv.first] : nullptr;
       if (stream_ordered) zt->buf.stream = pl->stream;
       ztable_bytes += bytes;
       if (!estimate) HIP_TRY(zt->buf.ensure((size_t)bytes), "normal table alloc");
@@ -930,6 +953,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
                           (double)pl->final_slots * sizeof(Partial) + pl->final_base.size() * 8.0 +
                           (double)pl->record_slots * sizeof(PathRecordDev) + 8 * 4096.0);
     for (auto& zt : pl->ztables) zt->buf.stream = nullptr;
+    for (auto& ot : pl->owen_tabs) ot->buf.stream = nullptr;
     return 0;
   }
 
@@ -1183,6 +1207,7 @@ int riga_plan_query(const riga_plan* plan, riga_plan_info* info) {
   info->device_bytes = (int64_t)(plan->arena.cap + plan->items_d.cap + plan->block_item.cap + plan->partials.cap +
                                  plan->finals.cap + plan->final_base_d.cap + plan->records_d.cap);
   for (const auto& zt : plan->ztables) info->device_bytes += (int64_t)zt->buf.cap;
+  for (const auto& ot : plan->owen_tabs) info->device_bytes += (int64_t)ot->buf.cap;
   info->normal_tables = (int32_t)plan->ztables.size();
   info->graph = plan->graph ? 1 : 0;
   info->paths_per_thread = plan->ppt;

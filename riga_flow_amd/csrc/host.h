@@ -37,10 +37,6 @@ const std::vector<uint32_t>& sobol_directions(int dims);
 // Halton: first `dims` primes and seed-keyed digit permutations, packed as
 // [bases(dims)][perm offsets(dims)][perms...] in uint32 words.
 std::vector<uint32_t> halton_tables(int dims, uint64_t seed);
-// Owen scramble of the top byte per raw dimension: table[d*256 + v] = the top
-// byte of owen_scramble(v << 24, d, seed) (qmc.rs:293-308), for d < dims.
-// Cached per (seed, dims); the pointer stays valid for the process lifetime.
-const std::vector<uint8_t>* owen_top8_table(uint64_t seed, int dims);
 
 // ---- request validation (crates/riga-engine/src/lib.rs:209-321, F factors)
 bool validate_request(const riga_request& r, Error* err);

@@ -27,10 +27,13 @@ hipError_t launch_phoenix_f64(const PhoenixLaunch& l, hipStream_t stream);
 hipError_t launch_phoenix_f32(const PhoenixLaunch& l, hipStream_t stream);
 
 // Shared normal table z[dim][path] (Real = double for FP64, float for FP32).
-hipError_t launch_normal_table_f64(const uint32_t* dirs, const uint8_t* owen_tab, uint64_t seed,
+hipError_t launch_normal_table_f64(const uint32_t* dirs, const uint16_t* owen_tab, uint64_t seed,
                                    int dims, int n_paths, int nbits, double* out, hipStream_t stream);
-hipError_t launch_normal_table_f32(const uint32_t* dirs, const uint8_t* owen_tab, uint64_t seed,
+hipError_t launch_normal_table_f32(const uint32_t* dirs, const uint16_t* owen_tab, uint64_t seed,
                                    int dims, int n_paths, int nbits, float* out, hipStream_t stream);
+
+// Owen prefix table: out[dim][v] = scrambled top 12 bits of v << 20, dims [0, dims).
+hipError_t launch_owen_table(uint64_t seed, int dims, uint16_t* out, hipStream_t stream);
 
 hipError_t launch_finalize(const ItemDesc* items, int n_items, const Partial* partials,
                            Partial* finals, const int64_t* final_base, hipStream_t stream);

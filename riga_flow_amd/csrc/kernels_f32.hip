@@ -34,7 +34,7 @@ hipError_t launch_phoenix_f32(const PhoenixLaunch& l, hipStream_t stream) {
   }
 }
 
-hipError_t launch_normal_table_f32(const uint32_t* dirs, const uint8_t* owen_tab, uint64_t seed,
+hipError_t launch_normal_table_f32(const uint32_t* dirs, const uint16_t* owen_tab, uint64_t seed,
                                    int dims, int n_paths, int nbits, float* out, hipStream_t stream) {
   dim3 grid((n_paths + 255) / 256, dims);
   hipLaunchKernelGGL((normal_table_kernel<float>), grid, dim3(256), 0, stream, dirs, owen_tab, seed,

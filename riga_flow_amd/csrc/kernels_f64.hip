@@ -131,7 +131,26 @@ hipError_t launch_debug_sobol(const uint32_t* dirs, int ladder_dims, uint64_t se
   return hipGetLastError();
 }
 
-hipError_t launch_normal_table_f64(const uint32_t* dirs, const uint8_t* owen_tab, uint64_t seed,
+// tab[dim][v] = top kOwenTableBits bits of owen_scramble(v << (32-L), key(dim)).
+__global__ __launch_bounds__(256) void owen_table_kernel(uint64_t seed, int dims, uint16_t* __restrict__ out) {
+  const uint32_t v = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t dim = blockIdx.y;
+  if (v >= (1u << kOwenTableBits) || (int)dim >= dims) return;
+  const uint64_t key = seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
+  const uint32_t x = v << (32 - kOwenTableBits);
+  uint32_t prefix = 0;
+  uint64_t P = 0;
+  for (int bit = 31; bit >= 32 - kOwenTableBits; --bit) owen_level(x, key, bit, prefix, P);
+  out[(size_t)dim * (1u << kOwenTableBits) + v] = (uint16_t)prefix;
+}
+
+hipError_t launch_owen_table(uint64_t seed, int dims, uint16_t* out, hipStream_t stream) {
+  dim3 grid((1u << kOwenTableBits) / 256, dims);
+  hipLaunchKernelGGL(owen_table_kernel, grid, dim3(256), 0, stream, seed, dims, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_normal_table_f64(const uint32_t* dirs, const uint16_t* owen_tab, uint64_t seed,
                                    int dims, int n_paths, int nbits, double* out, hipStream_t stream) {
   dim3 grid((n_paths + 255) / 256, dims);
   hipLaunchKernelGGL((normal_table_kernel<double>), grid, dim3(256), 0, stream, dirs, owen_tab, seed,

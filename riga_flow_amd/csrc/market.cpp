@@ -382,47 +382,7 @@ std::vector<uint32_t> halton_tables(int dims, uint64_t seed) {
   return out;
 }
 
-namespace {
 
-uint64_t splitmix_finalise(uint64_t z) {  // mix64, qmc.rs:310-314
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-
-std::mutex g_owen_mu;
-std::map<std::pair<uint64_t, int>, std::unique_ptr<std::vector<uint8_t>>> g_owen;
-
-}  // namespace
-
-const std::vector<uint8_t>* owen_top8_table(uint64_t seed, int dims) {
-  std::lock_guard<std::mutex> lock(g_owen_mu);
-  auto key = std::make_pair(seed, dims);
-  auto it = g_owen.find(key);
-  if (it != g_owen.end()) return it->second.get();
-  std::unique_ptr<std::vector<uint8_t>> tab(new std::vector<uint8_t>((size_t)dims * 256));
-  uint8_t flip[256];  // flip bit of tree node (level j, prefix p) at index (1 << j) + p
-  for (int d = 0; d < dims; ++d) {
-    const uint64_t dk = seed ^ ((uint64_t)d * 0x9E3779B97F4A7C15ull);
-    for (int j = 0; j < 8; ++j) {
-      const uint64_t bk = dk ^ ((uint64_t)(31 - j) * 0xD6E8FEB86659FD93ull);
-      for (uint32_t p = 0; p < (1u << j); ++p)
-        flip[(1u << j) + p] = (uint8_t)(splitmix_finalise(bk ^ ((uint64_t)p * 0xA24BAED4963EE407ull)) >> 63);
-    }
-    uint8_t* row = tab->data() + (size_t)d * 256;
-    for (uint32_t v = 0; v < 256; ++v) {
-      uint32_t prefix = 0;
-      for (int j = 0; j < 8; ++j) {
-        const uint32_t b = (v >> (7 - j)) & 1u;
-        prefix = (prefix << 1) | (b ^ flip[(1u << j) + prefix]);
-      }
-      row[v] = (uint8_t)prefix;
-    }
-  }
-  const std::vector<uint8_t>* out = tab.get();
-  g_owen[key] = std::move(tab);
-  return out;
-}
 
 // ---------------------------------------------------------------- validation
 bool validate_request(const riga_request& r, Error* err) {

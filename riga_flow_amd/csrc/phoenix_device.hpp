@@ -35,58 +35,76 @@ constexpr int kWaves = kWavesPerBlock;
 // prefix product is carried incrementally: P(prefix<<1 | s) = 2 P(prefix) + s*C3
 // (mod 2^64), and only the top bit of the second multiply is formed, since the
 // final xorshift of mix64 leaves bit 63 unchanged.
-__device__ __forceinline__ uint32_t owen_scramble(uint32_t x, uint64_t key) {
-  constexpr uint64_t C2 = 0xD6E8FEB86659FD93ull;
-  constexpr uint64_t C3 = 0xA24BAED4963EE407ull;
+constexpr uint64_t kOwenC2 = 0xD6E8FEB86659FD93ull;
+constexpr uint64_t kOwenC3 = 0xA24BAED4963EE407ull;
+constexpr int kOwenTableBits = 12;  // levels read from the per-dimension prefix table
+
+__device__ __forceinline__ void owen_level(uint32_t x, uint64_t key, int bit, uint32_t& prefix, uint64_t& P) {
   constexpr uint64_t C4 = 0xBF58476D1CE4E5B9ull;
   constexpr uint64_t C5 = 0x94D049BB133111EBull;
+  // z = h ^ (h >> 30) for h = kb ^ P, as (kb ^ kb>>30) ^ P ^ (P>>30): the key
+  // part is wave-uniform (scalar unit), the rest two three-input XORs
+  const uint64_t kb = key ^ ((uint64_t)bit * kOwenC2);
+  const uint64_t kk = kb ^ (kb >> 30);
+  const uint64_t ps = P >> 30;
+  const uint32_t zlo = __builtin_amdgcn_bitop3_b32((uint32_t)kk, (uint32_t)P, (uint32_t)ps, 0x96);
+  const uint32_t zhi = __builtin_amdgcn_bitop3_b32((uint32_t)(kk >> 32), (uint32_t)(P >> 32), (uint32_t)(ps >> 32), 0x96);
+  uint64_t z = ((uint64_t)zhi << 32) | zlo;
+  z *= C4;
+  z ^= z >> 27;
+  const uint32_t top = (uint32_t)((z * C5) >> 63);
+  const uint32_t s = ((x >> bit) & 1u) ^ top;
+  prefix = (prefix << 1) | s;
+  P = (P << 1) + (s ? kOwenC3 : 0ull);
+}
+
+__device__ __forceinline__ uint32_t owen_scramble(uint32_t x, uint64_t key) {
   uint32_t prefix = 0;
   uint64_t P = 0;
 #pragma unroll 4
-  for (int bit = 31; bit >= 0; --bit) {
-    uint64_t z = key ^ ((uint64_t)bit * C2) ^ P;
-    z ^= z >> 30;
-    z *= C4;
-    z ^= z >> 27;
-    const uint32_t top = (uint32_t)((z * C5) >> 63);
-    const uint32_t s = ((x >> bit) & 1u) ^ top;
-    prefix = (prefix << 1) | s;
-    P = (P << 1) + (s ? C3 : 0ull);
-  }
+  for (int bit = 31; bit >= 0; --bit) owen_level(x, key, bit, prefix, P);
   return prefix;
 }
 
-// The same scramble with its first 8 levels read from a table: the top byte of
-// the output depends only on the top byte of x (each level hashes the already
-// scrambled prefix), so tab[x >> 24] = owen(x)[31:24] for the dimension, and the
-// remaining 24 levels continue from prefix = that byte, P = prefix * C3.
-__device__ __forceinline__ uint32_t owen_scramble_top8(uint32_t x, uint64_t key, uint32_t top) {
-  constexpr uint64_t C2 = 0xD6E8FEB86659FD93ull;
-  constexpr uint64_t C3 = 0xA24BAED4963EE407ull;
-  constexpr uint64_t C4 = 0xBF58476D1CE4E5B9ull;
-  constexpr uint64_t C5 = 0x94D049BB133111EBull;
+// The same scramble with its first kOwenTableBits levels read from a table:
+// the top L output bits depend only on the top L bits of x (each level hashes
+// the already scrambled prefix), so tab[x >> (32-L)] = owen(x)[31:32-L] for the
+// dimension, and the remaining levels continue from prefix = that value,
+// P = prefix * C3.
+template <bool kFullUnroll>
+__device__ __forceinline__ uint32_t owen_scramble_below(uint32_t x, uint64_t key, uint32_t top) {
   uint32_t prefix = top;
-  uint64_t P = (uint64_t)top * C3;
+  uint64_t P = (uint64_t)top * kOwenC3;
+  if constexpr (kFullUnroll) {
+#pragma unroll
+    for (int bit = 31 - kOwenTableBits; bit >= 0; --bit) owen_level(x, key, bit, prefix, P);
+  } else {
 #pragma unroll 4
-  for (int bit = 23; bit >= 0; --bit) {
-    uint64_t z = key ^ ((uint64_t)bit * C2) ^ P;
-    z ^= z >> 30;
-    z *= C4;
-    z ^= z >> 27;
-    const uint32_t topbit = (uint32_t)((z * C5) >> 63);
-    const uint32_t s = ((x >> bit) & 1u) ^ topbit;
-    prefix = (prefix << 1) | s;
-    P = (P << 1) + (s ? C3 : 0ull);
+    for (int bit = 31 - kOwenTableBits; bit >= 0; --bit) owen_level(x, key, bit, prefix, P);
   }
   return prefix;
 }
 
 // Unscrambled Sobol coordinate (qmc.rs:269-279): XOR of the direction numbers
-// selected by the gray code of n.  `row` is wave-uniform.
-__device__ __forceinline__ uint32_t sobol_x(uint32_t gray, const uint32_t* __restrict__ row,
-                                            int nbits) {
+// selected by the gray code of n.  `row` is wave-uniform (32 words, read as
+// scalars); each bit is a sign-extended mask and one three-input logic op, with
+// no branch and no per-bit load.  Bits of the gray code at or above nbits are
+// zero, so a fixed 24- or 32-bit sweep gives the same XOR.
+__device__ __forceinline__ uint32_t sobol_x(uint32_t gray, const uint32_t* __restrict__ row, int nbits) {
+  // opaque copy: the masks are rebuilt here (one bit-field extract each)
+  // instead of 32 lane masks being held across the whole path
+  asm volatile("" : "+v"(gray));
+  // bitop3 table 0x6c: f(a, b, c) = b ^ (a & c), i.e. x ^= mask & row[k]
   uint32_t x = 0;
-  for (int k = 0; k < nbits; ++k) x ^= ((gray >> k) & 1u) ? row[k] : 0u;
+  if (nbits <= 24) {
+#pragma unroll
+    for (int k = 0; k < 24; ++k)
+      x = __builtin_amdgcn_bitop3_b32((uint32_t)__builtin_amdgcn_sbfe((int)gray, k, 1), x, row[k], 0x6c);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 32; ++k)
+      x = __builtin_amdgcn_bitop3_b32((uint32_t)__builtin_amdgcn_sbfe((int)gray, k, 1), x, row[k], 0x6c);
+  }
   return x;
 }
 
@@ -240,7 +258,7 @@ __device__ __forceinline__ float root_sd_of(const ItemDesc& it, float) { return 
 
 // Correlated normals of one bridge node: dims node*F + f (note.rs:89-98 with F
 // factors), y = L z with the reference's sequential row sums (correlation.rs:196-206).
-template <typename Real, int F>
+template <typename Real, int F, bool kUnrollOwen = true>
 __device__ __forceinline__ void node_normals(const ItemDesc& it, const unsigned char* arena,
                                              uint32_t n, uint32_t gray, int node, Real (&y)[F]) {
   Real z[F];
@@ -270,8 +288,9 @@ __device__ __forceinline__ void node_normals(const ItemDesc& it, const unsigned 
       const uint64_t key = it.seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
       uint32_t xs;
       if (it.flags & kOwenTable) {
-        const uint8_t* tab = at<uint8_t>(arena, it.off_owen) + (size_t)dim * 256;
-        xs = owen_scramble_top8(x, key, tab[x >> 24]);
+        const uint16_t* tab = it.owen_tab + ((size_t)dim << kOwenTableBits);
+        // full unroll where registers allow (the FP64 10-scenario lane keeps the loop)
+        xs = owen_scramble_below<kUnrollOwen>(x, key, tab[x >> (32 - kOwenTableBits)]);
       } else {
         xs = owen_scramble(x, key);
       }
@@ -363,7 +382,7 @@ __device__ __forceinline__ float lv_sigma_uniform(const float4* __restrict__ row
 // dimension, so the direction row, the key and the Owen table row are uniform.
 template <typename Real>
 __global__ __launch_bounds__(256) void normal_table_kernel(const uint32_t* __restrict__ dirs,
-                                                           const uint8_t* __restrict__ owen_tab,
+                                                           const uint16_t* __restrict__ owen_tab,
                                                            uint64_t seed, int n_paths, int nbits,
                                                            Real* __restrict__ out) {
   const int path = blockIdx.x * 256 + threadIdx.x;
@@ -373,7 +392,8 @@ __global__ __launch_bounds__(256) void normal_table_kernel(const uint32_t* __res
   const uint32_t gray = n ^ (n >> 1);
   const uint32_t x = sobol_x(gray, dirs + (size_t)dim * 32, nbits);
   const uint64_t key = seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
-  const uint32_t xs = owen_tab ? owen_scramble_top8(x, key, owen_tab[(size_t)dim * 256 + (x >> 24)])
+  const uint32_t xs = owen_tab ? owen_scramble_below<true>(x, key, owen_tab[((size_t)dim << kOwenTableBits) +
+                                                                   (x >> (32 - kOwenTableBits))])
                                : owen_scramble(x, key);
   const double u = ((double)xs + 0.5) / 4294967296.0;
   Real z;
@@ -453,10 +473,13 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     int ko_step[SMAX];
     Real accrual[SMAX], unpaid[SMAX], cpv[SMAX], mpv[SMAX];
     bool received[SMAX];
+    // FP32 tier: KO step + 1 (bits 0-19, 0 = not knocked out), unpaid memory
+    // coupons as a count (bits 20-30) and coupon received (bit 31) in one word
+    uint32_t pk[SMAX];
     Real worst_last[SMAX];
     {
       Real y[F];
-      node_normals<Real, F>(it, arena, n, gray, 0, y);
+      node_normals<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, arena, n, gray, 0, y);
 #pragma unroll
       for (int f = 0; f < F; ++f) {
         bs.top[f] = y[f] * root_sd_of(it, Real(0));  // W[T] = y_0 sqrt(T dt), qmc.rs:165
@@ -471,6 +494,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
       ko_step[s] = -1;
       accrual[s] = unpaid[s] = cpv[s] = mpv[s] = Real(0);
       received[s] = false;
+      pk[s] = 0u;
       worst_last[s] = kLogState ? Real(0) : Real(1);  // final_worst starts at 1.0 (note.rs:115)
     }
 
@@ -491,7 +515,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         for (int d = 0; d < si.n_desc; ++d) {
           const BridgeNode bn = bnodes[node_ptr++];
           Real y[F], wmid[F];
-          node_normals<Real, F>(it, arena, n, gray, bn.node, y);
+          node_normals<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, arena, n, gray, bn.node, y);
 #pragma unroll
           for (int f = 0; f < F; ++f)
             wmid[f] = Real(0.5) * (wlo[f] + bs.top[f]) + y[f] * sd_of(bn, Real(0));  // qmc.rs:185-187
@@ -601,29 +625,29 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
 #pragma unroll
           for (int f = 1; f < F; ++f) w = fminf(w, spot[s][f]);
           worst_last[s] = w;
-          const bool alive = ko_step[s] < 0;
+          const bool alive = (pk[s] & 0xFFFFFu) == 0u;
           if (new_period) accrual[s] = 0.0f;
           accrual[s] += w >= it.ln_cb32 ? inc : 0.0f;
           const bool ko_now = alive && call_active && w >= it.ln_ko32;
           if (ko_now) {  // note.rs:166-195
-            ko_step[s] = step;
             const float due = fixed_now ? (float)it.coupon_rate : accrual[s];
-            const float carry = due > 0.0f ? unpaid[s] : 0.0f;
+            const float carry = due > 0.0f ? (float)((pk[s] >> 20) & 0x7FFu) * (float)it.coupon_rate : 0.0f;
+            uint32_t recv = pk[s] & 0x80000000u;
             if (due + carry > 0.0f) {
               cpv[s] += due * (float)si.df;
               mpv[s] += carry * (float)si.df;
-              received[s] = true;
+              recv = 0x80000000u;
             }
+            pk[s] = (pk[s] & 0x7FF00000u) | recv | (uint32_t)(step + 1);
           }
           if (period_end && alive && !ko_now) {  // note.rs:197-218
             const float due = fixed_now ? (float)it.coupon_rate : accrual[s];
             if (due > 0.0f) {
               cpv[s] += due * (float)si.df;
-              mpv[s] += unpaid[s] * (float)si.df;
-              received[s] = true;
-              unpaid[s] = 0.0f;
+              mpv[s] += (float)((pk[s] >> 20) & 0x7FFu) * (float)it.coupon_rate * (float)si.df;
+              pk[s] = (pk[s] & 0x000FFFFFu) | 0x80000000u;  // paid: unpaid count back to 0
             } else if (it.flags & kMemoryCoupon) {
-              unpaid[s] += (float)it.coupon_rate;
+              pk[s] += 1u << 20;
             }
           }
         }
@@ -722,6 +746,10 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
 #pragma unroll
     for (int s = 0; s < SMAX; ++s) {
       if (s >= S) continue;
+      if constexpr (kLogState) {  // unpack the FP32 tier's state word
+        ko_step[s] = (int)(pk[s] & 0xFFFFFu) - 1;
+        received[s] = (pk[s] >> 31) != 0u;
+      }
       const bool ko = ko_step[s] >= 0;
       const double funding = ko ? stepinfo[ko_step[s]].df : it.put_df;
       const double fw = kLogState ? exp((double)worst_last[s]) : (double)worst_last[s];

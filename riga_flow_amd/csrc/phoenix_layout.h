@@ -24,7 +24,7 @@ enum ItemFlags : uint32_t {
   kTrace = 1u << 3,          // trace_level != Off
   kHalton = 1u << 4,         // VarianceReduction::Halton ladder (else Sobol)
   kRecords = 1u << 5,        // write one PathRecord per (scenario, path)
-  kOwenTable = 1u << 6,      // top 8 Owen levels read from a per-dimension table
+  kOwenTable = 1u << 6,      // top 12 Owen levels read from a per-dimension table
   kNormalTable = 1u << 7,    // ladder normals read from the batch's shared [dim][path] table
   kLvUniform = 1u << 8,      // FP32 tier: every surface has evenly spaced knots (slope rows)
 };
@@ -119,7 +119,7 @@ struct ItemDesc {
   int32_t n_blocks;
   int32_t paths_per_block;
   int32_t ladder_rows;   // ladder rows uploaded (min(ladder_dims, F*steps))
-  int64_t off_owen;      // uint8: [F*steps][256] scrambled top byte per raw dimension
+  const uint16_t* owen_tab; // device table [dims][4096]: scrambled top 12 bits per raw dimension
   // FP32 tier: constants pre-rounded on the host, so the path loop converts nothing
   float chol32[kMaxFactors * kMaxFactors];
   float neg_half_dt, ln_ko32, ln_cb32, root_sd32;

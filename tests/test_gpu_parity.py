@@ -110,6 +110,21 @@ def test_lv_fp32_tier_uneven_knots_within_1e4():
     assert_fp64_parity(f64, ora)
 
 
+def test_fp32_memory_coupons_accumulate_within_1e4():
+    # a coupon barrier above spot: most periods are missed and carried as
+    # memory coupons until a KO or a paying period (the FP32 tier counts them)
+    req = W.flat_book(paths=4096, steps=252)
+    req.trade.coupon_barrier = 1.02
+    req.trade.knock_out_barrier = 1.06
+    md = W.max_dimension_for(req)
+    ora = O.price(req, max_dim=md)
+    res = api.price_put_leg(req, "fp32", max_dimension=md)
+    assert ora.leg_memory_coupon > 0.0
+    assert rel(res.pv, ora.pv) <= FP32_REL
+    assert rel(res.legs.memory_coupon, ora.leg_memory_coupon) <= 1e-3
+    assert rel(res.legs.coupon, ora.leg_coupon) <= 1e-3
+
+
 def test_crn_scenarios_alone_equal_in_mix():
     names, reqs = W.crn_scenarios(paths=1024, steps=126, years=0.5)
     md = max(W.max_dimension_for(r) for r in reqs)
@@ -154,7 +169,7 @@ def test_reference_dimension_cap_wraps_like_the_reference():
 
 
 def test_owen_table_and_full_hash_agree():
-    # the tabulated top byte and the 32-level hash give identical integers
+    # the tabulated top 12 levels and the 32-level hash give identical integers
     req = _flat(2048, steps=252)
     md = W.max_dimension_for(req)
     a = api.price_batch([req], "fp64", max_dimension=md, owen_table=True)[0]
