@@ -892,7 +892,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     ot->seed = kv.first;
     ot->dims = kv.second;
     if (stream_ordered) ot->buf.stream = pl->stream;
-    const size_t bytes = (size_t)kv.second << 13;  // 4096 x u16 per dimension
+    const size_t bytes = (size_t)kv.second << (kOwenPrefixBits + 1);  // 2^bits x u16 per dimension
     ztable_bytes += (double)bytes;
     if (!estimate) {
       HIP_TRY(ot->buf.ensure(bytes), "owen table alloc");

@@ -37,7 +37,7 @@ constexpr int kWaves = kWavesPerBlock;
 // final xorshift of mix64 leaves bit 63 unchanged.
 constexpr uint64_t kOwenC2 = 0xD6E8FEB86659FD93ull;
 constexpr uint64_t kOwenC3 = 0xA24BAED4963EE407ull;
-constexpr int kOwenTableBits = 12;  // levels read from the per-dimension prefix table
+constexpr int kOwenTableBits = kOwenPrefixBits;  // levels read from the per-dimension prefix table
 
 __device__ __forceinline__ void owen_level(uint32_t x, uint64_t key, int bit, uint32_t& prefix, uint64_t& P) {
   constexpr uint64_t C4 = 0xBF58476D1CE4E5B9ull;
@@ -376,6 +376,13 @@ __device__ __forceinline__ float lv_sigma_uniform(const float4* __restrict__ row
   return fmaf(wt, v1 - v0, v0);
 }
 
+// The FP32 tier's lookup on unevenly spaced knots, out of line: a rare path
+// whose search loops would otherwise widen the register budget of the kernel.
+__device__ __noinline__ float lv_sigma_f32_general(const float* __restrict__ ls, int nx, float inv_dx, float wt,
+                                                   float x) {
+  return lv_sigma<float, true>(ls, nx, inv_dx, wt, x);
+}
+
 // ---------------------------------------------------------------- shared normal table
 // Ladder normals z[dim][path] for every raw dimension a batch of same-seed
 // trades uses (dims without a wrapped direction row); blockIdx.y is the
@@ -593,14 +600,18 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         }
         const float inc = fixed_now ? 0.0f : (float)si.accrual_inc;
         const unsigned char* lv_rows = reinterpret_cast<const unsigned char*>(lds);
+        // Phase 1: the Euler step of every (scenario, factor).  The chains are
+        // independent and the local-vol mode is chosen once per step, so the
+        // factors of a scenario form one straight-line block the scheduler can
+        // overlap (the LDS reads of one chain behind the arithmetic of another).
+        if (lv_uniform) {
 #pragma unroll
-        for (int s = 0; s < SMAX; ++s) {
-          if (s >= S) continue;
+          for (int s = 0; s < SMAX; ++s) {
+            if (s >= S) continue;
+            const int ss = s;
 #pragma unroll
-          for (int f = 0; f < F; ++f) {
-            const SfF32 c = sf[s * F + f];
-            float incr;
-            if (lv_uniform) {
+            for (int f = 0; f < F; ++f) {
+              const SfF32 c = sf[ss * F + f];
               float t = fmaf(spot[s][f], c.a, c.b);
               t = fminf(fmaxf(t, 0.0f), c.tmax);
               const int j = (int)t;
@@ -609,46 +620,72 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
               const float v0 = fmaf(wx, e.y, e.x);
               const float v1 = fmaf(wx, e.w, e.z);
               const float vol = fmaf(wt[f], v1 - v0, v0);
-              incr = fmaf(vol, dw[f], fmaf(vol * vol, it.neg_half_dt, c.c));
-            } else if (local_vol) {  // knots not evenly spaced: bracketing search
-              const int u = scen[s].lv[f];
-              const LvSurface& su = surf[u];
-              const float vol = lv_sigma<float, true>(lds + (size_t)u * kSurfWords * nxs, su.n_x, (float)su.inv_dx,
-                                                      wt[f], spot[s][f] + (float)it.ln_s0[f]);
-              incr = fmaf(vol, dw[f], fmaf(vol * vol, it.neg_half_dt, c.c));
-            } else {
-              incr = fmaf(c.a, dw[f], c.c);
+              spot[s][f] += fmaf(vol, dw[f], fmaf(vol * vol, it.neg_half_dt, c.c));
             }
-            spot[s][f] += incr;
           }
+        } else if (local_vol) {  // knots not evenly spaced: bracketing search
+#pragma unroll
+          for (int s = 0; s < SMAX; ++s) {
+            if (s >= S) continue;
+            const int ss = s;
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+              const SfF32 c = sf[ss * F + f];
+              const int u = scen[ss].lv[f];
+              const LvSurface& su = surf[u];
+              const float vol = lv_sigma_f32_general(lds + (size_t)u * kSurfWords * nxs, su.n_x, (float)su.inv_dx,
+                                                     wt[f], spot[s][f] + (float)it.ln_s0[f]);
+              spot[s][f] += fmaf(vol, dw[f], fmaf(vol * vol, it.neg_half_dt, c.c));
+            }
+          }
+        } else {
+#pragma unroll
+          for (int s = 0; s < SMAX; ++s) {
+            if (s >= S) continue;
+            const int ss = s;
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+              const SfF32 c = sf[ss * F + f];
+              spot[s][f] += fmaf(c.a, dw[f], c.c);
+            }
+          }
+        }
+        // Phase 2: worst-of, accrual and the daily KO with its payment as
+        // selects (note.rs:143-195); no divergent branch.
+        const float cr = (float)it.coupon_rate;
+        const float df = (float)si.df;
+#pragma unroll
+        for (int s = 0; s < SMAX; ++s) {
           float w = spot[s][0];
 #pragma unroll
           for (int f = 1; f < F; ++f) w = fminf(w, spot[s][f]);
           worst_last[s] = w;
           const bool alive = (pk[s] & 0xFFFFFu) == 0u;
-          if (new_period) accrual[s] = 0.0f;
-          accrual[s] += w >= it.ln_cb32 ? inc : 0.0f;
+          accrual[s] = (new_period ? 0.0f : accrual[s]) + (w >= it.ln_cb32 ? inc : 0.0f);
           const bool ko_now = alive && call_active && w >= it.ln_ko32;
-          if (ko_now) {  // note.rs:166-195
-            const float due = fixed_now ? (float)it.coupon_rate : accrual[s];
-            const float carry = due > 0.0f ? (float)((pk[s] >> 20) & 0x7FFu) * (float)it.coupon_rate : 0.0f;
-            uint32_t recv = pk[s] & 0x80000000u;
-            if (due + carry > 0.0f) {
-              cpv[s] += due * (float)si.df;
-              mpv[s] += carry * (float)si.df;
-              recv = 0x80000000u;
-            }
-            pk[s] = (pk[s] & 0x7FF00000u) | recv | (uint32_t)(step + 1);
-          }
-          if (period_end && alive && !ko_now) {  // note.rs:197-218
-            const float due = fixed_now ? (float)it.coupon_rate : accrual[s];
-            if (due > 0.0f) {
-              cpv[s] += due * (float)si.df;
-              mpv[s] += (float)((pk[s] >> 20) & 0x7FFu) * (float)it.coupon_rate * (float)si.df;
-              pk[s] = (pk[s] & 0x000FFFFFu) | 0x80000000u;  // paid: unpaid count back to 0
-            } else if (it.flags & kMemoryCoupon) {
-              pk[s] += 1u << 20;
-            }
+          const float due = fixed_now ? cr : accrual[s];
+          const float carry = due > 0.0f ? (float)((pk[s] >> 20) & 0x7FFu) * cr : 0.0f;
+          const bool pay = ko_now && due + carry > 0.0f;
+          cpv[s] += pay ? due * df : 0.0f;
+          mpv[s] += pay ? carry * df : 0.0f;
+          const uint32_t recv = pay ? 0x80000000u : (pk[s] & 0x80000000u);
+          pk[s] = ko_now ? ((pk[s] & 0x7FF00000u) | recv | (uint32_t)(step + 1)) : pk[s];
+        }
+        // Phase 3: period end (note.rs:197-218), a step-uniform branch.
+        if (period_end) {
+          const bool memory = (it.flags & kMemoryCoupon) != 0;
+#pragma unroll
+          for (int s = 0; s < SMAX; ++s) {
+            // lanes knocked out at this very step took their coupon above
+            const bool open = (pk[s] & 0xFFFFFu) == 0u;
+            const float due = fixed_now ? cr : accrual[s];
+            const bool pays = open && due > 0.0f;
+            cpv[s] += pays ? due * df : 0.0f;
+            mpv[s] += pays ? (float)((pk[s] >> 20) & 0x7FFu) * cr * df : 0.0f;
+            uint32_t q = pk[s];
+            q = pays ? ((q & 0x000FFFFFu) | 0x80000000u) : q;          // paid: unpaid count back to 0
+            q = (open && !pays && memory) ? q + (1u << 20) : q;       // missed: one more unpaid coupon
+            pk[s] = q;
           }
         }
         continue;

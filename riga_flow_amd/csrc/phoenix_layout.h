@@ -15,6 +15,11 @@ constexpr int kMaxFactors = 5;      // per-trade underlyings handled by the kern
 constexpr int kBridgeStack = 14;    // pending bridge intervals (host-checked per trade)
 constexpr int kTraceCap = 64;       // note.rs:57 TRACE_EVENT_CAP
 constexpr int kMaxLvX = 256;        // log-spot points per local-vol row staged in LDS
+#ifndef RIGA_OWEN_PREFIX_BITS
+#define RIGA_OWEN_PREFIX_BITS 12
+#endif
+constexpr int kOwenPrefixBits = RIGA_OWEN_PREFIX_BITS;  // Owen levels tabulated per dimension
+static_assert(kOwenPrefixBits >= 8 && kOwenPrefixBits <= 16, "prefix table width out of range");
 constexpr int kBlock = 256;         // threads per workgroup (4 waves of 64)
 
 enum ItemFlags : uint32_t {
@@ -119,7 +124,7 @@ struct ItemDesc {
   int32_t n_blocks;
   int32_t paths_per_block;
   int32_t ladder_rows;   // ladder rows uploaded (min(ladder_dims, F*steps))
-  const uint16_t* owen_tab; // device table [dims][4096]: scrambled top 12 bits per raw dimension
+  const uint16_t* owen_tab; // device table [dims][2^kOwenPrefixBits]: scrambled top bits per raw dimension
   // FP32 tier: constants pre-rounded on the host, so the path loop converts nothing
   float chol32[kMaxFactors * kMaxFactors];
   float neg_half_dt, ln_ko32, ln_cb32, root_sd32;
