@@ -51,6 +51,24 @@ def ops_per_path_step(F: int, S: int, nbits: int, local_vol: bool, tier: str) ->
     return node + S * scen
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "phoenix_kernel_lv3_crn10_fp32_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the FP32 batch kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated), or None."""
+    try:
+        vals = {}
+        for line in open(TRAFFIC_FILE):
+            if "=" in line and not line.startswith("#"):
+                k, v = line.strip().split("=", 1)
+                vals[k] = float(v)
+        return vals["fetch_kib_per_dispatch"] * 1024 * vals["fetch_scale"] + \
+            vals["write_kib_per_dispatch"] * 1024 * vals["write_scale"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 # ---------------------------------------------------------------- CPU baseline
 def cpu_baseline(seconds_target: float = 12.0):
     """The oracle (single thread, then all host threads) on a bounded sample of the
@@ -191,6 +209,7 @@ def main():
     ops = ops_per_path_step(3, 10, nbits, True, "fp32") / 10.0  # per (scenario) path-step
     dev_s = sum(dev32) / len(dev32) / 1e3
     achieved = ops * ps32 / dev_s
+    traffic = measured_traffic()
     ops64 = ops_per_path_step(3, 1, plain_paths.bit_length(), True, "fp64")
     dev64_s = sum(dev64) / len(dev64) / 1e3
     achieved64 = ops64 * ps64 / dev64_s
@@ -230,9 +249,14 @@ def main():
         "fp32_vs_fp64": {"max_rel_pv_err": rel_pv, **gerr,
                          "base_pv_fp32": got[0].pv, "base_pv_fp64": ref[0].pv},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
-                     "unit": "Tops/s", "frac": achieved / VALU_PEAK, "traffic": None,
+                     "unit": "Tops/s", "frac": achieved / VALU_PEAK, "traffic": traffic,
+                     "traffic_gbps": traffic / dev_s / 1e9 if traffic else None,
+                     "hbm_frac": traffic / dev_s / HBM_PEAK if traffic else None,
                      "kernel": "phoenix_kernel<float,3,10>", "device_ms": dev_s * 1e3,
-                     "ops_per_path_step": ops},
+                     "ops_per_path_step": ops,
+                     "note": "VALU-bound: no MFMA (hashing, scalar FP); traffic = rocprofv3 "
+                             "FETCH_SIZE+WRITE_SIZE bytes per launch, copy-calibrated, from "
+                             "profiles/phoenix_kernel_lv3_crn10_fp32_traffic.txt"},
         "autotune": tuning,
     }
     fp32.close()
