@@ -526,7 +526,8 @@ struct LaunchPlan {
   int F = 0, smax = 1, blocks = 0;
   int threads = kBlock;  // lanes per workgroup
   size_t lds = 0;
-  size_t block_off = 0;
+  size_t block_off = 0;     // first (item, chunk) pair of this launch in the block table
+  std::vector<int> items;   // item indices, longest first
 };
 
 // Launch-shape signature of a batch: precision and, per (factors, scenario
@@ -643,7 +644,7 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
     L.lds = lp.lds;
     L.threads = lp.threads;
     L.items = (const ItemDesc*)pl->items_d.p;
-    L.block_item = (const int32_t*)pl->block_item.p + lp.block_off;
+    L.block_item = (const int32_t*)pl->block_item.p + 2 * lp.block_off;
     L.arena = (const unsigned char*)pl->arena.p;
     L.partials = (Partial*)pl->partials.p;
     L.records = pl->records ? (PathRecordDev*)pl->records_d.p : nullptr;
@@ -784,7 +785,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     LaunchPlan lp;
     lp.F = kv.first.first;
     lp.smax = kv.first.second;
-    lp.block_off = all_blocks.size();
+    lp.items = idx;
     int stack_levels = 0, lv_words = 0;
     for (int i : idx) {
       stack_levels = std::max(stack_levels, items[i].depth - 1);
@@ -808,9 +809,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       d.stack_levels = stack_levels;
       d.paths_per_block = threads * ppt;
       d.n_blocks = (int32_t)((d.paths + d.paths_per_block - 1) / d.paths_per_block);
-      d.block_begin = blocks;
       blocks += d.n_blocks;
-      for (int b = 0; b < d.n_blocks; ++b) all_blocks.push_back(i);
       d.partial_base = pl->partial_slots;
       pl->partial_slots += (int64_t)d.S * d.n_blocks;
       d.record_base = pl->record_slots;
@@ -943,6 +942,33 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
         items[i].d.zt_stride = zt->paths;
       }
       pl->ztables.push_back(std::move(zt));
+    }
+  }
+  // block table: (item, chunk) pairs per launch.  Items reading a shared normal
+  // table go chunk-major (every item's chunk c before any item's chunk c+1), so
+  // the blocks on the device at once read the same path columns of the table
+  // and the rows stay in cache; other launches go item by item, longest first.
+  for (LaunchPlan& lp : pl->launches) {
+    lp.block_off = all_blocks.size() / 2;
+    bool shared_table = false;
+    int max_chunks = 0;
+    for (int i : lp.items) {
+      shared_table = shared_table || (items[i].d.flags & kNormalTable) != 0;
+      max_chunks = std::max(max_chunks, items[i].d.n_blocks);
+    }
+    if (shared_table) {
+      for (int c = 0; c < max_chunks; ++c)
+        for (int i : lp.items)
+          if (c < items[i].d.n_blocks) {
+            all_blocks.push_back(i);
+            all_blocks.push_back(c);
+          }
+    } else {
+      for (int i : lp.items)
+        for (int c = 0; c < items[i].d.n_blocks; ++c) {
+          all_blocks.push_back(i);
+          all_blocks.push_back(c);
+        }
     }
   }
   std::vector<ItemDesc> descs(items.size());

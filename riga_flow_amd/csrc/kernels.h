@@ -17,7 +17,7 @@ struct PhoenixLaunch {
   int threads; // lanes per workgroup: 64, 128 or 256
   size_t lds;  // dynamic LDS bytes (bridge stack and local-vol rows)
   const ItemDesc* items;
-  const int32_t* block_item;
+  const int32_t* block_item;  // (item, chunk) per block
   const unsigned char* arena;
   Partial* partials;
   PathRecordDev* records;

@@ -258,22 +258,34 @@ __device__ __forceinline__ float root_sd_of(const ItemDesc& it, float) { return 
 
 // Correlated normals of one bridge node: dims node*F + f (note.rs:89-98 with F
 // factors), y = L z with the reference's sequential row sums (correlation.rs:196-206).
+// Normals of one node from the batch's shared [dim][path] table (same ladder,
+// same point), and the correlation y = L z with the reference's sequential row
+// sums (correlation.rs:196-206).
+template <typename Real, int F>
+__device__ __forceinline__ void table_normals(const ItemDesc& it, int node, uint32_t n, Real (&z)[F]) {
+  const Real* zt = reinterpret_cast<const Real*>(it.ztable);
+#pragma unroll
+  for (int f = 0; f < F; ++f) z[f] = zt[(size_t)((uint32_t)node * F + f) * (size_t)it.zt_stride + (n - 1)];
+}
+
+template <typename Real, int F>
+__device__ __forceinline__ void correlate(const ItemDesc& it, const Real (&z)[F], Real (&y)[F]) {
+#pragma unroll
+  for (int i = 0; i < F; ++i) {
+    Real acc = Real(0);
+#pragma unroll
+    for (int j = 0; j <= i; ++j) acc += chol_of(it, i * kMaxFactors + j, Real(0)) * z[j];
+    y[i] = acc;
+  }
+}
+
 template <typename Real, int F, bool kUnrollOwen = true>
 __device__ __forceinline__ void node_normals(const ItemDesc& it, const unsigned char* arena,
                                              uint32_t n, uint32_t gray, int node, Real (&y)[F]) {
   Real z[F];
   if (it.flags & kNormalTable) {
-    // the batch's shared table holds exactly these normals (same ladder, same point)
-    const Real* zt = reinterpret_cast<const Real*>(it.ztable);
-#pragma unroll
-    for (int f = 0; f < F; ++f) z[f] = zt[(size_t)((uint32_t)node * F + f) * (size_t)it.zt_stride + (n - 1)];
-#pragma unroll
-    for (int i = 0; i < F; ++i) {
-      Real acc = Real(0);
-#pragma unroll
-      for (int j = 0; j <= i; ++j) acc += chol_of(it, i * kMaxFactors + j, Real(0)) * z[j];
-      y[i] = acc;
-    }
+    table_normals<Real, F>(it, node, n, z);
+    correlate<Real, F>(it, z, y);
     return;
   }
   const bool halton = (it.flags & kHalton) != 0;
@@ -422,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   __shared__ BlockShared sh[SMAX];
   __shared__ SfF32 sf_lds[sizeof(Real) == 4 ? SMAX * F : 1];  // FP32 Euler constants, read per step
 
-  const int item = block_item[blockIdx.x];
+  const int item = block_item[2 * blockIdx.x];  // block table: (item, chunk) pairs
   const ItemDesc& it = items[item];
   const int S = it.S;
   const int steps = it.steps;
@@ -431,7 +443,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   const int nwaves = nthreads >> 6;
   const int wave = tid >> 6;
   const int lane = tid & 63;
-  const int chunk = (int)blockIdx.x - it.block_begin;
+  const int chunk = block_item[2 * blockIdx.x + 1];
   const bool local_vol = (it.flags & kLocalVol) != 0;
   const bool trace = (it.flags & kTrace) != 0;
   const bool want_records = (it.flags & kRecords) != 0 && records != nullptr;
@@ -506,6 +518,13 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     }
 
     int node_ptr = 0;
+    // shared normal table: the next node's normals are loaded one descent ahead,
+    // so their latency overlaps the current step's scenario work
+    const bool ztab = (it.flags & kNormalTable) != 0;
+    Real zn[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) zn[f] = Real(0);
+    if (ztab && steps > 1) table_normals<Real, F>(it, bnodes[0].node, n, zn);
     int loaded_i0[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) loaded_i0[f] = -1;
@@ -522,7 +541,12 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         for (int d = 0; d < si.n_desc; ++d) {
           const BridgeNode bn = bnodes[node_ptr++];
           Real y[F], wmid[F];
-          node_normals<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, arena, n, gray, bn.node, y);
+          if (ztab) {  // prefetched a step ahead; fetch the following node's now
+            correlate<Real, F>(it, zn, y);
+            if (node_ptr < steps - 1) table_normals<Real, F>(it, bnodes[node_ptr].node, n, zn);
+          } else {
+            node_normals<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, arena, n, gray, bn.node, y);
+          }
 #pragma unroll
           for (int f = 0; f < F; ++f)
             wmid[f] = Real(0.5) * (wlo[f] + bs.top[f]) + y[f] * sd_of(bn, Real(0));  // qmc.rs:185-187

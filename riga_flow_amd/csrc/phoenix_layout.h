@@ -120,7 +120,7 @@ struct ItemDesc {
   int32_t pad2;
   int64_t partial_base;  // first Partial slot: [scenario][block_in_item]
   int64_t record_base;   // first PathRecord slot: [scenario][path]
-  int32_t block_begin;   // first block of this item in the launch
+  int32_t pad3;
   int32_t n_blocks;
   int32_t paths_per_block;
   int32_t ladder_rows;   // ladder rows uploaded (min(ladder_dims, F*steps))
