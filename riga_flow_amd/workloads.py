@@ -52,7 +52,8 @@ def phoenix_trade(trade_id: str, F: int, years: float, steps: int, *, ko: float 
 def flat_market(F: int = 3, vols=(0.20, 0.25, 0.30), rho: float = 0.5, r: float = 0.03,
                 q: float = 0.01, s0: float = 100.0) -> MarketSnapshot:
     corr = [[1.0 if i == j else rho for j in range(F)] for i in range(F)]
-    return MarketSnapshot(AS_OF, [s0] * F, list(vols)[:F], r, [q] * F, corr)
+    vols = [float(vols[i % len(vols)]) for i in range(F)]  # cycled beyond the given list
+    return MarketSnapshot(AS_OF, [s0] * F, vols, r, [q] * F, corr)
 
 
 def sobol_config(paths: int, steps: int, F: int, seed: int = 1, trace: str = "aggregated",
