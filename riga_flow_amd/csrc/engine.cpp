@@ -130,6 +130,19 @@ struct HostBuf {
 
 }  // namespace
 
+// Owen prefix table of one seed: u16[dims][2^kOwenPrefixBits], built on the
+// device, shared by every plan of the context (plans keep a reference, so a
+// table replaced by a larger one stays valid until its last plan is gone).
+struct OwenTable {
+  uint64_t seed = 0;
+  int dims = 0;
+  DevBuf buf;
+  hipEvent_t ready = nullptr;  // recorded after the build
+  ~OwenTable() {
+    if (ready) (void)hipEventDestroy(ready);
+  }
+};
+
 struct riga_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -139,6 +152,8 @@ struct riga_ctx {
   std::mutex mu;
   std::mutex tune_mu;
   std::map<std::string, std::pair<int, int>> tuned;  // shape signature -> (paths/thread, waves)
+  std::mutex owen_mu;
+  std::map<uint64_t, std::shared_ptr<struct OwenTable>> owen;  // per seed, grown on demand
   DevBuf arena, items, block_item, partials, finals, final_base, records;
   HostBuf staging, finals_host;
   PinnedPool pinned;  // plans' download buffers (plans end before their context)
@@ -580,12 +595,7 @@ struct riga_plan {
     DevBuf buf;
   };
   std::vector<std::unique_ptr<NormalTable>> ztables;
-  struct OwenTable {
-    uint64_t seed = 0;
-    int dims = 0;
-    DevBuf buf;
-  };
-  std::vector<std::unique_ptr<OwenTable>> owen_tabs;
+  std::vector<std::shared_ptr<OwenTable>> owen_tabs;  // the context's tables this plan reads
   int64_t off_dirs = 0;
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -599,7 +609,6 @@ struct riga_plan {
     // buffers go back to the pool on this stream before it is destroyed
     for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d}) b->release();
     for (auto& zt : ztables) zt->buf.release();
-    for (auto& ot : owen_tabs) ot->buf.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
   void bind_stream() {
@@ -608,6 +617,31 @@ struct riga_plan {
 };
 
 namespace {
+
+// The context's table for `seed` covering at least `dims` dimensions, building a
+// larger one (at least twice the old size, so growth is rare) on `stream` when
+// needed; callers wait on table->ready before their kernels read it.
+hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t stream,
+                          std::shared_ptr<OwenTable>* out) {
+  std::lock_guard<std::mutex> g(ctx->owen_mu);
+  auto f = ctx->owen.find(seed);
+  if (f != ctx->owen.end() && f->second->dims >= dims) {
+    *out = f->second;
+    return hipSuccess;
+  }
+  const int want = f == ctx->owen.end() ? dims : std::max(dims, 2 * f->second->dims);
+  std::shared_ptr<OwenTable> t(new OwenTable());
+  t->seed = seed;
+  t->dims = want;
+  hipError_t e = t->buf.ensure((size_t)want << (kOwenPrefixBits + 1));  // 2^bits x u16 per dimension
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&t->ready, hipEventDisableTiming);
+  if (e == hipSuccess) e = launch_owen_table(seed, want, (uint16_t*)t->buf.p, stream);
+  if (e == hipSuccess) e = hipEventRecord(t->ready, stream);
+  if (e != hipSuccess) return e;
+  ctx->owen[seed] = t;
+  *out = t;
+  return hipSuccess;
+}
 
 // Knots at x0 + i*dx to within 1e-9 of the spacing (linspace grids, as the
 // calibration writes them): the FP32 tier then locates a knot by one multiply.
@@ -886,20 +920,14 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   if (stream_ordered) pl->bind_stream();
   double ztable_bytes = 0.0;
   std::map<uint64_t, const uint16_t*> owen_ptr;
-  for (auto& kv : owen_dims) {
-    std::unique_ptr<riga_plan::OwenTable> ot(new riga_plan::OwenTable());
-    ot->seed = kv.first;
-    ot->dims = kv.second;
-    if (stream_ordered) ot->buf.stream = pl->stream;
-    const size_t bytes = (size_t)kv.second << (kOwenPrefixBits + 1);  // 2^bits x u16 per dimension
-    ztable_bytes += (double)bytes;
-    if (!estimate) {
-      HIP_TRY(ot->buf.ensure(bytes), "owen table alloc");
-      HIP_TRY(launch_owen_table(kv.first, kv.second, (uint16_t*)ot->buf.p, pl->stream), "owen table build");
+  if (!estimate)
+    for (auto& kv : owen_dims) {
+      std::shared_ptr<OwenTable> ot;
+      HIP_TRY(owen_table_for(ctx, kv.first, kv.second, pl->stream, &ot), "owen table");
+      HIP_TRY(hipStreamWaitEvent(pl->stream, ot->ready, 0), "owen table wait");
+      owen_ptr[kv.first] = (const uint16_t*)ot->buf.p;
+      pl->owen_tabs.push_back(std::move(ot));
     }
-    owen_ptr[kv.first] = (const uint16_t*)ot->buf.p;
-    pl->owen_tabs.push_back(std::move(ot));
-  }
   for (Item& it : items)
     if (it.d.flags & kOwenTable) it.d.owen_tab = owen_ptr[it.d.seed];
   // shared normal tables: same-seed Sobol trades without wrapped direction rows
@@ -979,7 +1007,6 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
                           (double)pl->final_slots * sizeof(Partial) + pl->final_base.size() * 8.0 +
                           (double)pl->record_slots * sizeof(PathRecordDev) + 8 * 4096.0);
     for (auto& zt : pl->ztables) zt->buf.stream = nullptr;
-    for (auto& ot : pl->owen_tabs) ot->buf.stream = nullptr;
     return 0;
   }
 
@@ -1233,7 +1260,6 @@ int riga_plan_query(const riga_plan* plan, riga_plan_info* info) {
   info->device_bytes = (int64_t)(plan->arena.cap + plan->items_d.cap + plan->block_item.cap + plan->partials.cap +
                                  plan->finals.cap + plan->final_base_d.cap + plan->records_d.cap);
   for (const auto& zt : plan->ztables) info->device_bytes += (int64_t)zt->buf.cap;
-  for (const auto& ot : plan->owen_tabs) info->device_bytes += (int64_t)ot->buf.cap;
   info->normal_tables = (int32_t)plan->ztables.size();
   info->graph = plan->graph ? 1 : 0;
   info->paths_per_thread = plan->ppt;

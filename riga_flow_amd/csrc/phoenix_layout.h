@@ -16,7 +16,7 @@ constexpr int kBridgeStack = 14;    // pending bridge intervals (host-checked pe
 constexpr int kTraceCap = 64;       // note.rs:57 TRACE_EVENT_CAP
 constexpr int kMaxLvX = 256;        // log-spot points per local-vol row staged in LDS
 #ifndef RIGA_OWEN_PREFIX_BITS
-#define RIGA_OWEN_PREFIX_BITS 12
+#define RIGA_OWEN_PREFIX_BITS 16
 #endif
 constexpr int kOwenPrefixBits = RIGA_OWEN_PREFIX_BITS;  // Owen levels tabulated per dimension
 static_assert(kOwenPrefixBits >= 8 && kOwenPrefixBits <= 16, "prefix table width out of range");
