@@ -30,7 +30,7 @@ from riga_flow_amd import api, workloads as W  # noqa: E402
 # Per generated coordinate (both tiers, integers identical):
 OWEN_ROUND = 27     # hash input xor 2, xorshift 4, 64-bit multiply 5, xorshift 4,
                     # top word of the 2nd multiply 4, bit logic 3, prefix 1, prefix*C3 4
-OWEN_ROUNDS = 20    # the top 12 of the 32 levels come from the per-dimension prefix table
+OWEN_ROUNDS = 16    # the top 16 of the 32 levels come from the per-dimension prefix table
 OWEN_TABLE = 3      # prefix table address and load
 PROBIT = 22         # rational (12) + divide (8) + tails (5% of log + sqrt + rational)
 UNIFORM = 4         # int->float, +0.5, scale, clamp
@@ -139,8 +139,10 @@ def main():
     ap.add_argument("--no-book", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--quick", action="store_true", help="smaller path counts (smoke)")
-    ap.add_argument("--ppt", type=int, default=0, help="paths per thread (0 = autotuned)")
-    ap.add_argument("--waves", type=int, default=0, help="waves per workgroup (0 = autotuned)")
+    ap.add_argument("--ppt", type=int, default=0, help="FP32 batch: paths per lane (0 = autotuned)")
+    ap.add_argument("--waves", type=int, default=0, help="FP32 batch: waves per workgroup (0 = autotuned)")
+    ap.add_argument("--ppt64", type=int, default=0, help="FP64 plain: paths per lane (0 = autotuned)")
+    ap.add_argument("--waves64", type=int, default=0, help="FP64 plain: waves per workgroup (0 = autotuned)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (shape 1 x 4 waves)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -165,13 +167,13 @@ def main():
     fp32 = api.Plan(crn, "fp32", paths_per_thread=ppt, waves_per_block=waves, use_graph=True,
                     max_dimension=md)
     plain = W.lv_book(paths=plain_paths, steps=756)
-    if not args.no_tune and not (args.ppt and args.waves):
+    if not args.no_tune and not (args.ppt64 and args.waves64):
         tuning["fp64_plain"] = api.autotune([plain], "fp64", use_graph=True, max_dimension=md,
                                             reps=1)
         log(f"autotune fp64: {tuning['fp64_plain']['paths_per_thread']} paths/lane x "
             f"{tuning['fp64_plain']['waves_per_block']} waves")
-    fp64 = api.Plan([plain], "fp64", paths_per_thread=ppt, waves_per_block=waves, use_graph=True,
-                    max_dimension=md)
+    fp64 = api.Plan([plain], "fp64", paths_per_thread=args.ppt64, waves_per_block=args.waves64,
+                    use_graph=True, max_dimension=md)
 
     wall32, dev32, res32 = timed(fp32, args.steps, args.warmup)
     log(f"fp32 batched: {wall32 / args.steps * 1e3:.1f} ms/step")
