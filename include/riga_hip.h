@@ -136,7 +136,7 @@ typedef struct riga_launch_config {
   int32_t normal_table;     /* 0: auto (share one [dim][path] normal table among same-seed trades
                                when they need 4x more normals than it holds); 1: force; -1: off */
   int32_t stream_ordered_alloc; /* 1: plan memory from the stream-ordered pool (serving) */
-  int32_t waves_per_block;  /* 1, 2 or 4 waves of 64 lanes per workgroup; 0: autotuned, else 4 */
+  int32_t waves_per_block;  /* 1-4 waves of 64 lanes per workgroup; 0: autotuned, else 4 */
   int32_t reserved[2];
 } riga_launch_config;
 
@@ -162,7 +162,7 @@ typedef struct riga_tune_trial {
   double device_ms;
 } riga_tune_trial;
 
-#define RIGA_TUNE_MAX_TRIALS 12
+#define RIGA_TUNE_MAX_TRIALS 16
 typedef struct riga_tune_report {
   int32_t n_trials;
   int32_t best;          /* index of the fastest trial */
@@ -171,7 +171,7 @@ typedef struct riga_tune_report {
 
 /* Autotuner (no reference counterpart: the reference launches a fixed 256-lane
  * shape, gpu.rs:240-271).  Runs the batch once per launch shape -- paths per lane
- * 1/2/4/8 x waves per workgroup 4/2/1 -- timing the path kernels with device
+ * 1/2/4/8 x waves per workgroup 4/3/2/1 -- timing the path kernels with device
  * events (best of `reps` after one warm-up), and keeps the fastest for the
  * batch's shape signature in the context: plans created afterwards with
  * paths_per_thread = 0 and waves_per_block = 0 use it.  cfg gives the other

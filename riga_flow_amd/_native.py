@@ -114,7 +114,7 @@ class TuneTrial(C.Structure):
 
 
 class TuneReport(C.Structure):
-    _fields_ = [("n_trials", C.c_int32), ("best", C.c_int32), ("trials", TuneTrial * 12)]
+    _fields_ = [("n_trials", C.c_int32), ("best", C.c_int32), ("trials", TuneTrial * 16)]
 
 
 class ServerConfig(C.Structure):

@@ -717,8 +717,8 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     set_err(err, "invalid_launch_config", "paths_per_thread must be 1, 2, 4 or 8");
     return RIGA_EINVAL;
   }
-  if (waves != 0 && waves != 1 && waves != 2 && waves != 4) {
-    set_err(err, "invalid_launch_config", "waves_per_block must be 1, 2 or 4");
+  if (waves < 0 || waves > 4) {
+    set_err(err, "invalid_launch_config", "waves_per_block must be 1, 2, 3 or 4");
     return RIGA_EINVAL;
   }
   std::unique_ptr<riga_plan> pl(new riga_plan());
@@ -1306,10 +1306,10 @@ int riga_autotune(riga_ctx* ctx, const riga_request* reqs, int n, int precision,
   std::vector<riga_result> res((size_t)n);
   std::string signature;
   static const int kPpt[4] = {1, 2, 4, 8};
-  static const int kWv[3] = {4, 2, 1};
+  static const int kWv[4] = {4, 3, 2, 1};
   double best = 0.0;
   for (int a = 0; a < 4; ++a)
-    for (int b = 0; b < 3; ++b) {
+    for (int b = 0; b < 4; ++b) {
       riga_tune_trial& t = rep.trials[rep.n_trials++];
       t.paths_per_thread = kPpt[a];
       t.waves_per_block = kWv[b];
