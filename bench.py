@@ -200,6 +200,7 @@ def main():
                         (vg.pv - base.pv) / 0.01))
         return out
 
+    per_scenario_err = [abs(a.pv - b.pv) / abs(b.pv) for a, b in zip(got, ref)]
     g32, g64 = greeks(got), greeks(ref)
     gerr = {}
     for i, k in enumerate(("delta", "gamma", "vega")):
@@ -248,7 +249,8 @@ def main():
                                     "kernel": "phoenix_kernel<double,3,1>",
                                     "ops_per_path_step": ops64}},
         "fp32_batched_over_fp64_plain": value / fp64_rate,
-        "fp32_vs_fp64": {"max_rel_pv_err": rel_pv, **gerr,
+        "fp32_vs_fp64": {"max_rel_pv_err": rel_pv, "rel_pv_err_per_scenario": dict(zip(names, per_scenario_err)),
+                         **gerr,
                          "base_pv_fp32": got[0].pv, "base_pv_fp64": ref[0].pv},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
                      "unit": "Tops/s", "frac": achieved / VALU_PEAK, "traffic": traffic,
@@ -264,6 +266,7 @@ def main():
     fp32.close()
     fp64.close()
     log("fp32 error check done")
+    out["crn_variance_reduction"] = crn_vrf_leg(args.quick)
     if not args.no_book:
         out["uneven_book"] = book_leg(args.quick)
     if not args.no_serving:
@@ -272,6 +275,27 @@ def main():
         out["cpu_baseline"] = cpu_baseline()
         log("cpu baseline done")
     print(json.dumps(out), flush=True)
+
+
+def crn_vrf_leg(quick: bool) -> dict:
+    """CRN variance-reduction factor of the asset-0 spot delta (riga-risk's
+    crn_variance_reduction_factor, lib.rs:296-336) on the 3-asset local-vol trade with
+    implied grids recalibrated per scenario; 16 replicates x 4 valuations, one batch."""
+    from riga_flow_amd import risk
+    from riga_flow_amd.types import LocalVolSpec
+    paths = 262_144 if not quick else 16_384
+    req = W.lv_book(paths=paths, steps=756, years=3.0)
+    sig = W.svi_sigmas(3)
+    req.local_vol = LocalVolSpec(grids=[W.svi_implied(sig[f]) for f in range(3)], grid_x=128)
+    t0 = time.perf_counter()
+    factor, crn, ind = risk.crn_variance_reduction_factor(
+        req, replicates=16, precision="fp32", max_dimension=W.max_dimension_for(req), details=True)
+    secs = time.perf_counter() - t0
+    log(f"crn variance reduction factor {factor:.1f} ({secs:.2f} s)")
+    return {"factor": factor, "replicates": 16, "paths": paths, "tier": "fp32",
+            "greek": "central spot delta of asset 0, 1% bump",
+            "delta_mean_crn": sum(crn) / len(crn), "delta_mean_independent": sum(ind) / len(ind),
+            "seconds_including_calibration": secs}
 
 
 def book_leg(quick: bool) -> dict:
