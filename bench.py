@@ -333,22 +333,35 @@ def serving_leg(quick: bool) -> dict:
     pool = [SV.sample_request(u, rng) for _ in range(24 if not quick else 6)]
     tiers = ["fp32" if rng.random() < 0.5 else "fp64" for _ in pool]
     log(f"serving pool ready: {len(pool)} requests, {sum(len(r) for r in pool)} trade-scenarios")
-    runs = []
-    for qps in (50.0, 200.0, 800.0):
+    sizes = [sum(q.config.paths * q.trade.steps for q in req) for req in pool]
+
+    def run(pool, tiers, qps, mean_size, tag):
         srv = SV.Server(hbm_budget_gib=96.0, max_inflight=3, max_batch_requests=4096,
                         max_queue_seconds=3.0, paths_per_thread=1, max_batch_path_steps=1e11)
         r = SV.poisson_load(srv, pool, tiers, qps, duration_s=5.0 if not quick else 1.0,
                             seed=int(qps))
         r["max_concurrent_plans"] = srv.stats()["max_concurrent"]
+        r["offered_path_steps_per_s"] = qps * mean_size
         srv.close()
-        log(f"serving qps={qps:g}: completed {r['completed']}, shed {r['shed']}, "
+        log(f"serving {tag} qps={qps:g}: completed {r['completed']}, shed {r['shed']}, "
             f"p50 {r['p50_s']}, p99 {r['p99_s']}, wall {r['wall_s']:.1f} s")
-        runs.append(r)
-    sizes = [sum(q.config.paths * q.trade.steps for q in req) for req in pool]
+        return r
+
+    runs = [run(pool, tiers, qps, float(np.mean(sizes)), "config") for qps in (50.0, 200.0, 800.0)]
+    # the same market and tiers with small requests (16k/65k paths, 1-8 trades,
+    # 1-4 bumps): the latency of the loop itself at a load one GPU can carry
+    small = [SV.sample_request(u, rng, path_choices=(16_384, 65_536), max_trades=8, max_bumps=4)
+             for _ in range(24 if not quick else 6)]
+    small_tiers = ["fp32" if rng.random() < 0.5 else "fp64" for _ in small]
+    small_sizes = [sum(q.config.paths * q.trade.steps for q in req) for req in small]
+    small_runs = [run(small, small_tiers, qps, float(np.mean(small_sizes)), "small")
+                  for qps in (10.0, 40.0)]
     return {"runs": runs, "mean_request_path_steps": float(np.mean(sizes)),
             "requests_in_pool": len(pool),
             "note": "offered load exceeds one GPU by orders of magnitude; tickets queued "
-                    "longer than 3 s are shed, p50/p99 are over completed tickets"}
+                    "longer than 3 s are shed, p50/p99 are over completed tickets",
+            "small_requests": {"runs": small_runs, "mean_request_path_steps": float(np.mean(small_sizes)),
+                               "pool": "16k/65k paths, 1-8 trades x 1-4 bumps, same market and tiers"}}
 
 
 if __name__ == "__main__":
