@@ -89,7 +89,8 @@ typedef struct riga_request {
   /* SimulationConfig (riga-core/src/lib.rs:77-86) */
   uint64_t paths, path_block_size, seed;
   int32_t trace_level;     /* riga_trace */
-  int32_t ladder;          /* riga_ladder: VarianceReduction kind */
+  int32_t ladder;          /* riga_ladder: VarianceReduction kind (NONE = the ChaCha8 Monte
+                              Carlo lane, note.rs:58,124-126; HALTON; SOBOL) */
   int32_t ladder_dimension;/* VarianceReduction dimension */
   int32_t max_dimension;   /* ladder cap: 512 reproduces qmc.rs:9; larger lifts the wrap */
   /* LocalVolSpec (riga-engine/src/lib.rs:31-34): one per factor, or NULL for flat vols.
@@ -280,6 +281,12 @@ int riga_debug_sobol(riga_ctx *ctx, uint64_t seed, int32_t ladder_dims, uint64_t
                      int32_t n_points, int32_t first_dim, int32_t n_dims,
                      uint32_t *out_unscrambled, uint32_t *out_scrambled, double *out_normal,
                      riga_error *err);
+
+/* ---- diagnostics: the Monte Carlo lane's stream parse.  out_starts[p] = index of
+ * the u64 word of ChaCha8Rng::seed_from_u64(seed) at which the StandardNormal draw
+ * number p * draws_per_path begins (the first draw of path p, note.rs:58,124-126). */
+int riga_debug_mc_starts(riga_ctx *ctx, uint64_t seed, uint64_t draws_per_path, uint64_t n_paths,
+                         uint64_t *out_starts, riga_error *err);
 
 #ifdef __cplusplus
 }

@@ -154,6 +154,15 @@ def lib():
         L.ro_price.argtypes = [C.POINTER(_Request), C.c_uint64, C.c_uint64, C.POINTER(_Result),
                                C.POINTER(PathRecord), C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t]
         L.ro_path_shocks.argtypes = [C.POINTER(_Request), C.c_uint64, _D]
+        _U32 = C.POINTER(C.c_uint32)
+        L.ro_chacha_block.argtypes = [_U32, C.c_int, _U32]
+        L.ro_seed_from_u64.argtypes = [C.c_uint64, _U32]
+        L.ro_rng_u64_at.restype = C.c_uint64
+        L.ro_rng_u64_at.argtypes = [C.c_uint64, C.c_uint64]
+        L.ro_zig_tables.argtypes = [_D, _D]
+        L.ro_std_normals.restype = C.c_uint64
+        L.ro_std_normals.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, _D]
+        L.ro_draw_starts.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]
         _LIB = L
     return _LIB
 
@@ -260,6 +269,47 @@ class Halton:
             lib().ro_halton_free(C.byref(self._h))
         except Exception:
             pass
+
+
+def chacha_block(words: Sequence[int], double_rounds: int) -> List[int]:
+    """The ChaCha block function on a 16-word input state (RFC 7539 section 2.3)."""
+    a = (C.c_uint32 * 16)(*words)
+    out = (C.c_uint32 * 16)()
+    lib().ro_chacha_block(a, double_rounds, out)
+    return list(out)
+
+
+def seed_from_u64(seed: int) -> List[int]:
+    """rand_core 0.9 ``seed_from_u64``: the eight key words (PCG32 expansion)."""
+    out = (C.c_uint32 * 8)()
+    lib().ro_seed_from_u64(seed, out)
+    return list(out)
+
+
+def rng_u64_at(seed: int, k: int) -> int:
+    """The k-th u64 of ``ChaCha8Rng::seed_from_u64(seed)``."""
+    return lib().ro_rng_u64_at(seed, k)
+
+
+def zig_tables():
+    x = np.zeros(257)
+    f = np.zeros(257)
+    lib().ro_zig_tables(_dptr(x), _dptr(f))
+    return x, f
+
+
+def std_normals(seed: int, n: int, start: int = 0):
+    """n ``StandardNormal`` draws from u64 index ``start``; returns (draws, next index)."""
+    out = np.zeros(max(n, 1))
+    end = lib().ro_std_normals(seed, start, n, _dptr(out))
+    return out[:n], end
+
+
+def draw_starts(seed: int, n: int, start: int = 0) -> np.ndarray:
+    """u64 index at which each of n consecutive ``StandardNormal`` draws starts."""
+    out = np.zeros(max(n, 1), dtype=np.uint64)
+    lib().ro_draw_starts(seed, start, n, out.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return out[:n]
 
 
 def brownian_increments(normals: Sequence[float], dt: float) -> np.ndarray:
@@ -467,7 +517,8 @@ def _pack(request, max_dim: int, lvs: Optional[List[LocalVol]]):
 
 def price(request, max_dim: int = 512, path_range=None, records: bool = False,
           lvs: Optional[List[LocalVol]] = None):
-    """``price_full_phoenix`` (note.rs:38-373) on the QMC lane.
+    """``price_full_phoenix`` (note.rs:38-373) on the QMC lanes and the ChaCha8
+    Monte Carlo lane (``variance_reduction.kind == "none"``).
 
     Returns ``OracleResult`` (and a numpy record array when ``records``)."""
     if lvs is None:

@@ -808,6 +808,182 @@ int ro_validate(const ro_request *r, char *code, size_t codelen, char *msg, size
 }
 
 /* ------------------------------------------------------------------------- */
+/* Monte Carlo lane: ChaCha8Rng + StandardNormal (note.rs:58,124-126)        */
+/* ------------------------------------------------------------------------- */
+static uint32_t rotl32(uint32_t v, int c) { return (v << c) | (v >> (32 - c)); }
+
+#define RO_QR(a, b, c, d)                                                                          \
+  do {                                                                                             \
+    x[a] += x[b];                                                                                  \
+    x[d] = rotl32(x[d] ^ x[a], 16);                                                                \
+    x[c] += x[d];                                                                                  \
+    x[b] = rotl32(x[b] ^ x[c], 12);                                                                \
+    x[a] += x[b];                                                                                  \
+    x[d] = rotl32(x[d] ^ x[a], 8);                                                                 \
+    x[c] += x[d];                                                                                  \
+    x[b] = rotl32(x[b] ^ x[c], 7);                                                                 \
+  } while (0)
+
+/* The ChaCha block function (Bernstein 2008; RFC 7539 section 2.3): column then
+ * diagonal quarter rounds, then the input added word by word. */
+void ro_chacha_block(const uint32_t in[16], int double_rounds, uint32_t out[16]) {
+  uint32_t x[16];
+  memcpy(x, in, sizeof x);
+  for (int r = 0; r < double_rounds; ++r) {
+    RO_QR(0, 4, 8, 12);
+    RO_QR(1, 5, 9, 13);
+    RO_QR(2, 6, 10, 14);
+    RO_QR(3, 7, 11, 15);
+    RO_QR(0, 5, 10, 15);
+    RO_QR(1, 6, 11, 12);
+    RO_QR(2, 7, 8, 13);
+    RO_QR(3, 4, 9, 14);
+  }
+  for (int i = 0; i < 16; ++i) out[i] = x[i] + in[i];
+}
+
+/* rand_core 0.9 SeedableRng::seed_from_u64: PCG32 (XSH-RR output), state stepped
+ * before each output, eight outputs as the little-endian key words. */
+void ro_seed_from_u64(uint64_t state, uint32_t key[8]) {
+  for (int i = 0; i < 8; ++i) {
+    state = state * 6364136223846793005ULL + 11634580027462260723ULL;
+    uint32_t xorshifted = (uint32_t)(((state >> 18) ^ state) >> 27);
+    uint32_t rot = (uint32_t)(state >> 59);
+    key[i] = (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
+  }
+}
+
+typedef struct {
+  uint32_t in[16];
+  uint32_t buf[16];
+  uint64_t block; /* index of the block in buf */
+  int word;       /* next word of buf */
+} chacha_rng;
+
+static void rng_load(chacha_rng *g, uint64_t block) {
+  g->in[12] = (uint32_t)block;
+  g->in[13] = (uint32_t)(block >> 32);
+  ro_chacha_block(g->in, 4, g->buf);
+  g->block = block;
+}
+
+/* Positioned at u64 index k (= word 2k of the stream). */
+static void rng_init(chacha_rng *g, uint64_t seed, uint64_t k) {
+  static const uint32_t sigma[4] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u};
+  memcpy(g->in, sigma, sizeof sigma);
+  ro_seed_from_u64(seed, g->in + 4);
+  g->in[14] = g->in[15] = 0;
+  rng_load(g, k / 8);
+  g->word = (int)(k % 8) * 2;
+}
+
+static uint64_t rng_next_u64(chacha_rng *g) {
+  if (g->word == 16) {
+    rng_load(g, g->block + 1);
+    g->word = 0;
+  }
+  uint64_t v = (uint64_t)g->buf[g->word] | ((uint64_t)g->buf[g->word + 1] << 32);
+  g->word += 2;
+  return v;
+}
+
+static uint64_t rng_pos(const chacha_rng *g) { return g->block * 8 + (uint64_t)(g->word / 2); }
+
+uint64_t ro_rng_u64_at(uint64_t seed, uint64_t k) {
+  chacha_rng g;
+  rng_init(&g, seed, k);
+  return rng_next_u64(&g);
+}
+
+#define ZIG_R 3.6541528853610088
+#define ZIG_V 0.00492867323399
+
+static double zig_pdf(double x) { return exp(-x * x / 2.0); }
+
+/* rand's ziggurat_tables.py (after Doornik 2005, zigNorInit): x_0 = V/f(R),
+ * x_1 = R, x_i = f^-1(V/x_{i-1} + f(x_{i-1})), x_256 = 0, f_i = f(x_i); each
+ * entry as the crate stores it, printed with 18 decimals and read back. */
+void ro_zig_tables(double x[257], double f[257]) {
+  double xv[257], fv[257];
+  xv[0] = ZIG_V / zig_pdf(ZIG_R);
+  xv[1] = ZIG_R;
+  for (int i = 2; i < 256; ++i) {
+    double last = xv[i - 1];
+    xv[i] = sqrt(-2.0 * log(ZIG_V / last + zig_pdf(last)));
+  }
+  xv[256] = 0.0;
+  for (int i = 0; i < 257; ++i) fv[i] = zig_pdf(xv[i]);
+  char buf[64];
+  for (int i = 0; i < 257; ++i) {
+    snprintf(buf, sizeof buf, "%.18f", xv[i]);
+    x[i] = strtod(buf, NULL);
+    snprintf(buf, sizeof buf, "%.18f", fv[i]);
+    f[i] = strtod(buf, NULL);
+  }
+}
+
+static double zig_x[257], zig_f[257];
+static int zig_ready;
+
+static double bits_to_double(uint64_t b) {
+  double d;
+  memcpy(&d, &b, sizeof d);
+  return d;
+}
+
+static double open01(chacha_rng *g) { /* rand 0.9 Open01 for f64 */
+  uint64_t v = rng_next_u64(g) >> 12;
+  return bits_to_double(v | (1023ULL << 52)) - (1.0 - 2.220446049250313e-16 / 2.0);
+}
+
+static double uniform53(chacha_rng *g) { /* rand 0.9 StandardUniform for f64 */
+  return (double)(rng_next_u64(g) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* rand_distr 0.5 StandardNormal for f64: the symmetric ziggurat, with the
+ * exponential tail sampler for layer 0. */
+static double std_normal(chacha_rng *g) {
+  if (!zig_ready) {
+    ro_zig_tables(zig_x, zig_f);
+    zig_ready = 1;
+  }
+  for (;;) {
+    uint64_t bits = rng_next_u64(g);
+    int i = (int)(bits & 0xff);
+    double u = bits_to_double((bits >> 12) | (1024ULL << 52)) - 3.0; /* [2,4) - 3 */
+    double x = u * zig_x[i];
+    if (fabs(x) < zig_x[i + 1]) return x;
+    if (i == 0) {
+      double tx = 1.0, ty = 0.0;
+      while (-2.0 * ty < tx * tx) {
+        double x_ = open01(g);
+        double y_ = open01(g);
+        tx = log(x_) / ZIG_R;
+        ty = log(y_);
+      }
+      return u < 0.0 ? tx - ZIG_R : ZIG_R - tx;
+    }
+    if (zig_f[i + 1] + (zig_f[i] - zig_f[i + 1]) * uniform53(g) < zig_pdf(x)) return x;
+  }
+}
+
+uint64_t ro_std_normals(uint64_t seed, uint64_t start, uint64_t n, double *out) {
+  chacha_rng g;
+  rng_init(&g, seed, start);
+  for (uint64_t i = 0; i < n; ++i) out[i] = std_normal(&g);
+  return rng_pos(&g);
+}
+
+void ro_draw_starts(uint64_t seed, uint64_t start, uint64_t n, uint64_t *out) {
+  chacha_rng g;
+  rng_init(&g, seed, start);
+  for (uint64_t i = 0; i < n; ++i) {
+    out[i] = rng_pos(&g);
+    (void)std_normal(&g);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
 /* Pricer, note.rs:38-373                                                     */
 /* ------------------------------------------------------------------------- */
 typedef struct {
@@ -830,6 +1006,7 @@ typedef struct {
   int kind;
   ro_sobol sobol;
   ro_halton halton;
+  chacha_rng rng; /* RO_VR_NONE: the request's single stream (note.rs:58) */
 } ladder;
 
 static double ladder_normal(const ladder *l, uint64_t point, int dim) { /* qmc.rs:203-208 */
@@ -858,7 +1035,21 @@ static int ladder_init(ladder *l, const ro_request *r) {
   if (r->vr_kind == RO_VR_SOBOL) return ro_sobol_init(&l->sobol, r->vr_dimension, r->seed, r->max_dim);
   if (r->vr_kind == RO_VR_HALTON)
     return ro_halton_init(&l->halton, r->vr_dimension, r->seed, r->max_dim);
+  if (r->vr_kind == RO_VR_NONE) {
+    rng_init(&l->rng, r->seed, 0);
+    return 0;
+  }
   return -1;
+}
+
+/* note.rs:123-126: F draws per step, correlated per step; shocks[f*steps + t]. */
+static void mc_path_shocks(ladder *lad, const double *chol, int F, int64_t steps, double *shocks) {
+  double z[64], y[64];
+  for (int64_t step = 0; step < steps; ++step) {
+    for (int f = 0; f < F; ++f) z[f] = std_normal(&lad->rng);
+    ro_chol_apply(F, chol, z, y);
+    for (int f = 0; f < F; ++f) shocks[f * steps + step] = y[f];
+  }
 }
 
 static void ladder_free(ladder *l) {
@@ -881,7 +1072,11 @@ int ro_path_shocks(const ro_request *r, uint64_t path_id, double *out) {
   }
   double dt = r->maturity_years / (double)r->steps;
   double *normals = malloc(sizeof(double) * (size_t)F * r->steps);
-  qmc_path_shocks(&lad, L, F, r->steps, dt, path_id, normals, out);
+  if (lad.kind == RO_VR_NONE) {
+    for (uint64_t p = 0; p <= path_id; ++p) mc_path_shocks(&lad, L, F, r->steps, out);
+  } else {
+    qmc_path_shocks(&lad, L, F, r->steps, dt, path_id, normals, out);
+  }
   free(normals);
   ladder_free(&lad);
   free(L);
@@ -893,8 +1088,8 @@ int ro_price(const ro_request *r, uint64_t path_begin, uint64_t path_end, ro_res
   if (ro_validate(r, code, codelen, msg, msglen)) return -1;
   const int F = r->factors;
   if (F > 64) FAIL("invalid_market_shape", "the oracle supports at most 64 factors");
-  if (r->vr_kind != RO_VR_SOBOL && r->vr_kind != RO_VR_HALTON)
-    FAIL("unsupported", "the oracle restates the QMC (Sobol/Halton) lanes only");
+  if (r->vr_kind != RO_VR_SOBOL && r->vr_kind != RO_VR_HALTON && r->vr_kind != RO_VR_NONE)
+    FAIL("unsupported", "unknown variance-reduction lane");
   const int64_t steps = r->steps;
   const int64_t periods = r->coupon_periods < 0 ? 9 : (r->coupon_periods < 1 ? 1 : r->coupon_periods);
   const int64_t call_start_step = r->call_start_step < 0 ? 0 : r->call_start_step;
@@ -920,9 +1115,14 @@ int ro_price(const ro_request *r, uint64_t path_begin, uint64_t path_end, ro_res
   memset(out, 0, sizeof *out);
   double spots[64];
 
+  if (lad.kind == RO_VR_NONE) /* the single stream: skip the draws of earlier paths */
+    for (uint64_t p = 0; p < path_begin; ++p) mc_path_shocks(&lad, L, F, steps, incr);
   for (uint64_t path_id = path_begin; path_id < path_end; ++path_id) {
     for (int f = 0; f < F; ++f) spots[f] = r->spots[f];
-    qmc_path_shocks(&lad, L, F, steps, dt, path_id, normals, incr);
+    if (lad.kind == RO_VR_NONE)
+      mc_path_shocks(&lad, L, F, steps, incr);
+    else
+      qmc_path_shocks(&lad, L, F, steps, dt, path_id, normals, incr);
     int knocked_out = 0;
     int64_t ko_step = -1;
     double redemption_time = r->maturity_years;

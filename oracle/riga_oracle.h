@@ -151,6 +151,34 @@ typedef struct {
   double final_worst, funding, put, coupon_pv, memory_pv, pv;
 } ro_path_record;
 
+/* Monte Carlo lane (VarianceReduction::None, note.rs:58,124-126): the reference
+ * draws from rand_chacha 0.9.0 ChaCha8Rng::seed_from_u64(seed) through rand_distr
+ * 0.5.1 StandardNormal (rand 0.9.5).  Those crates are not vendored; this is a
+ * restatement of their published algorithms:
+ *   seed_from_u64   rand_core 0.9 SeedableRng default: PCG32 expansion of the
+ *                   u64 into the 32-byte key
+ *   ChaCha8         8 rounds (4 double rounds), 64-bit block counter in words
+ *                   12-13 starting at 0, 64-bit stream 0 in words 14-15; the
+ *                   output is the blocks' words in order
+ *   next_u64        two consecutive output words, low word first
+ *   StandardNormal  256-layer ziggurat (Marsaglia-Tsang / Doornik) with the
+ *                   tables of rand's ziggurat_tables.py (R = 3.6541528853610088,
+ *                   V = 0.00492867323399), printed to 18 decimals in the crate
+ *   random::<f64>   (u64 >> 11) * 2^-53;  Open01: 52-bit fraction in [1,2) - (1 - eps/2)
+ * Pinned by the reference's golden PriceResult (fixtures/termsheet1.golden.json,
+ * the full result within 1e-12 at 30,000 paths, seed 1729) and by the ChaCha
+ * core's RFC 7539 known answers (20 rounds). */
+void ro_chacha_block(const uint32_t in[16], int double_rounds, uint32_t out[16]);
+void ro_seed_from_u64(uint64_t seed, uint32_t key[8]);
+/* The k-th u64 of the stream of ChaCha8Rng::seed_from_u64(seed). */
+uint64_t ro_rng_u64_at(uint64_t seed, uint64_t k);
+void ro_zig_tables(double x[257], double f[257]);
+/* Standard normals drawn from the stream starting at u64 index `start`; writes n
+ * values and returns the u64 index after the last draw. */
+uint64_t ro_std_normals(uint64_t seed, uint64_t start, uint64_t n, double *out);
+/* u64 index at which each of n consecutive draws starts (draw 0 starts at `start`). */
+void ro_draw_starts(uint64_t seed, uint64_t start, uint64_t n, uint64_t *out);
+
 /* Validation mirroring lib.rs:209-321 generalised to F factors.
  * Returns 0 or writes code/message. */
 int ro_validate(const ro_request *r, char *code, size_t codelen, char *msg, size_t msglen);
@@ -162,7 +190,9 @@ int ro_price(const ro_request *r, uint64_t path_begin, uint64_t path_end, ro_res
              ro_path_record *records, char *code, size_t codelen, char *msg, size_t msglen);
 
 /* Generate the correlated, bridged and 1/sqrt(dt)-scaled shock cube of one path:
- * out[f*steps + t].  Exposed for bridge/ladder parity tests. */
+ * out[f*steps + t].  Exposed for bridge/ladder parity tests.  On the Monte Carlo
+ * lane: the path's correlated draws chol * z per step (draws of earlier paths
+ * skipped, as the reference's single stream consumes them). */
 int ro_path_shocks(const ro_request *r, uint64_t path_id, double *out);
 
 #ifdef __cplusplus

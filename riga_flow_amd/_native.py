@@ -147,7 +147,7 @@ EXPORTS = ("riga_hip_init", "riga_hip_destroy", "riga_hip_device_name", "riga_hi
            "riga_server_create", "riga_server_submit", "riga_server_wait",
            "riga_server_stats_get", "riga_server_destroy",
            "riga_surface_create", "riga_surface_destroy", "riga_autotune",
-           "riga_lv_calibrate", "riga_lv_sigma", "riga_debug_sobol")
+           "riga_lv_calibrate", "riga_lv_sigma", "riga_debug_sobol", "riga_debug_mc_starts")
 
 _lib = None
 _lock = threading.Lock()
@@ -218,6 +218,8 @@ def lib():
                                            C.c_int32, C.c_int32, C.c_int32,
                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _D,
                                            C.POINTER(Error)]
+            L.riga_debug_mc_starts.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64,
+                                               C.POINTER(C.c_uint64), C.POINTER(Error)]
             _lib = L
     return _lib
 

@@ -511,3 +511,16 @@ def debug_sobol(seed: int, ladder_dims: int, first_point: int, n_points: int, fi
         u.ctypes.data_as(C.POINTER(C.c_uint32)), s.ctypes.data_as(C.POINTER(C.c_uint32)),
         N.dptr(z), C.byref(err)), err)
     return (u.reshape(n_points, n_dims), s.reshape(n_points, n_dims), z.reshape(n_points, n_dims))
+
+
+def debug_mc_starts(seed: int, draws_per_path: int, n_paths: int) -> np.ndarray:
+    """Device stream parse of the Monte Carlo lane: the u64 word of
+    ``ChaCha8Rng::seed_from_u64(seed)`` where each path's first StandardNormal
+    draw begins (path p starts at draw ``p * draws_per_path``)."""
+    gpu_backend()
+    out = np.zeros(n_paths, dtype=np.uint64)
+    err = N.Error()
+    N.check(N.lib().riga_debug_mc_starts(
+        N.Device.get().handle, seed, draws_per_path, n_paths,
+        out.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(err)), err)
+    return out

@@ -20,6 +20,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "host.h"
@@ -372,7 +373,8 @@ bool build_item(const std::vector<Prep>& preps, Item* it, int precision, bool re
   const int S = (int)it->members.size();
   it->smax = S <= 1 ? 1 : (S <= 4 ? 4 : 10);
   plan_bridge(T, dt, it);
-  if (it->depth > kBridgeStack) {
+  const bool mc = r.ladder == RIGA_VR_NONE;  // the Monte Carlo lane has no bridge
+  if (!mc && it->depth > kBridgeStack) {
     e->code = "invalid_schedule";
     e->message = "bridge depth exceeds the supported number of steps";
     return false;
@@ -393,7 +395,8 @@ bool build_item(const std::vector<Prep>& preps, Item* it, int precision, bool re
   d.call_start = (int32_t)(r.call_start_step < 0 ? 0 : r.call_start_step);
   d.flags = (r.memory_coupon ? kMemoryCoupon : 0) | (r.fixed_first_coupon ? kFixedFirst : 0) |
             (!p0.lv.empty() ? kLocalVol : 0) | (r.trace_level != RIGA_TRACE_OFF ? kTrace : 0) |
-            (r.ladder == RIGA_VR_HALTON ? kHalton : 0) | (records ? kRecords : 0);
+            (r.ladder == RIGA_VR_HALTON ? kHalton : 0) | (records ? kRecords : 0) | (mc ? kMonteCarlo : 0);
+  if (mc) chacha_key_from_seed(r.seed, d.mc_key);
   d.ladder_dims = p0.ladder_dims;
   d.ladder_rows = (int32_t)std::min<int64_t>(p0.ladder_dims, (int64_t)F * T);
   d.paths = r.paths;
@@ -539,6 +542,7 @@ void fill_result(const Partial& fr, uint64_t paths, riga_result* out) {
 // ------------------------------------------------------------ plans
 struct LaunchPlan {
   int F = 0, smax = 1, blocks = 0;
+  bool mc = false;       // Monte Carlo lane items
   int threads = kBlock;  // lanes per workgroup
   size_t lds = 0;
   size_t block_off = 0;     // first (item, chunk) pair of this launch in the block table
@@ -548,7 +552,9 @@ struct LaunchPlan {
 // Launch-shape signature of a batch: precision and, per (factors, scenario
 // bucket) launch, the item count and the log2 sizes of its paths and steps.
 // The autotuner's choices are kept per signature.
-std::string shape_signature(int precision, const std::map<std::pair<int, int>, std::vector<int>>& parts,
+using PartKey = std::tuple<int, int, int>;  // (factors, scenario bucket, Monte Carlo lane)
+
+std::string shape_signature(int precision, const std::map<PartKey, std::vector<int>>& parts,
                             const std::vector<Item>& items) {
   std::string k = precision == RIGA_FP64 ? "d" : "f";
   for (const auto& kv : parts) {
@@ -559,7 +565,8 @@ std::string shape_signature(int precision, const std::map<std::pair<int, int>, s
     }
     steps /= (double)kv.second.size();
     char buf[96];
-    snprintf(buf, sizeof buf, "|F%d.S%d.n%zu.p%d.t%d", kv.first.first, kv.first.second, kv.second.size(),
+    snprintf(buf, sizeof buf, "|%sF%d.S%d.n%zu.p%d.t%d", std::get<2>(kv.first) ? "mc." : "",
+             std::get<0>(kv.first), std::get<1>(kv.first), kv.second.size(),
              (int)std::ceil(std::log2(paths + 1.0)), (int)std::lround(std::log2(steps + 1.0)));
     k += buf;
   }
@@ -595,8 +602,21 @@ struct riga_plan {
     DevBuf buf;
   };
   std::vector<std::unique_ptr<NormalTable>> ztables;
+  // Monte Carlo lane: one stream parse per (seed, draws per path), giving the
+  // first stream word of paths [0, paths)
+  struct McParse {
+    uint64_t seed = 0, dpp = 0, paths = 0;
+    int chunk_words = 0, slow_cap = 0;
+    int64_t n_chunks = 0, off_key = 0;
+    DevBuf chunks, slow, entry, cnt, block_off, starts;
+    void each(void (*fn)(DevBuf&, hipStream_t), hipStream_t st) {
+      for (DevBuf* b : {&chunks, &slow, &entry, &cnt, &block_off, &starts}) fn(*b, st);
+    }
+  };
+  std::vector<std::unique_ptr<McParse>> mc;
+  DevBuf mc_status;  // [parse][2]: paths found, error flags
   std::vector<std::shared_ptr<OwenTable>> owen_tabs;  // the context's tables this plan reads
-  int64_t off_dirs = 0;
+  int64_t off_dirs = 0, off_zig = 0;
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipGraph_t g = nullptr;
@@ -609,10 +629,13 @@ struct riga_plan {
     // buffers go back to the pool on this stream before it is destroyed
     for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d}) b->release();
     for (auto& zt : ztables) zt->buf.release();
+    for (auto& mp : mc) mp->each([](DevBuf& b, hipStream_t) { b.release(); }, nullptr);
+    mc_status.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
   void bind_stream() {
-    for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d}) b->stream = stream;
+    for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d, &mc_status})
+      b->stream = stream;
   }
 };
 
@@ -643,6 +666,62 @@ hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t st
   return hipSuccess;
 }
 
+// Stream parse sizing for the first words of paths [0, mp->paths) at mp->dpp
+// draws per path: chunks are sized so a parse has ~32k or more of them (one
+// lane each), and the stream is cut 3% beyond the words the draws need (1.022
+// per draw on average) plus eight chunks.
+void mc_parse_size(riga_plan::McParse* mp) {
+  const double draws = (double)(mp->paths - 1) * (double)mp->dpp + 1.0;
+  int C = 512;
+  while (C < 4096 && draws * 1.03 / C > 32768.0) C *= 2;
+  mp->chunk_words = C;
+  mp->slow_cap = C / 16 + 32;
+  mp->n_chunks = (int64_t)std::ceil((draws * 1.03 + 8.0 * C) / C) + 1;
+}
+
+double mc_parse_bytes(const riga_plan::McParse& mp) {
+  const double nc = (double)mp.n_chunks;
+  return nc * (sizeof(McChunk) + (double)mp.slow_cap * 4 + 5) + ((mp.n_chunks + 255) / 256) * 8.0 +
+         (double)mp.paths * 8.0;
+}
+
+hipError_t mc_parse_alloc(riga_plan::McParse* mp) {
+  const size_t nc = (size_t)mp->n_chunks;
+  hipError_t e;
+  if ((e = mp->chunks.ensure(nc * sizeof(McChunk))) != hipSuccess) return e;
+  if ((e = mp->slow.ensure(nc * (size_t)mp->slow_cap * 4)) != hipSuccess) return e;
+  if ((e = mp->entry.ensure(nc)) != hipSuccess) return e;
+  if ((e = mp->cnt.ensure(nc * 4)) != hipSuccess) return e;
+  if ((e = mp->block_off.ensure((size_t)((mp->n_chunks + 255) / 256) * 8)) != hipSuccess) return e;
+  return mp->starts.ensure((size_t)mp->paths * 8);
+}
+
+hipError_t mc_parse_launch(const riga_plan::McParse& mp, const uint32_t* key, const double* zig, uint32_t* status,
+                           hipStream_t stream) {
+  McParseArgs pa;
+  pa.key = key;
+  pa.zig = zig;
+  pa.chunk_words = mp.chunk_words;
+  pa.slow_cap = mp.slow_cap;
+  pa.n_chunks = mp.n_chunks;
+  pa.slow = (uint32_t*)mp.slow.p;
+  pa.chunks = (McChunk*)mp.chunks.p;
+  McStartsArgs sa;
+  sa.chunks = pa.chunks;
+  sa.slow = pa.slow;
+  sa.slow_cap = mp.slow_cap;
+  sa.chunk_words = mp.chunk_words;
+  sa.n_chunks = mp.n_chunks;
+  sa.entry = (uint8_t*)mp.entry.p;
+  sa.cnt = (uint32_t*)mp.cnt.p;
+  sa.block_off = (uint64_t*)mp.block_off.p;
+  sa.dpp = mp.dpp;
+  sa.paths = mp.paths;
+  sa.starts = (uint64_t*)mp.starts.p;
+  sa.status = status;
+  return launch_mc_parse(pa, sa, stream);
+}
+
 // Knots at x0 + i*dx to within 1e-9 of the spacing (linspace grids, as the
 // calibration writes them): the FP32 tier then locates a knot by one multiply.
 bool evenly_spaced(const std::vector<double>& x) {
@@ -659,6 +738,12 @@ bool evenly_spaced(const std::vector<double>& x) {
 hipError_t enqueue(riga_plan* pl, bool timed) {
   hipError_t e;
   if (timed && (e = hipEventRecord(pl->e0, pl->stream)) != hipSuccess) return e;
+  for (size_t k = 0; k < pl->mc.size(); ++k) {
+    const unsigned char* a = (const unsigned char*)pl->arena.p;
+    e = mc_parse_launch(*pl->mc[k], (const uint32_t*)(a + pl->mc[k]->off_key), (const double*)(a + pl->off_zig),
+                        (uint32_t*)pl->mc_status.p + 2 * k, pl->stream);
+    if (e != hipSuccess) return e;
+  }
   for (const auto& zt : pl->ztables) {
     const unsigned char* a = (const unsigned char*)pl->arena.p;
     const uint32_t* dirs = (const uint32_t*)(a + pl->off_dirs);
@@ -674,6 +759,7 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
     PhoenixLaunch L;
     L.F = lp.F;
     L.smax = lp.smax;
+    L.mc = lp.mc;
     L.blocks = lp.blocks;
     L.lds = lp.lds;
     L.threads = lp.threads;
@@ -689,7 +775,10 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
   e = launch_finalize((const ItemDesc*)pl->items_d.p, (int)pl->items.size(), (const Partial*)pl->partials.p,
                       (Partial*)pl->finals.p, (const int64_t*)pl->final_base_d.p, pl->stream);
   if (e != hipSuccess) return e;
-  return hipMemcpyAsync(pl->finals_host.p, pl->finals.p, (size_t)pl->final_slots * sizeof(Partial),
+  const size_t fin = (size_t)pl->final_slots * sizeof(Partial);
+  e = hipMemcpyAsync(pl->finals_host.p, pl->finals.p, fin, hipMemcpyDeviceToHost, pl->stream);
+  if (e != hipSuccess || pl->mc.empty()) return e;
+  return hipMemcpyAsync((unsigned char*)pl->finals_host.p + fin, pl->mc_status.p, 8 * pl->mc.size(),
                         hipMemcpyDeviceToHost, pl->stream);
 }
 
@@ -771,8 +860,9 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   }
   // launches: one per (factors, scenario bucket), blocks longest-first
   const int real_size = precision == RIGA_FP64 ? 8 : 4;
-  std::map<std::pair<int, int>, std::vector<int>> parts;
-  for (size_t i = 0; i < items.size(); ++i) parts[{items[i].F, items[i].smax}].push_back((int)i);
+  std::map<PartKey, std::vector<int>> parts;
+  for (size_t i = 0; i < items.size(); ++i)
+    parts[PartKey(items[i].F, items[i].smax, (items[i].d.flags & kMonteCarlo) ? 1 : 0)].push_back((int)i);
   pl->signature = shape_signature(precision, parts, items);
   if (ppt == 0 || waves == 0) {  // the autotuned shape for this signature, else 1 path x 4 waves
     int tp = 1, tw = 4;
@@ -793,16 +883,19 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   Arena arena;
   int sobol_rows = 1;
   for (Item& it : items)
-    if (!(it.d.flags & kHalton)) sobol_rows = std::max(sobol_rows, it.d.ladder_rows);
+    if (!(it.d.flags & (kHalton | kMonteCarlo))) sobol_rows = std::max(sobol_rows, it.d.ladder_rows);
   const std::vector<uint32_t>& dirs = sobol_directions(sobol_rows);
   const int64_t off_dirs = arena.add(dirs.data(), (size_t)sobol_rows * 32);
+  bool any_mc = false;
+  for (Item& it : items) any_mc = any_mc || (it.d.flags & kMonteCarlo) != 0;
+  const int64_t off_zig = any_mc ? arena.add(ziggurat_tables().data(), 2 * kZigLen) : 0;
   // one Owen prefix table per seed (built on the device at creation), covering
   // the largest raw dimension used
   const bool owen_table = !(cfg && cfg->owen_table < 0);
   std::map<uint64_t, int> owen_dims;
   if (owen_table) {
     for (Item& it : items)
-      if (!(it.d.flags & kHalton)) {
+      if (!(it.d.flags & (kHalton | kMonteCarlo))) {
         int& m = owen_dims[it.d.seed];
         m = std::max(m, it.d.F * it.d.steps);
       }
@@ -817,8 +910,9 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     std::vector<int>& idx = kv.second;
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return items[a].cost > items[b].cost; });
     LaunchPlan lp;
-    lp.F = kv.first.first;
-    lp.smax = kv.first.second;
+    lp.F = std::get<0>(kv.first);
+    lp.smax = std::get<1>(kv.first);
+    lp.mc = std::get<2>(kv.first) != 0;
     lp.items = idx;
     int stack_levels = 0, lv_words = 0;
     for (int i : idx) {
@@ -828,7 +922,10 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       lv_words = std::max(lv_words, (int)items[i].surfaces.size() * (real_size == 4 ? 4 : 3) * max_nx);
     }
     lp.threads = threads;
-    lp.lds = (size_t)stack_levels * lp.F * threads * real_size + (size_t)lv_words * real_size;
+    if (lp.mc)  // stream ring and ziggurat tables instead of the bridge stack
+      lp.lds = (size_t)kMcRingSlots * threads * 8 + 2 * kZigLen * 8 + (size_t)lv_words * real_size;
+    else
+      lp.lds = (size_t)stack_levels * lp.F * threads * real_size + (size_t)lv_words * real_size;
     if (lp.lds + (size_t)lp.smax * sizeof(BlockShared) > ctx->lds_limit) {
       set_err(err, "invalid_local_vol",
               "the batch needs " + std::to_string(lp.lds) + " bytes of LDS per workgroup (limit " +
@@ -840,7 +937,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     for (int i : idx) {
       Item& it = items[i];
       ItemDesc& d = it.d;
-      d.stack_levels = stack_levels;
+      d.stack_levels = lp.mc ? 0 : stack_levels;
       d.paths_per_block = threads * ppt;
       d.n_blocks = (int32_t)((d.paths + d.paths_per_block - 1) / d.paths_per_block);
       blocks += d.n_blocks;
@@ -849,7 +946,8 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       d.record_base = pl->record_slots;
       if (d.flags & kRecords) pl->record_slots += (int64_t)d.S * (int64_t)d.paths;
       d.off_dirs = off_dirs;
-      if (owen_table && !(d.flags & kHalton)) d.flags |= kOwenTable;  // table address set below
+      if (owen_table && !(d.flags & (kHalton | kMonteCarlo))) d.flags |= kOwenTable;  // table address set below
+      d.off_zig = off_zig;
       d.off_nodes = arena.add(it.nodes.data(), it.nodes.size());
       d.off_steps = arena.add(it.steps.data(), it.steps.size());
       d.off_lvb = arena.add(it.lvb.data(), it.lvb.size());
@@ -938,7 +1036,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     std::map<uint64_t, std::vector<int>> by_seed;
     for (size_t i = 0; i < items.size(); ++i) {
       const ItemDesc& d = items[i].d;
-      if (!(d.flags & kHalton) && d.ladder_dims >= d.F * d.steps && d.path_begin == 0)
+      if (!(d.flags & (kHalton | kMonteCarlo)) && d.ladder_dims >= d.F * d.steps && d.path_begin == 0)
         by_seed[d.seed].push_back((int)i);
     }
     const size_t real = precision == RIGA_FP64 ? 8 : 4;
@@ -972,6 +1070,31 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       pl->ztables.push_back(std::move(zt));
     }
   }
+  // Monte Carlo lane: one stream parse per (seed, draws per path)
+  double mc_bytes = 0.0;
+  {
+    std::map<std::pair<uint64_t, uint64_t>, std::vector<int>> by_stream;
+    for (size_t i = 0; i < items.size(); ++i)
+      if (items[i].d.flags & kMonteCarlo)
+        by_stream[{items[i].d.seed, (uint64_t)items[i].d.F * (uint64_t)items[i].d.steps}].push_back((int)i);
+    for (auto& kv : by_stream) {
+      std::unique_ptr<riga_plan::McParse> mp(new riga_plan::McParse());
+      mp->seed = kv.first.first;
+      mp->dpp = kv.first.second;
+      for (int i : kv.second) mp->paths = std::max<uint64_t>(mp->paths, items[i].d.path_begin + items[i].d.paths);
+      mc_parse_size(mp.get());
+      uint32_t key[8];
+      chacha_key_from_seed(mp->seed, key);
+      mp->off_key = arena.add(key, 8);
+      mc_bytes += mc_parse_bytes(*mp);
+      if (stream_ordered) mp->each([](DevBuf& b, hipStream_t st) { b.stream = st; }, pl->stream);
+      if (!estimate) HIP_TRY(mc_parse_alloc(mp.get()), "stream parse alloc");
+      for (int i : kv.second) items[i].d.mc_start = (const uint64_t*)mp->starts.p;
+      pl->mc.push_back(std::move(mp));
+    }
+    if (!pl->mc.empty() && !estimate) HIP_TRY(pl->mc_status.ensure(8 * pl->mc.size()), "stream status alloc");
+  }
+  pl->off_zig = off_zig;
   // block table: (item, chunk) pairs per launch.  Items reading a shared normal
   // table go chunk-major (every item's chunk c before any item's chunk c+1), so
   // the blocks on the device at once read the same path columns of the table
@@ -1002,11 +1125,12 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   std::vector<ItemDesc> descs(items.size());
   for (size_t i = 0; i < items.size(); ++i) descs[i] = items[i].d;
   if (estimate) {
-    *estimate = (int64_t)(ztable_bytes + (double)arena.bytes.size() + descs.size() * sizeof(ItemDesc) +
+    *estimate = (int64_t)(ztable_bytes + mc_bytes + (double)arena.bytes.size() + descs.size() * sizeof(ItemDesc) +
                           all_blocks.size() * 4.0 + (double)pl->partial_slots * sizeof(Partial) +
                           (double)pl->final_slots * sizeof(Partial) + pl->final_base.size() * 8.0 +
                           (double)pl->record_slots * sizeof(PathRecordDev) + 8 * 4096.0);
     for (auto& zt : pl->ztables) zt->buf.stream = nullptr;
+    for (auto& mp : pl->mc) mp->each([](DevBuf& b, hipStream_t) { b.stream = nullptr; }, nullptr);
     return 0;
   }
 
@@ -1018,7 +1142,8 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   HIP_TRY(pl->partials.ensure((size_t)pl->partial_slots * sizeof(Partial)), "partials alloc");
   HIP_TRY(pl->finals.ensure((size_t)pl->final_slots * sizeof(Partial)), "finals alloc");
   HIP_TRY(pl->final_base_d.ensure(pl->final_base.size() * sizeof(int64_t)), "final index alloc");
-  HIP_TRY(pl->finals_host.ensure((size_t)pl->final_slots * sizeof(Partial)), "host finals alloc");
+  HIP_TRY(pl->finals_host.ensure((size_t)pl->final_slots * sizeof(Partial) + 8 * pl->mc.size()),
+          "host finals alloc");
   if (records) {
     HIP_TRY(pl->records_d.ensure((size_t)pl->record_slots * sizeof(PathRecordDev)), "records alloc");
     pl->recs_host.resize((size_t)pl->record_slots);
@@ -1076,6 +1201,14 @@ int plan_wait(riga_plan* pl, riga_result* results, riga_path_record* records, ri
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, pl->e0, pl->e1), "event timing");
   const Partial* finals = (const Partial*)pl->finals_host.p;
+  const uint32_t* mc_status = (const uint32_t*)(finals + pl->final_slots);
+  for (size_t k = 0; k < pl->mc.size(); ++k)
+    if (mc_status[2 * k] != pl->mc[k]->paths || mc_status[2 * k + 1] != 0) {
+      set_err(err, "mc_stream_error",
+              "Monte Carlo stream parse: " + std::to_string(mc_status[2 * k]) + " of " +
+                  std::to_string(pl->mc[k]->paths) + " path starts, flags " + std::to_string(mc_status[2 * k + 1]));
+      return RIGA_EGPU;
+    }
   for (size_t i = 0; i < pl->items.size(); ++i) {
     const Item& it = pl->items[i];
     for (int s = 0; s < it.d.S; ++s) {
@@ -1453,6 +1586,49 @@ int riga_debug_sobol(riga_ctx* ctx, uint64_t seed, int32_t ladder_dims, uint64_t
   HIP_TRY(hipMemcpyAsync(out_scrambled, d_s.p, cells * 4, hipMemcpyDeviceToHost, ctx->stream), "download");
   HIP_TRY(hipMemcpyAsync(out_normal, d_n.p, cells * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
   HIP_TRY(hipStreamSynchronize(ctx->stream), "debug kernel execution");
+  return 0;
+}
+
+int riga_debug_mc_starts(riga_ctx* ctx, uint64_t seed, uint64_t draws_per_path, uint64_t n_paths,
+                         uint64_t* out_starts, riga_error* err) {
+  clear_err(err);
+  if (!ctx) {
+    set_err(err, "gpu_unavailable", "no device context");
+    return RIGA_ENODEV;
+  }
+  if (draws_per_path == 0 || n_paths == 0 || !out_starts || (double)draws_per_path * (double)n_paths > 1e12) {
+    set_err(err, "invalid_argument", "bad debug_mc_starts sizes");
+    return RIGA_EINVAL;
+  }
+  riga_plan::McParse mp;
+  mp.seed = seed;
+  mp.dpp = draws_per_path;
+  mp.paths = n_paths;
+  mc_parse_size(&mp);
+  uint32_t key[8];
+  chacha_key_from_seed(seed, key);
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  DevBuf d_key, d_zig, d_status;
+  HIP_TRY(d_key.ensure(32), "alloc");
+  HIP_TRY(d_zig.ensure(2 * kZigLen * 8), "alloc");
+  HIP_TRY(d_status.ensure(8), "alloc");
+  HIP_TRY(mc_parse_alloc(&mp), "alloc");
+  HIP_TRY(hipMemcpyAsync(d_key.p, key, 32, hipMemcpyHostToDevice, ctx->stream), "upload");
+  HIP_TRY(hipMemcpyAsync(d_zig.p, ziggurat_tables().data(), 2 * kZigLen * 8, hipMemcpyHostToDevice, ctx->stream),
+          "upload");
+  HIP_TRY(mc_parse_launch(mp, (const uint32_t*)d_key.p, (const double*)d_zig.p, (uint32_t*)d_status.p, ctx->stream),
+          "stream parse");
+  uint32_t status[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(status, d_status.p, 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+  HIP_TRY(hipMemcpyAsync(out_starts, mp.starts.p, (size_t)n_paths * 8, hipMemcpyDeviceToHost, ctx->stream),
+          "download");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "stream parse execution");
+  if (status[0] != n_paths || status[1] != 0) {
+    set_err(err, "mc_stream_error", "stream parse found " + std::to_string(status[0]) + " of " +
+                                        std::to_string(n_paths) + " path starts, flags " + std::to_string(status[1]));
+    return RIGA_EGPU;
+  }
   return 0;
 }
 

@@ -38,6 +38,13 @@ const std::vector<uint32_t>& sobol_directions(int dims);
 // [bases(dims)][perm offsets(dims)][perms...] in uint32 words.
 std::vector<uint32_t> halton_tables(int dims, uint64_t seed);
 
+// ---- Monte Carlo lane: rand_core 0.9 seed_from_u64 (PCG32 expansion of the
+// seed into the ChaCha8 key) and rand_distr 0.5 ziggurat tables
+// (ziggurat_tables.py: R = 3.6541528853610088, V = 0.00492867323399, each
+// entry as printed to 18 decimals in the crate): x[257] then f(x)[257].
+void chacha_key_from_seed(uint64_t seed, uint32_t key[8]);
+const std::vector<double>& ziggurat_tables();
+
 // ---- request validation (crates/riga-engine/src/lib.rs:209-321, F factors)
 bool validate_request(const riga_request& r, Error* err);
 

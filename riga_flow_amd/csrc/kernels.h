@@ -13,6 +13,7 @@ constexpr int kScenarioBuckets[3] = {1, 4, 10};
 struct PhoenixLaunch {
   int F;       // factors (1..kMaxFactors)
   int smax;    // scenario bucket (1, 4 or 10)
+  bool mc;     // Monte Carlo lane items (ChaCha8 + ziggurat) instead of QMC
   int blocks;  // grid size
   int threads; // lanes per workgroup: 64, 128 or 256
   size_t lds;  // dynamic LDS bytes (bridge stack and local-vol rows)
@@ -32,6 +33,31 @@ hipError_t launch_normal_table_f64(const uint32_t* dirs, const uint16_t* owen_ta
 hipError_t launch_normal_table_f32(const uint32_t* dirs, const uint16_t* owen_tab, uint64_t seed,
                                    int dims, int n_paths, int nbits, float* out, hipStream_t stream);
 
+// Monte Carlo lane stream parse (kernels_mc.hip) for one (seed, draws per path).
+struct McParseArgs {
+  const uint32_t* key;   // ChaCha8 key words (device)
+  const double* zig;     // ziggurat tables: x[kZigLen] then f[kZigLen] (device)
+  int chunk_words;       // words per chunk, a multiple of 8
+  int slow_cap;          // slow-word list capacity per chunk
+  int64_t n_chunks;
+  uint32_t* slow;        // [n_chunks][slow_cap]
+  McChunk* chunks;       // [n_chunks]
+};
+struct McStartsArgs {
+  const McChunk* chunks;
+  const uint32_t* slow;
+  int slow_cap, chunk_words;
+  int64_t n_chunks;
+  uint8_t* entry;        // [n_chunks] resolved entry offset
+  uint32_t* cnt;         // [n_chunks] draws starting in the chunk
+  uint64_t* block_off;   // [ceil(n_chunks / 256)] block sums, then their exclusive prefix sums
+  uint64_t dpp;          // draws per path (steps * factors)
+  uint64_t paths;        // paths whose first word is wanted: [0, paths)
+  uint64_t* starts;      // [paths]
+  uint32_t* status;      // [2]: paths found, error flags
+};
+hipError_t launch_mc_parse(const McParseArgs& p, const McStartsArgs& s, hipStream_t stream);
+
 // Owen prefix table: out[dim][v] = scrambled top 12 bits of v << 20, dims [0, dims).
 hipError_t launch_owen_table(uint64_t seed, int dims, uint16_t* out, hipStream_t stream);
 
@@ -45,7 +71,5 @@ hipError_t launch_debug_sobol(const uint32_t* dirs, int ladder_dims, uint64_t se
                               int nbits, uint32_t* unscrambled, uint32_t* scrambled,
                               double* normal, hipStream_t stream);
 
-// Register use of a kernel variant (for the autotuner and the report).
-int phoenix_vgprs(int precision, int F, int smax);
 
 }  // namespace riga

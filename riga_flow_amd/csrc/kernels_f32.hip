@@ -8,8 +8,12 @@ namespace riga {
 template <int F, int SMAX>
 static hipError_t launch_one(const PhoenixLaunch& l, hipStream_t stream) {
   if (l.threads < 64 || l.threads > kBlock || l.threads % 64 != 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((phoenix_kernel<float, F, SMAX>), dim3(l.blocks), dim3(l.threads), l.lds,
-                     stream, l.items, l.block_item, l.arena, l.partials, l.records);
+  if (l.mc)
+    hipLaunchKernelGGL((phoenix_kernel<float, F, SMAX, true>), dim3(l.blocks), dim3(l.threads), l.lds,
+                       stream, l.items, l.block_item, l.arena, l.partials, l.records);
+  else
+    hipLaunchKernelGGL((phoenix_kernel<float, F, SMAX, false>), dim3(l.blocks), dim3(l.threads), l.lds,
+                       stream, l.items, l.block_item, l.arena, l.partials, l.records);
   return hipGetLastError();
 }
 

@@ -7,6 +7,8 @@
 //   qmc.rs:319-430, riga-engine/src/lib.rs:209-321.
 // Built with -ffp-contract=off (no fused multiply-add on any target).
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -384,6 +386,42 @@ std::vector<uint32_t> halton_tables(int dims, uint64_t seed) {
 
 
 
+// ---------------------------------------------------------------- Monte Carlo lane
+void chacha_key_from_seed(uint64_t seed, uint32_t key[8]) {
+  // PCG32 (XSH-RR): the state advances before each output
+  uint64_t st = seed;
+  for (int w = 0; w < 8; ++w) {
+    st = st * 6364136223846793005ULL + 11634580027462260723ULL;
+    const uint32_t v = (uint32_t)(((st >> 18) ^ st) >> 27);
+    const unsigned rot = (unsigned)(st >> 59);
+    key[w] = rot ? (v >> rot) | (v << (32 - rot)) : v;
+  }
+}
+
+const std::vector<double>& ziggurat_tables() {
+  static const std::vector<double> tab = [] {
+    const double R = 3.6541528853610088, V = 0.00492867323399;
+    auto pdf = [](double x) { return exp(-x * x / 2.0); };
+    double x[257];
+    x[0] = V / pdf(R);
+    x[1] = R;
+    for (int i = 2; i < 256; ++i) x[i] = sqrt(-2.0 * log(V / x[i - 1] + pdf(x[i - 1])));
+    x[256] = 0.0;
+    std::vector<double> out(2 * 257);
+    auto stored = [](double v) {  // the crate's 18-decimal literal, read back
+      char buf[64];
+      snprintf(buf, sizeof buf, "%.18f", v);
+      return strtod(buf, nullptr);
+    };
+    for (int i = 0; i < 257; ++i) {
+      out[i] = stored(x[i]);
+      out[257 + i] = stored(pdf(x[i]));
+    }
+    return out;
+  }();
+  return tab;
+}
+
 // ---------------------------------------------------------------- validation
 bool validate_request(const riga_request& r, Error* err) {
   const int F = r.factors;
@@ -421,9 +459,8 @@ bool validate_request(const riga_request& r, Error* err) {
     return fail(err, "invalid_trade", "coupon_rate must be finite and non-negative when present");
   if (r.strike <= 0.0 || r.knock_in_barrier <= 0.0)
     return fail(err, "invalid_trade", "strike must be positive and barriers finite");
-  if (r.ladder != RIGA_VR_SOBOL && r.ladder != RIGA_VR_HALTON)
-    return fail(err, "unsupported_variance_reduction",
-                "the gfx950 backend prices the QMC lanes (sobol, halton)");
+  if (r.ladder != RIGA_VR_SOBOL && r.ladder != RIGA_VR_HALTON && r.ladder != RIGA_VR_NONE)
+    return fail(err, "unsupported_variance_reduction", "unknown variance-reduction lane");
   return true;
 }
 

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mc_rng.hpp"
 #include "phoenix_layout.h"
 
 namespace riga {
@@ -424,7 +425,7 @@ __global__ __launch_bounds__(256) void normal_table_kernel(const uint32_t* __res
 }
 
 // ---------------------------------------------------------------- the kernel
-template <typename Real, int F, int SMAX>
+template <typename Real, int F, int SMAX, bool kMC>
 __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restrict__ items,
                                                           const int32_t* __restrict__ block_item,
                                                           const unsigned char* __restrict__ arena,
@@ -450,9 +451,14 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   constexpr bool kLogState = sizeof(Real) == 4;  // FP32 tier carries X = ln(S / S0)
   constexpr int kSurfWords = kLogState ? 4 : 3;   // LDS words per knot and surface
   const bool lv_uniform = kLogState && (it.flags & kLvUniform) != 0;
-  // dynamic LDS: [bridge stack: (depth-1) x F x lanes][local-vol rows: n_surf x kSurfWords x max_nx]
+  // dynamic LDS: QMC lanes [bridge stack: (depth-1) x F x lanes][local-vol rows:
+  // n_surf x kSurfWords x max_nx]; the Monte Carlo lane [stream ring: slots x
+  // lanes][ziggurat x_i, f(x_i)][local-vol rows]
+  uint64_t* mc_ring = reinterpret_cast<uint64_t*>(lds_dyn);
+  double* zx = reinterpret_cast<double*>(mc_ring + (kMC ? kMcRingSlots * nthreads : 0));
+  double* zf = zx + kZigLen;
   Real* lds_stack = reinterpret_cast<Real*>(lds_dyn);
-  Real* lds_w = lds_stack + (size_t)it.stack_levels * F * nthreads;
+  Real* lds_w = kMC ? reinterpret_cast<Real*>(zf + kZigLen) : lds_stack + (size_t)it.stack_levels * F * nthreads;
   const Real* lds = lds_w;
   const int nxs = it.max_nx;
   const StepInfo* stepinfo = at<StepInfo>(arena, it.off_steps);
@@ -471,7 +477,12 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     const SfF32* sfg = at<SfF32>(arena, it.off_sf32);
     for (int i = tid; i < S * F; i += nthreads) sf_lds[i] = sfg[i];
   }
+  if constexpr (kMC) {
+    const double* zg = at<double>(arena, it.off_zig);
+    for (int i = tid; i < 2 * kZigLen; i += nthreads) zx[i] = zg[i];
+  }
   __syncthreads();
+  const Real sqrt_dt_r = (Real)it.sqrt_dt;
 
   const int ppb = it.paths_per_block;
   const int batches = ppb / nthreads;
@@ -496,7 +507,13 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     // coupons as a count (bits 20-30) and coupon received (bit 31) in one word
     uint32_t pk[SMAX];
     Real worst_last[SMAX];
-    {
+    McRing ring;
+    if constexpr (kMC) {
+      // the path's words start where the parse found its first draw
+      ring.base = mc_ring + tid;
+      ring.stride = nthreads;
+      ring.seek(it.mc_key, it.mc_start[valid ? path : it.path_begin]);
+    } else {
       Real y[F];
       node_normals<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, arena, n, gray, 0, y);
 #pragma unroll
@@ -531,8 +548,22 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
 
     for (int step = 0; step < steps; ++step) {
       const StepInfo si = stepinfo[step];
+      Real shock[F];
+      Real dw[F];  // Brownian increment W[t+1] - W[t]
+      if constexpr (kMC) {
+        // ---- Monte Carlo lane (note.rs:123-126): F StandardNormals in stream
+        // order, shocks = L z.  A wave tops up every lane's ring together when
+        // any lane is short of F words, so the ChaCha blocks run converged.
+        if (__ballot(ring.fill < F) != 0ull && ring.fill <= kMcRingSlots - 8) ring.append(it.mc_key);
+        Real z[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) z[f] = (Real)mc_normal(ring, it.mc_key, zx, zf);
+        correlate<Real, F>(it, z, shock);
+#pragma unroll
+        for (int f = 0; f < F; ++f) dw[f] = shock[f] * sqrt_dt_r;
+      } else {
       // ---- streaming bridge: descend to the interval (step, step+1)
-      Real wnext[F], shock[F];
+      Real wnext[F];
       if (si.n_desc == 0) {
 #pragma unroll
         for (int f = 0; f < F; ++f) wnext[f] = bs.top[f];
@@ -558,12 +589,12 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
           }
         }
       }
-      Real dw[F];  // Brownian increment W[t+1] - W[t]
 #pragma unroll
       for (int f = 0; f < F; ++f) {
         dw[f] = wnext[f] - wlo[f];
         shock[f] = dw[f] * (Real)it.inv_sqrt_dt;  // qmc.rs:169, note.rs:101-105
         wlo[f] = wnext[f];
+      }
       }
 
       // ---- local-vol rows for this step (uniform across the block)

@@ -32,6 +32,30 @@ enum ItemFlags : uint32_t {
   kOwenTable = 1u << 6,      // top 12 Owen levels read from a per-dimension table
   kNormalTable = 1u << 7,    // ladder normals read from the batch's shared [dim][path] table
   kLvUniform = 1u << 8,      // FP32 tier: every surface has evenly spaced knots (slope rows)
+  kMonteCarlo = 1u << 9,     // VarianceReduction::None: ChaCha8 + ziggurat normals (note.rs:124-126)
+};
+
+// Monte Carlo lane (ChaCha8 stream + StandardNormal ziggurat).
+constexpr int kZigLen = 257;        // ziggurat table entries (256 layers + the x = 0 end)
+constexpr int kMcRingSlots = 24;    // stream words buffered per lane in LDS: three ChaCha blocks
+constexpr int kMcEntries = 16;      // chunk entry offsets tracked by the stream parse
+constexpr int kZigMaxLen = 250;     // longest draw accepted, in words (never reached in practice)
+
+// Stream parse, one record per chunk of `chunk_words` words: for each entry
+// offset e (the first word of the chunk that starts a draw, given how far the
+// previous chunk's last draw reached), the number of draws that start in the
+// chunk and the entry offset of the next chunk.
+enum McFlags : uint32_t {
+  kMcConstExit = 1u << 0,  // the next chunk's entry does not depend on this chunk's entry
+  kMcErrLong = 1u << 1,    // a draw longer than kZigMaxLen words
+  kMcErrSlow = 1u << 2,    // more slow draws than the chunk's list holds
+  kMcErrExit = 1u << 3,    // a draw reaching kMcEntries or more words into the next chunk
+};
+struct McChunk {
+  int32_t count[kMcEntries];
+  uint8_t exit[kMcEntries];
+  int32_t nslow;           // slow draws listed: (offset << 8) | words, in offset order
+  uint32_t flags;
 };
 
 // Per observation date, uniform across all paths of an item.
@@ -132,6 +156,11 @@ struct ItemDesc {
   int64_t off_wt32;      // float[steps * F]: local-vol tenor weight per (step, factor)
   uint64_t ztable;       // device address of the shared normal table (Real[dims][zt_stride])
   int64_t zt_stride;     // paths per table row
+  // Monte Carlo lane: the ChaCha8 key of seed_from_u64(seed), the first stream
+  // word of every path (from the parse) and the ziggurat tables in the arena
+  uint32_t mc_key[8];
+  const uint64_t* mc_start;  // [path]: stream word of the path's first draw
+  int64_t off_zig;           // double[2 * kZigLen]: x_i then f(x_i)
 };
 
 // Double-double accumulator (hi + lo).

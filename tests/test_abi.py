@@ -44,6 +44,16 @@ def _code(req, **trade):
     return e.value.code
 
 
+def _raw_ladder_code(ladder):
+    import ctypes as C
+    from riga_flow_amd import _native as N
+    p = api._Packed(W.flat_book(paths=16), 512)
+    p.r.ladder = ladder
+    err = N.Error()
+    assert N.lib().riga_validate(C.byref(p.r), C.byref(err)) != 0
+    return err.code.decode()
+
+
 def test_validation_error_codes():
     ok = W.flat_book(paths=16)
     api.validate(ok, max_dimension=W.max_dimension_for(ok))
@@ -63,7 +73,8 @@ def test_validation_error_codes():
     assert _code(req) == "invalid_market"
     req = W.flat_book(paths=16)
     req.config.variance_reduction = W.VarianceReduction("none")
-    assert _code(req) == "unsupported_variance_reduction"
+    api.validate(req)  # the ChaCha8 Monte Carlo lane
+    assert _raw_ladder_code(7) == "unsupported_variance_reduction"
     req = W.flat_book(paths=16)
     req.market.spots = [100.0] * 6
     req.market.vols = [0.2] * 6

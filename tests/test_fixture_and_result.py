@@ -3,9 +3,9 @@ assembly strings, pinned against the reference's fixture, its golden result
 (fixtures/termsheet1.golden.json) and its loader test (fixture.rs:471-538).
 
 The golden PriceResult comes from the ChaCha8/ziggurat Monte Carlo lane
-(rand 0.9.5 / rand_chacha 0.9.0 / rand_distr 0.5.1, not vendored, not
-importable here): its RNG-dependent numbers are not reproducible by this
-build (parity unpinned for that lane); the RNG-independent parts are checked.
+(rand 0.9.5 / rand_chacha 0.9.0 / rand_distr 0.5.1, not vendored): the oracle
+reproduces it in full (tests/test_oracle_mc.py) and the device within 1e-10
+(tests/test_gpu_mc.py); here the RNG-independent parts are checked.
 """
 import json
 import math

@@ -97,11 +97,6 @@ def test_fp32_tier_at_the_reference_dimension_cap():
 
 
 def test_rejected_requests_name_the_reference_codes():
-    req = _flat(1024, steps=63, years=0.25)
-    req.config.variance_reduction = W.VarianceReduction("none", 0)
-    with pytest.raises(RigaError) as e:
-        api.price_put_leg(req, "fp64")
-    assert e.value.code == "unsupported_variance_reduction"
     bad = _flat(1024, steps=63, years=0.25)
     bad.market.spots[0] = -1.0
     with pytest.raises(RigaError) as e:

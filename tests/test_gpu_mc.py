@@ -1,0 +1,127 @@
+"""GPU parity of the Monte Carlo lane (VarianceReduction::None: ChaCha8Rng +
+StandardNormal, note.rs:58,124-126) through the C ABI against the oracle.
+
+Bars: the stream parse's path-start word indices and the per-path KO/KI events
+bit-exact; FP64 prices within 1e-10 relative (the oracle itself meets the
+reference golden's 1e-12); FP32-tier prices within 1e-4 relative.  The
+ziggurat's accept/reject decisions call exp/log, so a decision could differ from
+the host's only if a device result differed from glibc's in the last bit exactly
+at the comparison boundary (~1e-16 per slow draw).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from parity import FP32_REL, assert_fp64_parity, rel
+from riga_flow_amd import api, workloads as W
+from riga_flow_amd.fixture import load_termsheet1_fixture
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _mc(req):
+    req.config.variance_reduction = W.VarianceReduction("none")
+    return req
+
+
+@pytest.mark.parametrize("seed,dpp,n_paths", [
+    (1729, 210, 30_000),   # the golden's stream: 105 steps x 2 factors, 6.3M draws
+    (7, 1, 20_000),        # every draw starts a path
+    (11, 3, 5_000),
+    (42, 2268, 700),       # 756 steps x 3 factors
+    (5, 97, 1),
+])
+def test_stream_parse_finds_every_path_start(seed, dpp, n_paths):
+    got = api.debug_mc_starts(seed, dpp, n_paths)
+    starts = O.draw_starts(seed, (n_paths - 1) * dpp + 1)
+    want = starts[np.arange(n_paths) * dpp]
+    assert np.array_equal(got, want)
+
+
+def test_termsheet1_golden_on_the_device():
+    lf = load_termsheet1_fixture(os.path.join(GOLDEN, "termsheet1.json"))
+    req = lf.request
+    golden = json.load(open(os.path.join(GOLDEN, "termsheet1.golden.json")))
+    res = api.price_put_leg(req, "fp64")
+    doc = api.result_to_json(res)
+    for mine, gold in zip(doc["ledger"], golden["ledger"]):
+        assert mine["population_after"] == gold["population_after"]
+        assert rel(mine["pv_impact"], gold["pv_impact"]) <= 1e-10
+    for k, v in golden["legs"].items():
+        assert abs(doc["legs"][k] - v) <= 1e-10 * max(abs(v), 1.0), k
+    for k in ("pv", "mc_error", "ki_probability"):
+        assert rel(doc[k], golden[k]) <= 1e-10, k
+    for mine, gold in zip(doc["stats"], golden["stats"]):
+        assert mine["output_count"] == gold["output_count"]
+        assert rel(mine["mean_payoff"], gold["mean_payoff"]) <= 1e-10
+    assert doc.get("trace_events", []) == golden.get("trace_events", [])
+    f32 = api.price_put_leg(req, "fp32")
+    assert rel(f32.pv, golden["pv"]) <= FP32_REL
+    assert rel(f32.legs.down_and_in_put, golden["legs"]["down_and_in_put"]) <= 1e-2
+
+
+@pytest.mark.parametrize("F", [1, 2, 3, 5])
+def test_mc_lane_every_factor_count(F):
+    req = W.lv_book(paths=700, steps=126, years=0.5, F=F)
+    _mc(req)
+    ora = O.price(req)
+    assert_fp64_parity(api.price_put_leg(req, "fp64"), ora)
+    assert rel(api.price_put_leg(req, "fp32").pv, ora.pv) <= FP32_REL
+
+
+def test_mc_lane_event_indices_bit_exact():
+    req = _mc(W.flat_book(paths=2048, steps=252))
+    recs, res = api.replay_paths(req, "fp64")
+    ora, orecs = O.price(req, records=True)
+    assert np.array_equal(recs["ko_step"], orecs["ko_step"])
+    assert np.array_equal(recs["hit_ki"], orecs["hit_ki"])
+    for f in ("final_worst", "funding", "put", "coupon_pv", "memory_pv", "pv"):
+        np.testing.assert_allclose(recs[f], orecs[f], rtol=1e-12, atol=1e-15)
+    assert_fp64_parity(res, ora)
+    dt = req.trade.maturity_years / req.trade.steps
+    want = [f"daily_global_ko:path={p}:t={(s + 1) * dt:.6f}" for p, s in ora.trace]
+    assert api.price_put_leg(req, "fp64").trace_events == want
+
+
+def test_mc_crn_scenarios_alone_equal_in_mix():
+    names, reqs = W.crn_scenarios(paths=1024, steps=126, years=0.5)
+    for r in reqs:
+        _mc(r)
+    mix = api.price_batch(reqs, "fp64")
+    for i in (0, 4, 9):
+        alone = api.price_put_leg(reqs[i], "fp64")
+        assert alone.pv == mix[i].pv and alone.legs.down_and_in_put == mix[i].legs.down_and_in_put
+    assert_fp64_parity(mix[2], O.price(reqs[2]))
+    f32 = api.price_batch(reqs, "fp32")
+    for a, b in zip(f32, mix):
+        assert rel(a.pv, b.pv) <= FP32_REL
+
+
+@pytest.mark.parametrize("ppt,waves", [(2, 4), (1, 1), (4, 2)])
+def test_mc_launch_shape_changes_nothing_but_the_shape(ppt, waves):
+    req = _mc(W.lv_book(paths=3000, steps=126, years=0.5))
+    a = api.price_batch([req], "fp64", paths_per_thread=1, waves_per_block=4)[0]
+    b = api.price_batch([req], "fp64", paths_per_thread=ppt, waves_per_block=waves)[0]
+    assert b.stats[0].output_count == a.stats[0].output_count
+    assert b.trace_events == a.trace_events
+    assert rel(b.pv, a.pv) <= 1e-14
+
+
+def test_mc_and_qmc_items_in_one_graph_replay_like_eager():
+    mc = _mc(W.flat_book(paths=4000, steps=63, years=0.25))
+    qmc = W.flat_book(paths=4000, steps=63, years=0.25)
+    reqs = [mc, qmc]
+    eager = api.price_batch(reqs, "fp32")
+    plan = api.Plan(reqs, "fp32", use_graph=True)
+    assert plan.info.graph == 1
+    for _ in range(3):
+        got = plan.run()
+        for a, b in zip(got, eager):
+            assert a.pv == b.pv and a.legs.down_and_in_put == b.legs.down_and_in_put
+        api.price_put_leg(mc, "fp64")
+    plan.close()
+    assert eager[0].pv != eager[1].pv
