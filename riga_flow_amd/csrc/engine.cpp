@@ -608,9 +608,9 @@ struct riga_plan {
     uint64_t seed = 0, dpp = 0, paths = 0;
     int chunk_words = 0, slow_cap = 0;
     int64_t n_chunks = 0, off_key = 0;
-    DevBuf chunks, slow, entry, cnt, block_off, starts;
+    DevBuf chunks, slow, slow_val, entry, cnt, block_off, starts;
     void each(void (*fn)(DevBuf&, hipStream_t), hipStream_t st) {
-      for (DevBuf* b : {&chunks, &slow, &entry, &cnt, &block_off, &starts}) fn(*b, st);
+      for (DevBuf* b : {&chunks, &slow, &slow_val, &entry, &cnt, &block_off, &starts}) fn(*b, st);
     }
   };
   std::vector<std::unique_ptr<McParse>> mc;
@@ -666,12 +666,12 @@ hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t st
   return hipSuccess;
 }
 
-// Stream parse sizing for the first words of paths [0, mp->paths) at mp->dpp
-// draws per path: chunks are sized so a parse has ~32k or more of them (one
-// lane each), and the stream is cut 3% beyond the words the draws need (1.022
-// per draw on average) plus eight chunks.
+// Stream parse sizing for paths [0, mp->paths) at mp->dpp draws per path:
+// chunks are sized so a parse has ~32k or more of them (one lane each), and the
+// stream is cut 3% beyond the words the draws need (1.022 per draw on average)
+// plus eight chunks.
 void mc_parse_size(riga_plan::McParse* mp) {
-  const double draws = (double)(mp->paths - 1) * (double)mp->dpp + 1.0;
+  const double draws = (double)mp->paths * (double)mp->dpp;
   int C = 512;
   while (C < 4096 && draws * 1.03 / C > 32768.0) C *= 2;
   mp->chunk_words = C;
@@ -681,7 +681,7 @@ void mc_parse_size(riga_plan::McParse* mp) {
 
 double mc_parse_bytes(const riga_plan::McParse& mp) {
   const double nc = (double)mp.n_chunks;
-  return nc * (sizeof(McChunk) + (double)mp.slow_cap * 4 + 5) + ((mp.n_chunks + 255) / 256) * 8.0 +
+  return nc * (sizeof(McChunk) + (double)mp.slow_cap * 12 + 5) + ((mp.n_chunks + 255) / 256) * 8.0 +
          (double)mp.paths * 8.0;
 }
 
@@ -690,6 +690,7 @@ hipError_t mc_parse_alloc(riga_plan::McParse* mp) {
   hipError_t e;
   if ((e = mp->chunks.ensure(nc * sizeof(McChunk))) != hipSuccess) return e;
   if ((e = mp->slow.ensure(nc * (size_t)mp->slow_cap * 4)) != hipSuccess) return e;
+  if ((e = mp->slow_val.ensure(nc * (size_t)mp->slow_cap * 8)) != hipSuccess) return e;
   if ((e = mp->entry.ensure(nc)) != hipSuccess) return e;
   if ((e = mp->cnt.ensure(nc * 4)) != hipSuccess) return e;
   if ((e = mp->block_off.ensure((size_t)((mp->n_chunks + 255) / 256) * 8)) != hipSuccess) return e;
@@ -705,6 +706,7 @@ hipError_t mc_parse_launch(const riga_plan::McParse& mp, const uint32_t* key, co
   pa.slow_cap = mp.slow_cap;
   pa.n_chunks = mp.n_chunks;
   pa.slow = (uint32_t*)mp.slow.p;
+  pa.slow_val = (double*)mp.slow_val.p;
   pa.chunks = (McChunk*)mp.chunks.p;
   McStartsArgs sa;
   sa.chunks = pa.chunks;
@@ -923,7 +925,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     }
     lp.threads = threads;
     if (lp.mc)  // stream ring and ziggurat tables instead of the bridge stack
-      lp.lds = (size_t)kMcRingSlots * threads * 8 + 2 * kZigLen * 8 + (size_t)lv_words * real_size;
+      lp.lds = (size_t)mc_ring_slots(lp.smax) * threads * 8 + 2 * kZigLen * 8 + (size_t)lv_words * real_size;
     else
       lp.lds = (size_t)stack_levels * lp.F * threads * real_size + (size_t)lv_words * real_size;
     if (lp.lds + (size_t)lp.smax * sizeof(BlockShared) > ctx->lds_limit) {
@@ -1089,7 +1091,16 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       mc_bytes += mc_parse_bytes(*mp);
       if (stream_ordered) mp->each([](DevBuf& b, hipStream_t st) { b.stream = st; }, pl->stream);
       if (!estimate) HIP_TRY(mc_parse_alloc(mp.get()), "stream parse alloc");
-      for (int i : kv.second) items[i].d.mc_start = (const uint64_t*)mp->starts.p;
+      for (int i : kv.second) {
+        ItemDesc& d = items[i].d;
+        d.mc_start = (const uint64_t*)mp->starts.p;
+        d.mc_chunks = (const McChunk*)mp->chunks.p;
+        d.mc_slow = (const uint32_t*)mp->slow.p;
+        d.mc_val = (const double*)mp->slow_val.p;
+        d.mc_n_chunks = mp->n_chunks;
+        d.mc_chunk_words = mp->chunk_words;
+        d.mc_slow_cap = mp->slow_cap;
+      }
       pl->mc.push_back(std::move(mp));
     }
     if (!pl->mc.empty() && !estimate) HIP_TRY(pl->mc_status.ensure(8 * pl->mc.size()), "stream status alloc");

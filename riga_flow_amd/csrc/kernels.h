@@ -41,6 +41,7 @@ struct McParseArgs {
   int slow_cap;          // slow-word list capacity per chunk
   int64_t n_chunks;
   uint32_t* slow;        // [n_chunks][slow_cap]
+  double* slow_val;      // [n_chunks][slow_cap]
   McChunk* chunks;       // [n_chunks]
 };
 struct McStartsArgs {
@@ -52,7 +53,7 @@ struct McStartsArgs {
   uint32_t* cnt;         // [n_chunks] draws starting in the chunk
   uint64_t* block_off;   // [ceil(n_chunks / 256)] block sums, then their exclusive prefix sums
   uint64_t dpp;          // draws per path (steps * factors)
-  uint64_t paths;        // paths whose first word is wanted: [0, paths)
+  uint64_t paths;        // paths priced from the stream: [0, paths)
   uint64_t* starts;      // [paths]
   uint32_t* status;      // [2]: paths found, error flags
 };

@@ -8,9 +8,10 @@
 // draw.  The parse finds it in three passes over the stream, cut into chunks of
 // `chunk_words` words:
 //   1. mc_parse_kernel, one lane per chunk: the words that fail the rectangle
-//      test ("slow" words, ~1.5%) and how many words a draw starting there
-//      takes; from that list, for every entry offset e < kMcEntries, the draws
-//      that start in the chunk and the next chunk's entry offset.
+//      test ("slow" words, ~1.5%), and for each the draw that would start there
+//      (its value and how many words it takes); from that list, for every
+//      entry offset e < kMcEntries, the draws that start in the chunk and the
+//      next chunk's entry offset.
 //   2. mc_resolve_kernel: each chunk's true entry (from the nearest earlier
 //      chunk whose exit does not depend on its entry) and its draw count;
 //      mc_scan_kernel: prefix sums of the counts.
@@ -37,29 +38,42 @@ __global__ __launch_bounds__(256) void mc_parse_kernel(McParseArgs a) {
   if (j >= a.n_chunks) return;
   const uint32_t* key = a.key;
   const int C = a.chunk_words;
-  const uint64_t first_block = ((uint64_t)j * (uint64_t)C) >> 3;
+  const uint64_t base = (uint64_t)j * (uint64_t)C;
   uint32_t* slow = a.slow + (size_t)j * (size_t)a.slow_cap;
+  double* val = a.slow_val + (size_t)j * (size_t)a.slow_cap;
   int ns = 0;
   uint32_t flags = 0;
+  // Pass 1: the words that fail the rectangle test, with their bits (kept in
+  // the value slot).  Their draws are finished in pass 2, one entry per
+  // iteration, so a wave does not run the wedge and tail tests word by word.
   for (int b = 0; b < C / 8; ++b) {
-    const uint64_t blk = first_block + (uint64_t)b;
     uint64_t w[8];
-    chacha8_block(key, blk, w);
+    chacha8_block(key, (base >> 3) + (uint64_t)b, w);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       double x;
       if (!zig_rect(w[q], zx, &x)) {
-        const int len = zig_slow_len(key, blk * 8 + (uint64_t)q, w[q], zx, zf);
-        if (len > kZigMaxLen) flags |= kMcErrLong;
-        if (ns < a.slow_cap)
-          slow[ns] = ((uint32_t)(b * 8 + q) << 8) | (uint32_t)(len > 255 ? 255 : len);
-        else
+        if (ns < a.slow_cap) {
+          slow[ns] = (uint32_t)(b * 8 + q) << 8;
+          val[ns] = bits_f64(w[q]);
+        } else {
           flags |= kMcErrSlow;
+        }
         ++ns;
       }
     }
   }
   const int n = ns < a.slow_cap ? ns : a.slow_cap;
+  // Pass 2: each listed draw's length and value.
+  for (int m = 0; m < n; ++m) {
+    const uint32_t off = slow[m] >> 8;
+    const uint64_t w0 = (uint64_t)__double_as_longlong(val[m]);
+    double x;
+    const int len = zig_slow_draw(key, base + off, w0, zx, zf, &x);
+    if (len > kZigMaxLen) flags |= kMcErrLong;
+    slow[m] = (off << 8) | (uint32_t)(len > 255 ? 255 : len);
+    val[m] = x;
+  }
   McChunk out;
   for (int e = 0; e < kMcEntries; ++e) {
     int pos = e, draws = 0;
@@ -187,6 +201,9 @@ __global__ __launch_bounds__(256) void mc_starts_kernel(McStartsArgs a) {
     ++found;
   }
   if (found) atomicAdd(&a.status[0], found);
+  // the parsed stream must hold every draw of every path (the pricing lanes
+  // take slow draws from it)
+  if (j == a.n_chunks - 1 && end < a.paths * a.dpp) atomicOr(&a.status[1], (uint32_t)kMcErrShort);
 }
 
 hipError_t launch_mc_parse(const McParseArgs& p, const McStartsArgs& s, hipStream_t stream) {

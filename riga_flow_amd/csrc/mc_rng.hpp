@@ -5,8 +5,9 @@
 //
 // The stream is one sequence per request, and a normal draw consumes a
 // variable number of words (one in ~98.5% of draws).  Paths therefore start at
-// data-dependent word offsets; the parse kernels (kernels_mc.hip) find them,
-// and each path lane then regenerates its own words from that offset.
+// data-dependent word offsets; the parse kernels (kernels_mc.hip) find them and
+// compute every multi-word ("slow") draw once, and each path lane then
+// regenerates its own words from its offset, taking slow draws from the parse.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -141,8 +142,9 @@ struct ZigSlow {
   }
 };
 
-// A lane's words in LDS: ring[slot * stride] for slot in [0, kMcRingSlots),
-// `fill` words buffered from `head`; ChaCha blocks are appended whole.
+// A lane's words in LDS: ring[slot * stride] for slot in [0, kSlots), `fill`
+// words buffered from `head`; ChaCha blocks are appended whole.
+template <int kSlots>
 struct McRing {
   uint64_t* base;  // this lane's slot 0
   int stride;      // lanes per workgroup
@@ -154,11 +156,11 @@ struct McRing {
     chacha8_block(key, next_block, w);
     ++next_block;
     int s = head + fill;
-    s = s >= kMcRingSlots ? s - kMcRingSlots : s;
+    s = s >= kSlots ? s - kSlots : s;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       base[s * stride] = w[j];
-      s = s + 1 == kMcRingSlots ? 0 : s + 1;
+      s = s + 1 == kSlots ? 0 : s + 1;
     }
     fill += 8;
   }
@@ -176,38 +178,104 @@ struct McRing {
   __device__ __forceinline__ uint64_t take(const uint32_t* __restrict__ key) {
     if (fill == 0) append(key);  // only after slow draws outran the wave's refills
     const uint64_t w = base[head * stride];
-    head = head + 1 == kMcRingSlots ? 0 : head + 1;
+    head = head + 1 == kSlots ? 0 : head + 1;
     --fill;
     return w;
   }
 };
 
-// One StandardNormal draw from the lane's ring.
-__device__ __forceinline__ double mc_normal(McRing& ring, const uint32_t* __restrict__ key,
-                                            const double* __restrict__ zx, const double* __restrict__ zf) {
+// A lane's place in the parse's slow-word lists: the stream index of the next
+// listed slow word at or after the lane's position.  A slow word that starts a
+// draw carries the draw's value and length, computed once by the parse, so
+// the pricing loop never runs the ziggurat's wedge or tail tests.
+struct McCursor {
+  const uint32_t* slow;    // [chunk][cap]: (offset << 8) | words
+  const double* val;       // [chunk][cap]: the draw starting at that word
+  const McChunk* chunks;
+  int64_t n_chunks;
+  int C, cap;
+  int64_t chunk;
+  int q, n;
+  uint64_t next;           // ~0: past the parsed stream (no path reads there)
+
+  __device__ __forceinline__ void advance(uint64_t pos) {
+    for (;;) {
+      if (q < n) {
+        const uint64_t a = (uint64_t)chunk * (uint64_t)C + (slow[chunk * cap + q] >> 8);
+        if (a >= pos) {
+          next = a;
+          return;
+        }
+        ++q;
+        continue;
+      }
+      ++chunk;
+      if (chunk >= n_chunks) {
+        next = ~0ull;
+        return;
+      }
+      q = 0;
+      n = chunks[chunk].nslow;
+    }
+  }
+
+  __device__ __forceinline__ void seek(uint64_t pos) {
+    chunk = (int64_t)(pos / (uint64_t)C);
+    if (chunk >= n_chunks) {
+      next = ~0ull;
+      n = q = 0;
+      return;
+    }
+    n = chunks[chunk].nslow;
+    const uint32_t off = (uint32_t)(pos - (uint64_t)chunk * (uint64_t)C);
+    int lo = 0, hi = n;  // first entry at or after `off`
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if ((slow[chunk * cap + mid] >> 8) < off)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    q = lo;
+    advance(pos);
+  }
+};
+
+// One StandardNormal draw: the word at stream index `wpos` either passes the
+// rectangle test (x = u * x_i) or is a listed slow word whose draw the parse
+// has already computed (its extra words are skipped).
+template <int kSlots>
+__device__ __forceinline__ double mc_draw(McRing<kSlots>& ring, McCursor& cur, uint64_t& wpos,
+                                          const uint32_t* __restrict__ key, const double* __restrict__ zx) {
   const uint64_t w = ring.take(key);
-  double x;
-  if (zig_rect(w, zx, &x)) return x;
-  ZigSlow s;
-  s.start(w, zx);
-  // a draw longer than kZigMaxLen words is flagged by the parse (the request
-  // fails), so the bound only keeps every lane's loop finite
-  for (int len = 2; len <= kZigMaxLen; ++len)
-    if (s.feed(ring.take(key), zx, zf, &x)) break;
-  return x;
+  const uint64_t at = wpos++;
+  if (at == cur.next) {
+    const int64_t e = cur.chunk * cur.cap + cur.q;
+    const double x = cur.val[e];
+    const int extra = (int)(cur.slow[e] & 0xffu) - 1;
+    for (int k = 0; k < extra; ++k) (void)ring.take(key);
+    wpos += (uint64_t)extra;
+    ++cur.q;
+    cur.advance(wpos);
+    return x;
+  }
+  const int i = (int)(w & 0xffu);
+  const double u = bits_f64((w >> 12) | 0x4000000000000000ull) - 3.0;
+  return u * zx[i];
 }
 
-// Words consumed by a draw whose first word `w0` sits at stream index `idx`
-// and fails the rectangle test (parse side: the following words are
-// regenerated by index).
-__device__ __forceinline__ int zig_slow_len(const uint32_t* __restrict__ key, uint64_t idx, uint64_t w0,
-                                            const double* __restrict__ zx, const double* __restrict__ zf) {
+// The draw whose first word `w0` sits at stream index `idx` and fails the
+// rectangle test: its value, and the words it takes (the following words
+// regenerated by index); kZigMaxLen + 1 words means too long (refused).
+__device__ __forceinline__ int zig_slow_draw(const uint32_t* __restrict__ key, uint64_t idx, uint64_t w0,
+                                             const double* __restrict__ zx, const double* __restrict__ zf,
+                                             double* value) {
   ZigSlow s;
   s.start(w0, zx);
-  double x;
   for (int len = 2; len <= kZigMaxLen; ++len)
-    if (s.feed(chacha8_word(key, idx + (uint64_t)(len - 1)), zx, zf, &x)) return len;
-  return kZigMaxLen + 1;  // flagged: the request is refused
+    if (s.feed(chacha8_word(key, idx + (uint64_t)(len - 1)), zx, zf, value)) return len;
+  *value = 0.0;
+  return kZigMaxLen + 1;
 }
 
 }  // namespace riga

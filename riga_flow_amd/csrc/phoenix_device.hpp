@@ -455,7 +455,8 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   // n_surf x kSurfWords x max_nx]; the Monte Carlo lane [stream ring: slots x
   // lanes][ziggurat x_i, f(x_i)][local-vol rows]
   uint64_t* mc_ring = reinterpret_cast<uint64_t*>(lds_dyn);
-  double* zx = reinterpret_cast<double*>(mc_ring + (kMC ? kMcRingSlots * nthreads : 0));
+  constexpr int kRingSlots = mc_ring_slots(SMAX);
+  double* zx = reinterpret_cast<double*>(mc_ring + (kMC ? kRingSlots * nthreads : 0));
   double* zf = zx + kZigLen;
   Real* lds_stack = reinterpret_cast<Real*>(lds_dyn);
   Real* lds_w = kMC ? reinterpret_cast<Real*>(zf + kZigLen) : lds_stack + (size_t)it.stack_levels * F * nthreads;
@@ -507,12 +508,22 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     // coupons as a count (bits 20-30) and coupon received (bit 31) in one word
     uint32_t pk[SMAX];
     Real worst_last[SMAX];
-    McRing ring;
+    McRing<kRingSlots> ring;
+    McCursor cur;
+    uint64_t wpos = 0;  // stream index of the lane's next word
     if constexpr (kMC) {
       // the path's words start where the parse found its first draw
+      wpos = it.mc_start[valid ? path : it.path_begin];
       ring.base = mc_ring + tid;
       ring.stride = nthreads;
-      ring.seek(it.mc_key, it.mc_start[valid ? path : it.path_begin]);
+      ring.seek(it.mc_key, wpos);
+      cur.slow = it.mc_slow;
+      cur.val = it.mc_val;
+      cur.chunks = it.mc_chunks;
+      cur.n_chunks = it.mc_n_chunks;
+      cur.C = it.mc_chunk_words;
+      cur.cap = it.mc_slow_cap;
+      cur.seek(wpos);
     } else {
       Real y[F];
       node_normals<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, arena, n, gray, 0, y);
@@ -554,10 +565,10 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         // ---- Monte Carlo lane (note.rs:123-126): F StandardNormals in stream
         // order, shocks = L z.  A wave tops up every lane's ring together when
         // any lane is short of F words, so the ChaCha blocks run converged.
-        if (__ballot(ring.fill < F) != 0ull && ring.fill <= kMcRingSlots - 8) ring.append(it.mc_key);
+        if (__ballot(ring.fill < F) != 0ull && ring.fill <= kRingSlots - 8) ring.append(it.mc_key);
         Real z[F];
 #pragma unroll
-        for (int f = 0; f < F; ++f) z[f] = (Real)mc_normal(ring, it.mc_key, zx, zf);
+        for (int f = 0; f < F; ++f) z[f] = (Real)mc_draw(ring, cur, wpos, it.mc_key, zx);
         correlate<Real, F>(it, z, shock);
 #pragma unroll
         for (int f = 0; f < F; ++f) dw[f] = shock[f] * sqrt_dt_r;

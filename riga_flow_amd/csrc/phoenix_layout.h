@@ -37,7 +37,10 @@ enum ItemFlags : uint32_t {
 
 // Monte Carlo lane (ChaCha8 stream + StandardNormal ziggurat).
 constexpr int kZigLen = 257;        // ziggurat table entries (256 layers + the x = 0 end)
-constexpr int kMcRingSlots = 24;    // stream words buffered per lane in LDS: three ChaCha blocks
+// Stream words buffered per lane in LDS: three ChaCha blocks, or two for the
+// 10-scenario lanes (whose local-vol rows and partials leave less LDS: two
+// workgroups must still fit a CU).
+constexpr int mc_ring_slots(int smax) { return smax >= 10 ? 16 : 24; }
 constexpr int kMcEntries = 16;      // chunk entry offsets tracked by the stream parse
 constexpr int kZigMaxLen = 250;     // longest draw accepted, in words (never reached in practice)
 
@@ -50,11 +53,13 @@ enum McFlags : uint32_t {
   kMcErrLong = 1u << 1,    // a draw longer than kZigMaxLen words
   kMcErrSlow = 1u << 2,    // more slow draws than the chunk's list holds
   kMcErrExit = 1u << 3,    // a draw reaching kMcEntries or more words into the next chunk
+  kMcErrShort = 1u << 4,   // the parsed stream ends before the last path's last draw
 };
 struct McChunk {
   int32_t count[kMcEntries];
   uint8_t exit[kMcEntries];
-  int32_t nslow;           // slow draws listed: (offset << 8) | words, in offset order
+  int32_t nslow;           // slow words listed: (offset << 8) | words of the draw that
+                           // would start there, in offset order, with that draw's value
   uint32_t flags;
 };
 
@@ -161,6 +166,11 @@ struct ItemDesc {
   uint32_t mc_key[8];
   const uint64_t* mc_start;  // [path]: stream word of the path's first draw
   int64_t off_zig;           // double[2 * kZigLen]: x_i then f(x_i)
+  const McChunk* mc_chunks;  // the parse of this item's stream
+  const uint32_t* mc_slow;   // [chunk][mc_slow_cap] slow words
+  const double* mc_val;      // [chunk][mc_slow_cap] their draws
+  int64_t mc_n_chunks;
+  int32_t mc_chunk_words, mc_slow_cap;
 };
 
 // Double-double accumulator (hi + lo).
