@@ -138,6 +138,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-book", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo (ChaCha8) lane leg")
     ap.add_argument("--quick", action="store_true", help="smaller path counts (smoke)")
     ap.add_argument("--ppt", type=int, default=0, help="FP32 batch: paths per lane (0 = autotuned)")
     ap.add_argument("--waves", type=int, default=0, help="FP32 batch: waves per workgroup (0 = autotuned)")
@@ -267,6 +268,8 @@ def main():
     fp64.close()
     log("fp32 error check done")
     out["crn_variance_reduction"] = crn_vrf_leg(args.quick)
+    if not args.no_mc:
+        out["mc_lane"] = mc_leg(args)
     if not args.no_book:
         out["uneven_book"] = book_leg(args.quick)
     if not args.no_serving:
@@ -275,6 +278,48 @@ def main():
         out["cpu_baseline"] = cpu_baseline()
         log("cpu baseline done")
     print(json.dumps(out), flush=True)
+
+
+def mc_leg(args) -> dict:
+    """The Monte Carlo lane (VarianceReduction::None: ChaCha8Rng + StandardNormal,
+    note.rs:58,124-126) on the same trade and batch shapes as the QMC legs; the
+    timed region includes the stream parse that finds each path's first word.
+    Also the reference's golden term sheet (30,000 paths) priced on the device."""
+    from riga_flow_amd.fixture import load_termsheet1_fixture
+    big = 1_048_576 if not args.quick else 65_536
+    plain_paths = 262_144 if not args.quick else 16_384
+    names, crn = W.crn_scenarios(paths=big, steps=756, years=3.0)
+    plain = W.lv_book(paths=plain_paths, steps=756)
+    for r in crn + [plain]:
+        r.config.variance_reduction = W.VarianceReduction("none")
+    p32 = api.Plan(crn, "fp32", use_graph=True)
+    p64 = api.Plan([plain], "fp64", use_graph=True)
+    w32, d32, r32 = timed(p32, args.steps, args.warmup)
+    w64, d64, r64 = timed(p64, args.steps, args.warmup)
+    ref = api.Plan(crn, "fp64").run()
+    err = max(abs(a.pv - b.pv) / abs(b.pv) for a, b in zip(r32, ref))
+    v32 = p32.path_steps * args.steps / w32
+    v64 = p64.path_steps * args.steps / w64
+    log(f"mc lane: fp32 batched {w32 / args.steps * 1e3:.1f} ms, fp64 plain {w64 / args.steps * 1e3:.1f} ms")
+    p32.close()
+    p64.close()
+    here = os.path.dirname(os.path.abspath(__file__))
+    ts = load_termsheet1_fixture(os.path.join(here, "tests", "golden", "termsheet1.json")).request
+    with open(os.path.join(here, "tests", "golden", "termsheet1.golden.json")) as fh:
+        golden = json.load(fh)
+    t0 = time.perf_counter()
+    g = api.price_put_leg(ts, "fp64")
+    gsec = time.perf_counter() - t0
+    return {"fp32_batched": {"value": v32, "unit": "path-steps/s", "ms_per_step": w32 / args.steps * 1e3,
+                             "device_ms": sum(d32) / len(d32), "paths": big, "scenarios": len(crn)},
+            "fp64_plain": {"value": v64, "unit": "path-steps/s", "ms_per_step": w64 / args.steps * 1e3,
+                           "device_ms": sum(d64) / len(d64), "paths": plain_paths},
+            "fp32_batched_over_fp64_plain": v32 / v64,
+            "fp32_max_rel_pv_err": err,
+            "termsheet1_golden": {"pv": g.pv, "golden_pv": golden["pv"], "abs_pv_diff": abs(g.pv - golden["pv"]),
+                                  "ko_count": g.stats[0].output_count,
+                                  "golden_ko_count": golden["stats"][0]["output_count"],
+                                  "seconds": gsec}}
 
 
 def crn_vrf_leg(quick: bool) -> dict:

@@ -44,3 +44,20 @@ def assert_fp64_parity(res, ora, tol=FP64_REL):
                        ("ki_var", res.stats[1].variance_payoff, ora.ki_var),
                        ("cpn_var", res.stats[2].variance_payoff, ora.cpn_var)]:
         assert abs(a - b) <= 1e-9 * max(abs(b), 1e-6), (name, a, b)
+
+
+def assert_json_close(actual, expected, path="result"):
+    """The reference's golden comparison (riga-engine/src/lib.rs:429-456): same keys
+    and lengths, numbers within 1e-12 absolute, everything else equal."""
+    if isinstance(expected, dict):
+        assert set(actual) == set(expected), path
+        for k in expected:
+            assert_json_close(actual[k], expected[k], f"{path}.{k}")
+    elif isinstance(expected, list):
+        assert len(actual) == len(expected), path
+        for i, (a, e) in enumerate(zip(actual, expected)):
+            assert_json_close(a, e, f"{path}[{i}]")
+    elif isinstance(expected, (int, float)) and not isinstance(expected, bool):
+        assert abs(actual - expected) <= 1e-12, (path, actual, expected)
+    else:
+        assert actual == expected, (path, actual, expected)

@@ -17,6 +17,7 @@ import pytest
 from scipy import stats
 
 import oracle as O
+from parity import assert_json_close
 from riga_flow_amd import api
 from riga_flow_amd import _native as N
 from riga_flow_amd.fixture import load_termsheet1_fixture
@@ -99,22 +100,6 @@ def _golden_doc(req, o):
     for i, (p, s) in enumerate(o.trace):
         r.trace_path[i], r.trace_step[i] = p, s
     return api.result_to_json(api._assemble(req, r))
-
-
-def assert_json_close(actual, expected, path="result"):
-    """lib.rs:429-456: same keys and lengths, numbers within 1e-12, the rest equal."""
-    if isinstance(expected, dict):
-        assert set(actual) == set(expected), path
-        for k in expected:
-            assert_json_close(actual[k], expected[k], f"{path}.{k}")
-    elif isinstance(expected, list):
-        assert len(actual) == len(expected), path
-        for i, (a, e) in enumerate(zip(actual, expected)):
-            assert_json_close(a, e, f"{path}[{i}]")
-    elif isinstance(expected, (int, float)) and not isinstance(expected, bool):
-        assert abs(actual - expected) <= 1e-12, (path, actual, expected)
-    else:
-        assert actual == expected, (path, actual, expected)
 
 
 def test_termsheet1_golden_full_result():
