@@ -133,7 +133,7 @@ typedef struct riga_launch_config {
   int32_t paths_per_thread; /* 1, 2, 4 or 8 paths per lane; 0: the autotuned choice (riga_autotune)
                                for the batch's shape, else 1 */
   int32_t use_graph;        /* 1: capture the launch sequence in a hipGraph at plan creation */
-  int32_t owen_table;       /* 0: tabulate the top 8 Owen levels per dimension; -1: hash all 32 */
+  int32_t owen_table;       /* 0: tabulate the top 16 Owen levels per dimension; -1: hash all 32 */
   int32_t normal_table;     /* 0: auto (share one [dim][path] normal table among same-seed trades
                                when they need 4x more normals than it holds); 1: force; -1: off */
   int32_t stream_ordered_alloc; /* 1: plan memory from the stream-ordered pool (serving) */
@@ -261,6 +261,41 @@ void riga_server_destroy(riga_server *server);
 int riga_replay_paths(riga_ctx *ctx, const riga_request *req, int precision,
                       riga_path_record *records, riga_result *out, riga_error *err);
 
+/* explain.rs:35-55 CouponPeriodRecord: one period of a path's coupon strip (nominal
+ * amounts; pv = paid * discount_factor; explain.rs's coupon_by_period[period] == pv). */
+typedef struct riga_coupon_row {
+  int64_t period;
+  double years;                 /* payment or period-end time; 0 after termination */
+  int64_t qualified_steps, period_steps;
+  double accrual, memory_paid, paid, memory_accrued, discount_factor, pv;
+  int32_t knocked_out, pad;
+} riga_coupon_row;
+
+/* explain.rs:90-316 run_paths and :319-334 reconstruct_path: records of the
+ * listed paths (path_ids NULL: all paths, n_ids ignored), in list order, and,
+ * when ledger is not NULL, their coupon strips ledger[i * periods + period]
+ * (periods = coupon_periods, 9 when unset).  A listed id >= paths gives
+ * path_out_of_range. */
+int riga_replay_ledger(riga_ctx *ctx, const riga_request *req, int precision, const uint64_t *path_ids,
+                       uint64_t n_ids, riga_path_record *records, riga_coupon_row *ledger,
+                       riga_error *err);
+
+/* explain.rs:78-88 PathCubeQuery (the sample is returned through the arrays). */
+typedef struct riga_worst_query {
+  double percentile, threshold_worst_final;
+  uint64_t selected, total, n_sample;
+} riga_worst_query;
+
+/* explain.rs:336-372 query_worst_percentile: every path's final worst-of, a
+ * stable ascending sort, selected = ceil(total * percentile), the threshold and
+ * the first min(selected, max_sample) paths: their ids, records and (when
+ * sample_ledger is not NULL) coupon strips [max_sample][periods].  Percentile
+ * outside [0, 1] gives invalid_percentile. */
+int riga_query_worst_percentile(riga_ctx *ctx, const riga_request *req, int precision, double percentile,
+                                uint64_t max_sample, riga_worst_query *out, uint64_t *sample_ids,
+                                riga_path_record *sample_records, riga_coupon_row *sample_ledger,
+                                riga_error *err);
+
 /* ---- resident market surfaces (gpu.rs:280-327 resident_lv keeps the local-vol
  * rows and log-strike axes on the device across runs).  A calibrated grid is
  * copied to device memory once; requests then name it in riga_request.surfaces.
@@ -281,6 +316,17 @@ int riga_debug_sobol(riga_ctx *ctx, uint64_t seed, int32_t ladder_dims, uint64_t
                      int32_t n_points, int32_t first_dim, int32_t n_dims,
                      uint32_t *out_unscrambled, uint32_t *out_scrambled, double *out_normal,
                      riga_error *err);
+
+/* ---- PathCube on demand (pathcube.rs:47-76 generate_correlated_block, :146-187
+ * gbm_evolve): a correlated block y = L z from ChaCha8Rng::seed_from_u64(seed),
+ * `factors` StandardNormals per (path, time) cell in stream order, as floats, and
+ * the GBM spot cube exp(log s0 + sum((r - q - vol^2/2) dt + vol sqrt(dt) y)) as
+ * floats, both [paths][times][factors].  out_block / out_cube may be NULL.  With
+ * in_block set, only gbm_evolve of that block runs (into out_cube). */
+int riga_path_cube(riga_ctx *ctx, uint64_t seed, uint64_t paths, int32_t times, int32_t factors,
+                   const double *correlation, const double *s0, const double *vols, double rate,
+                   const double *dividend_yields, double dt, const float *in_block, float *out_block,
+                   float *out_cube, riga_error *err);
 
 /* ---- diagnostics: the Monte Carlo lane's stream parse.  out_starts[p] = index of
  * the u64 word of ChaCha8Rng::seed_from_u64(seed) at which the StandardNormal draw

@@ -96,6 +96,18 @@ class PathRecord(C.Structure):
                 ("memory_pv", C.c_double), ("pv", C.c_double)]
 
 
+class CouponRow(C.Structure):
+    _fields_ = [("period", C.c_int64), ("years", C.c_double), ("qualified_steps", C.c_int64),
+                ("period_steps", C.c_int64), ("accrual", C.c_double), ("memory_paid", C.c_double),
+                ("paid", C.c_double), ("memory_accrued", C.c_double), ("discount_factor", C.c_double),
+                ("pv", C.c_double), ("knocked_out", C.c_int32), ("pad", C.c_int32)]
+
+
+COUPON_DTYPE = np.dtype([("period", "<i8"), ("years", "<f8"), ("qualified_steps", "<i8"),
+                         ("period_steps", "<i8"), ("accrual", "<f8"), ("memory_paid", "<f8"),
+                         ("paid", "<f8"), ("memory_accrued", "<f8"), ("discount_factor", "<f8"),
+                         ("pv", "<f8"), ("knocked_out", "<i4"), ("pad", "<i4")])
+
 RECORD_DTYPE = np.dtype([("ko_step", "<i4"), ("hit_ki", "<i4"), ("final_worst", "<f8"),
                          ("funding", "<f8"), ("put", "<f8"), ("coupon_pv", "<f8"),
                          ("memory_pv", "<f8"), ("pv", "<f8")])
@@ -152,7 +164,12 @@ def lib():
         L.ro_interp_linear.argtypes = [C.c_int, _D, _D, C.c_double]
         L.ro_validate.argtypes = [C.POINTER(_Request), C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t]
         L.ro_price.argtypes = [C.POINTER(_Request), C.c_uint64, C.c_uint64, C.POINTER(_Result),
-                               C.POINTER(PathRecord), C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t]
+                               C.POINTER(PathRecord), C.POINTER(CouponRow), C.c_char_p, C.c_size_t,
+                               C.c_char_p, C.c_size_t]
+        _F32 = C.POINTER(C.c_float)
+        L.ro_correlated_block.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_int, _D, _F32]
+        L.ro_gbm_evolve.argtypes = [_D, _D, C.c_double, _D, C.c_double, C.c_uint64, C.c_int64, C.c_int,
+                                    _F32, _F32]
         L.ro_path_shocks.argtypes = [C.POINTER(_Request), C.c_uint64, _D]
         _U32 = C.POINTER(C.c_uint32)
         L.ro_chacha_block.argtypes = [_U32, C.c_int, _U32]
@@ -310,6 +327,26 @@ def draw_starts(seed: int, n: int, start: int = 0) -> np.ndarray:
     out = np.zeros(max(n, 1), dtype=np.uint64)
     lib().ro_draw_starts(seed, start, n, out.ctypes.data_as(C.POINTER(C.c_uint64)))
     return out[:n]
+
+
+def correlated_block(seed: int, paths: int, times: int, correlation) -> np.ndarray:
+    """pathcube.rs:47-76 ``generate_correlated_block``: float32 [paths][times][factors]."""
+    L = cholesky(correlation)
+    F = L.shape[0]
+    out = np.zeros((paths, times, F), dtype=np.float32)
+    lib().ro_correlated_block(seed, paths, times, F, _dptr(L), out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def gbm_evolve(s0, vols, rate: float, dividend_yields, dt: float, block: np.ndarray) -> np.ndarray:
+    """pathcube.rs:146-187 ``gbm_evolve``: float32 spot cube of the block's shape."""
+    blk = np.ascontiguousarray(block, dtype=np.float32)
+    P, T, F = blk.shape
+    out = np.zeros_like(blk)
+    lib().ro_gbm_evolve(_dptr(_darr(s0)), _dptr(_darr(vols)), rate, _dptr(_darr(dividend_yields)), dt,
+                        P, T, F, blk.ctypes.data_as(C.POINTER(C.c_float)),
+                        out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
 
 
 def brownian_increments(normals: Sequence[float], dt: float) -> np.ndarray:
@@ -516,26 +553,33 @@ def _pack(request, max_dim: int, lvs: Optional[List[LocalVol]]):
 
 
 def price(request, max_dim: int = 512, path_range=None, records: bool = False,
-          lvs: Optional[List[LocalVol]] = None):
+          lvs: Optional[List[LocalVol]] = None, ledger: bool = False):
     """``price_full_phoenix`` (note.rs:38-373) on the QMC lanes and the ChaCha8
     Monte Carlo lane (``variance_reduction.kind == "none"``).
 
-    Returns ``OracleResult`` (and a numpy record array when ``records``)."""
+    Returns ``OracleResult`` (and a numpy record array when ``records``, and the
+    coupon strips ``[path][period]`` when ``ledger``, explain.rs:146-262)."""
     if lvs is None:
         lvs = local_vols_for(request)
     r, keep = _pack(request, max_dim, lvs)
     b, e = (0, request.config.paths) if path_range is None else path_range
     res = _Result()
-    recs = (PathRecord * max(e - b, 1))() if records else None
+    recs = (PathRecord * max(e - b, 1))() if (records or ledger) else None
+    periods = 9 if request.trade.coupon_periods is None else max(1, request.trade.coupon_periods)
+    rows = (CouponRow * max((e - b) * periods, 1))() if ledger else None
     code = C.create_string_buffer(64)
     msg = C.create_string_buffer(256)
-    rc = lib().ro_price(C.byref(r), b, e, C.byref(res), recs, code, 64, msg, 256)
+    rc = lib().ro_price(C.byref(r), b, e, C.byref(res), recs, rows, code, 64, msg, 256)
     if rc:
         raise OracleError(code.value.decode(), msg.value.decode())
     out = OracleResult(
         **{f: getattr(res, f) for f, _ in _Result._fields_ if f not in ("n_trace", "trace_path", "trace_step")},
         trace=[(res.trace_path[i], res.trace_step[i]) for i in range(res.n_trace)],
     )
+    if ledger:
+        arr = np.frombuffer(recs, dtype=RECORD_DTYPE, count=e - b).copy()
+        led = np.frombuffer(rows, dtype=COUPON_DTYPE, count=(e - b) * periods).copy()
+        return out, arr, led.reshape(e - b, periods)
     if records:
         arr = np.frombuffer(recs, dtype=RECORD_DTYPE, count=e - b).copy()
         return out, arr

@@ -983,6 +983,37 @@ void ro_draw_starts(uint64_t seed, uint64_t start, uint64_t n, uint64_t *out) {
   }
 }
 
+/* pathcube.rs:47-76: one stream, `factors` draws per (path, time) cell, y = L z. */
+void ro_correlated_block(uint64_t seed, uint64_t paths, int64_t times, int factors, const double *chol,
+                         float *out) {
+  chacha_rng g;
+  rng_init(&g, seed, 0);
+  double z[64], y[64];
+  for (uint64_t p = 0; p < paths; ++p)
+    for (int64_t t = 0; t < times; ++t) {
+      for (int f = 0; f < factors; ++f) z[f] = std_normal(&g);
+      ro_chol_apply(factors, chol, z, y);
+      for (int f = 0; f < factors; ++f) out[((size_t)p * (size_t)times + (size_t)t) * factors + f] = (float)y[f];
+    }
+}
+
+/* pathcube.rs:146-187: log S += (r - q - vol^2/2) dt + vol sqrt(dt) z, stored as exp in float. */
+void ro_gbm_evolve(const double *s0, const double *vols, double rate, const double *dividend_yields,
+                   double dt, uint64_t paths, int64_t times, int factors, const float *block, float *out) {
+  double sqrt_dt = sqrt(dt);
+  for (uint64_t p = 0; p < paths; ++p)
+    for (int f = 0; f < factors; ++f) {
+      double log_s = log(s0[f]);
+      double drift = (rate - dividend_yields[f] - 0.5 * vols[f] * vols[f]) * dt;
+      double diffusion = vols[f] * sqrt_dt;
+      for (int64_t t = 0; t < times; ++t) {
+        size_t i = ((size_t)p * (size_t)times + (size_t)t) * factors + f;
+        log_s += drift + diffusion * (double)block[i];
+        out[i] = (float)exp(log_s);
+      }
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* Pricer, note.rs:38-373                                                     */
 /* ------------------------------------------------------------------------- */
@@ -1084,7 +1115,8 @@ int ro_path_shocks(const ro_request *r, uint64_t path_id, double *out) {
 }
 
 int ro_price(const ro_request *r, uint64_t path_begin, uint64_t path_end, ro_result *out,
-             ro_path_record *records, char *code, size_t codelen, char *msg, size_t msglen) {
+             ro_path_record *records, ro_coupon_row *ledger, char *code, size_t codelen, char *msg,
+             size_t msglen) {
   if (ro_validate(r, code, codelen, msg, msglen)) return -1;
   const int F = r->factors;
   if (F > 64) FAIL("invalid_market_shape", "the oracle supports at most 64 factors");
@@ -1131,6 +1163,16 @@ int ro_price(const ro_request *r, uint64_t path_begin, uint64_t path_end, ro_res
     double path_coupon_pv = 0.0, path_memory_coupon_pv = 0.0;
     int path_received_coupon = 0;
     double final_worst = 1.0;
+    ro_coupon_row *row0 = ledger ? ledger + (path_id - path_begin) * (uint64_t)periods : NULL;
+    if (row0) /* explain.rs:147-164 */
+      for (int64_t p = 0; p < periods; ++p) {
+        ro_coupon_row *w = &row0[p];
+        memset(w, 0, sizeof *w);
+        w->period = p;
+        int64_t a = steps * (p + 1) / periods, b = steps * p / periods;
+        w->period_steps = a > b ? a - b : 0;
+        if (w->period_steps < 1) w->period_steps = 1;
+      }
     for (int64_t step = 0; step < steps; ++step) {
       int note_is_active = !knocked_out;
       double time = (double)(step + 1) * dt;
@@ -1156,6 +1198,7 @@ int ro_price(const ro_request *r, uint64_t path_begin, uint64_t path_end, ro_res
         int64_t period_steps = a > b ? a - b : 0; /* saturating_sub */
         if (period_steps < 1) period_steps = 1;
         accrual_this_period += coupon_rate / (double)period_steps;
+        if (row0) row0[period].qualified_steps += 1; /* explain.rs:209 */
       }
       int call_active = step >= call_start_step;
       int all_above = 1;
@@ -1177,6 +1220,17 @@ int ro_price(const ro_request *r, uint64_t path_begin, uint64_t path_end, ro_res
         double due = (period == 0 && r->fixed_first_coupon) ? coupon_rate : accrual_this_period;
         double carry_paid = due > 0.0 ? unpaid_memory : 0.0;
         double coupon_amount = due + carry_paid;
+        if (row0) { /* explain.rs:222-230 */
+          double df = exp(-rate * time);
+          ro_coupon_row *w = &row0[period];
+          w->years = time;
+          w->accrual = due;
+          w->memory_paid = carry_paid;
+          w->paid = coupon_amount;
+          w->discount_factor = df;
+          w->pv = coupon_amount * df;
+          w->knocked_out = 1;
+        }
         if (coupon_amount > 0.0) {
           double df = exp(-rate * time);
           path_coupon_pv += due * df;
@@ -1195,6 +1249,17 @@ int ro_price(const ro_request *r, uint64_t path_begin, uint64_t path_end, ro_res
       int is_period_end = (step + 1 == steps) || next_period != period;
       if (is_period_end) {
         double due = (period == 0 && r->fixed_first_coupon) ? coupon_rate : accrual_this_period;
+        if (row0) { /* explain.rs:242-255 */
+          double df = exp(-rate * time);
+          ro_coupon_row *w = &row0[period];
+          w->years = time;
+          w->accrual = due;
+          w->memory_paid = due > 0.0 ? unpaid_memory : 0.0;
+          w->paid = due > 0.0 ? due + unpaid_memory : 0.0;
+          w->discount_factor = df;
+          w->pv = w->paid * df;
+          w->memory_accrued = (due == 0.0 && r->memory_coupon) ? coupon_rate : 0.0;
+        }
         if (due > 0.0) {
           double df = exp(-rate * time);
           path_coupon_pv += due * df;

@@ -179,6 +179,23 @@ uint64_t ro_std_normals(uint64_t seed, uint64_t start, uint64_t n, double *out);
 /* u64 index at which each of n consecutive draws starts (draw 0 starts at `start`). */
 void ro_draw_starts(uint64_t seed, uint64_t start, uint64_t n, uint64_t *out);
 
+/* One period of a path's coupon strip, explain.rs:35-55 CouponPeriodRecord. */
+typedef struct {
+  int64_t period;
+  double years;
+  int64_t qualified_steps, period_steps;
+  double accrual, memory_paid, paid, memory_accrued, discount_factor, pv;
+  int32_t knocked_out, pad;
+} ro_coupon_row;
+
+/* pathcube.rs:47-76 generate_correlated_block: [paths][times][factors] floats of
+ * L z, z drawn `factors` at a time from ChaCha8Rng::seed_from_u64(seed). */
+void ro_correlated_block(uint64_t seed, uint64_t paths, int64_t times, int factors, const double *chol,
+                         float *out);
+/* pathcube.rs:146-187 gbm_evolve: spot cube from a block (same layout). */
+void ro_gbm_evolve(const double *s0, const double *vols, double rate, const double *dividend_yields,
+                   double dt, uint64_t paths, int64_t times, int factors, const float *block, float *out);
+
 /* Validation mirroring lib.rs:209-321 generalised to F factors.
  * Returns 0 or writes code/message. */
 int ro_validate(const ro_request *r, char *code, size_t codelen, char *msg, size_t msglen);
@@ -187,7 +204,9 @@ int ro_validate(const ro_request *r, char *code, size_t codelen, char *msg, size
  * in `out` cover only that range (path-order sequential, as the reference).
  * `records` (optional) receives one record per path in the range. */
 int ro_price(const ro_request *r, uint64_t path_begin, uint64_t path_end, ro_result *out,
-             ro_path_record *records, char *code, size_t codelen, char *msg, size_t msglen);
+             ro_path_record *records, ro_coupon_row *ledger, char *code, size_t codelen, char *msg,
+             size_t msglen);
+/* `ledger` (optional): periods rows per path in the range, explain.rs:146-262. */
 
 /* Generate the correlated, bridged and 1/sqrt(dt)-scaled shock cube of one path:
  * out[f*steps + t].  Exposed for bridge/ladder parity tests.  On the Monte Carlo

@@ -89,6 +89,24 @@ class PathRecord(C.Structure):
                 ("memory_pv", C.c_double), ("pv", C.c_double)]
 
 
+class CouponRow(C.Structure):
+    _fields_ = [("period", C.c_int64), ("years", C.c_double), ("qualified_steps", C.c_int64),
+                ("period_steps", C.c_int64), ("accrual", C.c_double), ("memory_paid", C.c_double),
+                ("paid", C.c_double), ("memory_accrued", C.c_double), ("discount_factor", C.c_double),
+                ("pv", C.c_double), ("knocked_out", C.c_int32), ("pad", C.c_int32)]
+
+
+COUPON_DTYPE = np.dtype([("period", "<i8"), ("years", "<f8"), ("qualified_steps", "<i8"),
+                         ("period_steps", "<i8"), ("accrual", "<f8"), ("memory_paid", "<f8"),
+                         ("paid", "<f8"), ("memory_accrued", "<f8"), ("discount_factor", "<f8"),
+                         ("pv", "<f8"), ("knocked_out", "<i4"), ("pad", "<i4")])
+
+
+class WorstQuery(C.Structure):
+    _fields_ = [("percentile", C.c_double), ("threshold_worst_final", C.c_double),
+                ("selected", C.c_uint64), ("total", C.c_uint64), ("n_sample", C.c_uint64)]
+
+
 RECORD_DTYPE = np.dtype([("ko_step", "<i4"), ("hit_ki", "<i4"), ("final_worst", "<f8"),
                          ("funding", "<f8"), ("put", "<f8"), ("coupon_pv", "<f8"),
                          ("memory_pv", "<f8"), ("pv", "<f8")])
@@ -147,7 +165,8 @@ EXPORTS = ("riga_hip_init", "riga_hip_destroy", "riga_hip_device_name", "riga_hi
            "riga_server_create", "riga_server_submit", "riga_server_wait",
            "riga_server_stats_get", "riga_server_destroy",
            "riga_surface_create", "riga_surface_destroy", "riga_autotune",
-           "riga_lv_calibrate", "riga_lv_sigma", "riga_debug_sobol", "riga_debug_mc_starts")
+           "riga_lv_calibrate", "riga_lv_sigma", "riga_debug_sobol", "riga_debug_mc_starts",
+           "riga_replay_ledger", "riga_query_worst_percentile", "riga_path_cube")
 
 _lib = None
 _lock = threading.Lock()
@@ -220,6 +239,17 @@ def lib():
                                            C.POINTER(Error)]
             L.riga_debug_mc_starts.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64,
                                                C.POINTER(C.c_uint64), C.POINTER(Error)]
+            _U64 = C.POINTER(C.c_uint64)
+            L.riga_replay_ledger.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int, _U64, C.c_uint64,
+                                             C.POINTER(PathRecord), C.POINTER(CouponRow), C.POINTER(Error)]
+            L.riga_query_worst_percentile.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int, C.c_double,
+                                                      C.c_uint64, C.POINTER(WorstQuery), _U64,
+                                                      C.POINTER(PathRecord), C.POINTER(CouponRow),
+                                                      C.POINTER(Error)]
+            _F32 = C.POINTER(C.c_float)
+            L.riga_path_cube.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int32, C.c_int32, _D, _D,
+                                         _D, C.c_double, _D, C.c_double, _F32, _F32, _F32,
+                                         C.POINTER(Error)]
             _lib = L
     return _lib
 

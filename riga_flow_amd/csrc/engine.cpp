@@ -179,6 +179,14 @@ namespace {
     }                                                                                    \
   } while (0)
 
+// Replay options: the paths to price (all when `ids` is null) and whether to
+// write their worst-of series and coupon strips.
+struct ReplayOpts {
+  const uint64_t* ids = nullptr;
+  uint64_t n_ids = 0;
+  bool ledger = false;
+};
+
 // ------------------------------------------------------------ per-request prep
 struct Prep {
   const riga_request* r = nullptr;
@@ -305,6 +313,7 @@ struct Item {
   std::vector<uint32_t> halton;
   int depth = 1;
   double cost = 0.0;
+  uint64_t span = 0;  // paths [0, span) of the request's stream that the item reads
 };
 
 // Streaming traversal of the reference bridge (qmc.rs:158-191): for every step,
@@ -477,6 +486,7 @@ bool build_item(const std::vector<Prep>& preps, Item* it, int precision, bool re
   }
   if (r.ladder == RIGA_VR_HALTON) it->halton = halton_tables(d.ladder_rows, r.seed);
   it->cost = (double)T * (double)F * (double)S * (double)r.paths;
+  it->span = d.path_begin + d.paths;
   return true;
 }
 
@@ -594,6 +604,10 @@ struct riga_plan {
   DevBuf arena, items_d, block_item, partials, finals, final_base_d, records_d;
   HostBuf finals_host;
   std::vector<PathRecordDev> recs_host;
+  // replay of listed paths / coupon strips
+  DevBuf path_list, series, ledger_d;
+  std::vector<CouponRowDev> ledger_host;
+  bool ledger = false;
   struct NormalTable {
     uint64_t seed = 0;
     int dims = 0, nbits = 0;
@@ -627,7 +641,9 @@ struct riga_plan {
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     // buffers go back to the pool on this stream before it is destroyed
-    for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d}) b->release();
+    for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d, &path_list,
+                      &series, &ledger_d})
+      b->release();
     for (auto& zt : ztables) zt->buf.release();
     for (auto& mp : mc) mp->each([](DevBuf& b, hipStream_t) { b.release(); }, nullptr);
     mc_status.release();
@@ -773,6 +789,11 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
     e = pl->precision == RIGA_FP64 ? launch_phoenix_f64(L, pl->stream) : launch_phoenix_f32(L, pl->stream);
     if (e != hipSuccess) return e;
   }
+  if (pl->ledger) {
+    e = launch_ledger((const ItemDesc*)pl->items_d.p, 0, pl->items[0].d.paths, (const unsigned char*)pl->arena.p,
+                      (CouponRowDev*)pl->ledger_d.p, pl->stream);
+    if (e != hipSuccess) return e;
+  }
   if (timed && (e = hipEventRecord(pl->e1, pl->stream)) != hipSuccess) return e;
   e = launch_finalize((const ItemDesc*)pl->items_d.p, (int)pl->items.size(), (const Partial*)pl->partials.p,
                       (Partial*)pl->finals.p, (const int64_t*)pl->final_base_d.p, pl->stream);
@@ -787,7 +808,8 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
 // With `estimate` set, only the device bytes the plan would hold are computed
 // (no stream, no allocation): the serving loop admits requests with it.
 int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, const riga_launch_config* cfg,
-                bool records, riga_plan** out, riga_error* err, int64_t* estimate = nullptr) {
+                bool records, riga_plan** out, riga_error* err, int64_t* estimate = nullptr,
+                const ReplayOpts* replay = nullptr) {
   clear_err(err);
   *out = nullptr;
   if (!ctx) {
@@ -858,8 +880,35 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       set_err(err, e.code, e.message);
       return RIGA_EINVAL;
     }
-    pl->path_steps += (double)it.d.S * (double)it.d.paths * (double)it.d.steps;
   }
+  std::vector<uint64_t> ids;
+  if (replay && replay->ids) {  // explain.rs:319-334: listed paths of the one request
+    if (items.size() != 1 || !records) {
+      set_err(err, "invalid_argument", "a path list replays one request");
+      return RIGA_EINVAL;
+    }
+    ItemDesc& d = items[0].d;
+    ids.assign(replay->ids, replay->ids + replay->n_ids);
+    uint64_t top = 0;
+    for (uint64_t id : ids) {
+      if (id >= d.paths) {
+        set_err(err, "path_out_of_range", "path_id " + std::to_string(id) +
+                                              " is outside the configured path count " + std::to_string(d.paths));
+        return RIGA_EINVAL;
+      }
+      top = std::max(top, id);
+    }
+    d.paths = ids.size();
+    d.nbits = 64 - __builtin_clzll(top + 1);
+    items[0].span = top + 1;
+    items[0].cost = (double)d.steps * d.F * (double)d.paths;
+  }
+  pl->ledger = replay && replay->ledger;
+  if (pl->ledger && (items.size() != 1 || !records)) {
+    set_err(err, "invalid_argument", "coupon strips are written for one replayed request");
+    return RIGA_EINVAL;
+  }
+  for (Item& it : items) pl->path_steps += (double)it.d.S * (double)it.d.paths * (double)it.d.steps;
   // launches: one per (factors, scenario bucket), blocks longest-first
   const int real_size = precision == RIGA_FP64 ? 8 : 4;
   std::map<PartKey, std::vector<int>> parts;
@@ -1083,7 +1132,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       std::unique_ptr<riga_plan::McParse> mp(new riga_plan::McParse());
       mp->seed = kv.first.first;
       mp->dpp = kv.first.second;
-      for (int i : kv.second) mp->paths = std::max<uint64_t>(mp->paths, items[i].d.path_begin + items[i].d.paths);
+      for (int i : kv.second) mp->paths = std::max<uint64_t>(mp->paths, items[i].span);
       mc_parse_size(mp.get());
       uint32_t key[8];
       chacha_key_from_seed(mp->seed, key);
@@ -1159,6 +1208,19 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     HIP_TRY(pl->records_d.ensure((size_t)pl->record_slots * sizeof(PathRecordDev)), "records alloc");
     pl->recs_host.resize((size_t)pl->record_slots);
   }
+  if (!ids.empty()) {
+    HIP_TRY(pl->path_list.ensure(ids.size() * 8), "path list alloc");
+    HIP_TRY(hipMemcpyAsync(pl->path_list.p, ids.data(), ids.size() * 8, hipMemcpyHostToDevice, pl->stream),
+            "path list upload");
+    descs[0].path_list = (const uint64_t*)pl->path_list.p;
+  }
+  if (pl->ledger) {
+    const ItemDesc& d = descs[0];
+    HIP_TRY(pl->series.ensure((size_t)d.paths * (size_t)d.steps * 8), "series alloc");
+    HIP_TRY(pl->ledger_d.ensure((size_t)d.paths * (size_t)d.periods * sizeof(CouponRowDev)), "ledger alloc");
+    pl->ledger_host.resize((size_t)d.paths * (size_t)d.periods);
+    descs[0].series = (double*)pl->series.p;
+  }
   HIP_TRY(hipMemcpyAsync(pl->arena.p, arena.bytes.data(), arena.bytes.size(), hipMemcpyHostToDevice, pl->stream),
           "arena upload");
   HIP_TRY(hipMemcpyAsync(pl->items_d.p, descs.data(), descs.size() * sizeof(ItemDesc), hipMemcpyHostToDevice,
@@ -1203,10 +1265,15 @@ int plan_launch(riga_plan* pl, riga_error* err) {
     HIP_TRY(hipMemcpyAsync(pl->recs_host.data(), pl->records_d.p, pl->recs_host.size() * sizeof(PathRecordDev),
                            hipMemcpyDeviceToHost, pl->stream),
             "records download");
+  if (pl->ledger)
+    HIP_TRY(hipMemcpyAsync(pl->ledger_host.data(), pl->ledger_d.p, pl->ledger_host.size() * sizeof(CouponRowDev),
+                           hipMemcpyDeviceToHost, pl->stream),
+            "ledger download");
   return 0;
 }
 
-int plan_wait(riga_plan* pl, riga_result* results, riga_path_record* records, riga_error* err) {
+int plan_wait(riga_plan* pl, riga_result* results, riga_path_record* records, riga_error* err,
+              riga_coupon_row* ledger = nullptr) {
   clear_err(err);
   HIP_TRY(hipStreamSynchronize(pl->stream), "kernel execution");
   float ms = 0.f;
@@ -1246,16 +1313,21 @@ int plan_wait(riga_plan* pl, riga_result* results, riga_path_record* records, ri
       }
     }
   }
+  if (ledger && pl->ledger) {
+    static_assert(sizeof(CouponRowDev) == sizeof(riga_coupon_row), "coupon row layout");
+    memcpy(ledger, pl->ledger_host.data(), pl->ledger_host.size() * sizeof(CouponRowDev));
+  }
   return 0;
 }
 
 int run_batch(riga_ctx* ctx, const riga_request* reqs, int n, int precision, const riga_launch_config* cfg,
-              riga_result* results, riga_path_record* records, riga_error* err) {
+              riga_result* results, riga_path_record* records, riga_error* err, const ReplayOpts* replay = nullptr,
+              riga_coupon_row* ledger = nullptr) {
   riga_plan* pl = nullptr;
-  int rc = plan_create(ctx, reqs, n, precision, cfg, records != nullptr, &pl, err);
+  int rc = plan_create(ctx, reqs, n, precision, cfg, records != nullptr, &pl, err, nullptr, replay);
   if (rc) return rc;
   rc = plan_launch(pl, err);
-  if (!rc) rc = plan_wait(pl, results, records, err);
+  if (!rc) rc = plan_wait(pl, results, records, err, ledger);
   delete pl;
   return rc;
 }
@@ -1437,6 +1509,64 @@ int riga_replay_paths(riga_ctx* ctx, const riga_request* req, int precision,
   return run_batch(ctx, req, 1, precision, nullptr, out, records, err);
 }
 
+int riga_replay_ledger(riga_ctx* ctx, const riga_request* req, int precision, const uint64_t* path_ids,
+                       uint64_t n_ids, riga_path_record* records, riga_coupon_row* ledger, riga_error* err) {
+  clear_err(err);
+  if (!records || !req) {
+    set_err(err, "invalid_argument", "request and records are required");
+    return RIGA_EINVAL;
+  }
+  if (path_ids && n_ids == 0) {
+    set_err(err, "invalid_argument", "an empty path list");
+    return RIGA_EINVAL;
+  }
+  ReplayOpts ro;
+  ro.ids = path_ids;
+  ro.n_ids = n_ids;
+  ro.ledger = ledger != nullptr;
+  return run_batch(ctx, req, 1, precision, nullptr, nullptr, records, err, &ro, ledger);
+}
+
+int riga_query_worst_percentile(riga_ctx* ctx, const riga_request* req, int precision, double percentile,
+                                uint64_t max_sample, riga_worst_query* out, uint64_t* sample_ids,
+                                riga_path_record* sample_records, riga_coupon_row* sample_ledger, riga_error* err) {
+  clear_err(err);
+  if (!req || !out) {
+    set_err(err, "invalid_argument", "request and output are required");
+    return RIGA_EINVAL;
+  }
+  if (!(percentile >= 0.0 && percentile <= 1.0)) {  // explain.rs:341-346
+    set_err(err, "invalid_percentile", "percentile must be within [0, 1]");
+    return RIGA_EINVAL;
+  }
+  // every path's record on the device, then the reference's stable ascending
+  // sort by final worst-of and the first ceil(total * percentile) paths
+  std::vector<riga_path_record> all((size_t)req->paths);
+  int rc = run_batch(ctx, req, 1, precision, nullptr, nullptr, all.data(), err);
+  if (rc) return rc;
+  std::vector<uint64_t> order(all.size());
+  std::iota(order.begin(), order.end(), (uint64_t)0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint64_t a, uint64_t b) { return all[a].final_worst < all[b].final_worst; });
+  const uint64_t total = all.size();
+  const uint64_t selected = (uint64_t)std::ceil((double)total * percentile);
+  out->percentile = percentile;
+  out->total = total;
+  out->selected = selected;
+  out->threshold_worst_final = selected == 0 ? 0.0 : all[order[selected - 1]].final_worst;
+  out->n_sample = std::min(selected, max_sample);
+  if (out->n_sample == 0) return 0;
+  for (uint64_t i = 0; i < out->n_sample; ++i)
+    if (sample_ids) sample_ids[i] = order[i];
+  if (!sample_records) return 0;
+  // the sample again, with coupon strips
+  ReplayOpts ro;
+  ro.ids = order.data();
+  ro.n_ids = out->n_sample;
+  ro.ledger = sample_ledger != nullptr;
+  return run_batch(ctx, req, 1, precision, nullptr, nullptr, sample_records, err, &ro, sample_ledger);
+}
+
 int riga_autotune(riga_ctx* ctx, const riga_request* reqs, int n, int precision, const riga_launch_config* cfg,
                   int reps, riga_tune_report* report, riga_error* err) {
   clear_err(err);
@@ -1597,6 +1727,106 @@ int riga_debug_sobol(riga_ctx* ctx, uint64_t seed, int32_t ladder_dims, uint64_t
   HIP_TRY(hipMemcpyAsync(out_scrambled, d_s.p, cells * 4, hipMemcpyDeviceToHost, ctx->stream), "download");
   HIP_TRY(hipMemcpyAsync(out_normal, d_n.p, cells * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
   HIP_TRY(hipStreamSynchronize(ctx->stream), "debug kernel execution");
+  return 0;
+}
+
+int riga_path_cube(riga_ctx* ctx, uint64_t seed, uint64_t paths, int32_t times, int32_t factors,
+                   const double* correlation, const double* s0, const double* vols, double rate,
+                   const double* dividend_yields, double dt, const float* in_block, float* out_block,
+                   float* out_cube, riga_error* err) {
+  clear_err(err);
+  if (!ctx) {
+    set_err(err, "gpu_unavailable", "no device context");
+    return RIGA_ENODEV;
+  }
+  if (paths == 0 || times <= 0 || factors < 1 || factors > kMaxFactors) {
+    set_err(err, "shape_mismatch", "paths and times must be positive and factors 1.." + std::to_string(kMaxFactors));
+    return RIGA_EINVAL;
+  }
+  const double cells = (double)paths * (double)times * (double)factors;
+  if (cells * 4.0 > 64e9) {
+    set_err(err, "invalid_argument", "a PathCube above 64 GB");
+    return RIGA_EINVAL;
+  }
+  if ((out_cube || in_block) && (!s0 || !vols || !dividend_yields)) {
+    set_err(err, "shape_mismatch", "s0, vols and dividend_yields are required to evolve spots");
+    return RIGA_EINVAL;
+  }
+  if (in_block && (!out_cube || out_block)) {
+    set_err(err, "invalid_argument", "evolving a given block writes only the cube");
+    return RIGA_EINVAL;
+  }
+  CubeArgs a{};
+  a.paths = paths;
+  a.times = times;
+  if (out_cube || in_block) {
+    const double sdt = sqrt(dt);
+    for (int f = 0; f < factors; ++f) {  // pathcube.rs:170-175
+      a.ln_s0[f] = log(s0[f]);
+      a.drift[f] = (rate - dividend_yields[f] - 0.5 * vols[f] * vols[f]) * dt;
+      a.diffusion[f] = vols[f] * sdt;
+    }
+  }
+  const size_t bytes = (size_t)cells * 4;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  DevBuf d_in, d_block, d_cube, d_key, d_zig, d_status;
+  riga_plan::McParse mp;
+  if (out_cube) HIP_TRY(d_cube.ensure(bytes), "cube alloc");
+  a.cube = (float*)d_cube.p;
+  if (in_block) {
+    HIP_TRY(d_in.ensure(bytes), "block alloc");
+    HIP_TRY(hipMemcpyAsync(d_in.p, in_block, bytes, hipMemcpyHostToDevice, ctx->stream), "block upload");
+    a.in_block = (const float*)d_in.p;
+  } else {
+    std::vector<double> chol;
+    Error e;
+    if (!correlation || !cholesky_factor(factors, correlation, &chol, &e)) {
+      set_err(err, "invalid_market", correlation ? e.message : "correlation is required");
+      return RIGA_EINVAL;
+    }
+    for (int i = 0; i < factors; ++i)
+      for (int j = 0; j < factors; ++j) a.chol[i * kMaxFactors + j] = chol[(size_t)i * factors + j];
+    if (out_block) HIP_TRY(d_block.ensure(bytes), "block alloc");
+    a.block = (float*)d_block.p;
+    // the stream parse: path p starts at draw p * times * factors
+    mp.seed = seed;
+    mp.dpp = (uint64_t)times * (uint64_t)factors;
+    mp.paths = paths;
+    mc_parse_size(&mp);
+    uint32_t key[8];
+    chacha_key_from_seed(seed, key);
+    HIP_TRY(d_key.ensure(32), "alloc");
+    HIP_TRY(d_zig.ensure(2 * kZigLen * 8), "alloc");
+    HIP_TRY(d_status.ensure(8), "alloc");
+    HIP_TRY(mc_parse_alloc(&mp), "stream parse alloc");
+    HIP_TRY(hipMemcpyAsync(d_key.p, key, 32, hipMemcpyHostToDevice, ctx->stream), "upload");
+    HIP_TRY(hipMemcpyAsync(d_zig.p, ziggurat_tables().data(), 2 * kZigLen * 8, hipMemcpyHostToDevice, ctx->stream),
+            "upload");
+    HIP_TRY(mc_parse_launch(mp, (const uint32_t*)d_key.p, (const double*)d_zig.p, (uint32_t*)d_status.p,
+                            ctx->stream),
+            "stream parse");
+    a.key = (const uint32_t*)d_key.p;
+    a.zig = (const double*)d_zig.p;
+    a.starts = (const uint64_t*)mp.starts.p;
+    a.chunks = (const McChunk*)mp.chunks.p;
+    a.slow = (const uint32_t*)mp.slow.p;
+    a.slow_val = (const double*)mp.slow_val.p;
+    a.n_chunks = mp.n_chunks;
+    a.chunk_words = mp.chunk_words;
+    a.slow_cap = mp.slow_cap;
+  }
+  HIP_TRY(launch_cube(a, factors, ctx->stream), "path cube");
+  uint32_t status[2] = {(uint32_t)paths, 0};
+  if (!in_block) HIP_TRY(hipMemcpyAsync(status, d_status.p, 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+  if (out_block) HIP_TRY(hipMemcpyAsync(out_block, d_block.p, bytes, hipMemcpyDeviceToHost, ctx->stream), "download");
+  if (out_cube) HIP_TRY(hipMemcpyAsync(out_cube, d_cube.p, bytes, hipMemcpyDeviceToHost, ctx->stream), "download");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "path cube execution");
+  if (status[0] != paths || status[1] != 0) {
+    set_err(err, "mc_stream_error", "stream parse found " + std::to_string(status[0]) + " of " +
+                                        std::to_string(paths) + " path starts, flags " + std::to_string(status[1]));
+    return RIGA_EGPU;
+  }
   return 0;
 }
 

@@ -59,8 +59,33 @@ struct McStartsArgs {
 };
 hipError_t launch_mc_parse(const McParseArgs& p, const McStartsArgs& s, hipStream_t stream);
 
+// PathCube generation from a parsed stream (draws per path = times * factors),
+// or gbm_evolve of `in_block` when it is set.  Outputs [paths][times][factors].
+struct CubeArgs {
+  const uint32_t* key;
+  const double* zig;
+  const uint64_t* starts;
+  const McChunk* chunks;
+  const uint32_t* slow;
+  const double* slow_val;
+  int64_t n_chunks;
+  int chunk_words, slow_cap;
+  uint64_t paths;
+  int times;
+  double chol[kMaxFactors * kMaxFactors];
+  double ln_s0[kMaxFactors], drift[kMaxFactors], diffusion[kMaxFactors];
+  const float* in_block;
+  float* block;  // null: not written
+  float* cube;   // null: not written
+};
+hipError_t launch_cube(const CubeArgs& a, int factors, hipStream_t stream);
+
 // Owen prefix table: out[dim][v] = scrambled top 12 bits of v << 20, dims [0, dims).
 hipError_t launch_owen_table(uint64_t seed, int dims, uint16_t* out, hipStream_t stream);
+
+// Coupon strips [path][period] of a replayed item from its worst-of series.
+hipError_t launch_ledger(const ItemDesc* items, int item, uint64_t paths, const unsigned char* arena,
+                         CouponRowDev* rows, hipStream_t stream);
 
 hipError_t launch_finalize(const ItemDesc* items, int n_items, const Partial* partials,
                            Partial* finals, const int64_t* final_base, hipStream_t stream);

@@ -154,6 +154,83 @@ hipError_t launch_owen_table(uint64_t seed, int dims, uint16_t* out, hipStream_t
   return hipGetLastError();
 }
 
+// Coupon strips of replayed paths (explain.rs:146-262): one lane per path walks
+// the path's worst-of series against the step table.  The daily KO is "every
+// name at or above the barrier", i.e. the worst-of at or above it.
+__global__ __launch_bounds__(256) void ledger_kernel(const ItemDesc* __restrict__ items, int item,
+                                                     const unsigned char* __restrict__ arena,
+                                                     CouponRowDev* __restrict__ rows) {
+  const ItemDesc& it = items[item];
+  const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= it.paths) return;
+  const StepInfo* st = reinterpret_cast<const StepInfo*>(arena + it.off_steps);
+  const int T = it.steps, P = it.periods;
+  CouponRowDev* row = rows + p * (uint64_t)P;
+  for (int q = 0; q < P; ++q) {
+    CouponRowDev r;
+    r.period = q;
+    r.years = 0.0;
+    r.qualified_steps = 0;
+    const int64_t a = (int64_t)T * (q + 1) / P, b = (int64_t)T * q / P;
+    r.period_steps = a > b ? a - b : 1;
+    r.accrual = r.memory_paid = r.paid = r.memory_accrued = r.discount_factor = r.pv = 0.0;
+    r.knocked_out = 0;
+    r.pad = 0;
+    row[q] = r;
+  }
+  const bool fixed_first = (it.flags & kFixedFirst) != 0;
+  const bool memory = (it.flags & kMemoryCoupon) != 0;
+  double accrual = 0.0, unpaid = 0.0;
+  for (int step = 0; step < T; ++step) {
+    const StepInfo si = st[step];
+    const double worst = it.series[(size_t)step * it.paths + p];
+    const int period = si.period;
+    const double time = (double)(step + 1) * it.dt;
+    if (si.flags & 4) accrual = 0.0;
+    const bool fixed_now = period == 0 && fixed_first;
+    if (!fixed_now && worst >= it.cb) {
+      accrual += si.accrual_inc;
+      row[period].qualified_steps += 1;
+    }
+    if ((si.flags & 2) && worst >= it.ko) {
+      const double due = fixed_now ? it.coupon_rate : accrual;
+      const double carry = due > 0.0 ? unpaid : 0.0;
+      const double amount = due + carry;
+      CouponRowDev& r = row[period];
+      r.years = time;
+      r.accrual = due;
+      r.memory_paid = carry;
+      r.paid = amount;
+      r.discount_factor = si.df;
+      r.pv = amount * si.df;
+      r.knocked_out = 1;
+      return;
+    }
+    if (si.flags & 1) {
+      const double due = fixed_now ? it.coupon_rate : accrual;
+      CouponRowDev& r = row[period];
+      r.years = time;
+      r.accrual = due;
+      r.memory_paid = due > 0.0 ? unpaid : 0.0;
+      r.paid = due > 0.0 ? due + unpaid : 0.0;
+      r.discount_factor = si.df;
+      r.pv = r.paid * si.df;
+      r.memory_accrued = (due == 0.0 && memory) ? it.coupon_rate : 0.0;
+      if (due > 0.0)
+        unpaid = 0.0;
+      else if (memory)
+        unpaid += it.coupon_rate;
+    }
+  }
+}
+
+hipError_t launch_ledger(const ItemDesc* items, int item, uint64_t paths, const unsigned char* arena,
+                         CouponRowDev* rows, hipStream_t stream) {
+  hipLaunchKernelGGL(ledger_kernel, dim3((unsigned)((paths + 255) / 256)), dim3(256), 0, stream, items, item,
+                     arena, rows);
+  return hipGetLastError();
+}
+
 hipError_t launch_normal_table_f64(const uint32_t* dirs, const uint16_t* owen_tab, uint64_t seed,
                                    int dims, int n_paths, int nbits, double* out, hipStream_t stream) {
   dim3 grid((n_paths + 255) / 256, dims);

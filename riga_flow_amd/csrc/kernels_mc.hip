@@ -206,6 +206,85 @@ __global__ __launch_bounds__(256) void mc_starts_kernel(McStartsArgs a) {
   if (j == a.n_chunks - 1 && end < a.paths * a.dpp) atomicOr(&a.status[1], (uint32_t)kMcErrShort);
 }
 
+// PathCube (pathcube.rs:47-76 generate_correlated_block, :146-187 gbm_evolve):
+// one lane per path regenerates its words like the pricing lanes, forms L z per
+// (path, time) cell in FP64, stores it as float, and evolves
+// log S += drift + diffusion * (double)(float)y, storing exp(log S) as float.
+template <int F>
+__global__ __launch_bounds__(256) void cube_kernel(CubeArgs a) {
+  __shared__ uint64_t ring_lds[24 * 256];
+  __shared__ double zx[kZigLen];
+  for (int i = threadIdx.x; i < kZigLen; i += blockDim.x) zx[i] = a.zig[i];
+  __syncthreads();
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = p < a.paths;
+  McRing<24> ring;
+  ring.base = ring_lds + threadIdx.x;
+  ring.stride = blockDim.x;
+  uint64_t wpos = a.starts[valid ? p : 0];
+  ring.seek(a.key, wpos);
+  McCursor cur;
+  cur.slow = a.slow;
+  cur.val = a.slow_val;
+  cur.chunks = a.chunks;
+  cur.n_chunks = a.n_chunks;
+  cur.C = a.chunk_words;
+  cur.cap = a.slow_cap;
+  cur.seek(wpos);
+  double log_s[F];
+#pragma unroll
+  for (int f = 0; f < F; ++f) log_s[f] = a.ln_s0[f];
+  for (int t = 0; t < a.times; ++t) {
+    if (__ballot(ring.fill < F) != 0ull && ring.fill <= 24 - 8) ring.append(a.key);
+    double z[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) z[f] = mc_draw(ring, cur, wpos, a.key, zx);
+    const size_t cell = ((size_t)p * (size_t)a.times + (size_t)t) * F;
+#pragma unroll
+    for (int i = 0; i < F; ++i) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j <= i; ++j) acc += a.chol[i * kMaxFactors + j] * z[j];
+      const float y = (float)acc;
+      if (valid && a.block) a.block[cell + i] = y;
+      log_s[i] += a.drift[i] + a.diffusion[i] * (double)y;
+      if (valid && a.cube) a.cube[cell + i] = (float)exp(log_s[i]);
+    }
+  }
+}
+
+// gbm_evolve of a given block: one lane per (path, factor).
+__global__ __launch_bounds__(256) void gbm_kernel(CubeArgs a, int factors) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.paths * (uint64_t)factors) return;
+  const uint64_t p = k / (uint64_t)factors;
+  const int f = (int)(k % (uint64_t)factors);
+  double log_s = a.ln_s0[f];
+  for (int t = 0; t < a.times; ++t) {
+    const size_t i = ((size_t)p * (size_t)a.times + (size_t)t) * (size_t)factors + (size_t)f;
+    log_s += a.drift[f] + a.diffusion[f] * (double)a.in_block[i];
+    a.cube[i] = (float)exp(log_s);
+  }
+}
+
+hipError_t launch_cube(const CubeArgs& a, int factors, hipStream_t stream) {
+  if (a.paths == 0 || a.times <= 0 || factors < 1 || factors > kMaxFactors) return hipErrorInvalidValue;
+  if (a.in_block) {
+    const uint64_t n = a.paths * (uint64_t)factors;
+    hipLaunchKernelGGL(gbm_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a, factors);
+    return hipGetLastError();
+  }
+  const dim3 grid((unsigned)((a.paths + 255) / 256)), block(256);
+  switch (factors) {
+    case 1: hipLaunchKernelGGL(cube_kernel<1>, grid, block, 0, stream, a); break;
+    case 2: hipLaunchKernelGGL(cube_kernel<2>, grid, block, 0, stream, a); break;
+    case 3: hipLaunchKernelGGL(cube_kernel<3>, grid, block, 0, stream, a); break;
+    case 4: hipLaunchKernelGGL(cube_kernel<4>, grid, block, 0, stream, a); break;
+    default: hipLaunchKernelGGL(cube_kernel<5>, grid, block, 0, stream, a); break;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_mc_parse(const McParseArgs& p, const McStartsArgs& s, hipStream_t stream) {
   if (p.chunk_words < 8 || p.chunk_words % 8 != 0 || p.chunk_words > (1 << 24) || p.n_chunks < 1 ||
       s.n_chunks != p.n_chunks)

@@ -490,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   for (int b = 0; b < batches; ++b) {
     const uint64_t path_local = (uint64_t)chunk * ppb + (uint64_t)b * nthreads + tid;
     const bool valid = path_local < it.paths;
-    const uint64_t path = it.path_begin + path_local;
+    const uint64_t path = it.path_list ? it.path_list[valid ? path_local : 0] : it.path_begin + path_local;
     const uint32_t n = (uint32_t)(path + 1);  // Sobol/Halton point index (qmc.rs:286, 121)
     const uint32_t gray = n ^ (n >> 1);
 
@@ -737,6 +737,8 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
           const uint32_t recv = pay ? 0x80000000u : (pk[s] & 0x80000000u);
           pk[s] = ko_now ? ((pk[s] & 0x7FF00000u) | recv | (uint32_t)(step + 1)) : pk[s];
         }
+        if (it.series != nullptr && valid)  // replay: the worst-of for the coupon strip
+          it.series[(size_t)step * it.paths + path_local] = exp((double)worst_last[0]);
         // Phase 3: period end (note.rs:197-218), a step-uniform branch.
         if (period_end) {
           const bool memory = (it.flags & kMemoryCoupon) != 0;
@@ -843,6 +845,8 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
           }
         }
       }
+      if (it.series != nullptr && valid)  // replay: the worst-of for the coupon strip
+        it.series[(size_t)step * it.paths + path_local] = (double)worst_last[0];
     }
 
     // ---- path end (note.rs:221-258) and reductions

@@ -171,6 +171,10 @@ struct ItemDesc {
   const double* mc_val;      // [chunk][mc_slow_cap] their draws
   int64_t mc_n_chunks;
   int32_t mc_chunk_words, mc_slow_cap;
+  // replay: the paths priced (null: path_begin + [0, paths)) and the worst-of
+  // per (step, path) for the coupon strips (null: not written)
+  const uint64_t* path_list;
+  double* series;        // [step][path]
 };
 
 // Double-double accumulator (hi + lo).
@@ -207,6 +211,16 @@ struct BlockShared {
   int32_t wave_ko[kWavesPerBlock];
   int32_t ev_path[kTraceCap];
   int32_t ev_step[kTraceCap];
+};
+
+// One period of a path's coupon strip (explain.rs:35-55 CouponPeriodRecord);
+// the layout of riga_coupon_row.
+struct CouponRowDev {
+  int64_t period;
+  double years;
+  int64_t qualified_steps, period_steps;
+  double accrual, memory_paid, paid, memory_accrued, discount_factor, pv;
+  int32_t knocked_out, pad;
 };
 
 // Per-path record (explain.rs:58-74 subset) written only for replay queries.
