@@ -83,3 +83,124 @@ def test_delta_convergence_and_rejections():
     with pytest.raises(RigaError) as e:
         risk.price_central_greeks(cal, max_dimension=md)
     assert e.value.code == "risk_surface_shape"
+
+
+def termsheet_small():
+    # riga-risk/src/lib.rs:724-732: the reference's risk tests run on this request
+    # (the ChaCha8 Monte Carlo lane, implied grids recalibrated per scenario)
+    import os
+    from riga_flow_amd.fixture import load_termsheet1_fixture
+    req = load_termsheet1_fixture(os.path.join(os.path.dirname(__file__), "golden", "termsheet1.json")).request
+    req.config.paths = 512
+    req.config.path_block_size = 128
+    req.config.seed = 81_281
+    return req
+
+
+def _same(a, b):
+    assert a.pv == b.pv and a.mc_error == b.mc_error and a.legs == b.legs
+    assert [s.output_count for s in a.stats] == [s.output_count for s in b.stats]
+
+
+def test_reference_central_greeks_tests_on_the_device():
+    # riga-risk/src/lib.rs:734-811, 847-883
+    req = termsheet_small()
+    cfg = risk.CentralDifferenceConfig()
+    first = risk.price_central_greeks(req, cfg)
+    second = risk.price_central_greeks(req, cfg)
+    assert first.base_price.pv == second.base_price.pv and first.seed == req.config.seed
+    assert len(first.assets) == 2
+    a = first.assets[0]
+    h = req.market.spots[0] * cfg.spot_relative_bump
+    up = req.clone()
+    up.market.spots[0] += h
+    _same(a.spot_up, api.price_put_leg(up, "fp64"))
+    assert a.delta.value == (a.spot_up.pv - a.spot_down.pv) / (2.0 * h)
+    assert a.gamma.value == (a.spot_up.pv - 2.0 * first.base_price.pv + a.spot_down.pv) / (h * h)
+    assert a.delta.mc_error_upper_bound == (a.spot_up.mc_error + a.spot_down.mc_error) / (2.0 * h)
+    assert a.gamma.mc_error_upper_bound == \
+        (a.spot_up.mc_error + 2.0 * first.base_price.mc_error + a.spot_down.mc_error) / (h * h)
+    assert a.vega.mc_error_upper_bound == (a.vol_up.mc_error + a.vol_down.mc_error) / (2.0 * 0.01)
+    for asset in first.assets:
+        for x in (asset.delta, asset.gamma, asset.vega):
+            assert math.isfinite(x.mc_error_upper_bound)
+    # sticky-strike vega: every implied node of the asset shifted, recalibrated
+    vu, vd = req.clone(), req.clone()
+    for q, d in ((vu, 0.01), (vd, -0.01)):
+        q.market.vols[0] += d
+        g = q.local_vol.grids[0]
+        g.vols = [[v + d for v in row] for row in g.vols]
+    _same(a.vol_up, api.price_put_leg(vu, "fp64"))
+    _same(a.vol_down, api.price_put_leg(vd, "fp64"))
+    assert abs(a.vega.value) > 1e-8
+
+
+def test_reference_crn_tests_on_the_device():
+    # riga-risk/src/lib.rs:835-845, 919-939
+    req = termsheet_small()
+    assert risk.crn_variance_reduction_factor(req, replicates=40) > 1.5
+    coarse = risk.crn_delta_convergence(req, replicates=16)
+    fine = risk.crn_delta_convergence(req, replicates=64)
+    assert coarse.standard_error > 0 and fine.standard_error > 0
+    assert fine.standard_error <= coarse.standard_error
+
+
+def test_reference_batch_risk_tests_on_the_device():
+    # riga-risk/src/lib.rs:941-978
+    req = termsheet_small()
+    cfg = risk.CentralDifferenceConfig()
+    rep = risk.price_risk_batch(req, cfg)
+    assert rep.scenario_count == 1 + 4 * 2 + 4 and len(rep.assets) == 2
+    assert math.isfinite(rep.base_pv) and math.isfinite(rep.dv01()) and math.isfinite(rep.correlation_sensitivity())
+    rate = [m for m in risk.pnl_explain(rep, [0.0, 0.0], [0.0, 0.0], 0.0001, 0.0) if m.label == "rate"][0]
+    assert abs(rate.contribution - rep.dv01() * 0.0001) < 1e-15
+    pf = risk.aggregate_portfolio([(10_000.0, rep), (5_000.0, rep)])
+    assert pf.trades == 2 and abs(pf.pv - 15_000.0 * rep.base_pv) < 1e-9 and len(pf.delta) == 2
+    h = req.market.spots[0] * cfg.spot_relative_bump
+    bumped = req.clone()
+    bumped.market.spots[0] += h
+    actual = api.price_put_leg(bumped, "fp64").pv - rep.base_pv
+    total = sum(m.contribution for m in risk.pnl_explain(rep, [h, 0.0], [0.0, 0.0], 0.0, 0.0))
+    assert abs(total - actual) < 1e-3
+    # the rate and correlation scenarios, each priced alone
+    ru = req.clone()
+    ru.market.risk_free_rate += 1e-4
+    assert api.price_put_leg(ru, "fp64").pv == rep.rate_up_pv
+    cd = req.clone()
+    cd.market.correlation = [[1.0, req.market.correlation[0][1] - 0.01], [req.market.correlation[1][0] - 0.01, 1.0]]
+    assert api.price_put_leg(cd, "fp64").pv == rep.correlation_down_pv
+    # transition risk: the KO transition's delta from its own bump-pair impacts
+    names = [t.id for t in rep.transitions]
+    assert names == ["daily_global_ko", "final_fixing_eki", "memory_coupon"]
+
+
+def test_adapter_batches_equal_their_scenarios_priced_alone():
+    # riga-mcp/src/main.rs:1216-1278, 1355-1537 with every valuation in one batch
+    req = termsheet_small()
+    det = risk.risk_detailed(req)
+    assert det["scenario_count"] == 13 + 1 + 4
+    half = [s * 0.01 for s in req.market.spots]
+    pv = {}
+    for di, dj in ((1, 1), (1, -1), (-1, 1), (-1, -1)):
+        q = req.clone()
+        q.market.spots[0] += di * half[0]
+        q.market.spots[1] += dj * half[1]
+        pv[(di, dj)] = api.price_put_leg(q, "fp64").pv
+    cg = (pv[(1, 1)] - pv[(1, -1)] - pv[(-1, 1)] + pv[(-1, -1)]) / (4.0 * half[0] * half[1])
+    assert det["cross_gamma"][0][1] == cg == det["cross_gamma"][1][0]
+    dec = req.clone()
+    dec.trade.maturity_years = max(req.trade.maturity_years - 1 / 365, 1 / 365)
+    assert det["theta"] == api.price_put_leg(dec, "fp64").pv - det["base_pv"]
+    td = risk.trade_design(req)
+    assert td["scenario_count"] == 6 and [p["parameter"] for p in td["parameters"]] == \
+        ["strike", "knock_in_barrier", "knock_out_barrier", "coupon_barrier", "coupon_rate"]
+    q = req.clone()
+    q.trade.strike += 0.01
+    assert td["parameters"][0]["delta_pv"] == api.price_put_leg(q, "fp64").pv - td["base"]["pv"]
+    conv = risk.mc_convergence(req)
+    assert [p["paths"] for p in conv["points"]] == risk.convergence_ladder(512)
+    q = req.clone()
+    q.config.paths = 64
+    r = api.price_put_leg(q, "fp64")
+    p64 = [p for p in conv["points"] if p["paths"] == 64][0]
+    assert p64["pv"] == r.pv and p64["se"] == r.mc_error and conv["final"]["paths"] == 512

@@ -45,3 +45,71 @@ def test_calibrated_only_surfaces_cannot_take_a_vega_bump():
     with pytest.raises(RigaError) as e:
         risk.bump_volatility_request(req, 0, 0.01)
     assert e.value.code == "risk_surface_shape"
+
+
+def termsheet_small():
+    # riga-risk/src/lib.rs:724-732
+    import os
+    from riga_flow_amd.fixture import load_termsheet1_fixture
+    req = load_termsheet1_fixture(os.path.join(os.path.dirname(__file__), "golden", "termsheet1.json")).request
+    req.config.paths = 512
+    req.config.path_block_size = 128
+    req.config.seed = 81_281
+    return req
+
+
+def test_reference_rejections_before_pricing():
+    # riga-risk/src/lib.rs:813-833, 885-917 (the host checks; no valuation runs)
+    cfg = risk.CentralDifferenceConfig()
+    risk._validate(termsheet_small(), cfg)
+    with pytest.raises(RigaError) as e:
+        risk._validate(termsheet_small(), risk.CentralDifferenceConfig(volatility_absolute_bump=1.0))
+    assert e.value.code == "invalid_volatility_bump"
+    req = termsheet_small()
+    req.market.spots[0] = 5e-324  # f64::from_bits(1)
+    with pytest.raises(RigaError) as e:
+        risk._validate(req, cfg)
+    assert e.value.code == "invalid_spot_bump"
+    req = termsheet_small()
+    req.config.paths = 1
+    with pytest.raises(RigaError) as e:
+        risk._validate(req, cfg)
+    assert e.value.code == "risk_paths_required"
+    req = termsheet_small()
+    req.market.correlation = [[1.0, 0.995], [0.995, 1.0]]
+    with pytest.raises(RigaError) as e:
+        risk._validate(req, cfg)
+    assert e.value.code == "invalid_correlation_bump"
+    req = W.flat_book(paths=64, steps=10, years=10 / 252)
+    req.market = W.flat_market(1)
+    req.trade.initial_spots = req.trade.initial_spots[:1]
+    with pytest.raises(RigaError) as e:
+        risk._validate(req, cfg)
+    assert e.value.code == "risk_input_shape"
+
+
+def _report(d=(0.5, -0.2), g=(0.01, 0.02), v=(0.3, 0.1)):
+    assets = [risk.BatchAssetGreeks(i, d[i], g[i], v[i], 1e-3, 1e-3) for i in range(2)]
+    return risk.BatchRiskReport(1.01, 13, 7, assets, [], 1.02, 1.00, 1.015, 1.005, 1e-4, 1e-2, 1e-3)
+
+
+def test_dv01_pnl_explain_and_portfolio_formulas():
+    # riga-risk/src/lib.rs:556-565, 637-720
+    rep = _report()
+    assert rep.dv01() == (1.02 - 1.00) / (2.0 * 1e-4)
+    assert rep.correlation_sensitivity() == (1.015 - 1.005) / (2.0 * 1e-2)
+    moves = {m.label: m.contribution for m in risk.pnl_explain(rep, [2.0, 0.0], [0.0, 0.01], 1e-4, 0.02)}
+    assert moves["spot[0]"] == 0.5 * 2.0 + 0.5 * 0.01 * 2.0 * 2.0
+    assert moves["spot[1]"] == 0.0 and moves["vol[1]"] == 0.1 * 0.01
+    assert moves["rate"] == rep.dv01() * 1e-4 and moves["correlation"] == rep.correlation_sensitivity() * 0.02
+    assert "vol[0]" in moves  # reported because vega != 0 (lib.rs:661)
+    pf = risk.aggregate_portfolio([(10_000.0, rep), (5_000.0, _report(d=(1.0, 1.0)))])
+    assert pf.trades == 2 and abs(pf.pv - 15_000.0 * 1.01) < 1e-9
+    assert pf.delta == [10_000.0 * 0.5 + 5_000.0 * 1.0, 10_000.0 * -0.2 + 5_000.0 * 1.0]
+
+
+def test_convergence_ladder():
+    # riga-mcp/src/main.rs:1216-1227
+    assert risk.convergence_ladder(2000) == [62, 125, 250, 500, 1000, 2000]
+    assert risk.convergence_ladder(1) == [1]
+    assert risk.convergence_ladder(40) == [1, 2, 5, 10, 20, 40]
