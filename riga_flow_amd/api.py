@@ -383,6 +383,21 @@ def price_put_leg(request: PriceRequest, precision: str = "fp64",
     return price_batch([request], precision, max_dimension=max_dimension)[0]
 
 
+def price_trade(request_json, precision: str = "fp64", max_dimension: int = 512) -> dict:
+    """riga-mcp's ``price_trade`` (main.rs:1821-1831): a ``PriceRequest`` as JSON (a
+    dict or text) in, the ``PriceResult`` JSON out; a request that does not decode is
+    ``invalid_price_request``, engine errors keep their codes."""
+    import json as _json
+    from .types import request_from_json
+    if isinstance(request_json, (str, bytes)):
+        try:
+            request_json = _json.loads(request_json)
+        except ValueError as e:
+            raise RigaError("invalid_price_request", f"request must match riga_engine::PriceRequest: {e}")
+    req = request_from_json(request_json)
+    return result_to_json(price_put_leg(req, precision, max_dimension=max_dimension))
+
+
 def replay_paths(request: PriceRequest, precision: str = "fp64",
                  max_dimension: int = 512) -> Tuple[np.ndarray, PriceResult]:
     """Per-path records (explain.rs:90-316); the only call that writes paths to HBM."""

@@ -237,3 +237,138 @@ class PriceResult:
     ledger: List[LedgerEntry]
     trace_events: List[str]
     determinism: DeterminismEnvelope
+
+
+# ---------------------------------------------------------------- PriceRequest JSON
+def _bad(msg: str) -> RigaError:
+    return RigaError("invalid_price_request", f"request must match riga_engine::PriceRequest: {msg}")
+
+
+def _num(doc, key, where, optional=False):
+    if key not in doc or doc[key] is None:
+        if optional:
+            return None
+        raise _bad(f"missing field `{key}` in {where}")
+    v = doc[key]
+    if isinstance(v, bool) or not isinstance(v, (int, float)):
+        raise _bad(f"invalid type for `{key}` in {where}: expected f64")
+    return float(v)
+
+
+def _uint(doc, key, where, optional=False):
+    if key not in doc or doc[key] is None:
+        if optional:
+            return None
+        raise _bad(f"missing field `{key}` in {where}")
+    v = doc[key]
+    if isinstance(v, bool) or not isinstance(v, int) or v < 0:
+        raise _bad(f"invalid value for `{key}` in {where}: expected usize")
+    return v
+
+
+def _vec(doc, key, where):
+    if key not in doc:
+        raise _bad(f"missing field `{key}` in {where}")
+    v = doc[key]
+    if not isinstance(v, list) or any(isinstance(x, bool) or not isinstance(x, (int, float)) for x in v):
+        raise _bad(f"invalid type for `{key}` in {where}: expected a sequence of f64")
+    return [float(x) for x in v]
+
+
+def _str(doc, key, where):
+    if key not in doc:
+        raise _bad(f"missing field `{key}` in {where}")
+    if not isinstance(doc[key], str):
+        raise _bad(f"invalid type for `{key}` in {where}: expected a string")
+    return doc[key]
+
+
+def _bool(doc, key, where, default=None):
+    if key not in doc:
+        if default is not None:
+            return default
+        raise _bad(f"missing field `{key}` in {where}")
+    if not isinstance(doc[key], bool):
+        raise _bad(f"invalid type for `{key}` in {where}: expected a boolean")
+    return doc[key]
+
+
+def request_from_json(doc) -> "PriceRequest":
+    """serde's decoding of ``PriceRequest`` (riga-engine/src/lib.rs:36-46 with the
+    riga-core field rules): required fields, ``Option`` fields absent or null,
+    ``#[serde(default)]`` fields defaulted, snake_case enums, the
+    ``variance_reduction`` kind tag.  Errors are ``invalid_price_request``."""
+    if not isinstance(doc, dict):
+        raise _bad("expected a JSON object")
+    for k in ("trade", "market", "config"):
+        if not isinstance(doc.get(k), dict):
+            raise _bad(f"missing field `{k}`" if k not in doc else f"invalid type for `{k}`: expected a struct")
+    t, m, c = doc["trade"], doc["market"], doc["config"]
+    trade = PhoenixTrade(
+        trade_id=_str(t, "trade_id", "trade"), initial_spots=_vec(t, "initial_spots", "trade"),
+        maturity_years=_num(t, "maturity_years", "trade"), steps=_uint(t, "steps", "trade"),
+        strike=_num(t, "strike", "trade"), knock_in_barrier=_num(t, "knock_in_barrier", "trade"),
+        knock_out_barrier=_num(t, "knock_out_barrier", "trade", optional=True),
+        coupon_barrier=_num(t, "coupon_barrier", "trade", optional=True),
+        coupon_rate=_num(t, "coupon_rate", "trade", optional=True),
+        memory_coupon=_bool(t, "memory_coupon", "trade"),
+        coupon_periods=_uint(t, "coupon_periods", "trade", optional=True),
+        fixed_first_coupon=_bool(t, "fixed_first_coupon", "trade", default=False),
+        call_start_step=_uint(t, "call_start_step", "trade", optional=True))
+    corr = m.get("correlation")
+    if not isinstance(corr, list) or not all(isinstance(r, list) for r in corr):
+        raise _bad("missing field `correlation` in market" if corr is None
+                   else "invalid type for `correlation` in market: expected a sequence of sequences")
+    market = MarketSnapshot(
+        as_of=_str(m, "as_of", "market"), spots=_vec(m, "spots", "market"), vols=_vec(m, "vols", "market"),
+        risk_free_rate=_num(m, "risk_free_rate", "market"),
+        dividend_yields=_vec(m, "dividend_yields", "market"),
+        correlation=[_vec({"row": r}, "row", "market.correlation") for r in corr])
+    level = _str(c, "trace_level", "config")
+    if level not in TRACE_LEVELS:
+        raise _bad(f"unknown variant `{level}` for trace_level, expected one of {', '.join(TRACE_LEVELS)}")
+    vr_doc = c.get("variance_reduction", {"kind": "none"})
+    if not isinstance(vr_doc, dict) or vr_doc.get("kind") not in ("none", "halton", "sobol"):
+        raise _bad("variance_reduction must be {\"kind\": \"none\" | \"halton\" | \"sobol\"}")
+    vr = VarianceReduction(vr_doc["kind"], 0 if vr_doc["kind"] == "none"
+                           else _uint(vr_doc, "dimension", "variance_reduction"))
+    config = SimulationConfig(paths=_uint(c, "paths", "config"), path_block_size=_uint(c, "path_block_size", "config"),
+                              seed=_uint(c, "seed", "config"), trace_level=level, variance_reduction=vr)
+    lv = None
+    if doc.get("local_vol") is not None:
+        lvd = doc["local_vol"]
+        if not isinstance(lvd, dict) or not isinstance(lvd.get("grids"), list):
+            raise _bad("local_vol must be {\"grids\": [...]}")
+        grids = []
+        for g in lvd["grids"]:
+            if not isinstance(g, dict) or not isinstance(g.get("vols"), list):
+                raise _bad("each local_vol grid needs maturities_years, strikes and vols")
+            grids.append(ImpliedVolGrid(_vec(g, "maturities_years", "grid"), _vec(g, "strikes", "grid"),
+                                        [_vec({"row": r}, "row", "grid.vols") for r in g["vols"]]))
+        lv = LocalVolSpec(grids=grids)
+    return PriceRequest(trade=trade, market=market, config=config, as_of=_str(doc, "as_of", "request"),
+                        local_vol=lv)
+
+
+def request_to_json(req: "PriceRequest") -> dict:
+    """serde's encoding of ``PriceRequest`` (field order as declared; ``local_vol``
+    omitted when absent)."""
+    t, m, c = req.trade, req.market, req.config
+    out = {
+        "trade": {"trade_id": t.trade_id, "initial_spots": list(t.initial_spots), "maturity_years": t.maturity_years,
+                  "steps": t.steps, "strike": t.strike, "knock_in_barrier": t.knock_in_barrier,
+                  "knock_out_barrier": t.knock_out_barrier, "coupon_barrier": t.coupon_barrier,
+                  "coupon_rate": t.coupon_rate, "memory_coupon": t.memory_coupon,
+                  "coupon_periods": t.coupon_periods, "fixed_first_coupon": t.fixed_first_coupon,
+                  "call_start_step": t.call_start_step},
+        "market": {"as_of": m.as_of, "spots": list(m.spots), "vols": list(m.vols),
+                   "risk_free_rate": m.risk_free_rate, "dividend_yields": list(m.dividend_yields),
+                   "correlation": [list(r) for r in m.correlation]},
+        "config": {"paths": c.paths, "path_block_size": c.path_block_size, "seed": c.seed,
+                   "trace_level": c.trace_level, "variance_reduction": c.variance_reduction.to_json()},
+        "as_of": req.as_of,
+    }
+    if req.local_vol is not None:
+        out["local_vol"] = {"grids": [{"maturities_years": list(g.maturities_years), "strikes": list(g.strikes),
+                                       "vols": [list(r) for r in g.vols]} for g in req.local_vol.grids]}
+    return out

@@ -117,3 +117,13 @@ def test_mc_and_qmc_items_in_one_graph_replay_like_eager():
         api.price_put_leg(mc, "fp64")
     plan.close()
     assert eager[0].pv != eager[1].pv
+
+
+def test_price_trade_json_front_end_reproduces_the_golden():
+    # riga-mcp price_trade (main.rs:1821-1831): request JSON in, PriceResult JSON out
+    from riga_flow_amd.types import request_to_json
+    lf = load_termsheet1_fixture(os.path.join(GOLDEN, "termsheet1.json"))
+    text = json.dumps(request_to_json(lf.request))
+    with open(os.path.join(GOLDEN, "termsheet1.golden.json")) as fh:
+        golden = json.load(fh)
+    assert_json_close(api.price_trade(text), golden)
