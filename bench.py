@@ -292,6 +292,11 @@ def mc_leg(args) -> dict:
     plain = W.lv_book(paths=plain_paths, steps=756)
     for r in crn + [plain]:
         r.config.variance_reduction = W.VarianceReduction("none")
+    shapes = {}
+    if not args.no_tune:
+        for key, reqs, tier in (("fp32_batched", crn, "fp32"), ("fp64_plain", [plain], "fp64")):
+            t = api.autotune(reqs, tier, use_graph=True, reps=1)
+            shapes[key] = {"paths_per_thread": t["paths_per_thread"], "waves_per_block": t["waves_per_block"]}
     p32 = api.Plan(crn, "fp32", use_graph=True)
     p64 = api.Plan([plain], "fp64", use_graph=True)
     w32, d32, r32 = timed(p32, args.steps, args.warmup)
@@ -301,6 +306,8 @@ def mc_leg(args) -> dict:
     v32 = p32.path_steps * args.steps / w32
     v64 = p64.path_steps * args.steps / w64
     log(f"mc lane: fp32 batched {w32 / args.steps * 1e3:.1f} ms, fp64 plain {w64 / args.steps * 1e3:.1f} ms")
+    launch = {"fp32_batched": [p32.info.paths_per_thread, p32.info.waves_per_block],
+              "fp64_plain": [p64.info.paths_per_thread, p64.info.waves_per_block]}
     p32.close()
     p64.close()
     here = os.path.dirname(os.path.abspath(__file__))
@@ -316,6 +323,7 @@ def mc_leg(args) -> dict:
                            "device_ms": sum(d64) / len(d64), "paths": plain_paths},
             "fp32_batched_over_fp64_plain": v32 / v64,
             "fp32_max_rel_pv_err": err,
+            "launch_shapes": launch, "autotuned": bool(shapes),
             "termsheet1_golden": {"pv": g.pv, "golden_pv": golden["pv"], "abs_pv_diff": abs(g.pv - golden["pv"]),
                                   "ko_count": g.stats[0].output_count,
                                   "golden_ko_count": golden["stats"][0]["output_count"],

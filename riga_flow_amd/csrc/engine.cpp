@@ -683,13 +683,14 @@ hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t st
 }
 
 // Stream parse sizing for paths [0, mp->paths) at mp->dpp draws per path:
-// chunks are sized so a parse has ~32k or more of them (one lane each), and the
-// stream is cut 3% beyond the words the draws need (1.022 per draw on average)
-// plus eight chunks.
+// chunks (one lane each) are sized so a parse fills the device (3 waves per
+// SIMD at the parse's 130 VGPRs: ~196k lanes) before they grow past 512 words,
+// and the stream is cut 3% beyond the words the draws need (1.022 per draw on
+// average) plus eight chunks.
 void mc_parse_size(riga_plan::McParse* mp) {
   const double draws = (double)mp->paths * (double)mp->dpp;
   int C = 512;
-  while (C < 4096 && draws * 1.03 / C > 32768.0) C *= 2;
+  while (C < 4096 && draws * 1.03 / C > 196608.0) C *= 2;
   mp->chunk_words = C;
   mp->slow_cap = C / 16 + 32;
   mp->n_chunks = (int64_t)std::ceil((draws * 1.03 + 8.0 * C) / C) + 1;
