@@ -103,3 +103,24 @@ def test_resident_market_tickets_alone_equal_in_the_mix():
         alone = api.price_batch(reqs, tier, max_dimension=MD)
         for a, b in zip(g.results, alone):
             assert a.pv == b.pv and a.legs.down_and_in_put == b.legs.down_and_in_put
+
+
+def test_monte_carlo_lane_tickets_alone_equal_in_the_mix():
+    # the reference's default lane (ChaCha8) through the server, beside Sobol tickets:
+    # each ticket's stream parse runs inside its own plan
+    names, crn = W.crn_scenarios(paths=3000, steps=126, years=0.5)
+    mc = [r.clone() for r in crn[:5]]
+    for r in mc:
+        r.config.variance_reduction = W.VarianceReduction("none")
+    flat = W.flat_book(paths=5000, steps=63, years=0.25)
+    flat.config.variance_reduction = W.VarianceReduction("none")
+    tickets = [(mc, "fp64"), (crn[:2], "fp64"), ([flat], "fp32"), (mc[:2], "fp32")]
+    srv = serving.Server(max_inflight=3, max_batch_requests=64)
+    ids = [srv.submit(reqs, tier) for reqs, tier in tickets]
+    got = [srv.wait(t) for t in ids]
+    srv.close()
+    assert all(g.state == serving.DONE for g in got)
+    for (reqs, tier), g in zip(tickets, got):
+        alone = api.price_batch(reqs, tier, max_dimension=MD)
+        for a, b in zip(g.results, alone):
+            assert a.pv == b.pv and a.stats[0].output_count == b.stats[0].output_count
