@@ -328,6 +328,40 @@ int riga_path_cube(riga_ctx *ctx, uint64_t seed, uint64_t paths, int32_t times, 
                    const double *dividend_yields, double dt, const float *in_block, float *out_block,
                    float *out_cube, riga_error *err);
 
+/* ---- the reference's device module (crates/riga-engine/src/gpu.rs), for its callers
+ * (examples/gpu_bench.rs, examples/gpu_lv_bench.rs).  FP32, synchronous like the
+ * reference; resident handles keep their inputs in device memory. */
+typedef struct riga_resident_worst_of riga_resident_worst_of;
+typedef struct riga_resident_lv riga_resident_lv;
+
+/* gpu.rs:171-205, :515-517 worst_of: out[i] = a[i] < b[i] ? a[i] : b[i]. */
+int riga_worst_of(riga_ctx *ctx, const float *a, const float *b, uint64_t n, float *out, riga_error *err);
+/* gpu.rs:331-357 resident_worst_of and ResidentWorstOf::{reduce, download, synchronize} (gpu.rs:362-405). */
+int riga_resident_worst_of_create(riga_ctx *ctx, const float *a, const float *b, uint64_t n,
+                                  riga_resident_worst_of **out, riga_error *err);
+int riga_resident_worst_of_reduce(riga_resident_worst_of *h, riga_error *err);
+int riga_resident_worst_of_download(riga_resident_worst_of *h, float *out, riga_error *err);
+int riga_resident_worst_of_synchronize(riga_resident_worst_of *h, riga_error *err);
+void riga_resident_worst_of_destroy(riga_resident_worst_of *h);
+
+/* gpu.rs:207-278, :538-566 lv_terminal_spots (kernel gpu.rs:37-75): two factors, shocks
+ * normals[paths][steps][2], local-vol rows[2][steps][gx], evenly spaced log-spot axes
+ * ls_axis[2][gx]; outputs terminal spots [paths][2] and the first step both names
+ * stand at or above call_barrier x reference (or -1). */
+int riga_lv_terminal_spots(riga_ctx *ctx, const float *normals, const float *rows, const float *ls_axis,
+                           int32_t gx, const float *s0, const float *references, float call_barrier,
+                           int64_t paths, int64_t steps, float dt, float rate, const float *dividends,
+                           float *out_terminal, int32_t *out_ko, riga_error *err);
+/* gpu.rs:280-328 resident_lv and ResidentLv::{run, download, synchronize} (gpu.rs:407-489). */
+int riga_resident_lv_create(riga_ctx *ctx, const float *normals, const float *rows, const float *ls_axis,
+                            int32_t gx, int64_t paths, int64_t steps, float dt, riga_resident_lv **out,
+                            riga_error *err);
+int riga_resident_lv_run(riga_resident_lv *h, const float *s0, const float *references, float call_barrier,
+                         float rate, const float *dividends, riga_error *err);
+int riga_resident_lv_download(riga_resident_lv *h, float *out_terminal, int32_t *out_ko, riga_error *err);
+int riga_resident_lv_synchronize(riga_resident_lv *h, riga_error *err);
+void riga_resident_lv_destroy(riga_resident_lv *h);
+
 /* ---- diagnostics: the Monte Carlo lane's stream parse.  out_starts[p] = index of
  * the u64 word of ChaCha8Rng::seed_from_u64(seed) at which the StandardNormal draw
  * number p * draws_per_path begins (the first draw of path p, note.rs:58,124-126). */

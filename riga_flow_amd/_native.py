@@ -166,7 +166,12 @@ EXPORTS = ("riga_hip_init", "riga_hip_destroy", "riga_hip_device_name", "riga_hi
            "riga_server_stats_get", "riga_server_destroy",
            "riga_surface_create", "riga_surface_destroy", "riga_autotune",
            "riga_lv_calibrate", "riga_lv_sigma", "riga_debug_sobol", "riga_debug_mc_starts",
-           "riga_replay_ledger", "riga_query_worst_percentile", "riga_path_cube")
+           "riga_replay_ledger", "riga_query_worst_percentile", "riga_path_cube",
+           "riga_worst_of", "riga_resident_worst_of_create", "riga_resident_worst_of_reduce",
+           "riga_resident_worst_of_download", "riga_resident_worst_of_synchronize",
+           "riga_resident_worst_of_destroy", "riga_lv_terminal_spots", "riga_resident_lv_create",
+           "riga_resident_lv_run", "riga_resident_lv_download", "riga_resident_lv_synchronize",
+           "riga_resident_lv_destroy")
 
 _lib = None
 _lock = threading.Lock()
@@ -247,6 +252,26 @@ def lib():
                                                       C.POINTER(PathRecord), C.POINTER(CouponRow),
                                                       C.POINTER(Error)]
             _F32 = C.POINTER(C.c_float)
+            _I32 = C.POINTER(C.c_int32)
+            _VP = C.c_void_p
+            L.riga_worst_of.argtypes = [_VP, _F32, _F32, C.c_uint64, _F32, C.POINTER(Error)]
+            L.riga_resident_worst_of_create.argtypes = [_VP, _F32, _F32, C.c_uint64, C.POINTER(_VP),
+                                                        C.POINTER(Error)]
+            for name in ("riga_resident_worst_of_reduce", "riga_resident_worst_of_synchronize",
+                         "riga_resident_lv_synchronize"):
+                getattr(L, name).argtypes = [_VP, C.POINTER(Error)]
+            L.riga_resident_worst_of_download.argtypes = [_VP, _F32, C.POINTER(Error)]
+            L.riga_resident_worst_of_destroy.argtypes = [_VP]
+            L.riga_resident_worst_of_destroy.restype = None
+            L.riga_lv_terminal_spots.argtypes = [_VP, _F32, _F32, _F32, C.c_int32, _F32, _F32, C.c_float,
+                                                 C.c_int64, C.c_int64, C.c_float, C.c_float, _F32, _F32, _I32,
+                                                 C.POINTER(Error)]
+            L.riga_resident_lv_create.argtypes = [_VP, _F32, _F32, _F32, C.c_int32, C.c_int64, C.c_int64,
+                                                  C.c_float, C.POINTER(_VP), C.POINTER(Error)]
+            L.riga_resident_lv_run.argtypes = [_VP, _F32, _F32, C.c_float, C.c_float, _F32, C.POINTER(Error)]
+            L.riga_resident_lv_download.argtypes = [_VP, _F32, _I32, C.POINTER(Error)]
+            L.riga_resident_lv_destroy.argtypes = [_VP]
+            L.riga_resident_lv_destroy.restype = None
             L.riga_path_cube.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int32, C.c_int32, _D, _D,
                                          _D, C.c_double, _D, C.c_double, _F32, _F32, _F32,
                                          C.POINTER(Error)]

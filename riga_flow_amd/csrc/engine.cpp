@@ -1831,6 +1831,260 @@ int riga_path_cube(riga_ctx* ctx, uint64_t seed, uint64_t paths, int32_t times, 
   return 0;
 }
 
+// ---- the reference's device module (gpu.rs:159-570)
+}  // extern "C"
+
+struct riga_resident_worst_of {
+  riga_ctx* ctx = nullptr;
+  DevBuf a, b, out;
+  uint64_t n = 0;
+};
+
+struct riga_resident_lv {
+  riga_ctx* ctx = nullptr;
+  DevBuf normals, rows, ls, s0, out, ko;
+  int paths = 0, steps = 0, gx = 0;
+  float dt = 0.f;
+};
+
+namespace {
+
+int lv_check(const float* normals, const float* rows, const float* ls_axis, int32_t gx, int64_t paths,
+             int64_t steps, riga_error* err) {
+  if (!normals || !rows || !ls_axis || gx < 2 || paths < 0 || steps < 0 || paths > INT32_MAX ||
+      steps > INT32_MAX || (double)paths * (double)steps * 2.0 > 8e9) {
+    set_err(err, "shape_mismatch", "normals [paths][steps][2], rows [2][steps][gx], ls_axis [2][gx] with gx >= 2");
+    return RIGA_EINVAL;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int riga_worst_of(riga_ctx* ctx, const float* a, const float* b, uint64_t n, float* out, riga_error* err) {
+  clear_err(err);
+  if (!ctx) {
+    set_err(err, "gpu_unavailable", "no device context");
+    return RIGA_ENODEV;
+  }
+  if (n == 0) return 0;
+  if (!a || !b || !out) {
+    set_err(err, "shape_mismatch", "worst-of inputs must have equal length");
+    return RIGA_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  DevBuf da, db, dout;
+  HIP_TRY(da.ensure(n * 4), "alloc");
+  HIP_TRY(db.ensure(n * 4), "alloc");
+  HIP_TRY(dout.ensure(n * 4), "alloc");
+  HIP_TRY(hipMemcpyAsync(da.p, a, n * 4, hipMemcpyHostToDevice, ctx->stream), "H2D a");
+  HIP_TRY(hipMemcpyAsync(db.p, b, n * 4, hipMemcpyHostToDevice, ctx->stream), "H2D b");
+  HIP_TRY(launch_worst_of((const float*)da.p, (const float*)db.p, (float*)dout.p, n, ctx->stream), "launch");
+  HIP_TRY(hipMemcpyAsync(out, dout.p, n * 4, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "worst_of");
+  return 0;
+}
+
+int riga_resident_worst_of_create(riga_ctx* ctx, const float* a, const float* b, uint64_t n,
+                                  riga_resident_worst_of** out, riga_error* err) {
+  clear_err(err);
+  if (!ctx || !out) {
+    set_err(err, "gpu_unavailable", "no device context");
+    return RIGA_ENODEV;
+  }
+  *out = nullptr;
+  if (n > 0 && (!a || !b)) {
+    set_err(err, "shape_mismatch", "worst-of inputs must have equal length");
+    return RIGA_EINVAL;
+  }
+  std::unique_ptr<riga_resident_worst_of> h(new riga_resident_worst_of());
+  h->ctx = ctx;
+  h->n = n;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  HIP_TRY(h->a.ensure(std::max<uint64_t>(n, 1) * 4), "alloc");
+  HIP_TRY(h->b.ensure(std::max<uint64_t>(n, 1) * 4), "alloc");
+  HIP_TRY(h->out.ensure(std::max<uint64_t>(n, 1) * 4), "alloc");
+  if (n) {
+    HIP_TRY(hipMemcpyAsync(h->a.p, a, n * 4, hipMemcpyHostToDevice, ctx->stream), "H2D a");
+    HIP_TRY(hipMemcpyAsync(h->b.p, b, n * 4, hipMemcpyHostToDevice, ctx->stream), "H2D b");
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "upload");
+  *out = h.release();
+  return 0;
+}
+
+int riga_resident_worst_of_reduce(riga_resident_worst_of* h, riga_error* err) {
+  clear_err(err);
+  if (!h) {
+    set_err(err, "invalid_argument", "handle is NULL");
+    return RIGA_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(h->ctx->mu);
+  HIP_TRY(hipSetDevice(h->ctx->device), "hipSetDevice");
+  HIP_TRY(launch_worst_of((const float*)h->a.p, (const float*)h->b.p, (float*)h->out.p, h->n, h->ctx->stream),
+          "launch");
+  return 0;
+}
+
+int riga_resident_worst_of_download(riga_resident_worst_of* h, float* out, riga_error* err) {
+  clear_err(err);
+  if (!h || (h->n && !out)) {
+    set_err(err, "invalid_argument", "handle or output is NULL");
+    return RIGA_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(h->ctx->mu);
+  HIP_TRY(hipSetDevice(h->ctx->device), "hipSetDevice");
+  if (h->n) HIP_TRY(hipMemcpyAsync(out, h->out.p, h->n * 4, hipMemcpyDeviceToHost, h->ctx->stream), "D2H");
+  HIP_TRY(hipStreamSynchronize(h->ctx->stream), "download");
+  return 0;
+}
+
+int riga_resident_worst_of_synchronize(riga_resident_worst_of* h, riga_error* err) {
+  clear_err(err);
+  if (!h) {
+    set_err(err, "invalid_argument", "handle is NULL");
+    return RIGA_EINVAL;
+  }
+  HIP_TRY(hipStreamSynchronize(h->ctx->stream), "synchronize");
+  return 0;
+}
+
+void riga_resident_worst_of_destroy(riga_resident_worst_of* h) {
+  if (!h) return;
+  (void)hipStreamSynchronize(h->ctx->stream);
+  delete h;
+}
+
+int riga_lv_terminal_spots(riga_ctx* ctx, const float* normals, const float* rows, const float* ls_axis,
+                           int32_t gx, const float* s0, const float* references, float call_barrier, int64_t paths,
+                           int64_t steps, float dt, float rate, const float* dividends, float* out_terminal,
+                           int32_t* out_ko, riga_error* err) {
+  clear_err(err);
+  if (!ctx) {
+    set_err(err, "gpu_unavailable", "no device context");
+    return RIGA_ENODEV;
+  }
+  int rc = lv_check(normals, rows, ls_axis, gx, paths, steps, err);
+  if (rc) return rc;
+  if (!s0 || !references || !dividends || !out_terminal || !out_ko) {
+    set_err(err, "shape_mismatch", "s0, references and dividends hold two values; outputs are required");
+    return RIGA_EINVAL;
+  }
+  if (paths == 0) return 0;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  DevBuf dn, dr, dl, ds, dout, dko;
+  const size_t nn = (size_t)paths * steps * 2, nr = (size_t)2 * steps * gx;
+  HIP_TRY(dn.ensure(std::max<size_t>(nn, 1) * 4), "alloc");
+  HIP_TRY(dr.ensure(std::max<size_t>(nr, 1) * 4), "alloc");
+  HIP_TRY(dl.ensure((size_t)2 * gx * 4), "alloc");
+  HIP_TRY(ds.ensure(8), "alloc");
+  HIP_TRY(dout.ensure((size_t)paths * 8), "alloc");
+  HIP_TRY(dko.ensure((size_t)paths * 4), "alloc");
+  if (nn) HIP_TRY(hipMemcpyAsync(dn.p, normals, nn * 4, hipMemcpyHostToDevice, ctx->stream), "H2D normals");
+  if (nr) HIP_TRY(hipMemcpyAsync(dr.p, rows, nr * 4, hipMemcpyHostToDevice, ctx->stream), "H2D rows");
+  HIP_TRY(hipMemcpyAsync(dl.p, ls_axis, (size_t)2 * gx * 4, hipMemcpyHostToDevice, ctx->stream), "H2D ls");
+  HIP_TRY(hipMemcpyAsync(ds.p, s0, 8, hipMemcpyHostToDevice, ctx->stream), "H2D s0");
+  LvPropagateArgs a{(const float*)dn.p, (const float*)dr.p, (const float*)dl.p, (const float*)ds.p,
+                    (float*)dout.p, (int*)dko.p, references[0], references[1], call_barrier, (int)paths,
+                    (int)steps, gx, dt, rate, dividends[0], dividends[1]};
+  HIP_TRY(launch_lv_propagate(a, ctx->stream), "lv launch");
+  HIP_TRY(hipMemcpyAsync(out_terminal, dout.p, (size_t)paths * 8, hipMemcpyDeviceToHost, ctx->stream), "D2H out");
+  HIP_TRY(hipMemcpyAsync(out_ko, dko.p, (size_t)paths * 4, hipMemcpyDeviceToHost, ctx->stream), "D2H ko");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "lv_terminal_spots");
+  return 0;
+}
+
+int riga_resident_lv_create(riga_ctx* ctx, const float* normals, const float* rows, const float* ls_axis,
+                            int32_t gx, int64_t paths, int64_t steps, float dt, riga_resident_lv** out,
+                            riga_error* err) {
+  clear_err(err);
+  if (!ctx || !out) {
+    set_err(err, "gpu_unavailable", "no device context");
+    return RIGA_ENODEV;
+  }
+  *out = nullptr;
+  int rc = lv_check(normals, rows, ls_axis, gx, paths, steps, err);
+  if (rc) return rc;
+  std::unique_ptr<riga_resident_lv> h(new riga_resident_lv());
+  h->ctx = ctx;
+  h->paths = (int)paths;
+  h->steps = (int)steps;
+  h->gx = gx;
+  h->dt = dt;
+  const size_t nn = (size_t)paths * steps * 2, nr = (size_t)2 * steps * gx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  HIP_TRY(h->normals.ensure(std::max<size_t>(nn, 1) * 4), "alloc");
+  HIP_TRY(h->rows.ensure(std::max<size_t>(nr, 1) * 4), "alloc");
+  HIP_TRY(h->ls.ensure((size_t)2 * gx * 4), "alloc");
+  HIP_TRY(h->s0.ensure(8), "alloc");
+  HIP_TRY(h->out.ensure(std::max<size_t>((size_t)paths, 1) * 8), "alloc");
+  HIP_TRY(h->ko.ensure(std::max<size_t>((size_t)paths, 1) * 4), "alloc");
+  if (nn) HIP_TRY(hipMemcpyAsync(h->normals.p, normals, nn * 4, hipMemcpyHostToDevice, ctx->stream), "H2D normals");
+  if (nr) HIP_TRY(hipMemcpyAsync(h->rows.p, rows, nr * 4, hipMemcpyHostToDevice, ctx->stream), "H2D rows");
+  HIP_TRY(hipMemcpyAsync(h->ls.p, ls_axis, (size_t)2 * gx * 4, hipMemcpyHostToDevice, ctx->stream), "H2D ls");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "upload");
+  *out = h.release();
+  return 0;
+}
+
+int riga_resident_lv_run(riga_resident_lv* h, const float* s0, const float* references, float call_barrier,
+                         float rate, const float* dividends, riga_error* err) {
+  clear_err(err);
+  if (!h || !s0 || !references || !dividends) {
+    set_err(err, "invalid_argument", "handle, s0, references and dividends are required");
+    return RIGA_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(h->ctx->mu);
+  HIP_TRY(hipSetDevice(h->ctx->device), "hipSetDevice");
+  // ordered on the stream after the previous run's kernel (which reads s0);
+  // a pageable source is staged before the call returns
+  HIP_TRY(hipMemcpyAsync(h->s0.p, s0, 8, hipMemcpyHostToDevice, h->ctx->stream), "H2D s0");
+  LvPropagateArgs a{(const float*)h->normals.p, (const float*)h->rows.p, (const float*)h->ls.p,
+                    (const float*)h->s0.p, (float*)h->out.p, (int*)h->ko.p, references[0], references[1],
+                    call_barrier, h->paths, h->steps, h->gx, h->dt, rate, dividends[0], dividends[1]};
+  HIP_TRY(launch_lv_propagate(a, h->ctx->stream), "lv launch");
+  return 0;
+}
+
+int riga_resident_lv_download(riga_resident_lv* h, float* out_terminal, int32_t* out_ko, riga_error* err) {
+  clear_err(err);
+  if (!h || (h->paths && (!out_terminal || !out_ko))) {
+    set_err(err, "invalid_argument", "handle or outputs are NULL");
+    return RIGA_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(h->ctx->mu);
+  HIP_TRY(hipSetDevice(h->ctx->device), "hipSetDevice");
+  if (h->paths) {
+    HIP_TRY(hipMemcpyAsync(out_terminal, h->out.p, (size_t)h->paths * 8, hipMemcpyDeviceToHost, h->ctx->stream),
+            "D2H out");
+    HIP_TRY(hipMemcpyAsync(out_ko, h->ko.p, (size_t)h->paths * 4, hipMemcpyDeviceToHost, h->ctx->stream), "D2H ko");
+  }
+  HIP_TRY(hipStreamSynchronize(h->ctx->stream), "download");
+  return 0;
+}
+
+int riga_resident_lv_synchronize(riga_resident_lv* h, riga_error* err) {
+  clear_err(err);
+  if (!h) {
+    set_err(err, "invalid_argument", "handle is NULL");
+    return RIGA_EINVAL;
+  }
+  HIP_TRY(hipStreamSynchronize(h->ctx->stream), "synchronize");
+  return 0;
+}
+
+void riga_resident_lv_destroy(riga_resident_lv* h) {
+  if (!h) return;
+  (void)hipStreamSynchronize(h->ctx->stream);
+  delete h;
+}
+
 int riga_debug_mc_starts(riga_ctx* ctx, uint64_t seed, uint64_t draws_per_path, uint64_t n_paths,
                          uint64_t* out_starts, riga_error* err) {
   clear_err(err);

@@ -80,6 +80,21 @@ struct CubeArgs {
 };
 hipError_t launch_cube(const CubeArgs& a, int factors, hipStream_t stream);
 
+// The reference device module's kernels (gpu.rs:27-75), FP32.
+struct LvPropagateArgs {
+  const float* normals;  // [paths][steps][2]
+  const float* rows;     // [2][steps][gx]
+  const float* ls_axis;  // [2][gx]
+  const float* s0;       // [2]
+  float* out;            // [paths][2] terminal spots
+  int* ko_out;           // [paths] first KO step or -1
+  float ref0, ref1, ko_bar;
+  int paths, steps, gx;
+  float dt, rate, q0, q1;
+};
+hipError_t launch_worst_of(const float* a, const float* b, float* o, uint64_t n, hipStream_t stream);
+hipError_t launch_lv_propagate(const LvPropagateArgs& a, hipStream_t stream);
+
 // Owen prefix table: out[dim][v] = scrambled top 12 bits of v << 20, dims [0, dims).
 hipError_t launch_owen_table(uint64_t seed, int dims, uint16_t* out, hipStream_t stream);
 

@@ -38,6 +38,70 @@ hipError_t launch_phoenix_f32(const PhoenixLaunch& l, hipStream_t stream) {
   }
 }
 
+// The reference's device module (crates/riga-engine/src/gpu.rs:27-75), for callers
+// of its worst_of / lv_terminal_spots API.  Elementwise worst-of of two
+// performance vectors, with the reference's `a < b ? a : b` (a NaN in a gives b).
+__global__ __launch_bounds__(256) void worst_of_kernel(const float* __restrict__ a, const float* __restrict__ b,
+                                                       float* __restrict__ o, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) o[i] = a[i] < b[i] ? a[i] : b[i];
+}
+
+// Two-factor FP32 local-vol propagation from given shocks, one lane per path:
+// normals [path][step][2], rows [factor][step][gx] (local vol per log-spot knot),
+// ls_axis [factor][gx] (evenly spaced knots).  Per factor and step: the knot
+// position (log S - x0) / (x_last - x0) * (gx - 1), clamped to the grid, a linear
+// blend of the row, then log S += (r - q - sigma^2/2) dt + sigma sqrt(dt) z; the
+// first step with both names at or above the call barrier (S / reference), or -1.
+__global__ __launch_bounds__(256) void lv_propagate_kernel(LvPropagateArgs a) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= a.paths) return;
+  const int gx = a.gx;
+  float x[2], lo[2], span[2];
+#pragma unroll
+  for (int f = 0; f < 2; ++f) {
+    x[f] = logf(a.s0[f]);
+    lo[f] = a.ls_axis[f * gx];
+    span[f] = 1.0f / (a.ls_axis[f * gx + gx - 1] - lo[f]);
+  }
+  const float sqdt = sqrtf(a.dt);
+  const float top = (float)(gx - 1);
+  const float q[2] = {a.q0, a.q1};
+  const float ref[2] = {a.ref0, a.ref1};
+  int ko = -1;
+  for (int s = 0; s < a.steps; ++s) {
+    const float* z = a.normals + ((size_t)p * a.steps + s) * 2;
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      float k = (x[f] - lo[f]) * span[f] * top;
+      k = k < 0.0f ? 0.0f : (k > top ? top : k);
+      int j = (int)k;
+      j = j > gx - 2 ? gx - 2 : j;
+      const float w = k - (float)j;
+      const float* row = a.rows + ((size_t)f * a.steps + s) * gx;
+      const float sig = row[j] * (1.0f - w) + row[j + 1] * w;
+      x[f] += (a.rate - q[f] - 0.5f * sig * sig) * a.dt + sig * sqdt * z[f];
+    }
+    if (ko < 0 && expf(x[0]) / ref[0] >= a.ko_bar && expf(x[1]) / ref[1] >= a.ko_bar) ko = s;
+  }
+  a.out[2 * p] = expf(x[0]);
+  a.out[2 * p + 1] = expf(x[1]);
+  a.ko_out[p] = ko;
+}
+
+hipError_t launch_worst_of(const float* a, const float* b, float* o, uint64_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(worst_of_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a, b, o, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_lv_propagate(const LvPropagateArgs& a, hipStream_t stream) {
+  if (a.paths <= 0) return hipSuccess;
+  if (a.gx < 2 || a.steps < 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(lv_propagate_kernel, dim3((unsigned)((a.paths + 255) / 256)), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_normal_table_f32(const uint32_t* dirs, const uint16_t* owen_tab, uint64_t seed,
                                    int dims, int n_paths, int nbits, float* out, hipStream_t stream) {
   dim3 grid((n_paths + 255) / 256, dims);
