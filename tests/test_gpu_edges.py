@@ -102,3 +102,35 @@ def test_rejected_requests_name_the_reference_codes():
     with pytest.raises(RigaError) as e:
         api.price_put_leg(bad, "fp64")
     assert e.value.code == "invalid_market"
+
+
+def test_concurrent_callers_get_their_own_results():
+    # the C ABI is called from several host threads at once (the reference's MCP
+    # handlers run on a multi-threaded runtime): each call's result equals the
+    # same call made alone
+    import threading
+    reqs = []
+    for i in range(6):
+        r = _flat(2048 + 512 * i, steps=63, years=0.25)
+        if i % 2:
+            r.config.variance_reduction = W.VarianceReduction("none")
+        reqs.append(r)
+    alone = [api.price_put_leg(r, "fp64") for r in reqs]
+    got = [None] * len(reqs)
+    errors = []
+
+    def worker(i):
+        try:
+            for _ in range(3):
+                got[i] = api.price_put_leg(reqs[i], "fp64")
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(len(reqs))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors and all(not t.is_alive() for t in threads)
+    for a, b in zip(got, alone):
+        assert a.pv == b.pv and a.stats[0].output_count == b.stats[0].output_count
