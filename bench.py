@@ -268,6 +268,7 @@ def main():
     fp64.close()
     log("fp32 error check done")
     out["crn_variance_reduction"] = crn_vrf_leg(args.quick)
+    out["reference_published"] = published_leg()
     if not args.no_mc:
         out["mc_lane"] = mc_leg(args)
     if not args.no_book:
@@ -328,6 +329,48 @@ def mc_leg(args) -> dict:
                                   "ko_count": g.stats[0].output_count,
                                   "golden_ko_count": golden["stats"][0]["output_count"],
                                   "seconds": gsec}}
+
+
+def published_leg() -> dict:
+    """The reference's published experiment figures (docs/plan/25-implementation-evidence.md
+    :88,:107,:110), recomputed on the device from the same inputs: the CRN variance-reduction
+    factor (riga-risk/src/lib.rs:835-845), the paired delta standard errors (lib.rs:919-939)
+    and the QMC put-variance ratios (riga-engine/src/lib.rs:653-688)."""
+    from riga_flow_amd import risk
+    from riga_flow_amd.fixture import load_termsheet1_fixture
+    fx = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "termsheet1.json")
+
+    def small():
+        r = load_termsheet1_fixture(fx).request
+        r.config.paths, r.config.path_block_size, r.config.seed = 512, 128, 81_281
+        return r
+
+    t0 = time.perf_counter()
+    vrf = risk.crn_variance_reduction_factor(small(), replicates=40)
+    se16 = risk.crn_delta_convergence(small(), replicates=16).standard_error
+    se64 = risk.crn_delta_convergence(small(), replicates=64).standard_error
+    base = load_termsheet1_fixture(fx).request
+    dim = base.trade.steps * 2
+    var = {}
+    for kind in ("none", "halton", "sobol"):
+        reqs = []
+        for seed in range(1, 13):
+            r = base.clone()
+            r.config.paths, r.config.path_block_size, r.config.seed = 4096, 4096, seed
+            r.config.variance_reduction = W.VarianceReduction(kind, 0 if kind == "none" else dim)
+            reqs.append(r)
+        s = [x.legs.down_and_in_put for x in api.price_batch(reqs, "fp64")]
+        m = sum(s) / len(s)
+        var[kind] = sum((v - m) ** 2 for v in s) / (len(s) - 1)
+    secs = time.perf_counter() - t0
+    log(f"published figures: vrf {vrf:.2f}, se {se16:.2e}/{se64:.2e}, qmc {var['none'] / var['halton']:.2f}/"
+        f"{var['none'] / var['sobol']:.2f} ({secs:.2f} s)")
+    return {"crn_variance_reduction_factor": {"device": vrf, "published": 140.18},
+            "delta_standard_error_16": {"device": se16, "published": 2.6e-5},
+            "delta_standard_error_64": {"device": se64, "published": 1.1e-5},
+            "put_variance_mc_over_halton": {"device": var["none"] / var["halton"], "published": 1.05},
+            "put_variance_mc_over_sobol": {"device": var["none"] / var["sobol"], "published": 1.04},
+            "tier": "fp64", "seconds": secs}
 
 
 def crn_vrf_leg(quick: bool) -> dict:
