@@ -27,10 +27,13 @@ from riga_flow_amd import api, workloads as W  # noqa: E402
 # ---------------------------------------------------------------- roofline model
 # Algorithmic work of the code path each tier runs, in 32-bit lane operations
 # (an FP64 operation counts one: MI355X issues FP64 at the full vector rate).
-# Per generated coordinate (both tiers, integers identical):
+# Per generated coordinate:
 OWEN_ROUND = 27     # hash input xor 2, xorshift 4, 64-bit multiply 5, xorshift 4,
                     # top word of the 2nd multiply 4, bit logic 3, prefix 1, prefix*C3 4
-OWEN_ROUNDS = 16    # the top 16 of the 32 levels come from the per-dimension prefix table
+# hashed levels: the top 16 of the 32 come from the per-dimension prefix table; the FP32
+# tier hashes levels 15..8 (a float's 24 bits) and levels 7..0 only for coordinates
+# within 2^-10 of 0 or 1 (a fraction 2^-9 of them)
+OWEN_ROUNDS = {"fp64": 16, "fp32": 8 + 8 * 2.0 ** -9}
 OWEN_TABLE = 3      # prefix table address and load
 PROBIT = 22         # rational (12) + divide (8) + tails (5% of log + sqrt + rational)
 UNIFORM = 4         # int->float, +0.5, scale, clamp
@@ -45,7 +48,7 @@ HBM_PEAK = 8.0e12
 
 def ops_per_path_step(F: int, S: int, nbits: int, local_vol: bool, tier: str) -> float:
     """Lane operations of one path advanced one step for all S scenarios."""
-    coord = OWEN_ROUNDS * OWEN_ROUND + OWEN_TABLE + 2 * nbits + UNIFORM + PROBIT
+    coord = OWEN_ROUNDS[tier] * OWEN_ROUND + OWEN_TABLE + 2 * nbits + UNIFORM + PROBIT
     node = F * coord + F * (F + 1) + 3 * F + 2 * F  # coordinates, L z, bridge node, shock
     scen = F * ((LV_LOOKUP[tier] if local_vol else 0) + EULER[tier]) + 9 * F + 5  # + worst/KO/coupon
     return node + S * scen

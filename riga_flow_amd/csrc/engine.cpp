@@ -552,7 +552,7 @@ void fill_result(const Partial& fr, uint64_t paths, riga_result* out) {
 // ------------------------------------------------------------ plans
 struct LaunchPlan {
   int F = 0, smax = 1, blocks = 0;
-  bool mc = false;       // Monte Carlo lane items
+  int mode = kLadderDirect;  // how the launch's items get their normals (kernel template mode)
   int threads = kBlock;  // lanes per workgroup
   size_t lds = 0;
   size_t block_off = 0;     // first (item, chunk) pair of this launch in the block table
@@ -562,7 +562,7 @@ struct LaunchPlan {
 // Launch-shape signature of a batch: precision and, per (factors, scenario
 // bucket) launch, the item count and the log2 sizes of its paths and steps.
 // The autotuner's choices are kept per signature.
-using PartKey = std::tuple<int, int, int>;  // (factors, scenario bucket, Monte Carlo lane)
+using PartKey = std::tuple<int, int, int>;  // (factors, scenario bucket, ladder mode)
 
 std::string shape_signature(int precision, const std::map<PartKey, std::vector<int>>& parts,
                             const std::vector<Item>& items) {
@@ -575,7 +575,9 @@ std::string shape_signature(int precision, const std::map<PartKey, std::vector<i
     }
     steps /= (double)kv.second.size();
     char buf[96];
-    snprintf(buf, sizeof buf, "|%sF%d.S%d.n%zu.p%d.t%d", std::get<2>(kv.first) ? "mc." : "",
+    const int mode = std::get<2>(kv.first);
+    snprintf(buf, sizeof buf, "|%sF%d.S%d.n%zu.p%d.t%d",
+             mode == kLadderMonteCarlo ? "mc." : mode == kLadderTable ? "tab." : "",
              std::get<0>(kv.first), std::get<1>(kv.first), kv.second.size(),
              (int)std::ceil(std::log2(paths + 1.0)), (int)std::lround(std::log2(steps + 1.0)));
     k += buf;
@@ -778,7 +780,7 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
     PhoenixLaunch L;
     L.F = lp.F;
     L.smax = lp.smax;
-    L.mc = lp.mc;
+    L.mode = lp.mode;
     L.blocks = lp.blocks;
     L.lds = lp.lds;
     L.threads = lp.threads;
@@ -849,6 +851,10 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       set_err(err, e.code, "request " + std::to_string(i) + ": " + e.message);
       return RIGA_EINVAL;
     }
+    if (reqs[i].steps >= (int64_t)1 << 20) {  // KO steps are packed in 20 bits on the device
+      set_err(err, "invalid_schedule", "request " + std::to_string(i) + ": the device takes at most 1,048,575 steps");
+      return RIGA_EINVAL;
+    }
   }
   // CRN grouping (records keep one scenario per item so path order is per request)
   std::vector<Item>& items = pl->items;
@@ -910,11 +916,45 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     return RIGA_EINVAL;
   }
   for (Item& it : items) pl->path_steps += (double)it.d.S * (double)it.d.paths * (double)it.d.steps;
-  // launches: one per (factors, scenario bucket), blocks longest-first
+  // shared normal tables: same-seed Sobol trades without wrapped direction rows
+  // read their normals from one [dim][path] table when they need many more
+  // normals than it holds (an uneven book), instead of hashing them per trade.
+  // Decided here, since the items reading a table launch as their own kernel mode.
+  const int zmode = cfg ? cfg->normal_table : 0;
+  std::map<uint64_t, std::vector<int>> table_items;  // seed -> items reading its table
+  if (zmode >= 0 && !records) {
+    std::map<uint64_t, std::vector<int>> by_seed;
+    for (size_t i = 0; i < items.size(); ++i) {
+      const ItemDesc& d = items[i].d;
+      if (!(d.flags & (kHalton | kMonteCarlo)) && d.ladder_dims >= d.F * d.steps && d.path_begin == 0)
+        by_seed[d.seed].push_back((int)i);
+    }
+    const size_t real = precision == RIGA_FP64 ? 8 : 4;
+    for (auto& kv : by_seed) {
+      int dims = 0;
+      uint64_t paths = 0;
+      double demand = 0.0;
+      for (int i : kv.second) {
+        const ItemDesc& d = items[i].d;
+        dims = std::max(dims, d.F * d.steps);
+        paths = std::max<uint64_t>(paths, d.paths);
+        demand += (double)d.paths * d.F * d.steps;
+      }
+      const double size = (double)dims * (double)paths;
+      if (size * real > 32e9 || (zmode == 0 && demand < 4.0 * size)) continue;
+      table_items[kv.first] = kv.second;
+    }
+  }
+  std::vector<int> item_mode(items.size(), kLadderDirect);
+  for (size_t i = 0; i < items.size(); ++i)
+    if (items[i].d.flags & kMonteCarlo) item_mode[i] = kLadderMonteCarlo;
+  for (auto& kv : table_items)
+    for (int i : kv.second) item_mode[i] = kLadderTable;
+  // launches: one per (factors, scenario bucket, ladder mode), blocks longest-first
   const int real_size = precision == RIGA_FP64 ? 8 : 4;
   std::map<PartKey, std::vector<int>> parts;
   for (size_t i = 0; i < items.size(); ++i)
-    parts[PartKey(items[i].F, items[i].smax, (items[i].d.flags & kMonteCarlo) ? 1 : 0)].push_back((int)i);
+    parts[PartKey(items[i].F, items[i].smax, item_mode[i])].push_back((int)i);
   pl->signature = shape_signature(precision, parts, items);
   if (ppt == 0 || waves == 0) {  // the autotuned shape for this signature, else 1 path x 4 waves
     int tp = 1, tw = 4;
@@ -964,7 +1004,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     LaunchPlan lp;
     lp.F = std::get<0>(kv.first);
     lp.smax = std::get<1>(kv.first);
-    lp.mc = std::get<2>(kv.first) != 0;
+    lp.mode = std::get<2>(kv.first);
     lp.items = idx;
     int stack_levels = 0, lv_words = 0;
     for (int i : idx) {
@@ -974,7 +1014,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       lv_words = std::max(lv_words, (int)items[i].surfaces.size() * (real_size == 4 ? 4 : 3) * max_nx);
     }
     lp.threads = threads;
-    if (lp.mc)  // stream ring and ziggurat tables instead of the bridge stack
+    if (lp.mode == kLadderMonteCarlo)  // stream ring and ziggurat tables instead of the bridge stack
       lp.lds = (size_t)mc_ring_slots(lp.smax) * threads * 8 + 2 * kZigLen * 8 + (size_t)lv_words * real_size;
     else
       lp.lds = (size_t)stack_levels * lp.F * threads * real_size + (size_t)lv_words * real_size;
@@ -989,7 +1029,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     for (int i : idx) {
       Item& it = items[i];
       ItemDesc& d = it.d;
-      d.stack_levels = lp.mc ? 0 : stack_levels;
+      d.stack_levels = lp.mode == kLadderMonteCarlo ? 0 : stack_levels;
       d.paths_per_block = threads * ppt;
       d.n_blocks = (int32_t)((d.paths + d.paths_per_block - 1) / d.paths_per_block);
       blocks += d.n_blocks;
@@ -1080,31 +1120,17 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     }
   for (Item& it : items)
     if (it.d.flags & kOwenTable) it.d.owen_tab = owen_ptr[it.d.seed];
-  // shared normal tables: same-seed Sobol trades without wrapped direction rows
-  // read their normals from one [dim][path] table when they need many more
-  // normals than it holds (an uneven book), instead of hashing them per trade
-  const int zmode = cfg ? cfg->normal_table : 0;
-  if (zmode >= 0 && !records) {
-    std::map<uint64_t, std::vector<int>> by_seed;
-    for (size_t i = 0; i < items.size(); ++i) {
-      const ItemDesc& d = items[i].d;
-      if (!(d.flags & (kHalton | kMonteCarlo)) && d.ladder_dims >= d.F * d.steps && d.path_begin == 0)
-        by_seed[d.seed].push_back((int)i);
-    }
+  // the shared normal tables decided above
+  {
     const size_t real = precision == RIGA_FP64 ? 8 : 4;
-    for (auto& kv : by_seed) {
+    for (auto& kv : table_items) {
       int dims = 0;
       uint64_t paths = 0;
-      double demand = 0.0;
       for (int i : kv.second) {
-        const ItemDesc& d = items[i].d;
-        dims = std::max(dims, d.F * d.steps);
-        paths = std::max<uint64_t>(paths, d.paths);
-        demand += (double)d.paths * d.F * d.steps;
+        dims = std::max(dims, items[i].d.F * items[i].d.steps);
+        paths = std::max<uint64_t>(paths, items[i].d.paths);
       }
-      const double size = (double)dims * (double)paths;
-      const double bytes = size * real;
-      if (bytes > 32e9 || (zmode == 0 && demand < 4.0 * size)) continue;
+      const double bytes = (double)dims * (double)paths * real;
       std::unique_ptr<riga_plan::NormalTable> zt(new riga_plan::NormalTable());
       zt->seed = kv.first;
       zt->dims = dims;

@@ -13,7 +13,7 @@ constexpr int kScenarioBuckets[3] = {1, 4, 10};
 struct PhoenixLaunch {
   int F;       // factors (1..kMaxFactors)
   int smax;    // scenario bucket (1, 4 or 10)
-  bool mc;     // Monte Carlo lane items (ChaCha8 + ziggurat) instead of QMC
+  int mode;    // kLadderDirect, kLadderTable or kLadderMonteCarlo
   int blocks;  // grid size
   int threads; // lanes per workgroup: 64, 128 or 256
   size_t lds;  // dynamic LDS bytes (bridge stack and local-vol rows)

@@ -8,12 +8,22 @@ namespace riga {
 template <int F, int SMAX>
 static hipError_t launch_one(const PhoenixLaunch& l, hipStream_t stream) {
   if (l.threads < 64 || l.threads > kBlock || l.threads % 64 != 0) return hipErrorInvalidValue;
-  if (l.mc)
-    hipLaunchKernelGGL((phoenix_kernel<float, F, SMAX, true>), dim3(l.blocks), dim3(l.threads), l.lds,
-                       stream, l.items, l.block_item, l.arena, l.partials, l.records);
-  else
-    hipLaunchKernelGGL((phoenix_kernel<float, F, SMAX, false>), dim3(l.blocks), dim3(l.threads), l.lds,
-                       stream, l.items, l.block_item, l.arena, l.partials, l.records);
+  switch (l.mode) {
+    case kLadderDirect:
+      hipLaunchKernelGGL((phoenix_kernel<float, F, SMAX, kLadderDirect>), dim3(l.blocks), dim3(l.threads), l.lds,
+                         stream, l.items, l.block_item, l.arena, l.partials, l.records);
+      break;
+    case kLadderTable:
+      hipLaunchKernelGGL((phoenix_kernel<float, F, SMAX, kLadderTable>), dim3(l.blocks), dim3(l.threads), l.lds,
+                         stream, l.items, l.block_item, l.arena, l.partials, l.records);
+      break;
+    case kLadderMonteCarlo:
+      hipLaunchKernelGGL((phoenix_kernel<float, F, SMAX, kLadderMonteCarlo>), dim3(l.blocks), dim3(l.threads),
+                         l.lds, stream, l.items, l.block_item, l.arena, l.partials, l.records);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -86,6 +86,53 @@ __device__ __forceinline__ uint32_t owen_scramble_below(uint32_t x, uint64_t key
   return prefix;
 }
 
+// Levels kHi..kLo of the scramble, continuing from (prefix, P).
+template <int kHi, int kLo>
+__device__ __forceinline__ void owen_levels(uint32_t x, uint64_t key, uint32_t& prefix, uint64_t& P) {
+#pragma unroll
+  for (int bit = kHi; bit >= kLo; --bit) owen_level(x, key, bit, prefix, P);
+}
+
+// The FP32 tier's scramble.  A float carries 24 significant bits, so the tier
+// hashes the top 24 levels and takes the middle of the remaining 8-bit cell
+// (x = p24 << 8 | 0x80, within 2^-25 of the 32-level uniform).  Where the
+// inverse normal is steep -- u < 2^-10 or u > 1 - 2^-10, i.e. the top 10 bits
+// of p24 all 0 or all 1 -- the last 8 levels are hashed too, so those
+// coordinates are the full 32-level scramble.  Elsewhere z moves by at most
+// 2^-25 / phi(z(2^-10)) = 9.1e-6.
+constexpr uint32_t kF32TailCells = 1u << 14;  // 24-bit cells below u = 2^-10
+
+// Top 24 levels; `top` is the prefix-table entry for x when `has_table`.
+__device__ __forceinline__ uint32_t owen_top24(uint32_t x, uint64_t key, bool has_table, uint32_t top) {
+  if (has_table) {
+    uint32_t prefix = top;
+    uint64_t P = (uint64_t)top * kOwenC3;
+    owen_levels<31 - kOwenTableBits, 8>(x, key, prefix, P);
+    return prefix;
+  }
+  uint32_t prefix = 0;
+  uint64_t P = 0;
+#pragma unroll 4
+  for (int bit = 31; bit >= 8; --bit) owen_level(x, key, bit, prefix, P);
+  return prefix;
+}
+
+__device__ __forceinline__ bool owen_f32_tail(uint32_t p24) {
+  return p24 < kF32TailCells || p24 >= (1u << 24) - kF32TailCells;
+}
+
+__device__ __forceinline__ uint32_t owen_low8(uint32_t x, uint64_t key, uint32_t p24) {
+  uint32_t prefix = p24;
+  uint64_t P = (uint64_t)p24 * kOwenC3;
+  owen_levels<7, 0>(x, key, prefix, P);
+  return prefix;
+}
+
+__device__ __forceinline__ uint32_t owen_f32(uint32_t x, uint64_t key, const uint16_t* __restrict__ tab_row) {
+  const uint32_t p = owen_top24(x, key, tab_row != nullptr, tab_row ? tab_row[x >> (32 - kOwenTableBits)] : 0u);
+  return owen_f32_tail(p) ? owen_low8(x, key, p) : ((p << 8) | 0x80u);
+}
+
 // Unscrambled Sobol coordinate (qmc.rs:269-279): XOR of the direction numbers
 // selected by the gray code of n.  `row` is wave-uniform (32 words, read as
 // scalars); each bit is a sign-extended mask and one three-input logic op, with
@@ -257,8 +304,6 @@ __device__ __forceinline__ float sd_of(const BridgeNode& b, float) { return b.sd
 __device__ __forceinline__ double root_sd_of(const ItemDesc& it, double) { return it.root_sd; }
 __device__ __forceinline__ float root_sd_of(const ItemDesc& it, float) { return it.root_sd32; }
 
-// Correlated normals of one bridge node: dims node*F + f (note.rs:89-98 with F
-// factors), y = L z with the reference's sequential row sums (correlation.rs:196-206).
 // Normals of one node from the batch's shared [dim][path] table (same ladder,
 // same point), and the correlation y = L z with the reference's sequential row
 // sums (correlation.rs:196-206).
@@ -280,53 +325,108 @@ __device__ __forceinline__ void correlate(const ItemDesc& it, const Real (&z)[F]
   }
 }
 
+// The Sobol ladder of one bridge node in two halves, so a lane can issue the
+// next node's loads a step ahead: the fetch forms the unscrambled coordinates
+// (dims node*F + f) and gathers their Owen prefix-table entries; the finish
+// hashes the remaining levels and applies the inverse normal.
+template <int F>
+struct SobolFetch {
+  uint32_t x[F];
+  uint32_t top[F];  // prefix-table entries (unused without a table)
+};
+
+template <int F>
+__device__ __forceinline__ void sobol_fetch(const ItemDesc& it, const unsigned char* arena, uint32_t gray, int node,
+                                            SobolFetch<F>& sf) {
+  const bool tab = (it.flags & kOwenTable) != 0;
+#pragma unroll
+  for (int f = 0; f < F; ++f) {
+    const uint32_t dim = (uint32_t)node * F + f;
+    const uint32_t d = dim % (uint32_t)it.ladder_dims;
+    const uint32_t* row = at<uint32_t>(arena, it.off_dirs) + (size_t)d * 32;
+    sf.x[f] = sobol_x(gray, row, it.nbits);
+    sf.top[f] = tab ? (uint32_t)it.owen_tab[((size_t)dim << kOwenTableBits) + (sf.x[f] >> (32 - kOwenTableBits))] : 0u;
+  }
+}
+
+template <typename Real, int F, bool kUnrollOwen>
+__device__ __forceinline__ void sobol_finish(const ItemDesc& it, int node, const SobolFetch<F>& sf, Real (&z)[F]) {
+  const bool tab = (it.flags & kOwenTable) != 0;
+  if constexpr (sizeof(Real) == 4) {
+    // FP32 tier: 24 hashed levels per coordinate; the few lanes with a
+    // coordinate in a tail then finish its last 8 levels, one coordinate per
+    // lane per pass, while any lane of the wave has one left.
+    uint32_t xs[F];
+    uint64_t keys[F];
+    uint32_t pend = 0;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      keys[f] = it.seed ^ ((uint64_t)((uint32_t)node * F + f) * 0x9E3779B97F4A7C15ull);
+      const uint32_t p = owen_top24(sf.x[f], keys[f], tab, sf.top[f]);
+      xs[f] = (p << 8) | 0x80u;
+      pend |= (uint32_t)owen_f32_tail(p) << f;
+    }
+    while (__ballot(pend != 0u) != 0ull) {
+      if (pend) {
+        const int g = __builtin_ctz(pend);
+        pend &= pend - 1u;
+        uint32_t xg = sf.x[0], pg = xs[0];
+        uint64_t kg = keys[0];
+#pragma unroll
+        for (int f = 1; f < F; ++f)
+          if (g == f) {
+            xg = sf.x[f];
+            pg = xs[f];
+            kg = keys[f];
+          }
+        const uint32_t full = owen_low8(xg, kg, pg >> 8);
+#pragma unroll
+        for (int f = 0; f < F; ++f)
+          if (g == f) xs[f] = full;
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < F; ++f) z[f] = probit_f32(((double)xs[f] + 0.5) / 4294967296.0);
+  } else {
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const uint64_t key = it.seed ^ ((uint64_t)((uint32_t)node * F + f) * 0x9E3779B97F4A7C15ull);
+      // full unroll where registers allow (the FP64 10-scenario lane keeps the loop)
+      const uint32_t xs = tab ? owen_scramble_below<kUnrollOwen>(sf.x[f], key, sf.top[f]) : owen_scramble(sf.x[f], key);
+      z[f] = probit_f64(((double)xs + 0.5) / 4294967296.0);
+    }
+  }
+}
+
+// Correlated normals of one bridge node: dims node*F + f (note.rs:89-98 with F
+// factors) from the batch's ladder -- Sobol with Owen scrambling, Halton, or the
+// shared [dim][path] table -- and y = L z with the reference's sequential row
+// sums (correlation.rs:196-206).
 template <typename Real, int F, bool kUnrollOwen = true>
 __device__ __forceinline__ void node_normals(const ItemDesc& it, const unsigned char* arena,
                                              uint32_t n, uint32_t gray, int node, Real (&y)[F]) {
   Real z[F];
   if (it.flags & kNormalTable) {
     table_normals<Real, F>(it, node, n, z);
-    correlate<Real, F>(it, z, y);
-    return;
-  }
-  const bool halton = (it.flags & kHalton) != 0;
+  } else if (it.flags & kHalton) {
 #pragma unroll
-  for (int f = 0; f < F; ++f) {
-    const uint32_t dim = (uint32_t)node * F + f;
-    const uint32_t d = dim % (uint32_t)it.ladder_dims;
-    double u;
-    if (!halton) {
-      const uint32_t* row = at<uint32_t>(arena, it.off_dirs) + (size_t)d * 32;
-      const uint32_t x = sobol_x(gray, row, it.nbits);
-      const uint64_t key = it.seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
-      uint32_t xs;
-      if (it.flags & kOwenTable) {
-        const uint16_t* tab = it.owen_tab + ((size_t)dim << kOwenTableBits);
-        // full unroll where registers allow (the FP64 10-scenario lane keeps the loop)
-        xs = owen_scramble_below<kUnrollOwen>(x, key, tab[x >> (32 - kOwenTableBits)]);
-      } else {
-        xs = owen_scramble(x, key);
-      }
-      u = ((double)xs + 0.5) / 4294967296.0;
-    } else {
+    for (int f = 0; f < F; ++f) {
+      const uint32_t d = ((uint32_t)node * F + f) % (uint32_t)it.ladder_dims;
       const uint32_t* h = at<uint32_t>(arena, it.off_halton);
-      const uint32_t base = h[d];
-      const uint32_t* perm = h + h[it.ladder_rows + d];
-      u = halton_u(n, base, perm);
+      const double u = halton_u(n, h[d], h + h[it.ladder_rows + d]);
+      if constexpr (sizeof(Real) == 8)
+        z[f] = probit_f64(u);
+      else
+        z[f] = probit_f32(u);
     }
-    if constexpr (sizeof(Real) == 8)
-      z[f] = probit_f64(u);
-    else
-      z[f] = probit_f32(u);
+  } else {
+    SobolFetch<F> sf;
+    sobol_fetch<F>(it, arena, gray, node, sf);
+    sobol_finish<Real, F, kUnrollOwen>(it, node, sf, z);
   }
-#pragma unroll
-  for (int i = 0; i < F; ++i) {
-    Real acc = Real(0);
-#pragma unroll
-    for (int j = 0; j <= i; ++j) acc += chol_of(it, i * kMaxFactors + j, Real(0)) * z[j];
-    y[i] = acc;
-  }
+  correlate<Real, F>(it, z, y);
 }
+
 
 // exp/log per tier: FP64 uses the correctly rounded-to-1ulp device libm.
 __device__ __forceinline__ double tier_exp(double x) { return exp(x); }
@@ -412,20 +512,25 @@ __global__ __launch_bounds__(256) void normal_table_kernel(const uint32_t* __res
   const uint32_t gray = n ^ (n >> 1);
   const uint32_t x = sobol_x(gray, dirs + (size_t)dim * 32, nbits);
   const uint64_t key = seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
-  const uint32_t xs = owen_tab ? owen_scramble_below<true>(x, key, owen_tab[((size_t)dim << kOwenTableBits) +
-                                                                   (x >> (32 - kOwenTableBits))])
-                               : owen_scramble(x, key);
-  const double u = ((double)xs + 0.5) / 4294967296.0;
   Real z;
-  if constexpr (sizeof(Real) == 8)
-    z = probit_f64(u);
-  else
-    z = probit_f32(u);
+  if constexpr (sizeof(Real) == 8) {
+    const uint32_t xs = owen_tab ? owen_scramble_below<true>(x, key, owen_tab[((size_t)dim << kOwenTableBits) +
+                                                                     (x >> (32 - kOwenTableBits))])
+                                 : owen_scramble(x, key);
+    z = probit_f64(((double)xs + 0.5) / 4294967296.0);
+  } else {
+    // the FP32 tier's 24-level scramble, as the inline ladder computes it
+    const uint32_t xs = owen_f32(x, key, owen_tab ? owen_tab + ((size_t)dim << kOwenTableBits) : nullptr);
+    z = probit_f32(((double)xs + 0.5) / 4294967296.0);
+  }
   out[(size_t)dim * n_paths + path] = z;
 }
 
 // ---------------------------------------------------------------- the kernel
-template <typename Real, int F, int SMAX, bool kMC>
+// kMode: how the items of the launch get their normals (kLadderDirect: Sobol
+// or Halton per lane; kLadderTable: the shared normal table; kLadderMonteCarlo:
+// the ChaCha8 stream).  Each mode compiles only its own ladder code.
+template <typename Real, int F, int SMAX, int kMode>
 __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restrict__ items,
                                                           const int32_t* __restrict__ block_item,
                                                           const unsigned char* __restrict__ arena,
@@ -449,6 +554,8 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   const bool trace = (it.flags & kTrace) != 0;
   const bool want_records = (it.flags & kRecords) != 0 && records != nullptr;
   constexpr bool kLogState = sizeof(Real) == 4;  // FP32 tier carries X = ln(S / S0)
+  constexpr bool kMC = kMode == kLadderMonteCarlo;
+  constexpr bool kTab = kMode == kLadderTable;
   constexpr int kSurfWords = kLogState ? 4 : 3;   // LDS words per knot and surface
   const bool lv_uniform = kLogState && (it.flags & kLvUniform) != 0;
   // dynamic LDS: QMC lanes [bridge stack: (depth-1) x F x lanes][local-vol rows:
@@ -484,6 +591,10 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   }
   __syncthreads();
   const Real sqrt_dt_r = (Real)it.sqrt_dt;
+  // -dt/2 for the FP32 step, held in a vector register so the step loop does
+  // not reload it through the scalar cache (whose wait also stalls LDS reads)
+  float nhdt = it.neg_half_dt;
+  asm volatile("" : "+v"(nhdt));
 
   const int ppb = it.paths_per_block;
   const int batches = ppb / nthreads;
@@ -526,7 +637,13 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
       cur.seek(wpos);
     } else {
       Real y[F];
-      node_normals<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, arena, n, gray, 0, y);
+      if constexpr (kTab) {
+        Real z[F];
+        table_normals<Real, F>(it, 0, n, z);
+        correlate<Real, F>(it, z, y);
+      } else {
+        node_normals<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, arena, n, gray, 0, y);
+      }
 #pragma unroll
       for (int f = 0; f < F; ++f) {
         bs.top[f] = y[f] * root_sd_of(it, Real(0));  // W[T] = y_0 sqrt(T dt), qmc.rs:165
@@ -546,13 +663,22 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     }
 
     int node_ptr = 0;
-    // shared normal table: the next node's normals are loaded one descent ahead,
-    // so their latency overlaps the current step's scenario work
-    const bool ztab = (it.flags & kNormalTable) != 0;
+    // The next node's loads are issued one descent ahead, so their latency
+    // overlaps the current step's scenario work: its normals from the shared
+    // table, or its Sobol coordinates and Owen prefix-table entries.
+    const bool sobol = kMode == kLadderDirect && (it.flags & (kNormalTable | kHalton)) == 0;
     Real zn[F];
+    SobolFetch<F> sf;
 #pragma unroll
-    for (int f = 0; f < F; ++f) zn[f] = Real(0);
-    if (ztab && steps > 1) table_normals<Real, F>(it, bnodes[0].node, n, zn);
+    for (int f = 0; f < F; ++f) {
+      zn[f] = Real(0);
+      sf.x[f] = sf.top[f] = 0u;
+    }
+    if constexpr (kTab) {
+      if (steps > 1) table_normals<Real, F>(it, bnodes[0].node, n, zn);
+    } else {
+      if (sobol && steps > 1) sobol_fetch<F>(it, arena, gray, bnodes[0].node, sf);
+    }
     int loaded_i0[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) loaded_i0[f] = -1;
@@ -583,9 +709,14 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         for (int d = 0; d < si.n_desc; ++d) {
           const BridgeNode bn = bnodes[node_ptr++];
           Real y[F], wmid[F];
-          if (ztab) {  // prefetched a step ahead; fetch the following node's now
+          if constexpr (kTab) {  // prefetched a step ahead; fetch the following node's now
             correlate<Real, F>(it, zn, y);
             if (node_ptr < steps - 1) table_normals<Real, F>(it, bnodes[node_ptr].node, n, zn);
+          } else if (sobol) {
+            Real z[F];
+            sobol_finish<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, bn.node, sf, z);
+            correlate<Real, F>(it, z, y);
+            if (node_ptr < steps - 1) sobol_fetch<F>(it, arena, gray, bnodes[node_ptr].node, sf);
           } else {
             node_normals<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, arena, n, gray, bn.node, y);
           }
@@ -671,10 +802,12 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         // factors of a scenario form one straight-line block the scheduler can
         // overlap (the LDS reads of one chain behind the arithmetic of another).
         if (lv_uniform) {
+          // one straight-line block over every scenario slot, so the LDS reads
+          // of one (scenario, factor) chain overlap the arithmetic of others;
+          // slots past S reread scenario S-1's constants (in bounds, unused)
 #pragma unroll
           for (int s = 0; s < SMAX; ++s) {
-            if (s >= S) continue;
-            const int ss = s;
+            const int ss = s < S ? s : S - 1;
 #pragma unroll
             for (int f = 0; f < F; ++f) {
               const SfF32 c = sf[ss * F + f];
@@ -686,7 +819,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
               const float v0 = fmaf(wx, e.y, e.x);
               const float v1 = fmaf(wx, e.w, e.z);
               const float vol = fmaf(wt[f], v1 - v0, v0);
-              spot[s][f] += fmaf(vol, dw[f], fmaf(vol * vol, it.neg_half_dt, c.c));
+              spot[s][f] += fmaf(vol, dw[f], fmaf(vol * vol, nhdt, c.c));
             }
           }
         } else if (local_vol) {  // knots not evenly spaced: bracketing search
@@ -701,7 +834,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
               const LvSurface& su = surf[u];
               const float vol = lv_sigma_f32_general(lds + (size_t)u * kSurfWords * nxs, su.n_x, (float)su.inv_dx,
                                                      wt[f], spot[s][f] + (float)it.ln_s0[f]);
-              spot[s][f] += fmaf(vol, dw[f], fmaf(vol * vol, it.neg_half_dt, c.c));
+              spot[s][f] += fmaf(vol, dw[f], fmaf(vol * vol, nhdt, c.c));
             }
           }
         } else {
@@ -720,12 +853,13 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         // selects (note.rs:143-195); no divergent branch.
         const float cr = (float)it.coupon_rate;
         const float df = (float)si.df;
+        float w_first = 0.0f;  // scenario 0's worst-of, for replay series
 #pragma unroll
         for (int s = 0; s < SMAX; ++s) {
           float w = spot[s][0];
 #pragma unroll
           for (int f = 1; f < F; ++f) w = fminf(w, spot[s][f]);
-          worst_last[s] = w;
+          if (s == 0) w_first = w;
           const bool alive = (pk[s] & 0xFFFFFu) == 0u;
           accrual[s] = (new_period ? 0.0f : accrual[s]) + (w >= it.ln_cb32 ? inc : 0.0f);
           const bool ko_now = alive && call_active && w >= it.ln_ko32;
@@ -738,7 +872,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
           pk[s] = ko_now ? ((pk[s] & 0x7FF00000u) | recv | (uint32_t)(step + 1)) : pk[s];
         }
         if (it.series != nullptr && valid)  // replay: the worst-of for the coupon strip
-          it.series[(size_t)step * it.paths + path_local] = exp((double)worst_last[0]);
+          it.series[(size_t)step * it.paths + path_local] = exp((double)w_first);
         // Phase 3: period end (note.rs:197-218), a step-uniform branch.
         if (period_end) {
           const bool memory = (it.flags & kMemoryCoupon) != 0;
@@ -850,6 +984,17 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     }
 
     // ---- path end (note.rs:221-258) and reductions
+    if constexpr (kLogState) {
+      // the FP32 tier's final worst-of, from the last step's log spots (steps >= 1),
+      // instead of carrying it through the step loop
+#pragma unroll
+      for (int s = 0; s < SMAX; ++s) {
+        float w = spot[s][0];
+#pragma unroll
+        for (int f = 1; f < F; ++f) w = fminf(w, spot[s][f]);
+        worst_last[s] = w;
+      }
+    }
 #pragma unroll
     for (int s = 0; s < SMAX; ++s) {
       if (s >= S) continue;
@@ -916,10 +1061,8 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         int base = sh[s].ev_n;
         for (int w = 0; w < wave; ++w) base += sh[s].wave_ko[w];
         const int rank = base + (int)__popcll(b_ko & ((1ull << lane) - 1ull));
-        if (valid && ko && rank < kTraceCap) {
-          sh[s].ev_path[rank] = (int32_t)path_local;
-          sh[s].ev_step[rank] = ko_step[s];
-        }
+        if (valid && ko && rank < kTraceCap)  // paths_per_block <= 2048, steps < 2^20
+          sh[s].ev[rank] = ((uint32_t)(b * nthreads + tid) << 20) | (uint32_t)ko_step[s];
         __syncthreads();
         if (tid == 0) {
           int tot = sh[s].ev_n;
@@ -948,8 +1091,8 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
       const int ne = trace ? sh[s].ev_n : 0;
       out->n_events = ne;
       for (int e = 0; e < ne; ++e) {
-        out->ev_path_lo[e] = sh[s].ev_path[e];
-        out->ev_step[e] = sh[s].ev_step[e];
+        out->ev_path_lo[e] = (int32_t)((int64_t)chunk * it.paths_per_block + (int64_t)(sh[s].ev[e] >> 20));
+        out->ev_step[e] = (int32_t)(sh[s].ev[e] & 0xFFFFFu);
       }
     }
   }

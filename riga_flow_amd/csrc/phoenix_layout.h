@@ -14,6 +14,10 @@ namespace riga {
 constexpr int kMaxFactors = 5;      // per-trade underlyings handled by the kernels
 constexpr int kBridgeStack = 14;    // pending bridge intervals (host-checked per trade)
 constexpr int kTraceCap = 64;       // note.rs:57 TRACE_EVENT_CAP
+// How a launch's items get their normals; the path kernel is compiled once per mode.
+constexpr int kLadderDirect = 0;      // Sobol (Owen) or Halton points generated per lane
+constexpr int kLadderTable = 1;       // read from the batch's shared [dim][path] normal table
+constexpr int kLadderMonteCarlo = 2;  // ChaCha8 + ziggurat stream (VarianceReduction::None)
 constexpr int kMaxLvX = 256;        // log-spot points per local-vol row staged in LDS
 #ifndef RIGA_OWEN_PREFIX_BITS
 #define RIGA_OWEN_PREFIX_BITS 16
@@ -29,7 +33,7 @@ enum ItemFlags : uint32_t {
   kTrace = 1u << 3,          // trace_level != Off
   kHalton = 1u << 4,         // VarianceReduction::Halton ladder (else Sobol)
   kRecords = 1u << 5,        // write one PathRecord per (scenario, path)
-  kOwenTable = 1u << 6,      // top 12 Owen levels read from a per-dimension table
+  kOwenTable = 1u << 6,      // top kOwenPrefixBits Owen levels read from a per-dimension table
   kNormalTable = 1u << 7,    // ladder normals read from the batch's shared [dim][path] table
   kLvUniform = 1u << 8,      // FP32 tier: every surface has evenly spaced knots (slope rows)
   kMonteCarlo = 1u << 9,     // VarianceReduction::None: ChaCha8 + ziggurat normals (note.rs:124-126)
@@ -209,8 +213,7 @@ struct BlockShared {
   uint64_t cnt[kWavesPerBlock][kNumC];
   int32_t ev_n;                      // KO events collected so far in this block
   int32_t wave_ko[kWavesPerBlock];
-  int32_t ev_path[kTraceCap];
-  int32_t ev_step[kTraceCap];
+  uint32_t ev[kTraceCap];             // (path within the block << 20) | KO step
 };
 
 // One period of a path's coupon strip (explain.rs:35-55 CouponPeriodRecord);
