@@ -317,6 +317,18 @@ int riga_debug_sobol(riga_ctx *ctx, uint64_t seed, int32_t ladder_dims, uint64_t
                      uint32_t *out_unscrambled, uint32_t *out_scrambled, double *out_normal,
                      riga_error *err);
 
+/* The FP32 tier's Sobol ladder as its path kernels run it, for points
+ * n = first_point + p (1-based) and dims node*factors + f, node < n_nodes:
+ * out_scrambled[p][dim] is the integer the tier turns into its uniform -- the
+ * top 24 Owen levels with the low byte 0x80, or all 32 levels where the top 10
+ * scrambled bits are all 0 or all 1 (u within 2^-10 of 0 or 1) -- and
+ * out_normal[p][dim] its FP32 inverse normal.  owen_table != 0 reads the top
+ * levels from the context's prefix table for the seed, as priced batches do.
+ * No counterpart in the reference (a diagnostic for the parity tests). */
+int riga_debug_sobol_f32(riga_ctx *ctx, uint64_t seed, int32_t ladder_dims, uint64_t first_point,
+                         int32_t n_points, int32_t n_nodes, int32_t factors, int32_t owen_table,
+                         uint32_t *out_scrambled, float *out_normal, riga_error *err);
+
 /* ---- PathCube on demand (pathcube.rs:47-76 generate_correlated_block, :146-187
  * gbm_evolve): a correlated block y = L z from ChaCha8Rng::seed_from_u64(seed),
  * `factors` StandardNormals per (path, time) cell in stream order, as floats, and

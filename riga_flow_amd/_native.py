@@ -165,7 +165,8 @@ EXPORTS = ("riga_hip_init", "riga_hip_destroy", "riga_hip_device_name", "riga_hi
            "riga_server_create", "riga_server_submit", "riga_server_wait",
            "riga_server_stats_get", "riga_server_destroy",
            "riga_surface_create", "riga_surface_destroy", "riga_autotune",
-           "riga_lv_calibrate", "riga_lv_sigma", "riga_debug_sobol", "riga_debug_mc_starts",
+           "riga_lv_calibrate", "riga_lv_sigma", "riga_debug_sobol", "riga_debug_sobol_f32",
+           "riga_debug_mc_starts",
            "riga_replay_ledger", "riga_query_worst_percentile", "riga_path_cube",
            "riga_worst_of", "riga_resident_worst_of_create", "riga_resident_worst_of_reduce",
            "riga_resident_worst_of_download", "riga_resident_worst_of_synchronize",
@@ -242,6 +243,10 @@ def lib():
                                            C.c_int32, C.c_int32, C.c_int32,
                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _D,
                                            C.POINTER(Error)]
+            L.riga_debug_sobol_f32.argtypes = [C.c_void_p, C.c_uint64, C.c_int32, C.c_uint64,
+                                               C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                               C.POINTER(C.c_uint32), C.POINTER(C.c_float),
+                                               C.POINTER(Error)]
             L.riga_debug_mc_starts.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64,
                                                C.POINTER(C.c_uint64), C.POINTER(Error)]
             _U64 = C.POINTER(C.c_uint64)

@@ -528,6 +528,23 @@ def debug_sobol(seed: int, ladder_dims: int, first_point: int, n_points: int, fi
     return (u.reshape(n_points, n_dims), s.reshape(n_points, n_dims), z.reshape(n_points, n_dims))
 
 
+def debug_sobol_f32(seed: int, ladder_dims: int, first_point: int, n_points: int, n_nodes: int,
+                    factors: int, owen_table: bool = True):
+    """The FP32 tier's ladder as its path kernels run it: the integers it turns into
+    uniforms (24 Owen levels with low byte 0x80, all 32 levels within 2^-10 of 0 or 1)
+    and its FP32 normals, [point][node * factors + f]."""
+    gpu_backend()
+    dims = n_nodes * factors
+    s = np.zeros(n_points * dims, dtype=np.uint32)
+    z = np.zeros(n_points * dims, dtype=np.float32)
+    err = N.Error()
+    N.check(N.lib().riga_debug_sobol_f32(
+        N.Device.get().handle, seed, ladder_dims, first_point, n_points, n_nodes, factors,
+        1 if owen_table else 0, s.ctypes.data_as(C.POINTER(C.c_uint32)),
+        z.ctypes.data_as(C.POINTER(C.c_float)), C.byref(err)), err)
+    return s.reshape(n_points, dims), z.reshape(n_points, dims)
+
+
 def debug_mc_starts(seed: int, draws_per_path: int, n_paths: int) -> np.ndarray:
     """Device stream parse of the Monte Carlo lane: the u64 word of
     ``ChaCha8Rng::seed_from_u64(seed)`` where each path's first StandardNormal

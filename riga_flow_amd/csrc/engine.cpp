@@ -1757,6 +1757,48 @@ int riga_debug_sobol(riga_ctx* ctx, uint64_t seed, int32_t ladder_dims, uint64_t
   return 0;
 }
 
+int riga_debug_sobol_f32(riga_ctx* ctx, uint64_t seed, int32_t ladder_dims, uint64_t first_point,
+                         int32_t n_points, int32_t n_nodes, int32_t factors, int32_t owen_table,
+                         uint32_t* out_scrambled, float* out_normal, riga_error* err) {
+  clear_err(err);
+  if (!ctx) {
+    set_err(err, "gpu_unavailable", "no device context");
+    return RIGA_ENODEV;
+  }
+  if (n_points <= 0 || n_nodes <= 0 || factors < 1 || factors > kMaxFactors || ladder_dims <= 0 ||
+      first_point == 0 || first_point + (uint64_t)n_points >= (1ull << 32) ||
+      (int64_t)n_nodes * factors > (int64_t)1 << 24) {
+    set_err(err, "invalid_argument", "bad debug_sobol_f32 ranges");
+    return RIGA_EINVAL;
+  }
+  const int dims = n_nodes * factors;
+  const int rows = std::min(ladder_dims, dims);
+  const std::vector<uint32_t>& dirs = sobol_directions(rows);
+  const int nbits = 64 - __builtin_clzll(first_point + (uint64_t)n_points);
+  const size_t cells = (size_t)n_points * dims;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  std::shared_ptr<OwenTable> ot;
+  if (owen_table) {
+    HIP_TRY(owen_table_for(ctx, seed, dims, ctx->stream, &ot), "owen table");
+    HIP_TRY(hipStreamWaitEvent(ctx->stream, ot->ready, 0), "owen table wait");
+  }
+  DevBuf d_dirs, d_s, d_z;
+  HIP_TRY(d_dirs.ensure((size_t)rows * 32 * 4), "alloc");
+  HIP_TRY(d_s.ensure(cells * 4), "alloc");
+  HIP_TRY(d_z.ensure(cells * 4), "alloc");
+  HIP_TRY(hipMemcpyAsync(d_dirs.p, dirs.data(), (size_t)rows * 32 * 4, hipMemcpyHostToDevice, ctx->stream),
+          "upload");
+  HIP_TRY(launch_debug_sobol_f32((const uint32_t*)d_dirs.p, ladder_dims, ot ? (const uint16_t*)ot->buf.p : nullptr,
+                                 seed, first_point, n_points, n_nodes, factors, nbits, (uint32_t*)d_s.p,
+                                 (float*)d_z.p, ctx->stream),
+          "debug kernel");
+  HIP_TRY(hipMemcpyAsync(out_scrambled, d_s.p, cells * 4, hipMemcpyDeviceToHost, ctx->stream), "download");
+  HIP_TRY(hipMemcpyAsync(out_normal, d_z.p, cells * 4, hipMemcpyDeviceToHost, ctx->stream), "download");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "debug kernel execution");
+  return 0;
+}
+
 int riga_path_cube(riga_ctx* ctx, uint64_t seed, uint64_t paths, int32_t times, int32_t factors,
                    const double* correlation, const double* s0, const double* vols, double rate,
                    const double* dividend_yields, double dt, const float* in_block, float* out_block,

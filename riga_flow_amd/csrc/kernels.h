@@ -25,6 +25,9 @@ struct PhoenixLaunch {
 };
 
 hipError_t launch_phoenix_f64(const PhoenixLaunch& l, hipStream_t stream);
+hipError_t launch_debug_sobol_f32(const uint32_t* dirs, int ladder_dims, const uint16_t* owen_tab, uint64_t seed,
+                                  uint64_t first_point, int n_points, int n_nodes, int factors, int nbits,
+                                  uint32_t* xs_out, float* z_out, hipStream_t stream);
 hipError_t launch_phoenix_f32(const PhoenixLaunch& l, hipStream_t stream);
 
 // Shared normal table z[dim][path] (Real = double for FP64, float for FP32).

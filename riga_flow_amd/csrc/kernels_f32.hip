@@ -37,6 +37,62 @@ static hipError_t launch_f(const PhoenixLaunch& l, hipStream_t stream) {
   }
 }
 
+// The FP32 tier's Sobol ladder as the path kernel runs it (prefix-table gather,
+// 24 hashed levels, the wave's tail passes, FP32 inverse normal) for points
+// first_point + p and dims node*F + f: xs_out/z_out[p][node*F + f].
+template <int F>
+__global__ __launch_bounds__(256) void debug_sobol_f32_kernel(const uint32_t* __restrict__ dirs, int ladder_dims,
+                                                              const uint16_t* __restrict__ owen_tab, uint64_t seed,
+                                                              uint64_t first_point, int n_points, int n_nodes,
+                                                              int nbits, uint32_t* __restrict__ xs_out,
+                                                              float* __restrict__ z_out) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n_points) return;
+  ItemDesc it{};
+  it.flags = owen_tab ? kOwenTable : 0u;
+  it.ladder_dims = ladder_dims;
+  it.off_dirs = 0;
+  it.nbits = nbits;
+  it.owen_tab = owen_tab;
+  it.seed = seed;
+  const uint32_t n = (uint32_t)(first_point + (uint64_t)p);
+  const uint32_t gray = n ^ (n >> 1);
+  const size_t row = (size_t)p * n_nodes * F;
+  for (int node = 0; node < n_nodes; ++node) {
+    SobolFetch<F> sf;
+    sobol_fetch<F>(it, reinterpret_cast<const unsigned char*>(dirs), gray, node, sf);
+    uint32_t xs[F];
+    sobol_ints_f32<F>(it, node, sf, xs);
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      xs_out[row + (size_t)node * F + f] = xs[f];
+      z_out[row + (size_t)node * F + f] = probit_f32(((double)xs[f] + 0.5) / 4294967296.0);
+    }
+  }
+}
+
+hipError_t launch_debug_sobol_f32(const uint32_t* dirs, int ladder_dims, const uint16_t* owen_tab, uint64_t seed,
+                                  uint64_t first_point, int n_points, int n_nodes, int factors, int nbits,
+                                  uint32_t* xs_out, float* z_out, hipStream_t stream) {
+  const dim3 grid((n_points + 255) / 256);
+#define RIGA_DEBUG_F32(FF)                                                                                  \
+  case FF:                                                                                                  \
+    hipLaunchKernelGGL(debug_sobol_f32_kernel<FF>, grid, dim3(256), 0, stream, dirs, ladder_dims, owen_tab, \
+                       seed, first_point, n_points, n_nodes, nbits, xs_out, z_out);                         \
+    break;
+  switch (factors) {
+    RIGA_DEBUG_F32(1)
+    RIGA_DEBUG_F32(2)
+    RIGA_DEBUG_F32(3)
+    RIGA_DEBUG_F32(4)
+    RIGA_DEBUG_F32(5)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef RIGA_DEBUG_F32
+  return hipGetLastError();
+}
+
 hipError_t launch_phoenix_f32(const PhoenixLaunch& l, hipStream_t stream) {
   switch (l.F) {
     case 1: return launch_f<1>(l, stream);

@@ -203,16 +203,13 @@ __device__ __forceinline__ double probit_f64(double p) {
 
 // FP32 inverse normal for the fast tier.  The uniform arrives in FP64 so the
 // tail argument (p or 1-p) and the central offset p-0.5 are exact before the
-// rounding to float.
+// rounding to float.  The central rational's two polynomials are evaluated in
+// FP64 (the same issue rate as FP32 here): in FP32 their mixed-sign terms
+// cancel to a relative error of 1.9e-4 near |z| = 1.95; this way the quotient
+// of their float roundings is within 4e-7 of the exact inverse normal.
 __device__ __forceinline__ float probit_f32(double p) {
   if (p < 1.0e-12) p = 1.0e-12;
   if (p > 1.0 - 1.0e-12) p = 1.0 - 1.0e-12;
-  constexpr float A0 = -3.969683028665376e+01f, A1 = 2.209460984245205e+02f,
-                  A2 = -2.759285104469687e+02f, A3 = 1.383577518672690e+02f,
-                  A4 = -3.066479806614716e+01f, A5 = 2.506628277459239e+00f;
-  constexpr float B0 = -5.447609879822406e+01f, B1 = 1.615858368580409e+02f,
-                  B2 = -1.556989798598866e+02f, B3 = 6.680131188771972e+01f,
-                  B4 = -1.328068155288572e+01f;
   constexpr float C0 = -7.784894002430293e-03f, C1 = -3.223964580411365e-01f,
                   C2 = -2.400758277161838e+00f, C3 = -2.549732539343734e+00f,
                   C4 = 4.374664141464968e+00f, C5 = 2.938163982698783e+00f;
@@ -227,10 +224,18 @@ __device__ __forceinline__ float probit_f32(double p) {
     const float z = __fdividef(num, den);
     return lower ? z : -z;
   }
-  const float q = (float)(p - 0.5);
-  const float r = q * q;
-  return __fdividef((((((A0 * r + A1) * r + A2) * r + A3) * r + A4) * r + A5) * q,
-                    (((((B0 * r + B1) * r + B2) * r + B3) * r + B4) * r + 1.0f));
+  // the central coefficients at full precision (qmc.rs:39-91)
+  constexpr double DA0 = -3.969683028665376e+01, DA1 = 2.209460984245205e+02,
+                   DA2 = -2.759285104469687e+02, DA3 = 1.383577518672690e+02,
+                   DA4 = -3.066479806614716e+01, DA5 = 2.506628277459239e+00;
+  constexpr double DB0 = -5.447609879822406e+01, DB1 = 1.615858368580409e+02,
+                   DB2 = -1.556989798598866e+02, DB3 = 6.680131188771972e+01,
+                   DB4 = -1.328068155288572e+01;
+  const double q = p - 0.5;
+  const double r = q * q;
+  const double num = (((((DA0 * r + DA1) * r + DA2) * r + DA3) * r + DA4) * r + DA5) * q;
+  const double den = ((((DB0 * r + DB1) * r + DB2) * r + DB3) * r + DB4) * r + 1.0;
+  return __fdividef((float)num, (float)den);
 }
 
 // ---------------------------------------------------------------- double-double
@@ -349,42 +354,50 @@ __device__ __forceinline__ void sobol_fetch(const ItemDesc& it, const unsigned c
   }
 }
 
+// The FP32 tier's scrambled integers of one node: 24 hashed levels per
+// coordinate; the few lanes with a coordinate in a tail then finish its last 8
+// levels, one coordinate per lane per pass, while any lane of the wave has one
+// left.
+template <int F>
+__device__ __forceinline__ void sobol_ints_f32(const ItemDesc& it, int node, const SobolFetch<F>& sf,
+                                               uint32_t (&xs)[F]) {
+  const bool tab = (it.flags & kOwenTable) != 0;
+  uint64_t keys[F];
+  uint32_t pend = 0;
+#pragma unroll
+  for (int f = 0; f < F; ++f) {
+    keys[f] = it.seed ^ ((uint64_t)((uint32_t)node * F + f) * 0x9E3779B97F4A7C15ull);
+    const uint32_t p = owen_top24(sf.x[f], keys[f], tab, sf.top[f]);
+    xs[f] = (p << 8) | 0x80u;
+    pend |= (uint32_t)owen_f32_tail(p) << f;
+  }
+  while (__ballot(pend != 0u) != 0ull) {
+    if (pend) {
+      const int g = __builtin_ctz(pend);
+      pend &= pend - 1u;
+      uint32_t xg = sf.x[0], pg = xs[0];
+      uint64_t kg = keys[0];
+#pragma unroll
+      for (int f = 1; f < F; ++f)
+        if (g == f) {
+          xg = sf.x[f];
+          pg = xs[f];
+          kg = keys[f];
+        }
+      const uint32_t full = owen_low8(xg, kg, pg >> 8);
+#pragma unroll
+      for (int f = 0; f < F; ++f)
+        if (g == f) xs[f] = full;
+    }
+  }
+}
+
 template <typename Real, int F, bool kUnrollOwen>
 __device__ __forceinline__ void sobol_finish(const ItemDesc& it, int node, const SobolFetch<F>& sf, Real (&z)[F]) {
   const bool tab = (it.flags & kOwenTable) != 0;
   if constexpr (sizeof(Real) == 4) {
-    // FP32 tier: 24 hashed levels per coordinate; the few lanes with a
-    // coordinate in a tail then finish its last 8 levels, one coordinate per
-    // lane per pass, while any lane of the wave has one left.
     uint32_t xs[F];
-    uint64_t keys[F];
-    uint32_t pend = 0;
-#pragma unroll
-    for (int f = 0; f < F; ++f) {
-      keys[f] = it.seed ^ ((uint64_t)((uint32_t)node * F + f) * 0x9E3779B97F4A7C15ull);
-      const uint32_t p = owen_top24(sf.x[f], keys[f], tab, sf.top[f]);
-      xs[f] = (p << 8) | 0x80u;
-      pend |= (uint32_t)owen_f32_tail(p) << f;
-    }
-    while (__ballot(pend != 0u) != 0ull) {
-      if (pend) {
-        const int g = __builtin_ctz(pend);
-        pend &= pend - 1u;
-        uint32_t xg = sf.x[0], pg = xs[0];
-        uint64_t kg = keys[0];
-#pragma unroll
-        for (int f = 1; f < F; ++f)
-          if (g == f) {
-            xg = sf.x[f];
-            pg = xs[f];
-            kg = keys[f];
-          }
-        const uint32_t full = owen_low8(xg, kg, pg >> 8);
-#pragma unroll
-        for (int f = 0; f < F; ++f)
-          if (g == f) xs[f] = full;
-      }
-    }
+    sobol_ints_f32<F>(it, node, sf, xs);
 #pragma unroll
     for (int f = 0; f < F; ++f) z[f] = probit_f32(((double)xs[f] + 0.5) / 4294967296.0);
   } else {

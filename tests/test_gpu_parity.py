@@ -40,6 +40,35 @@ def test_sobol_integers_bit_exact(seed, ladder_dims, first_point, n_points, firs
             assert z[p, d] == pytest.approx(sob.normal(n - 1, dim), rel=1e-15, abs=1e-15)
 
 
+@pytest.mark.parametrize("factors,owen_table", [(3, True), (5, True), (3, False)])
+def test_fp32_tier_ladder_keeps_24_owen_levels_and_full_tails(factors, owen_table):
+    """The FP32 tier's integers, as its path kernels compute them (prefix table, 24
+    hashed levels, the wave's tail passes), against the rule applied to the full
+    32-level scramble (whose integers the oracle pins): the top 24 bits with low byte
+    0x80, and all 32 bits where the top 10 scrambled bits are all 0 or all 1.  The
+    FP32 normals are the inverse normal of that uniform to FP32 accuracy."""
+    from scipy.special import ndtri
+    seed, ladder, nodes, n_pts = 81_281, 2268, 6, 65_536
+    xs, z = api.debug_sobol_f32(seed, ladder, 1, n_pts, nodes, factors, owen_table)
+    _, full, _ = api.debug_sobol(seed, ladder, 1, n_pts, 0, nodes * factors)
+    p24 = full >> np.uint32(8)
+    tail = (p24 < (1 << 14)) | (p24 >= (1 << 24) - (1 << 14))
+    want = np.where(tail, full, (p24 << np.uint32(8)) | np.uint32(0x80)).astype(np.uint32)
+    assert np.array_equal(xs, want)
+    # the tail passes ran, including lanes with two tail coordinates in one node
+    per_node = tail.reshape(n_pts, nodes, factors).sum(axis=2)
+    assert tail.sum() > 1000 and (per_node >= 2).any()
+    # the full integers of a sample of cells, tail cells among them, from the oracle
+    sob = O.Sobol(ladder, seed, max_dim=ladder)
+    rng = np.random.default_rng(7)
+    cells = list(zip(rng.integers(0, n_pts, 200), rng.integers(0, nodes * factors, 200)))
+    cells += list(zip(*np.nonzero(tail)))[:100]
+    for p, d in cells:
+        assert int(full[p, d]) == sob.scrambled(int(p) + 1, int(d))
+    zr = ndtri((want.astype(np.float64) + 0.5) / 2.0 ** 32)  # Acklam is within 1.2e-9 of it
+    assert np.max(np.abs(z - zr) / np.maximum(1.0, np.abs(zr))) < 5e-6
+
+
 def _flat(paths, steps=252, **kw):
     req = W.flat_book(paths=paths, steps=steps)
     for k, v in kw.items():
