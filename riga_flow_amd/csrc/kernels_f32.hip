@@ -38,7 +38,7 @@ static hipError_t launch_f(const PhoenixLaunch& l, hipStream_t stream) {
 }
 
 // The FP32 tier's Sobol ladder as the path kernel runs it (prefix-table gather,
-// 24 hashed levels, the wave's tail passes, FP32 inverse normal) for points
+// 24 hashed levels, the wave's tail passes, the node's FP32 inverse normals) for points
 // first_point + p and dims node*F + f: xs_out/z_out[p][node*F + f].
 template <int F>
 __global__ __launch_bounds__(256) void debug_sobol_f32_kernel(const uint32_t* __restrict__ dirs, int ladder_dims,
@@ -63,10 +63,15 @@ __global__ __launch_bounds__(256) void debug_sobol_f32_kernel(const uint32_t* __
     sobol_fetch<F>(it, reinterpret_cast<const unsigned char*>(dirs), gray, node, sf);
     uint32_t xs[F];
     sobol_ints_f32<F>(it, node, sf, xs);
+    double u[F];
+    float z[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) u[f] = ((double)xs[f] + 0.5) / 4294967296.0;
+    probit_node<float, F>(u, z);
 #pragma unroll
     for (int f = 0; f < F; ++f) {
       xs_out[row + (size_t)node * F + f] = xs[f];
-      z_out[row + (size_t)node * F + f] = probit_f32(((double)xs[f] + 0.5) / 4294967296.0);
+      z_out[row + (size_t)node * F + f] = z[f];
     }
   }
 }

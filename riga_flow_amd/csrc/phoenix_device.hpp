@@ -171,34 +171,41 @@ __device__ __forceinline__ double halton_u(uint32_t n, uint32_t base,
   return result;
 }
 
-// Acklam inverse normal (qmc.rs:37-91) with the ladder's clamp (qmc.rs:284-289).
-__device__ __forceinline__ double probit_f64(double p) {
-  if (p < 1.0e-12) p = 1.0e-12;
-  if (p > 1.0 - 1.0e-12) p = 1.0 - 1.0e-12;
+// Acklam inverse normal (qmc.rs:37-91) with the ladder's clamp (qmc.rs:284-289),
+// as its two regions: the central rational for plow <= p <= 1 - plow, and the
+// tails (4.85% of uniforms), whose clamp leaves the region test unchanged.
+__device__ __forceinline__ bool probit_is_tail(double p) { return p < 0.02425 || p > 1.0 - 0.02425; }
+
+__device__ __forceinline__ double probit_f64_central(double p) {
   constexpr double A0 = -3.969683028665376e+01, A1 = 2.209460984245205e+02,
                    A2 = -2.759285104469687e+02, A3 = 1.383577518672690e+02,
                    A4 = -3.066479806614716e+01, A5 = 2.506628277459239e+00;
   constexpr double B0 = -5.447609879822406e+01, B1 = 1.615858368580409e+02,
                    B2 = -1.556989798598866e+02, B3 = 6.680131188771972e+01,
                    B4 = -1.328068155288572e+01;
+  const double q = p - 0.5;
+  const double r = q * q;
+  return (((((A0 * r + A1) * r + A2) * r + A3) * r + A4) * r + A5) * q /
+         (((((B0 * r + B1) * r + B2) * r + B3) * r + B4) * r + 1.0);
+}
+
+__device__ __forceinline__ double probit_f64_tail(double p) {
   constexpr double C0 = -7.784894002430293e-03, C1 = -3.223964580411365e-01,
                    C2 = -2.400758277161838e+00, C3 = -2.549732539343734e+00,
                    C4 = 4.374664141464968e+00, C5 = 2.938163982698783e+00;
   constexpr double D0 = 7.784695709041462e-03, D1 = 3.224671290700398e-01,
                    D2 = 2.445134137142996e+00, D3 = 3.754408661907416e+00;
-  constexpr double plow = 0.02425;
-  constexpr double phigh = 1.0 - plow;
-  if (p < plow || p > phigh) {
-    const bool lower = p < plow;
-    const double q = sqrt(-2.0 * log(lower ? p : 1.0 - p));
-    const double num = ((((C0 * q + C1) * q + C2) * q + C3) * q + C4) * q + C5;
-    const double den = (((D0 * q + D1) * q + D2) * q + D3) * q + 1.0;
-    return lower ? num / den : -num / den;
-  }
-  const double q = p - 0.5;
-  const double r = q * q;
-  return (((((A0 * r + A1) * r + A2) * r + A3) * r + A4) * r + A5) * q /
-         (((((B0 * r + B1) * r + B2) * r + B3) * r + B4) * r + 1.0);
+  if (p < 1.0e-12) p = 1.0e-12;
+  if (p > 1.0 - 1.0e-12) p = 1.0 - 1.0e-12;
+  const bool lower = p < 0.02425;
+  const double q = sqrt(-2.0 * log(lower ? p : 1.0 - p));
+  const double num = ((((C0 * q + C1) * q + C2) * q + C3) * q + C4) * q + C5;
+  const double den = (((D0 * q + D1) * q + D2) * q + D3) * q + 1.0;
+  return lower ? num / den : -num / den;
+}
+
+__device__ __forceinline__ double probit_f64(double p) {
+  return probit_is_tail(p) ? probit_f64_tail(p) : probit_f64_central(p);
 }
 
 // FP32 inverse normal for the fast tier.  The uniform arrives in FP64 so the
@@ -207,35 +214,75 @@ __device__ __forceinline__ double probit_f64(double p) {
 // FP64 (the same issue rate as FP32 here): in FP32 their mixed-sign terms
 // cancel to a relative error of 1.9e-4 near |z| = 1.95; this way the quotient
 // of their float roundings is within 4e-7 of the exact inverse normal.
-__device__ __forceinline__ float probit_f32(double p) {
-  if (p < 1.0e-12) p = 1.0e-12;
-  if (p > 1.0 - 1.0e-12) p = 1.0 - 1.0e-12;
+__device__ __forceinline__ float probit_f32_central(double p) {
+  constexpr double A0 = -3.969683028665376e+01, A1 = 2.209460984245205e+02,
+                   A2 = -2.759285104469687e+02, A3 = 1.383577518672690e+02,
+                   A4 = -3.066479806614716e+01, A5 = 2.506628277459239e+00;
+  constexpr double B0 = -5.447609879822406e+01, B1 = 1.615858368580409e+02,
+                   B2 = -1.556989798598866e+02, B3 = 6.680131188771972e+01,
+                   B4 = -1.328068155288572e+01;
+  const double q = p - 0.5;
+  const double r = q * q;
+  const double num = (((((A0 * r + A1) * r + A2) * r + A3) * r + A4) * r + A5) * q;
+  const double den = ((((B0 * r + B1) * r + B2) * r + B3) * r + B4) * r + 1.0;
+  return __fdividef((float)num, (float)den);
+}
+
+__device__ __forceinline__ float probit_f32_tail(double p) {
   constexpr float C0 = -7.784894002430293e-03f, C1 = -3.223964580411365e-01f,
                   C2 = -2.400758277161838e+00f, C3 = -2.549732539343734e+00f,
                   C4 = 4.374664141464968e+00f, C5 = 2.938163982698783e+00f;
   constexpr float D0 = 7.784695709041462e-03f, D1 = 3.224671290700398e-01f,
                   D2 = 2.445134137142996e+00f, D3 = 3.754408661907416e+00f;
-  if (p < 0.02425 || p > 1.0 - 0.02425) {
-    const bool lower = p < 0.5;
-    const float t = (float)(lower ? p : 1.0 - p);
-    const float q = __fsqrt_rn(-2.0f * __logf(t));
-    const float num = ((((C0 * q + C1) * q + C2) * q + C3) * q + C4) * q + C5;
-    const float den = (((D0 * q + D1) * q + D2) * q + D3) * q + 1.0f;
-    const float z = __fdividef(num, den);
-    return lower ? z : -z;
+  if (p < 1.0e-12) p = 1.0e-12;
+  if (p > 1.0 - 1.0e-12) p = 1.0 - 1.0e-12;
+  const bool lower = p < 0.5;
+  const float t = (float)(lower ? p : 1.0 - p);
+  const float q = __fsqrt_rn(-2.0f * __logf(t));
+  const float num = ((((C0 * q + C1) * q + C2) * q + C3) * q + C4) * q + C5;
+  const float den = (((D0 * q + D1) * q + D2) * q + D3) * q + 1.0f;
+  const float z = __fdividef(num, den);
+  return lower ? z : -z;
+}
+
+__device__ __forceinline__ float probit_f32(double p) {
+  return probit_is_tail(p) ? probit_f32_tail(p) : probit_f32_central(p);
+}
+
+__device__ __forceinline__ double probit_tier(double p, double) { return probit_f64(p); }
+__device__ __forceinline__ float probit_tier(double p, float) { return probit_f32(p); }
+__device__ __forceinline__ double probit_tail_tier(double p, double) { return probit_f64_tail(p); }
+__device__ __forceinline__ float probit_tail_tier(double p, float) { return probit_f32_tail(p); }
+__device__ __forceinline__ double probit_central_tier(double p, double) { return probit_f64_central(p); }
+__device__ __forceinline__ float probit_central_tier(double p, float) { return probit_f32_central(p); }
+
+// Inverse normals of one node's F uniforms: the central rational for every
+// coordinate, then the tail coordinates one per lane per pass while any lane of
+// the wave has one.  A branch per coordinate would run both regions for nearly
+// every wave (a wave of 64 lanes almost always holds a tail uniform); this way
+// the tails cost about one pass per node.  Each z is the same value either way.
+template <typename Real, int F>
+__device__ __forceinline__ void probit_node(const double (&u)[F], Real (&z)[F]) {
+  uint32_t pend = 0;
+#pragma unroll
+  for (int f = 0; f < F; ++f) {
+    z[f] = probit_central_tier(u[f], Real(0));
+    pend |= (uint32_t)probit_is_tail(u[f]) << f;
   }
-  // the central coefficients at full precision (qmc.rs:39-91)
-  constexpr double DA0 = -3.969683028665376e+01, DA1 = 2.209460984245205e+02,
-                   DA2 = -2.759285104469687e+02, DA3 = 1.383577518672690e+02,
-                   DA4 = -3.066479806614716e+01, DA5 = 2.506628277459239e+00;
-  constexpr double DB0 = -5.447609879822406e+01, DB1 = 1.615858368580409e+02,
-                   DB2 = -1.556989798598866e+02, DB3 = 6.680131188771972e+01,
-                   DB4 = -1.328068155288572e+01;
-  const double q = p - 0.5;
-  const double r = q * q;
-  const double num = (((((DA0 * r + DA1) * r + DA2) * r + DA3) * r + DA4) * r + DA5) * q;
-  const double den = ((((DB0 * r + DB1) * r + DB2) * r + DB3) * r + DB4) * r + 1.0;
-  return __fdividef((float)num, (float)den);
+  while (__ballot(pend != 0u) != 0ull) {
+    if (pend) {
+      const int g = __builtin_ctz(pend);
+      pend &= pend - 1u;
+      double ug = u[0];
+#pragma unroll
+      for (int f = 1; f < F; ++f)
+        if (g == f) ug = u[f];
+      const Real zg = probit_tail_tier(ug, Real(0));
+#pragma unroll
+      for (int f = 0; f < F; ++f)
+        if (g == f) z[f] = zg;
+    }
+  }
 }
 
 // ---------------------------------------------------------------- double-double
@@ -398,16 +445,20 @@ __device__ __forceinline__ void sobol_finish(const ItemDesc& it, int node, const
   if constexpr (sizeof(Real) == 4) {
     uint32_t xs[F];
     sobol_ints_f32<F>(it, node, sf, xs);
+    double u[F];
 #pragma unroll
-    for (int f = 0; f < F; ++f) z[f] = probit_f32(((double)xs[f] + 0.5) / 4294967296.0);
+    for (int f = 0; f < F; ++f) u[f] = ((double)xs[f] + 0.5) / 4294967296.0;
+    probit_node<Real, F>(u, z);
   } else {
+    double u[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) {
       const uint64_t key = it.seed ^ ((uint64_t)((uint32_t)node * F + f) * 0x9E3779B97F4A7C15ull);
       // full unroll where registers allow (the FP64 10-scenario lane keeps the loop)
       const uint32_t xs = tab ? owen_scramble_below<kUnrollOwen>(sf.x[f], key, sf.top[f]) : owen_scramble(sf.x[f], key);
-      z[f] = probit_f64(((double)xs + 0.5) / 4294967296.0);
+      u[f] = ((double)xs + 0.5) / 4294967296.0;
     }
+    probit_node<Real, F>(u, z);
   }
 }
 
@@ -422,16 +473,14 @@ __device__ __forceinline__ void node_normals(const ItemDesc& it, const unsigned 
   if (it.flags & kNormalTable) {
     table_normals<Real, F>(it, node, n, z);
   } else if (it.flags & kHalton) {
+    double u[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) {
       const uint32_t d = ((uint32_t)node * F + f) % (uint32_t)it.ladder_dims;
       const uint32_t* h = at<uint32_t>(arena, it.off_halton);
-      const double u = halton_u(n, h[d], h + h[it.ladder_rows + d]);
-      if constexpr (sizeof(Real) == 8)
-        z[f] = probit_f64(u);
-      else
-        z[f] = probit_f32(u);
+      u[f] = halton_u(n, h[d], h + h[it.ladder_rows + d]);
     }
+    probit_node<Real, F>(u, z);
   } else {
     SobolFetch<F> sf;
     sobol_fetch<F>(it, arena, gray, node, sf);
