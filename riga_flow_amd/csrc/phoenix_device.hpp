@@ -864,26 +864,39 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         // factors of a scenario form one straight-line block the scheduler can
         // overlap (the LDS reads of one chain behind the arithmetic of another).
         if (lv_uniform) {
-          // one straight-line block over every scenario slot, so the LDS reads
-          // of one (scenario, factor) chain overlap the arithmetic of others;
-          // slots past S reread scenario S-1's constants (in bounds, unused)
+          // the Euler step of scenario slot s; slots past S (when S < SMAX) reread
+          // scenario S-1's constants (in bounds) and their state is never read
+#define RIGA_LV_EULER(s)                                                                   \
+  {                                                                                        \
+    const int ss_ = (s) < S ? (s) : S - 1;                                                 \
+    _Pragma("unroll") for (int f = 0; f < F; ++f) {                                        \
+      const SfF32 c = sf[ss_ * F + f];                                                     \
+      float t = fmaf(spot[s][f], c.a, c.b);                                                \
+      t = fminf(fmaxf(t, 0.0f), c.tmax);                                                   \
+      const int j = (int)t;                                                                \
+      const float wx = t - (float)j;                                                       \
+      const float4 e = *reinterpret_cast<const float4*>(lv_rows + c.lds_off + j * 16);     \
+      const float v0 = fmaf(wx, e.y, e.x);                                                 \
+      const float v1 = fmaf(wx, e.w, e.z);                                                 \
+      const float vol = fmaf(wt[f], v1 - v0, v0);                                          \
+      spot[s][f] += fmaf(vol, dw[f], fmaf(vol * vol, nhdt, c.c));                          \
+    }                                                                                      \
+  }
+          if (F * SMAX <= 30 && S == SMAX) {
+            // a full bucket: one straight-line block over every scenario, so the LDS
+            // reads of one (scenario, factor) chain overlap the arithmetic of others
+            // (up to 30 chains; more hold too many registers)
 #pragma unroll
-          for (int s = 0; s < SMAX; ++s) {
-            const int ss = s < S ? s : S - 1;
+            for (int s = 0; s < SMAX; ++s) RIGA_LV_EULER(s)
+          } else {
+            // a partly filled bucket computes only its scenarios
 #pragma unroll
-            for (int f = 0; f < F; ++f) {
-              const SfF32 c = sf[ss * F + f];
-              float t = fmaf(spot[s][f], c.a, c.b);
-              t = fminf(fmaxf(t, 0.0f), c.tmax);
-              const int j = (int)t;
-              const float wx = t - (float)j;
-              const float4 e = *reinterpret_cast<const float4*>(lv_rows + c.lds_off + j * 16);
-              const float v0 = fmaf(wx, e.y, e.x);
-              const float v1 = fmaf(wx, e.w, e.z);
-              const float vol = fmaf(wt[f], v1 - v0, v0);
-              spot[s][f] += fmaf(vol, dw[f], fmaf(vol * vol, nhdt, c.c));
+            for (int s = 0; s < SMAX; ++s) {
+              if (s >= S) continue;
+              RIGA_LV_EULER(s)
             }
           }
+#undef RIGA_LV_EULER
         } else if (local_vol) {  // knots not evenly spaced: bracketing search
 #pragma unroll
           for (int s = 0; s < SMAX; ++s) {
@@ -922,6 +935,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
 #pragma unroll
           for (int f = 1; f < F; ++f) w = fminf(w, spot[s][f]);
           if (s == 0) w_first = w;
+          worst_last[s] = w;
           const bool alive = (pk[s] & 0xFFFFFu) == 0u;
           accrual[s] = (new_period ? 0.0f : accrual[s]) + (w >= it.ln_cb32 ? inc : 0.0f);
           const bool ko_now = alive && call_active && w >= it.ln_ko32;
@@ -1046,17 +1060,6 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     }
 
     // ---- path end (note.rs:221-258) and reductions
-    if constexpr (kLogState) {
-      // the FP32 tier's final worst-of, from the last step's log spots (steps >= 1),
-      // instead of carrying it through the step loop
-#pragma unroll
-      for (int s = 0; s < SMAX; ++s) {
-        float w = spot[s][0];
-#pragma unroll
-        for (int f = 1; f < F; ++f) w = fminf(w, spot[s][f]);
-        worst_last[s] = w;
-      }
-    }
 #pragma unroll
     for (int s = 0; s < SMAX; ++s) {
       if (s >= S) continue;
