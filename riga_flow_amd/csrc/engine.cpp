@@ -847,12 +847,12 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   std::vector<Prep> preps(n);
   for (int i = 0; i < n; ++i) {
     Error e;
-    if (!prepare(reqs[i], &preps[i], &e)) {
-      set_err(err, e.code, "request " + std::to_string(i) + ": " + e.message);
-      return RIGA_EINVAL;
-    }
     if (reqs[i].steps >= (int64_t)1 << 20) {  // KO steps are packed in 20 bits on the device
       set_err(err, "invalid_schedule", "request " + std::to_string(i) + ": the device takes at most 1,048,575 steps");
+      return RIGA_EINVAL;
+    }
+    if (!prepare(reqs[i], &preps[i], &e)) {
+      set_err(err, e.code, "request " + std::to_string(i) + ": " + e.message);
       return RIGA_EINVAL;
     }
   }

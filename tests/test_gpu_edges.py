@@ -104,6 +104,16 @@ def test_rejected_requests_name_the_reference_codes():
     assert e.value.code == "invalid_market"
 
 
+def test_step_counts_beyond_the_packed_event_width_are_refused():
+    # KO steps travel in 20 bits on the device: 2^20 steps and more are refused
+    # before anything is laid out, in both tiers, rather than mispriced
+    long = _flat(64, steps=1 << 20, years=4000.0)
+    for tier in ("fp64", "fp32"):
+        with pytest.raises(RigaError) as e:
+            api.price_put_leg(long, tier)
+        assert e.value.code == "invalid_schedule" and "1,048,575" in e.value.message
+
+
 def test_concurrent_callers_get_their_own_results():
     # the C ABI is called from several host threads at once (the reference's MCP
     # handlers run on a multi-threaded runtime): each call's result equals the
