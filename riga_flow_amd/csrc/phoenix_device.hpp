@@ -925,7 +925,9 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
           }
         }
         // Phase 2: worst-of, accrual and the daily KO with its payment as
-        // selects (note.rs:143-195); no divergent branch.
+        // selects (note.rs:143-195).  The payment runs only in waves where some
+        // lane of the scenario knocks out at this step (a few percent of wave
+        // steps; a wave-uniform branch): for every other lane it changes nothing.
         const float cr = (float)it.coupon_rate;
         const float df = (float)si.df;
         float w_first = 0.0f;  // scenario 0's worst-of, for replay series
@@ -939,13 +941,15 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
           const bool alive = (pk[s] & 0xFFFFFu) == 0u;
           accrual[s] = (new_period ? 0.0f : accrual[s]) + (w >= it.ln_cb32 ? inc : 0.0f);
           const bool ko_now = alive && call_active && w >= it.ln_ko32;
-          const float due = fixed_now ? cr : accrual[s];
-          const float carry = due > 0.0f ? (float)((pk[s] >> 20) & 0x7FFu) * cr : 0.0f;
-          const bool pay = ko_now && due + carry > 0.0f;
-          cpv[s] += pay ? due * df : 0.0f;
-          mpv[s] += pay ? carry * df : 0.0f;
-          const uint32_t recv = pay ? 0x80000000u : (pk[s] & 0x80000000u);
-          pk[s] = ko_now ? ((pk[s] & 0x7FF00000u) | recv | (uint32_t)(step + 1)) : pk[s];
+          if (__ballot(ko_now && s < S) != 0ull) {
+            const float due = fixed_now ? cr : accrual[s];
+            const float carry = due > 0.0f ? (float)((pk[s] >> 20) & 0x7FFu) * cr : 0.0f;
+            const bool pay = ko_now && due + carry > 0.0f;
+            cpv[s] += pay ? due * df : 0.0f;
+            mpv[s] += pay ? carry * df : 0.0f;
+            const uint32_t recv = pay ? 0x80000000u : (pk[s] & 0x80000000u);
+            pk[s] = ko_now ? ((pk[s] & 0x7FF00000u) | recv | (uint32_t)(step + 1)) : pk[s];
+          }
         }
         if (it.series != nullptr && valid)  // replay: the worst-of for the coupon strip
           it.series[(size_t)step * it.paths + path_local] = exp((double)w_first);
