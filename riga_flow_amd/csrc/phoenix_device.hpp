@@ -589,6 +589,11 @@ __global__ __launch_bounds__(256) void normal_table_kernel(const uint32_t* __res
 }
 
 // ---------------------------------------------------------------- the kernel
+// LDS-address-space views (32-bit addresses) for reads from a staged address.
+typedef __attribute__((address_space(3))) const unsigned char LdsBytes;
+typedef float F32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const F32x4 LdsF32x4;
+
 // kMode: how the items of the launch get their normals (kLadderDirect: Sobol
 // or Halton per lane; kLadderTable: the shared normal table; kLadderMonteCarlo:
 // the ChaCha8 stream).  Each mode compiles only its own ladder code.
@@ -645,7 +650,14 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   }
   if constexpr (sizeof(Real) == 4) {
     const SfF32* sfg = at<SfF32>(arena, it.off_sf32);
-    for (int i = tid; i < S * F; i += nthreads) sf_lds[i] = sfg[i];
+    // surface offsets are staged as absolute LDS addresses, so a slope-row read
+    // is one multiply-add of the knot index away from its address
+    const int32_t lv_base = (int32_t)(uint32_t)(uintptr_t)((const LdsBytes*)lds_w);
+    for (int i = tid; i < S * F; i += nthreads) {
+      SfF32 c = sfg[i];
+      c.lds_off += lv_base;
+      sf_lds[i] = c;
+    }
   }
   if constexpr (kMC) {
     const double* zg = at<double>(arena, it.off_zig);
@@ -858,7 +870,6 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
           for (int f = 0; f < F; ++f) wt[f] = at<float>(arena, it.off_wt32)[step * F + f];
         }
         const float inc = fixed_now ? 0.0f : (float)si.accrual_inc;
-        const unsigned char* lv_rows = reinterpret_cast<const unsigned char*>(lds);
         // Phase 1: the Euler step of every (scenario, factor).  The chains are
         // independent and the local-vol mode is chosen once per step, so the
         // factors of a scenario form one straight-line block the scheduler can
@@ -871,15 +882,16 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     const int ss_ = (s) < S ? (s) : S - 1;                                                 \
     _Pragma("unroll") for (int f = 0; f < F; ++f) {                                        \
       const SfF32 c = sf[ss_ * F + f];                                                     \
-      float t = fmaf(spot[s][f], c.a, c.b);                                                \
-      t = fminf(fmaxf(t, 0.0f), c.tmax);                                                   \
+      /* knot position clamped to [0, tmax] (one med3), its index and weight */            \
+      const float t = __builtin_amdgcn_fmed3f(fmaf(spot[s][f], c.a, c.b), 0.0f, c.tmax);   \
       const int j = (int)t;                                                                \
-      const float wx = t - (float)j;                                                       \
-      const float4 e = *reinterpret_cast<const float4*>(lv_rows + c.lds_off + j * 16);     \
+      const float wx = __builtin_amdgcn_fractf(t); /* = t - j exactly, t >= 0 */           \
+      const F32x4 e = *(const LdsF32x4*)(uintptr_t)(uint32_t)(c.lds_off + j * 16);     \
       const float v0 = fmaf(wx, e.y, e.x);                                                 \
       const float v1 = fmaf(wx, e.w, e.z);                                                 \
       const float vol = fmaf(wt[f], v1 - v0, v0);                                          \
-      spot[s][f] += fmaf(vol, dw[f], fmaf(vol * vol, nhdt, c.c));                          \
+      /* X += (r - q) dt + vol dW - vol^2 dt / 2 */                                        \
+      spot[s][f] += fmaf(vol, fmaf(vol, nhdt, dw[f]), c.c);                                \
     }                                                                                      \
   }
           if (F * SMAX <= 30 && S == SMAX) {

@@ -115,7 +115,7 @@ struct SfF32 {
   float b;          // local vol: knot position of ln(S/S0) = 0
   float c;          // local vol: (rate - q) dt; flat vol: (rate - q - vol^2/2) dt
   float tmax;       // largest float below n_x - 1 (knot positions are clamped to [0, tmax])
-  int32_t lds_off;  // byte offset of the surface's slope rows in the local-vol LDS area
+  int32_t lds_off;  // byte offset of the surface's slope rows in the local-vol LDS area (the kernel adds the area's offset in dynamic LDS)
   int32_t pad[3];
 };
 
