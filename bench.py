@@ -28,8 +28,8 @@ from riga_flow_amd import api, workloads as W  # noqa: E402
 # Algorithmic work of the code path each tier runs, in 32-bit lane operations
 # (an FP64 operation counts one: MI355X issues FP64 at the full vector rate).
 # Per generated coordinate:
-OWEN_ROUND = 27     # hash input xor 2, xorshift 4, 64-bit multiply 5, xorshift 4,
-                    # top word of the 2nd multiply 4, bit logic 3, prefix 1, prefix*C3 4
+OWEN_ROUND = 23     # hash input xor 2 (three-input), xorshift 2, 64-bit multiply 4,
+                    # xorshift 3, top bit of the 2nd multiply 5, bit logic 4, prefix*C3 update 3
 # hashed levels: the top 16 of the 32 come from the per-dimension prefix table; the FP32
 # tier hashes levels 15..8 (a float's 24 bits) and levels 7..0 only for coordinates
 # within 2^-10 of 0 or 1 (a fraction 2^-9 of them)
@@ -39,9 +39,14 @@ PROBIT = 22         # rational (12) + divide (8) + tails (5% of log + sqrt + rat
 UNIFORM = 4         # int->float, +0.5, scale, clamp
 # Per scenario and factor:
 LV_LOOKUP = {"fp64": 41,  # log 15, knot bracket 6, divide 8, three lerps 12
-             "fp32": 12}  # knot position 2, clamp 2, index 3, address 1, slope-row lerps 4
+             "fp32": 9}   # knot position 1, clamp 1 (med3), index 1, weight 1, address 1,
+                          # slope-row lerps 4
 EULER = {"fp64": 22,      # drift 4, increment 2, exp 15, multiply 1
-         "fp32": 6}       # drift 3, increment 2, X += incr (log state: no exp)
+         "fp32": 3}       # X += c + vol (dW - vol dt/2): two FMAs and an add (no exp)
+# Phoenix scan per scenario step: the FP64 tier's ratios to the initial spots (F divides),
+# worst-of, KO test, accrual and payment; the FP32 tier's worst-of on log spots (F - 1),
+# accrual 4, KO test 3, payment branch and its rare body (~2)
+SCAN = {"fp64": lambda F: 9 * F + 5, "fp32": lambda F: F + 10}
 VALU_PEAK = 256 * 4 * 16 * 2.4e9  # CUs x SIMDs x lanes x clock = 3.93e13 lane-ops/s
 HBM_PEAK = 8.0e12
 
@@ -50,7 +55,7 @@ def ops_per_path_step(F: int, S: int, nbits: int, local_vol: bool, tier: str) ->
     """Lane operations of one path advanced one step for all S scenarios."""
     coord = OWEN_ROUNDS[tier] * OWEN_ROUND + OWEN_TABLE + 2 * nbits + UNIFORM + PROBIT
     node = F * coord + F * (F + 1) + 3 * F + 2 * F  # coordinates, L z, bridge node, shock
-    scen = F * ((LV_LOOKUP[tier] if local_vol else 0) + EULER[tier]) + 9 * F + 5  # + worst/KO/coupon
+    scen = F * ((LV_LOOKUP[tier] if local_vol else 0) + EULER[tier]) + SCAN[tier](F)
     return node + S * scen
 
 
