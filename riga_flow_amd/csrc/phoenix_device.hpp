@@ -249,8 +249,6 @@ __device__ __forceinline__ float probit_f32(double p) {
   return probit_is_tail(p) ? probit_f32_tail(p) : probit_f32_central(p);
 }
 
-__device__ __forceinline__ double probit_tier(double p, double) { return probit_f64(p); }
-__device__ __forceinline__ float probit_tier(double p, float) { return probit_f32(p); }
 __device__ __forceinline__ double probit_tail_tier(double p, double) { return probit_f64_tail(p); }
 __device__ __forceinline__ float probit_tail_tier(double p, float) { return probit_f32_tail(p); }
 __device__ __forceinline__ double probit_central_tier(double p, double) { return probit_f64_central(p); }
