@@ -28,8 +28,9 @@ from riga_flow_amd import api, workloads as W  # noqa: E402
 # Algorithmic work of the code path each tier runs, in 32-bit lane operations
 # (an FP64 operation counts one: MI355X issues FP64 at the full vector rate).
 # Per generated coordinate:
-OWEN_ROUND = 23     # hash input xor 2 (three-input), xorshift 2, 64-bit multiply 4,
-                    # xorshift 3, top bit of the 2nd multiply 5, bit logic 4, prefix*C3 update 3
+OWEN_ROUND = 21     # hash input xor 2 (three-input), xorshift 2, 64-bit multiply 4,
+                    # xorshift 3, top bit of the 2nd multiply 5, scrambled bit 2, prefix*C3 update 3
+                    # (the prefix itself is one multiply of P by C3^-1 after the last level)
 # hashed levels: the top 16 of the 32 come from the per-dimension prefix table; the FP32
 # tier hashes levels 15..8 (a float's 24 bits) and levels 7..0 only for coordinates
 # within 2^-10 of 0 or 1 (a fraction 2^-9 of them)

@@ -152,10 +152,9 @@ __global__ __launch_bounds__(256) void owen_table_kernel(uint64_t seed, int dims
   if (v >= (1u << kOwenTableBits) || (int)dim >= dims) return;
   const uint64_t key = seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
   const uint32_t x = v << (32 - kOwenTableBits);
-  uint32_t prefix = 0;
   uint64_t P = 0;
-  for (int bit = 31; bit >= 32 - kOwenTableBits; --bit) owen_level(x, key, bit, prefix, P);
-  out[(size_t)dim * (1u << kOwenTableBits) + v] = (uint16_t)prefix;
+  for (int bit = 31; bit >= 32 - kOwenTableBits; --bit) owen_level(x, key, bit, P);
+  out[(size_t)dim * (1u << kOwenTableBits) + v] = (uint16_t)owen_prefix(P);
 }
 
 hipError_t launch_owen_table(uint64_t seed, int dims, uint16_t* out, hipStream_t stream) {

@@ -40,7 +40,15 @@ constexpr uint64_t kOwenC2 = 0xD6E8FEB86659FD93ull;
 constexpr uint64_t kOwenC3 = 0xA24BAED4963EE407ull;
 constexpr int kOwenTableBits = kOwenPrefixBits;  // levels read from the per-dimension prefix table
 
-__device__ __forceinline__ void owen_level(uint32_t x, uint64_t key, int bit, uint32_t& prefix, uint64_t& P) {
+// The scrambled prefix is not carried level by level: C3 is odd, so
+// prefix = P * C3^-1 (mod 2^64), and its low 32 bits -- all of a prefix -- are
+// P_lo * (C3_lo)^-1 (mod 2^32), one multiply after the last level.
+constexpr uint32_t kOwenC3InvLo = 0xF69349B7u;  // (C3 mod 2^32)^-1 mod 2^32
+static_assert((uint32_t)kOwenC3 * kOwenC3InvLo == 1u, "C3 inverse");
+
+__device__ __forceinline__ uint32_t owen_prefix(uint64_t P) { return (uint32_t)P * kOwenC3InvLo; }
+
+__device__ __forceinline__ void owen_level(uint32_t x, uint64_t key, int bit, uint64_t& P) {
   constexpr uint64_t C4 = 0xBF58476D1CE4E5B9ull;
   constexpr uint64_t C5 = 0x94D049BB133111EBull;
   // z = h ^ (h >> 30) for h = kb ^ P, as (kb ^ kb>>30) ^ P ^ (P>>30): the key
@@ -54,43 +62,39 @@ __device__ __forceinline__ void owen_level(uint32_t x, uint64_t key, int bit, ui
   z *= C4;
   z ^= z >> 27;
   const uint32_t top = (uint32_t)((z * C5) >> 63);
-  const uint32_t s = ((x >> bit) & 1u) ^ top;
-  prefix = (prefix << 1) | s;
-  P = (P << 1) + (s ? kOwenC3 : 0ull);
+  const bool s = ((x >> bit) & 1u) != top;  // the scrambled bit
+  P = (P << 1) + (s ? kOwenC3 : 0ull);      // prefix' = 2 prefix + s
 }
 
 __device__ __forceinline__ uint32_t owen_scramble(uint32_t x, uint64_t key) {
-  uint32_t prefix = 0;
   uint64_t P = 0;
 #pragma unroll 4
-  for (int bit = 31; bit >= 0; --bit) owen_level(x, key, bit, prefix, P);
-  return prefix;
+  for (int bit = 31; bit >= 0; --bit) owen_level(x, key, bit, P);
+  return owen_prefix(P);
 }
 
 // The same scramble with its first kOwenTableBits levels read from a table:
 // the top L output bits depend only on the top L bits of x (each level hashes
 // the already scrambled prefix), so tab[x >> (32-L)] = owen(x)[31:32-L] for the
-// dimension, and the remaining levels continue from prefix = that value,
-// P = prefix * C3.
+// dimension, and the remaining levels continue from P = that prefix * C3.
 template <bool kFullUnroll>
 __device__ __forceinline__ uint32_t owen_scramble_below(uint32_t x, uint64_t key, uint32_t top) {
-  uint32_t prefix = top;
   uint64_t P = (uint64_t)top * kOwenC3;
   if constexpr (kFullUnroll) {
 #pragma unroll
-    for (int bit = 31 - kOwenTableBits; bit >= 0; --bit) owen_level(x, key, bit, prefix, P);
+    for (int bit = 31 - kOwenTableBits; bit >= 0; --bit) owen_level(x, key, bit, P);
   } else {
 #pragma unroll 4
-    for (int bit = 31 - kOwenTableBits; bit >= 0; --bit) owen_level(x, key, bit, prefix, P);
+    for (int bit = 31 - kOwenTableBits; bit >= 0; --bit) owen_level(x, key, bit, P);
   }
-  return prefix;
+  return owen_prefix(P);
 }
 
-// Levels kHi..kLo of the scramble, continuing from (prefix, P).
+// Levels kHi..kLo of the scramble, continuing from P.
 template <int kHi, int kLo>
-__device__ __forceinline__ void owen_levels(uint32_t x, uint64_t key, uint32_t& prefix, uint64_t& P) {
+__device__ __forceinline__ void owen_levels(uint32_t x, uint64_t key, uint64_t& P) {
 #pragma unroll
-  for (int bit = kHi; bit >= kLo; --bit) owen_level(x, key, bit, prefix, P);
+  for (int bit = kHi; bit >= kLo; --bit) owen_level(x, key, bit, P);
 }
 
 // The FP32 tier's scramble.  A float carries 24 significant bits, so the tier
@@ -105,16 +109,14 @@ constexpr uint32_t kF32TailCells = 1u << 14;  // 24-bit cells below u = 2^-10
 // Top 24 levels; `top` is the prefix-table entry for x when `has_table`.
 __device__ __forceinline__ uint32_t owen_top24(uint32_t x, uint64_t key, bool has_table, uint32_t top) {
   if (has_table) {
-    uint32_t prefix = top;
     uint64_t P = (uint64_t)top * kOwenC3;
-    owen_levels<31 - kOwenTableBits, 8>(x, key, prefix, P);
-    return prefix;
+    owen_levels<31 - kOwenTableBits, 8>(x, key, P);
+    return owen_prefix(P);
   }
-  uint32_t prefix = 0;
   uint64_t P = 0;
 #pragma unroll 4
-  for (int bit = 31; bit >= 8; --bit) owen_level(x, key, bit, prefix, P);
-  return prefix;
+  for (int bit = 31; bit >= 8; --bit) owen_level(x, key, bit, P);
+  return owen_prefix(P);
 }
 
 __device__ __forceinline__ bool owen_f32_tail(uint32_t p24) {
@@ -122,10 +124,9 @@ __device__ __forceinline__ bool owen_f32_tail(uint32_t p24) {
 }
 
 __device__ __forceinline__ uint32_t owen_low8(uint32_t x, uint64_t key, uint32_t p24) {
-  uint32_t prefix = p24;
   uint64_t P = (uint64_t)p24 * kOwenC3;
-  owen_levels<7, 0>(x, key, prefix, P);
-  return prefix;
+  owen_levels<7, 0>(x, key, P);
+  return owen_prefix(P);
 }
 
 __device__ __forceinline__ uint32_t owen_f32(uint32_t x, uint64_t key, const uint16_t* __restrict__ tab_row) {
