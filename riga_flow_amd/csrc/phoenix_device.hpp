@@ -668,6 +668,10 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   // not reload it through the scalar cache (whose wait also stalls LDS reads)
   float nhdt = it.neg_half_dt;
   asm volatile("" : "+v"(nhdt));
+  // the FP32 scan's barriers and coupon rate, held the same way: reloading them
+  // per scenario through the scalar cache would stall on every in-flight LDS read
+  float ln_ko32 = it.ln_ko32, ln_cb32 = it.ln_cb32, cr32 = (float)it.coupon_rate;
+  asm volatile("" : "+v"(ln_ko32), "+v"(ln_cb32), "+v"(cr32));
 
   const int ppb = it.paths_per_block;
   const int batches = ppb / nthreads;
@@ -939,7 +943,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         // selects (note.rs:143-195).  The payment runs only in waves where some
         // lane of the scenario knocks out at this step (a few percent of wave
         // steps; a wave-uniform branch): for every other lane it changes nothing.
-        const float cr = (float)it.coupon_rate;
+        const float cr = cr32;
         const float df = (float)si.df;
         float w_first = 0.0f;  // scenario 0's worst-of, for replay series
 #pragma unroll
@@ -950,8 +954,8 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
           if (s == 0) w_first = w;
           worst_last[s] = w;
           const bool alive = (pk[s] & 0xFFFFFu) == 0u;
-          accrual[s] = (new_period ? 0.0f : accrual[s]) + (w >= it.ln_cb32 ? inc : 0.0f);
-          const bool ko_now = alive && call_active && w >= it.ln_ko32;
+          accrual[s] = (new_period ? 0.0f : accrual[s]) + (w >= ln_cb32 ? inc : 0.0f);
+          const bool ko_now = alive & call_active & (w >= ln_ko32);  // no short-circuit branch
           if (__ballot(ko_now && s < S) != 0ull) {
             const float due = fixed_now ? cr : accrual[s];
             const float carry = due > 0.0f ? (float)((pk[s] >> 20) & 0x7FFu) * cr : 0.0f;
