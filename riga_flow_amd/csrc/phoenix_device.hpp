@@ -672,6 +672,25 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   // per scenario through the scalar cache would stall on every in-flight LDS read
   float ln_ko32 = it.ln_ko32, ln_cb32 = it.ln_cb32, cr32 = (float)it.coupon_rate;
   asm volatile("" : "+v"(ln_ko32), "+v"(ln_cb32), "+v"(cr32));
+  // and the FP64 step's constants (its kernels have registers to spare: their
+  // bridge stack in LDS holds them at two workgroups per CU)
+  double dt64 = 0.0, sqdt64 = 0.0, ko64 = 0.0, cb64 = 0.0, cr64 = 0.0;
+  double s0_64[F];
+#pragma unroll
+  for (int f = 0; f < F; ++f) s0_64[f] = 1.0;
+  if constexpr (!kLogState) {
+    dt64 = it.dt;
+    sqdt64 = it.sqrt_dt;
+    ko64 = it.ko;
+    cb64 = it.cb;
+    cr64 = it.coupon_rate;
+    asm volatile("" : "+v"(dt64), "+v"(sqdt64), "+v"(ko64), "+v"(cb64), "+v"(cr64));
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      s0_64[f] = it.initial_spot[f];
+      asm volatile("" : "+v"(s0_64[f]));
+    }
+  }
 
   const int ppb = it.paths_per_block;
   const int batches = ppb / nthreads;
@@ -1013,8 +1032,8 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
               vol = lv_sigma<Real, false>(lds + (size_t)u * kSurfWords * nxs, su.n_x, (Real)su.inv_dx,
                                           (Real)lvb[step * F + f].wt, spot[s][f]);
             }
-            const Real drift = ((Real)sc.rq[f] - Real(0.5) * vol * vol) * (Real)it.dt;
-            incr = drift + vol * (Real)it.sqrt_dt * shock[f];
+            const Real drift = ((Real)sc.rq[f] - Real(0.5) * vol * vol) * (Real)dt64;
+            incr = drift + vol * (Real)sqdt64 * shock[f];
           } else {
             incr = (Real)sc.drift[f] + (Real)sc.vsd[f] * shock[f];
           }
@@ -1030,15 +1049,15 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
 #pragma unroll
           for (int f = 1; f < F; ++f) worst = fminf(worst, spot[s][f]);
         } else {
-          worst = spot[s][0] / (Real)it.initial_spot[0];
+          worst = spot[s][0] / (Real)s0_64[0];
 #pragma unroll
-          for (int f = 1; f < F; ++f) worst = fmin(worst, spot[s][f] / (Real)it.initial_spot[f]);
+          for (int f = 1; f < F; ++f) worst = fmin(worst, spot[s][f] / (Real)s0_64[f]);
         }
         worst_last[s] = worst;
         if (ko_step[s] >= 0) continue;  // note.rs:146-148: cash flows stop after KO
         // ---- coupon accrual (note.rs:151-163)
         if (new_period) accrual[s] = Real(0);
-        const bool above_coupon = kLogState ? worst >= (Real)it.ln_cb : (double)worst >= it.cb;
+        const bool above_coupon = kLogState ? worst >= (Real)it.ln_cb : (double)worst >= cb64;
         if (!fixed_now && above_coupon) accrual[s] += (Real)si.accrual_inc;
         // ---- daily global KO (note.rs:166-195): every name at or above the
         // barrier, i.e. the worst-of at or above it
@@ -1048,11 +1067,11 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         } else {
 #pragma unroll
           for (int f = 0; f < F; ++f)
-            all_above = all_above && ((double)(spot[s][f] / (Real)it.initial_spot[f]) >= it.ko);
+            all_above = all_above && ((double)(spot[s][f] / (Real)s0_64[f]) >= ko64);
         }
         if (all_above) {
           ko_step[s] = step;
-          const Real due = fixed_now ? (Real)it.coupon_rate : accrual[s];
+          const Real due = fixed_now ? (Real)cr64 : accrual[s];
           const Real carry = due > Real(0) ? unpaid[s] : Real(0);
           if (due + carry > Real(0)) {
             cpv[s] += due * (Real)si.df;
@@ -1063,14 +1082,14 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         }
         // ---- period end (note.rs:197-218)
         if (period_end) {
-          const Real due = fixed_now ? (Real)it.coupon_rate : accrual[s];
+          const Real due = fixed_now ? (Real)cr64 : accrual[s];
           if (due > Real(0)) {
             cpv[s] += due * (Real)si.df;
             mpv[s] += unpaid[s] * (Real)si.df;
             received[s] = true;
             unpaid[s] = Real(0);
           } else if (it.flags & kMemoryCoupon) {
-            unpaid[s] += (Real)it.coupon_rate;
+            unpaid[s] += (Real)cr64;
           }
         }
       }
