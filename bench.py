@@ -156,8 +156,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (shape 1 x 4 waves)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: the hot path shards by path range with a single "
-              f"partial merge (see DESIGN.md); --gpus {args.gpus} is not run here.",
+        print(f"bench.py measures one GPU: the hot path shards by contiguous path range "
+              f"(Sobol point = path + 1) with one merge of the per-block double-double "
+              f"partials, so --gpus {args.gpus} is not run here.",
               file=sys.stderr)
         sys.exit(2)
 
