@@ -138,14 +138,19 @@ __device__ __forceinline__ uint32_t owen_f32(uint32_t x, uint64_t key, const uin
 // selected by the gray code of n.  `row` is wave-uniform (32 words, read as
 // scalars); each bit is a sign-extended mask and one three-input logic op, with
 // no branch and no per-bit load.  Bits of the gray code at or above nbits are
-// zero, so a fixed 24- or 32-bit sweep gives the same XOR.
+// zero, so a fixed 21-, 24- or 32-bit sweep gives the same XOR.
 __device__ __forceinline__ uint32_t sobol_x(uint32_t gray, const uint32_t* __restrict__ row, int nbits) {
   // opaque copy: the masks are rebuilt here (one bit-field extract each)
   // instead of 32 lane masks being held across the whole path
   asm volatile("" : "+v"(gray));
   // bitop3 table 0x6c: f(a, b, c) = b ^ (a & c), i.e. x ^= mask & row[k]
   uint32_t x = 0;
-  if (nbits <= 24) {
+  if (nbits <= 21) {
+    // up to 2^21 - 1 points (2 Mi paths per item)
+#pragma unroll
+    for (int k = 0; k < 21; ++k)
+      x = __builtin_amdgcn_bitop3_b32((uint32_t)__builtin_amdgcn_sbfe((int)gray, k, 1), x, row[k], 0x6c);
+  } else if (nbits <= 24) {
 #pragma unroll
     for (int k = 0; k < 24; ++k)
       x = __builtin_amdgcn_bitop3_b32((uint32_t)__builtin_amdgcn_sbfe((int)gray, k, 1), x, row[k], 0x6c);
