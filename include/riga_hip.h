@@ -138,7 +138,12 @@ typedef struct riga_launch_config {
                                when they need 4x more normals than it holds); 1: force; -1: off */
   int32_t stream_ordered_alloc; /* 1: plan memory from the stream-ordered pool (serving) */
   int32_t waves_per_block;  /* 1-4 waves of 64 lanes per workgroup; 0: autotuned, else 4 */
-  int32_t reserved[2];
+  int32_t serial_launches;  /* 0: a batch's launches (one per factor count, scenario bucket and
+                               ladder mode) run concurrently on streams of their own; 1: one
+                               after another on the plan's stream */
+  int32_t fp32_full_ladder; /* FP32 tier: 1 hashes all 32 Owen levels of every coordinate (the FP64
+                               tier's integers) instead of the top 24 plus the cell middle; no
+                               shared normal table then.  For pricing what the 24-level rule costs */
 } riga_launch_config;
 
 /* A batch laid out in device memory once (surfaces, factors, schedules, Sobol
@@ -255,6 +260,9 @@ int riga_server_submit(riga_server *server, const riga_request *reqs, int n, int
 int riga_server_wait(riga_server *server, uint64_t ticket, double timeout_s, riga_result *results,
                      riga_ticket_info *info, riga_error *err);
 int riga_server_stats_get(riga_server *server, riga_server_stats *out);
+/* Finishes the running plans, sheds the queued tickets (code "shed"), wakes every
+ * caller blocked in riga_server_wait and returns once they have left.  No call on
+ * the server may start after destroy has begun. */
 void riga_server_destroy(riga_server *server);
 
 /* ---- replay (explain.rs:90-316): per-path records for paths [0, req->paths) ---- */
@@ -317,13 +325,32 @@ int riga_debug_sobol(riga_ctx *ctx, uint64_t seed, int32_t ladder_dims, uint64_t
                      uint32_t *out_unscrambled, uint32_t *out_scrambled, double *out_normal,
                      riga_error *err);
 
+/* The FP64 tier's exp and log as its path kernels evaluate them (the Euler step,
+ * the local-vol lookup, the inverse-normal tails, the ziggurat tests): out_exp[i] =
+ * exp(x[i]), out_log[i] = log(x[i]), n <= 2^28.  No counterpart in the reference
+ * (a diagnostic for the parity tests: the oracle builds the same functions). */
+int riga_debug_exp_log(riga_ctx *ctx, const double *x, uint64_t n, double *out_exp, double *out_log,
+                       riga_error *err);
+
+/* The Halton ladder (qmc.rs:94-153) as the path kernels evaluate it, for points
+ * n = first_point + p (1-based, qmc.rs:121) and dims first_dim + d (the base and
+ * permutation of dim % ladder_dims): out_digits[(p*n_dims + d)*32 + k] is the
+ * permuted k-th base-b digit perm[digit_k(n)] (least significant first,
+ * 0xFFFFFFFF after the last), out_uniform the radical inverse and out_normal its
+ * FP64 inverse normal.  ladder_dims <= 7762.  No counterpart in the reference (a
+ * diagnostic for the parity tests). */
+int riga_debug_halton(riga_ctx *ctx, uint64_t seed, int32_t ladder_dims, uint64_t first_point, int32_t n_points,
+                      int32_t first_dim, int32_t n_dims, uint32_t *out_digits, double *out_uniform,
+                      double *out_normal, riga_error *err);
+
 /* The FP32 tier's Sobol ladder as its path kernels run it, for points
  * n = first_point + p (1-based) and dims node*factors + f, node < n_nodes:
  * out_scrambled[p][dim] is the integer the tier turns into its uniform -- the
  * top 24 Owen levels with the low byte 0x80, or all 32 levels where the top 10
  * scrambled bits are all 0 or all 1 (u within 2^-10 of 0 or 1) -- and
- * out_normal[p][dim] its FP32 inverse normal.  owen_table != 0 reads the top
- * levels from the context's prefix table for the seed, as priced batches do.
+ * out_normal[p][dim] its FP32 inverse normal.  owen_table bit 0 reads the top
+ * levels from the context's prefix table for the seed, as priced batches do;
+ * bit 1 hashes all 32 levels of every coordinate (the fp32_full_ladder option).
  * No counterpart in the reference (a diagnostic for the parity tests). */
 int riga_debug_sobol_f32(riga_ctx *ctx, uint64_t seed, int32_t ladder_dims, uint64_t first_point,
                          int32_t n_points, int32_t n_nodes, int32_t factors, int32_t owen_table,
@@ -357,8 +384,9 @@ int riga_resident_worst_of_synchronize(riga_resident_worst_of *h, riga_error *er
 void riga_resident_worst_of_destroy(riga_resident_worst_of *h);
 
 /* gpu.rs:207-278, :538-566 lv_terminal_spots (kernel gpu.rs:37-75): two factors, shocks
- * normals[paths][steps][2], local-vol rows[2][steps][gx], evenly spaced log-spot axes
- * ls_axis[2][gx]; outputs terminal spots [paths][2] and the first step both names
+ * normals[paths][steps][2], local-vol rows[2][steps][gx] (2 <= gx <= 3072), evenly spaced
+ * log-spot axes ls_axis[2][gx] (only their end knots are read, as in the reference);
+ * outputs terminal spots [paths][2] and the first step both names
  * stand at or above call_barrier x reference (or -1). */
 int riga_lv_terminal_spots(riga_ctx *ctx, const float *normals, const float *rows, const float *ls_axis,
                            int32_t gx, const float *s0, const float *references, float call_barrier,

@@ -180,8 +180,34 @@ def lib():
         L.ro_std_normals.restype = C.c_uint64
         L.ro_std_normals.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, _D]
         L.ro_draw_starts.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.ro_set_device_math.argtypes = [C.c_int]
+        L.ro_set_device_math.restype = None
+        L.ro_get_device_math.restype = C.c_int
+        for name in ("ro_shared_exp", "ro_shared_log"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [C.c_double]
         _LIB = L
     return _LIB
+
+
+# exp and log inside the path: False = glibc, the reference's own (the default);
+# True = the device FP64 tier's table-driven pair (riga_flow_amd/csrc/exp_log.h),
+# for comparing device paths bit for bit.  Applied per call, in the calling thread.
+DEVICE_MATH = False
+
+
+def _math(device_math=None):
+    lib().ro_set_device_math(1 if (DEVICE_MATH if device_math is None else device_math) else 0)
+
+
+def shared_exp(x: float) -> float:
+    """The device FP64 tier's exp (exp_log.h), built on the host."""
+    return lib().ro_shared_exp(x)
+
+
+def shared_log(x: float) -> float:
+    """The device FP64 tier's log (exp_log.h), built on the host."""
+    return lib().ro_shared_log(x)
 
 
 def _darr(values) -> np.ndarray:
@@ -194,6 +220,7 @@ def _dptr(a: np.ndarray):
 
 # ---------------------------------------------------------------- primitives
 def probit(p: float) -> float:
+    _math()
     return lib().ro_probit(p)
 
 
@@ -247,6 +274,7 @@ class Sobol:
         return lib().ro_sobol_uniform(C.byref(self._s), n, dim)
 
     def normal(self, point: int, dim: int) -> float:
+        _math()
         return lib().ro_sobol_normal(C.byref(self._s), point, dim)
 
     def __del__(self):
@@ -279,6 +307,7 @@ class Halton:
         return lib().ro_halton_uniform(C.byref(self._h), n, dim)
 
     def normal(self, point: int, dim: int) -> float:
+        _math()
         return lib().ro_halton_normal(C.byref(self._h), point, dim)
 
     def __del__(self):
@@ -318,6 +347,7 @@ def zig_tables():
 def std_normals(seed: int, n: int, start: int = 0):
     """n ``StandardNormal`` draws from u64 index ``start``; returns (draws, next index)."""
     out = np.zeros(max(n, 1))
+    _math()
     end = lib().ro_std_normals(seed, start, n, _dptr(out))
     return out[:n], end
 
@@ -325,6 +355,7 @@ def std_normals(seed: int, n: int, start: int = 0):
 def draw_starts(seed: int, n: int, start: int = 0) -> np.ndarray:
     """u64 index at which each of n consecutive ``StandardNormal`` draws starts."""
     out = np.zeros(max(n, 1), dtype=np.uint64)
+    _math()
     lib().ro_draw_starts(seed, start, n, out.ctypes.data_as(C.POINTER(C.c_uint64)))
     return out[:n]
 
@@ -333,6 +364,7 @@ def correlated_block(seed: int, paths: int, times: int, correlation) -> np.ndarr
     """pathcube.rs:47-76 ``generate_correlated_block``: float32 [paths][times][factors]."""
     L = cholesky(correlation)
     F = L.shape[0]
+    _math()
     out = np.zeros((paths, times, F), dtype=np.float32)
     lib().ro_correlated_block(seed, paths, times, F, _dptr(L), out.ctypes.data_as(C.POINTER(C.c_float)))
     return out
@@ -553,9 +585,10 @@ def _pack(request, max_dim: int, lvs: Optional[List[LocalVol]]):
 
 
 def price(request, max_dim: int = 512, path_range=None, records: bool = False,
-          lvs: Optional[List[LocalVol]] = None, ledger: bool = False):
+          lvs: Optional[List[LocalVol]] = None, ledger: bool = False, device_math=None):
     """``price_full_phoenix`` (note.rs:38-373) on the QMC lanes and the ChaCha8
-    Monte Carlo lane (``variance_reduction.kind == "none"``).
+    Monte Carlo lane (``variance_reduction.kind == "none"``).  ``device_math``
+    chooses the path's exp/log (None: the module's ``DEVICE_MATH``).
 
     Returns ``OracleResult`` (and a numpy record array when ``records``, and the
     coupon strips ``[path][period]`` when ``ledger``, explain.rs:146-262)."""
@@ -569,6 +602,7 @@ def price(request, max_dim: int = 512, path_range=None, records: bool = False,
     rows = (CouponRow * max((e - b) * periods, 1))() if ledger else None
     code = C.create_string_buffer(64)
     msg = C.create_string_buffer(256)
+    _math(device_math)
     rc = lib().ro_price(C.byref(r), b, e, C.byref(res), recs, rows, code, 64, msg, 256)
     if rc:
         raise OracleError(code.value.decode(), msg.value.decode())
@@ -597,6 +631,7 @@ def validate(request, max_dim: int = 512):
 def path_shocks(request, path_id: int, max_dim: int = 512) -> np.ndarray:
     r, keep = _pack(request, max_dim, None)
     F, T = request.factors, request.trade.steps
+    _math()
     out = np.zeros(F * T)
     if lib().ro_path_shocks(C.byref(r), path_id, _dptr(out)):
         raise OracleError("invalid_market", "path_shocks failed")

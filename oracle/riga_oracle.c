@@ -30,6 +30,24 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "../riga_flow_amd/csrc/exp_log.h"
+
+/* ------------------------------------------------------------------------- */
+/* exp and log inside the path (Euler step, local-vol lookup, inverse-normal   */
+/* tails, ziggurat tests): glibc's, as the reference's f64::exp / f64::ln     */
+/* call them, or -- per calling thread -- the table-driven exp_log.h pair the */
+/* device's FP64 tier evaluates, so that device and oracle paths agree bit for */
+/* bit.  Everything computed once per request (discount factors, calibration) */
+/* uses glibc in both modes, as the device's host side does.                  */
+/* ------------------------------------------------------------------------- */
+static _Thread_local int ro_device_math = 0;
+void ro_set_device_math(int on) { ro_device_math = on != 0; }
+int ro_get_device_math(void) { return ro_device_math; }
+double ro_shared_exp(double x) { return riga_exp(x); }
+double ro_shared_log(double x) { return riga_log(x); }
+static double path_exp(double x) { return ro_device_math ? riga_exp(x) : exp(x); }
+static double path_log(double x) { return ro_device_math ? riga_log(x) : log(x); }
+
 /* ------------------------------------------------------------------------- */
 /* probit: Acklam's rational approximation, qmc.rs:37-91                      */
 /* ------------------------------------------------------------------------- */
@@ -51,7 +69,7 @@ double ro_probit(double p) {
   const double plow = 0.02425;
   const double phigh = 1.0 - plow;
   if (p < plow) {
-    double q = sqrt(-2.0 * log(p));
+    double q = sqrt(-2.0 * path_log(p));
     return (((((PC[0] * q + PC[1]) * q + PC[2]) * q + PC[3]) * q + PC[4]) * q + PC[5]) /
            ((((PD[0] * q + PD[1]) * q + PD[2]) * q + PD[3]) * q + 1.0);
   } else if (p <= phigh) {
@@ -60,7 +78,7 @@ double ro_probit(double p) {
     return (((((PA[0] * r + PA[1]) * r + PA[2]) * r + PA[3]) * r + PA[4]) * r + PA[5]) * q /
            (((((PB[0] * r + PB[1]) * r + PB[2]) * r + PB[3]) * r + PB[4]) * r + 1.0);
   } else {
-    double q = sqrt(-2.0 * log(1.0 - p));
+    double q = sqrt(-2.0 * path_log(1.0 - p));
     return -(((((PC[0] * q + PC[1]) * q + PC[2]) * q + PC[3]) * q + PC[4]) * q + PC[5]) /
            ((((PD[0] * q + PD[1]) * q + PD[2]) * q + PD[3]) * q + 1.0);
   }
@@ -708,7 +726,7 @@ static void t_bracket(const ro_lv *lv, double t, int *i0, int *i1, double *wt) {
 }
 
 double ro_lv_sigma(const ro_lv *lv, double t, double s) { /* lib.rs:305-343 */
-  double x = log(fmax(s, 1e-12));
+  double x = path_log(fmax(s, 1e-12));
   int n = lv->n_x, lo, hi;
   if (x <= lv->log_s[0]) {
     lo = hi = 0;
@@ -958,12 +976,12 @@ static double std_normal(chacha_rng *g) {
       while (-2.0 * ty < tx * tx) {
         double x_ = open01(g);
         double y_ = open01(g);
-        tx = log(x_) / ZIG_R;
-        ty = log(y_);
+        tx = path_log(x_) / ZIG_R;
+        ty = path_log(y_);
       }
       return u < 0.0 ? tx - ZIG_R : ZIG_R - tx;
     }
-    if (zig_f[i + 1] + (zig_f[i] - zig_f[i + 1]) * uniform53(g) < zig_pdf(x)) return x;
+    if (zig_f[i + 1] + (zig_f[i] - zig_f[i + 1]) * uniform53(g) < path_exp(-x * x / 2.0)) return x;
   }
 }
 
@@ -1179,7 +1197,7 @@ int ro_price(const ro_request *r, uint64_t path_begin, uint64_t path_end, ro_res
       for (int f = 0; f < F; ++f) {
         double vol = r->lv ? ro_lv_sigma(&r->lv[f], (double)step * dt, spots[f]) : r->vols[f];
         double drift = (rate - r->dividend_yields[f] - 0.5 * vol * vol) * dt;
-        spots[f] *= exp(drift + vol * sqrt(dt) * incr[f * steps + step]);
+        spots[f] *= path_exp(drift + vol * sqrt(dt) * incr[f * steps + step]);
       }
       double worst = spots[0] / r->initial_spots[0];
       for (int f = 1; f < F; ++f) worst = fmin(worst, spots[f] / r->initial_spots[f]);

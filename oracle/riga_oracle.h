@@ -214,6 +214,13 @@ int ro_price(const ro_request *r, uint64_t path_begin, uint64_t path_end, ro_res
  * skipped, as the reference's single stream consumes them). */
 int ro_path_shocks(const ro_request *r, uint64_t path_id, double *out);
 
+/* exp and log inside the path: 0 = glibc (the reference's), 1 = the device FP64
+ * tier's table-driven pair (riga_flow_amd/csrc/exp_log.h); per calling thread. */
+void ro_set_device_math(int on);
+int ro_get_device_math(void);
+double ro_shared_exp(double x);
+double ro_shared_log(double x);
+
 #ifdef __cplusplus
 }
 #endif

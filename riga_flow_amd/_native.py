@@ -17,7 +17,9 @@ import numpy as np
 from .types import RigaError
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libriga_hip.so")
+# RIGA_HIP_LIB names another build of the library (e.g. one built with other
+# compile-time options); by default the in-tree one is loaded.
+LIB_PATH = os.environ.get("RIGA_HIP_LIB") or os.path.join(HERE, "libriga_hip.so")
 TRACE_CAP = 64
 
 FP64 = 0
@@ -116,7 +118,7 @@ class LaunchConfig(C.Structure):
     _fields_ = [("paths_per_thread", C.c_int32), ("use_graph", C.c_int32),
                 ("owen_table", C.c_int32), ("normal_table", C.c_int32),
                 ("stream_ordered_alloc", C.c_int32), ("waves_per_block", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("serial_launches", C.c_int32), ("fp32_full_ladder", C.c_int32)]
 
 
 class PlanInfo(C.Structure):
@@ -166,7 +168,7 @@ EXPORTS = ("riga_hip_init", "riga_hip_destroy", "riga_hip_device_name", "riga_hi
            "riga_server_stats_get", "riga_server_destroy",
            "riga_surface_create", "riga_surface_destroy", "riga_autotune",
            "riga_lv_calibrate", "riga_lv_sigma", "riga_debug_sobol", "riga_debug_sobol_f32",
-           "riga_debug_mc_starts",
+           "riga_debug_halton", "riga_debug_exp_log", "riga_debug_mc_starts",
            "riga_replay_ledger", "riga_query_worst_percentile", "riga_path_cube",
            "riga_worst_of", "riga_resident_worst_of_create", "riga_resident_worst_of_reduce",
            "riga_resident_worst_of_download", "riga_resident_worst_of_synchronize",
@@ -247,6 +249,10 @@ def lib():
                                                C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                                C.POINTER(C.c_uint32), C.POINTER(C.c_float),
                                                C.POINTER(Error)]
+            L.riga_debug_halton.argtypes = [C.c_void_p, C.c_uint64, C.c_int32, C.c_uint64,
+                                            C.c_int32, C.c_int32, C.c_int32,
+                                            C.POINTER(C.c_uint32), _D, _D, C.POINTER(Error)]
+            L.riga_debug_exp_log.argtypes = [C.c_void_p, _D, C.c_uint64, _D, _D, C.POINTER(Error)]
             L.riga_debug_mc_starts.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64,
                                                C.POINTER(C.c_uint64), C.POINTER(Error)]
             _U64 = C.POINTER(C.c_uint64)

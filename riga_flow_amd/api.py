@@ -279,7 +279,7 @@ def result_to_json(res: PriceResult) -> dict:
 def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",
                 paths_per_thread: int = 0, max_dimension: int = 512,
                 raw: bool = False, owen_table: bool = True, normal_table: int = 0,
-                waves_per_block: int = 0):
+                waves_per_block: int = 0, fp32_full_ladder: bool = False):
     """Price many requests in one launch; CRN scenarios share their Sobol points."""
     gpu_backend()
     for q in requests:
@@ -289,7 +289,7 @@ def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",
     arr = (N.Request * n)(*[p.r for p in packed])
     res = (N.Result * n)()
     cfg = N.LaunchConfig(paths_per_thread, 0, 0 if owen_table else -1, normal_table, 0,
-                         waves_per_block)
+                         waves_per_block, 0, 1 if fp32_full_ladder else 0)
     err = N.Error()
     dev = N.Device.get()
     N.check(N.lib().riga_price_batch(dev.handle, arr, n, PRECISIONS[precision], C.byref(cfg), res,
@@ -326,7 +326,8 @@ class Plan:
 
     def __init__(self, requests: Sequence[PriceRequest], precision: str = "fp64",
                  paths_per_thread: int = 0, use_graph: bool = False, max_dimension: int = 512,
-                 owen_table: bool = True, normal_table: int = 0, waves_per_block: int = 0):
+                 owen_table: bool = True, normal_table: int = 0, waves_per_block: int = 0,
+                 serial_launches: bool = False, fp32_full_ladder: bool = False):
         gpu_backend()
         for q in requests:
             _check_as_of(q)
@@ -335,7 +336,8 @@ class Plan:
         n = len(packed)
         arr = (N.Request * n)(*[p.r for p in packed])
         cfg = N.LaunchConfig(paths_per_thread, 1 if use_graph else 0, 0 if owen_table else -1,
-                             normal_table, 0, waves_per_block)
+                             normal_table, 0, waves_per_block, 1 if serial_launches else 0,
+                             1 if fp32_full_ladder else 0)
         err = N.Error()
         h = C.c_void_p()
         N.check(N.lib().riga_plan_create(N.Device.get().handle, arr, n, PRECISIONS[precision],
@@ -529,7 +531,7 @@ def debug_sobol(seed: int, ladder_dims: int, first_point: int, n_points: int, fi
 
 
 def debug_sobol_f32(seed: int, ladder_dims: int, first_point: int, n_points: int, n_nodes: int,
-                    factors: int, owen_table: bool = True):
+                    factors: int, owen_table: bool = True, full_ladder: bool = False):
     """The FP32 tier's ladder as its path kernels run it: the integers it turns into
     uniforms (24 Owen levels with low byte 0x80, all 32 levels within 2^-10 of 0 or 1)
     and its FP32 normals, [point][node * factors + f]."""
@@ -540,9 +542,37 @@ def debug_sobol_f32(seed: int, ladder_dims: int, first_point: int, n_points: int
     err = N.Error()
     N.check(N.lib().riga_debug_sobol_f32(
         N.Device.get().handle, seed, ladder_dims, first_point, n_points, n_nodes, factors,
-        1 if owen_table else 0, s.ctypes.data_as(C.POINTER(C.c_uint32)),
+        (1 if owen_table else 0) | (2 if full_ladder else 0), s.ctypes.data_as(C.POINTER(C.c_uint32)),
         z.ctypes.data_as(C.POINTER(C.c_float)), C.byref(err)), err)
     return s.reshape(n_points, dims), z.reshape(n_points, dims)
+
+
+def debug_halton(seed: int, ladder_dims: int, first_point: int, n_points: int, first_dim: int,
+                 n_dims: int):
+    """The device's Halton ladder: permuted digits [point][dim][32] (least significant
+    first, 0xFFFFFFFF after the last), radical inverses and FP64 normals [point][dim]."""
+    gpu_backend()
+    cells = n_points * n_dims
+    dig = np.zeros(cells * 32, dtype=np.uint32)
+    u = np.zeros(cells, dtype=np.float64)
+    z = np.zeros(cells, dtype=np.float64)
+    err = N.Error()
+    N.check(N.lib().riga_debug_halton(
+        N.Device.get().handle, seed, ladder_dims, first_point, n_points, first_dim, n_dims,
+        dig.ctypes.data_as(C.POINTER(C.c_uint32)), N.dptr(u), N.dptr(z), C.byref(err)), err)
+    return (dig.reshape(n_points, n_dims, 32), u.reshape(n_points, n_dims), z.reshape(n_points, n_dims))
+
+
+def debug_exp_log(x) -> Tuple[np.ndarray, np.ndarray]:
+    """The FP64 tier's exp and log of each x, as its path kernels evaluate them."""
+    gpu_backend()
+    xs = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    e = np.zeros_like(xs)
+    lg = np.zeros_like(xs)
+    err = N.Error()
+    N.check(N.lib().riga_debug_exp_log(N.Device.get().handle, N.dptr(xs), xs.size, N.dptr(e), N.dptr(lg),
+                                       C.byref(err)), err)
+    return e, lg
 
 
 def debug_mc_starts(seed: int, draws_per_path: int, n_paths: int) -> np.ndarray:

@@ -131,7 +131,7 @@ struct HostBuf {
 
 }  // namespace
 
-// Owen prefix table of one seed: u16[dims][2^kOwenPrefixBits], built on the
+// Owen prefix table of one seed: OwenEntry[dims][2^kOwenPrefixBits], built on the
 // device, shared by every plan of the context (plans keep a reference, so a
 // table replaced by a larger one stays valid until its last plan is gone).
 struct OwenTable {
@@ -551,7 +551,9 @@ void fill_result(const Partial& fr, uint64_t paths, riga_result* out) {
 
 // ------------------------------------------------------------ plans
 struct LaunchPlan {
-  int F = 0, smax = 1, blocks = 0;
+  int F = 0, smax = 1, blocks = 0;  // blocks: the (item, chunk) units of the launch
+  int grid = 0;                     // workgroups claiming them
+  double cost = 0.0;                // the longest item's path-steps x factors (launch order)
   int mode = kLadderDirect;  // how the launch's items get their normals (kernel template mode)
   int threads = kBlock;  // lanes per workgroup
   size_t lds = 0;
@@ -604,6 +606,12 @@ struct riga_plan {
   int blocks = 0;
   size_t lds_max = 0;
   DevBuf arena, items_d, block_item, partials, finals, final_base_d, records_d;
+  DevBuf queues;  // one unit counter per launch, kQueueStride words apart
+  // launches after the first run on their own streams, forked from and joined
+  // back into the plan's stream, so the persistent kernels of an uneven book
+  // share the device instead of draining one after another
+  std::vector<hipStream_t> aux;
+  std::vector<hipEvent_t> fork_join;  // [0]: fork; [k]: launch k done
   HostBuf finals_host;
   std::vector<PathRecordDev> recs_host;
   // replay of listed paths / coupon strips
@@ -614,7 +622,7 @@ struct riga_plan {
     uint64_t seed = 0;
     int dims = 0, nbits = 0;
     int64_t paths = 0;
-    const uint16_t* owen = nullptr;
+    const OwenEntry* owen = nullptr;
     DevBuf buf;
   };
   std::vector<std::unique_ptr<NormalTable>> ztables;
@@ -635,6 +643,7 @@ struct riga_plan {
   int64_t off_dirs = 0, off_zig = 0;
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool serial = false;     // launches one after another on `stream` (no fork)
   hipGraph_t g = nullptr;
   hipGraphExec_t gexec = nullptr;
   ~riga_plan() {
@@ -642,9 +651,11 @@ struct riga_plan {
     if (g) (void)hipGraphDestroy(g);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
+    for (hipEvent_t ev : fork_join) (void)hipEventDestroy(ev);
+    for (hipStream_t st : aux) (void)hipStreamDestroy(st);
     // buffers go back to the pool on this stream before it is destroyed
     for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d, &path_list,
-                      &series, &ledger_d})
+                      &series, &ledger_d, &queues})
       b->release();
     for (auto& zt : ztables) zt->buf.release();
     for (auto& mp : mc) mp->each([](DevBuf& b, hipStream_t) { b.release(); }, nullptr);
@@ -652,7 +663,8 @@ struct riga_plan {
     if (stream) (void)hipStreamDestroy(stream);
   }
   void bind_stream() {
-    for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d, &mc_status})
+    for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d, &mc_status,
+                      &queues})
       b->stream = stream;
   }
 };
@@ -674,9 +686,9 @@ hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t st
   std::shared_ptr<OwenTable> t(new OwenTable());
   t->seed = seed;
   t->dims = want;
-  hipError_t e = t->buf.ensure((size_t)want << (kOwenPrefixBits + 1));  // 2^bits x u16 per dimension
+  hipError_t e = t->buf.ensure(((size_t)want << kOwenPrefixBits) * sizeof(OwenEntry));  // 2^bits entries per dimension
   if (e == hipSuccess) e = hipEventCreateWithFlags(&t->ready, hipEventDisableTiming);
-  if (e == hipSuccess) e = launch_owen_table(seed, want, (uint16_t*)t->buf.p, stream);
+  if (e == hipSuccess) e = launch_owen_table(seed, want, (OwenEntry*)t->buf.p, stream);
   if (e == hipSuccess) e = hipEventRecord(t->ready, stream);
   if (e != hipSuccess) return e;
   ctx->owen[seed] = t;
@@ -755,6 +767,24 @@ bool evenly_spaced(const std::vector<double>& x) {
   return true;
 }
 
+PhoenixLaunch phoenix_launch_of(const riga_plan* pl, const LaunchPlan& lp, size_t k) {
+  PhoenixLaunch L;
+  L.F = lp.F;
+  L.smax = lp.smax;
+  L.mode = lp.mode;
+  L.blocks = lp.grid;
+  L.lds = lp.lds;
+  L.threads = lp.threads;
+  L.items = (const ItemDesc*)pl->items_d.p;
+  L.block_item = (const int32_t*)pl->block_item.p + 2 * lp.block_off;
+  L.n_units = lp.blocks;
+  L.queue = (uint32_t*)pl->queues.p + k * kQueueStride;
+  L.arena = (const unsigned char*)pl->arena.p;
+  L.partials = (Partial*)pl->partials.p;
+  L.records = pl->records ? (PathRecordDev*)pl->records_d.p : nullptr;
+  return L;
+}
+
 // Enqueue the kernels, the finalize merge and the download of the merged records.
 hipError_t enqueue(riga_plan* pl, bool timed) {
   hipError_t e;
@@ -768,7 +798,7 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
   for (const auto& zt : pl->ztables) {
     const unsigned char* a = (const unsigned char*)pl->arena.p;
     const uint32_t* dirs = (const uint32_t*)(a + pl->off_dirs);
-    const uint16_t* ot = zt->owen;
+    const OwenEntry* ot = zt->owen;
     e = pl->precision == RIGA_FP64
             ? launch_normal_table_f64(dirs, ot, zt->seed, zt->dims, (int)zt->paths, zt->nbits, (double*)zt->buf.p,
                                       pl->stream)
@@ -776,21 +806,22 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
                                       pl->stream);
     if (e != hipSuccess) return e;
   }
-  for (const LaunchPlan& lp : pl->launches) {
-    PhoenixLaunch L;
-    L.F = lp.F;
-    L.smax = lp.smax;
-    L.mode = lp.mode;
-    L.blocks = lp.blocks;
-    L.lds = lp.lds;
-    L.threads = lp.threads;
-    L.items = (const ItemDesc*)pl->items_d.p;
-    L.block_item = (const int32_t*)pl->block_item.p + 2 * lp.block_off;
-    L.arena = (const unsigned char*)pl->arena.p;
-    L.partials = (Partial*)pl->partials.p;
-    L.records = pl->records ? (PathRecordDev*)pl->records_d.p : nullptr;
-    e = pl->precision == RIGA_FP64 ? launch_phoenix_f64(L, pl->stream) : launch_phoenix_f32(L, pl->stream);
+  e = launch_queue_reset((uint32_t*)pl->queues.p, (int)pl->launches.size(), pl->stream);
+  if (e != hipSuccess) return e;
+  const size_t nl = pl->launches.size();
+  const bool fork = nl > 1 && !pl->serial;
+  if (fork && (e = hipEventRecord(pl->fork_join[0], pl->stream)) != hipSuccess) return e;
+  for (size_t k = 0; k < nl; ++k) {
+    const LaunchPlan& lp = pl->launches[k];
+    hipStream_t st = k == 0 || !fork ? pl->stream : pl->aux[k - 1];
+    if (k > 0 && fork && (e = hipStreamWaitEvent(st, pl->fork_join[0], 0)) != hipSuccess) return e;
+    const PhoenixLaunch L = phoenix_launch_of(pl, lp, k);
+    e = pl->precision == RIGA_FP64 ? launch_phoenix_f64(L, st) : launch_phoenix_f32(L, st);
     if (e != hipSuccess) return e;
+    if (k > 0 && fork) {
+      if ((e = hipEventRecord(pl->fork_join[k], st)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(pl->stream, pl->fork_join[k], 0)) != hipSuccess) return e;
+    }
   }
   if (pl->ledger) {
     e = launch_ledger((const ItemDesc*)pl->items_d.p, 0, pl->items[0].d.paths, (const unsigned char*)pl->arena.p,
@@ -849,6 +880,13 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     Error e;
     if (reqs[i].steps >= (int64_t)1 << 20) {  // KO steps are packed in 20 bits on the device
       set_err(err, "invalid_schedule", "request " + std::to_string(i) + ": the device takes at most 1,048,575 steps");
+      return RIGA_EINVAL;
+    }
+    // Sobol and Halton point indices n = path + 1 are 32-bit on the device (the
+    // reference's are u64, qmc.rs:269-282) and trace-event path ids are int32:
+    // beyond 2^31 - 1 paths the device would reuse points, so refuse
+    if (reqs[i].paths > (uint64_t)INT32_MAX) {
+      set_err(err, "invalid_paths", "request " + std::to_string(i) + ": the device takes at most 2,147,483,647 paths");
       return RIGA_EINVAL;
     }
     if (!prepare(reqs[i], &preps[i], &e)) {
@@ -920,7 +958,11 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   // read their normals from one [dim][path] table when they need many more
   // normals than it holds (an uneven book), instead of hashing them per trade.
   // Decided here, since the items reading a table launch as their own kernel mode.
-  const int zmode = cfg ? cfg->normal_table : 0;
+  const bool full_ladder = precision == RIGA_FP32 && cfg && cfg->fp32_full_ladder != 0;
+  const int zmode = full_ladder ? -1 : (cfg ? cfg->normal_table : 0);
+  if (full_ladder)
+    for (Item& it : items)
+      if (!(it.d.flags & (kHalton | kMonteCarlo))) it.d.flags |= kFullLadder;
   std::map<uint64_t, std::vector<int>> table_items;  // seed -> items reading its table
   if (zmode >= 0 && !records) {
     std::map<uint64_t, std::vector<int>> by_seed;
@@ -955,7 +997,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   std::map<PartKey, std::vector<int>> parts;
   for (size_t i = 0; i < items.size(); ++i)
     parts[PartKey(items[i].F, items[i].smax, item_mode[i])].push_back((int)i);
-  pl->signature = shape_signature(precision, parts, items);
+  pl->signature = shape_signature(precision, parts, items) + (full_ladder ? "|full" : "");
   if (ppt == 0 || waves == 0) {  // the autotuned shape for this signature, else 1 path x 4 waves
     int tp = 1, tw = 4;
     {
@@ -1100,22 +1142,27 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       d.max_nx = max_nx;
     }
     lp.blocks = blocks;
+    lp.cost = items[idx.front()].cost;
     pl->blocks += blocks;
     pl->launches.push_back(lp);
   }
+  // the launch holding the longest item is dispatched first (with the items of
+  // each launch longest first, the device's work is claimed longest first)
+  std::stable_sort(pl->launches.begin(), pl->launches.end(),
+                   [](const LaunchPlan& a, const LaunchPlan& b) { return a.cost > b.cost; });
   pl->off_dirs = off_dirs;
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
   if (!estimate) HIP_TRY(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "stream");
   const bool stream_ordered = cfg && cfg->stream_ordered_alloc != 0;
   if (stream_ordered) pl->bind_stream();
   double ztable_bytes = 0.0;
-  std::map<uint64_t, const uint16_t*> owen_ptr;
+  std::map<uint64_t, const OwenEntry*> owen_ptr;
   if (!estimate)
     for (auto& kv : owen_dims) {
       std::shared_ptr<OwenTable> ot;
       HIP_TRY(owen_table_for(ctx, kv.first, kv.second, pl->stream, &ot), "owen table");
       HIP_TRY(hipStreamWaitEvent(pl->stream, ot->ready, 0), "owen table wait");
-      owen_ptr[kv.first] = (const uint16_t*)ot->buf.p;
+      owen_ptr[kv.first] = (const OwenEntry*)ot->buf.p;
       pl->owen_tabs.push_back(std::move(ot));
     }
   for (Item& it : items)
@@ -1211,11 +1258,13 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   }
   std::vector<ItemDesc> descs(items.size());
   for (size_t i = 0; i < items.size(); ++i) descs[i] = items[i].d;
+  const double device_bytes = ztable_bytes + mc_bytes + (double)arena.bytes.size() +
+                              descs.size() * sizeof(ItemDesc) + all_blocks.size() * 4.0 +
+                              (double)pl->partial_slots * sizeof(Partial) + (double)pl->final_slots * sizeof(Partial) +
+                              pl->final_base.size() * 8.0 + (double)pl->record_slots * sizeof(PathRecordDev) +
+                              8 * 4096.0;
   if (estimate) {
-    *estimate = (int64_t)(ztable_bytes + mc_bytes + (double)arena.bytes.size() + descs.size() * sizeof(ItemDesc) +
-                          all_blocks.size() * 4.0 + (double)pl->partial_slots * sizeof(Partial) +
-                          (double)pl->final_slots * sizeof(Partial) + pl->final_base.size() * 8.0 +
-                          (double)pl->record_slots * sizeof(PathRecordDev) + 8 * 4096.0);
+    *estimate = (int64_t)device_bytes;
     for (auto& zt : pl->ztables) zt->buf.stream = nullptr;
     for (auto& mp : pl->mc) mp->each([](DevBuf& b, hipStream_t) { b.stream = nullptr; }, nullptr);
     return 0;
@@ -1229,6 +1278,16 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   HIP_TRY(pl->partials.ensure((size_t)pl->partial_slots * sizeof(Partial)), "partials alloc");
   HIP_TRY(pl->finals.ensure((size_t)pl->final_slots * sizeof(Partial)), "finals alloc");
   HIP_TRY(pl->final_base_d.ensure(pl->final_base.size() * sizeof(int64_t)), "final index alloc");
+  HIP_TRY(pl->queues.ensure(pl->launches.size() * kQueueStride * sizeof(uint32_t)), "unit queue alloc");
+  // one workgroup per unit (each claims its unit from the launch's counter)
+  for (LaunchPlan& lp : pl->launches) lp.grid = lp.blocks;
+  pl->serial = cfg && cfg->serial_launches != 0;
+  if (pl->launches.size() > 1 && !pl->serial) {
+    pl->fork_join.assign(pl->launches.size(), nullptr);
+    for (hipEvent_t& ev : pl->fork_join) HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
+    pl->aux.assign(pl->launches.size() - 1, nullptr);
+    for (hipStream_t& st : pl->aux) HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+  }
   HIP_TRY(pl->finals_host.ensure((size_t)pl->final_slots * sizeof(Partial) + 8 * pl->mc.size()),
           "host finals alloc");
   if (records) {
@@ -1757,6 +1816,65 @@ int riga_debug_sobol(riga_ctx* ctx, uint64_t seed, int32_t ladder_dims, uint64_t
   return 0;
 }
 
+int riga_debug_exp_log(riga_ctx* ctx, const double* x, uint64_t n, double* out_exp, double* out_log,
+                       riga_error* err) {
+  clear_err(err);
+  if (!ctx) {
+    set_err(err, "gpu_unavailable", "no device context");
+    return RIGA_ENODEV;
+  }
+  if (n == 0) return 0;
+  if (!x || !out_exp || !out_log || n > ((uint64_t)1 << 28)) {
+    set_err(err, "invalid_argument", "bad debug_exp_log arguments");
+    return RIGA_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  DevBuf dx, de, dl;
+  HIP_TRY(dx.ensure(n * 8), "alloc");
+  HIP_TRY(de.ensure(n * 8), "alloc");
+  HIP_TRY(dl.ensure(n * 8), "alloc");
+  HIP_TRY(hipMemcpyAsync(dx.p, x, n * 8, hipMemcpyHostToDevice, ctx->stream), "upload");
+  HIP_TRY(launch_debug_exp_log((const double*)dx.p, n, (double*)de.p, (double*)dl.p, ctx->stream), "debug kernel");
+  HIP_TRY(hipMemcpyAsync(out_exp, de.p, n * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+  HIP_TRY(hipMemcpyAsync(out_log, dl.p, n * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "debug kernel execution");
+  return 0;
+}
+
+int riga_debug_halton(riga_ctx* ctx, uint64_t seed, int32_t ladder_dims, uint64_t first_point, int32_t n_points,
+                      int32_t first_dim, int32_t n_dims, uint32_t* out_digits, double* out_uniform,
+                      double* out_normal, riga_error* err) {
+  clear_err(err);
+  if (!ctx) {
+    set_err(err, "gpu_unavailable", "no device context");
+    return RIGA_ENODEV;
+  }
+  if (n_points <= 0 || n_dims <= 0 || first_dim < 0 || ladder_dims <= 0 || ladder_dims > 7762 ||
+      !out_digits || !out_uniform || !out_normal || first_point + (uint64_t)n_points >= (1ull << 32)) {
+    set_err(err, "invalid_argument", "bad debug_halton ranges");
+    return RIGA_EINVAL;
+  }
+  const std::vector<uint32_t> tab = halton_tables(ladder_dims, seed);
+  const size_t cells = (size_t)n_points * n_dims;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  DevBuf d_tab, d_dig, d_u, d_n;
+  HIP_TRY(d_tab.ensure(tab.size() * 4), "alloc");
+  HIP_TRY(d_dig.ensure(cells * 32 * 4), "alloc");
+  HIP_TRY(d_u.ensure(cells * 8), "alloc");
+  HIP_TRY(d_n.ensure(cells * 8), "alloc");
+  HIP_TRY(hipMemcpyAsync(d_tab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, ctx->stream), "upload");
+  HIP_TRY(launch_debug_halton((const uint32_t*)d_tab.p, ladder_dims, first_point, n_points, first_dim, n_dims,
+                              (uint32_t*)d_dig.p, (double*)d_u.p, (double*)d_n.p, ctx->stream),
+          "debug kernel");
+  HIP_TRY(hipMemcpyAsync(out_digits, d_dig.p, cells * 32 * 4, hipMemcpyDeviceToHost, ctx->stream), "download");
+  HIP_TRY(hipMemcpyAsync(out_uniform, d_u.p, cells * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+  HIP_TRY(hipMemcpyAsync(out_normal, d_n.p, cells * 8, hipMemcpyDeviceToHost, ctx->stream), "download");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "debug kernel execution");
+  return 0;
+}
+
 int riga_debug_sobol_f32(riga_ctx* ctx, uint64_t seed, int32_t ladder_dims, uint64_t first_point,
                          int32_t n_points, int32_t n_nodes, int32_t factors, int32_t owen_table,
                          uint32_t* out_scrambled, float* out_normal, riga_error* err) {
@@ -1779,7 +1897,7 @@ int riga_debug_sobol_f32(riga_ctx* ctx, uint64_t seed, int32_t ladder_dims, uint
   std::lock_guard<std::mutex> lock(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
   std::shared_ptr<OwenTable> ot;
-  if (owen_table) {
+  if (owen_table & 1) {
     HIP_TRY(owen_table_for(ctx, seed, dims, ctx->stream, &ot), "owen table");
     HIP_TRY(hipStreamWaitEvent(ctx->stream, ot->ready, 0), "owen table wait");
   }
@@ -1789,9 +1907,9 @@ int riga_debug_sobol_f32(riga_ctx* ctx, uint64_t seed, int32_t ladder_dims, uint
   HIP_TRY(d_z.ensure(cells * 4), "alloc");
   HIP_TRY(hipMemcpyAsync(d_dirs.p, dirs.data(), (size_t)rows * 32 * 4, hipMemcpyHostToDevice, ctx->stream),
           "upload");
-  HIP_TRY(launch_debug_sobol_f32((const uint32_t*)d_dirs.p, ladder_dims, ot ? (const uint16_t*)ot->buf.p : nullptr,
-                                 seed, first_point, n_points, n_nodes, factors, nbits, (uint32_t*)d_s.p,
-                                 (float*)d_z.p, ctx->stream),
+  HIP_TRY(launch_debug_sobol_f32((const uint32_t*)d_dirs.p, ladder_dims, ot ? (const OwenEntry*)ot->buf.p : nullptr,
+                                 seed, first_point, n_points, n_nodes, factors, nbits, (owen_table & 2) != 0,
+                                 (uint32_t*)d_s.p, (float*)d_z.p, ctx->stream),
           "debug kernel");
   HIP_TRY(hipMemcpyAsync(out_scrambled, d_s.p, cells * 4, hipMemcpyDeviceToHost, ctx->stream), "download");
   HIP_TRY(hipMemcpyAsync(out_normal, d_z.p, cells * 4, hipMemcpyDeviceToHost, ctx->stream), "download");
@@ -1910,21 +2028,56 @@ struct riga_resident_worst_of {
 
 struct riga_resident_lv {
   riga_ctx* ctx = nullptr;
-  DevBuf normals, rows, ls, s0, out, ko;
+  DevBuf normals, rows, out, ko;
   int paths = 0, steps = 0, gx = 0;
   float dt = 0.f;
+  float ls_lo[2] = {0.f, 0.f}, ls_hi[2] = {0.f, 0.f};  // end knots of the two log-spot axes
 };
 
 namespace {
 
 int lv_check(const float* normals, const float* rows, const float* ls_axis, int32_t gx, int64_t paths,
              int64_t steps, riga_error* err) {
-  if (!normals || !rows || !ls_axis || gx < 2 || paths < 0 || steps < 0 || paths > INT32_MAX ||
-      steps > INT32_MAX || (double)paths * (double)steps * 2.0 > 8e9) {
-    set_err(err, "shape_mismatch", "normals [paths][steps][2], rows [2][steps][gx], ls_axis [2][gx] with gx >= 2");
+  if (!normals || !rows || !ls_axis || gx < 2 || gx > kLvMaxKnots || paths < 0 || steps < 0 ||
+      paths > INT32_MAX || steps > INT32_MAX || (double)paths * (double)steps * 2.0 > 8e9) {
+    set_err(err, "shape_mismatch",
+            "normals [paths][steps][2], rows [2][steps][gx], ls_axis [2][gx] with 2 <= gx <= " +
+                std::to_string(kLvMaxKnots));
     return RIGA_EINVAL;
   }
+  for (int f = 0; f < 2; ++f)
+    if (!(ls_axis[(size_t)f * gx + gx - 1] > ls_axis[(size_t)f * gx])) {
+      set_err(err, "invalid_local_vol", "each log-spot axis must increase from its first to its last knot");
+      return RIGA_EINVAL;
+    }
   return 0;
+}
+
+// The propagation's per-factor constants, formed in FP64 and rounded once: the
+// knot position (gx - 1) (x - x_first) / (x_last - x_first) as one multiply-add
+// of x, the drift (r - q) dt and the KO level ln(call_barrier * reference).
+LvPropagateArgs lv_args(const float* normals, const float* rows, float* out, int* ko, const float ls_lo[2],
+                        const float ls_hi[2], int gx, int64_t paths, int64_t steps, float dt, const float* s0,
+                        const float* references, float call_barrier, float rate, const float* dividends) {
+  LvPropagateArgs a{};
+  a.normals = normals;
+  a.rows = rows;
+  a.out = out;
+  a.ko_out = ko;
+  for (int f = 0; f < 2; ++f) {
+    const double scale = (double)(gx - 1) / ((double)ls_hi[f] - (double)ls_lo[f]);
+    a.x0[f] = (float)std::log((double)s0[f]);
+    a.knot_scale[f] = (float)scale;
+    a.knot_offset[f] = (float)(-(double)ls_lo[f] * scale);
+    a.drift[f] = (float)(((double)rate - (double)dividends[f]) * (double)dt);
+    a.ln_ko[f] = (float)std::log((double)call_barrier * (double)references[f]);
+  }
+  a.sqrt_dt = (float)std::sqrt((double)dt);
+  a.neg_half_dt = (float)(-0.5 * (double)dt);
+  a.paths = (int)paths;
+  a.steps = (int)steps;
+  a.gx = gx;
+  return a;
 }
 
 }  // namespace
@@ -2045,21 +2198,17 @@ int riga_lv_terminal_spots(riga_ctx* ctx, const float* normals, const float* row
   if (paths == 0) return 0;
   std::lock_guard<std::mutex> lock(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
-  DevBuf dn, dr, dl, ds, dout, dko;
+  DevBuf dn, dr, dout, dko;
   const size_t nn = (size_t)paths * steps * 2, nr = (size_t)2 * steps * gx;
   HIP_TRY(dn.ensure(std::max<size_t>(nn, 1) * 4), "alloc");
   HIP_TRY(dr.ensure(std::max<size_t>(nr, 1) * 4), "alloc");
-  HIP_TRY(dl.ensure((size_t)2 * gx * 4), "alloc");
-  HIP_TRY(ds.ensure(8), "alloc");
   HIP_TRY(dout.ensure((size_t)paths * 8), "alloc");
   HIP_TRY(dko.ensure((size_t)paths * 4), "alloc");
   if (nn) HIP_TRY(hipMemcpyAsync(dn.p, normals, nn * 4, hipMemcpyHostToDevice, ctx->stream), "H2D normals");
   if (nr) HIP_TRY(hipMemcpyAsync(dr.p, rows, nr * 4, hipMemcpyHostToDevice, ctx->stream), "H2D rows");
-  HIP_TRY(hipMemcpyAsync(dl.p, ls_axis, (size_t)2 * gx * 4, hipMemcpyHostToDevice, ctx->stream), "H2D ls");
-  HIP_TRY(hipMemcpyAsync(ds.p, s0, 8, hipMemcpyHostToDevice, ctx->stream), "H2D s0");
-  LvPropagateArgs a{(const float*)dn.p, (const float*)dr.p, (const float*)dl.p, (const float*)ds.p,
-                    (float*)dout.p, (int*)dko.p, references[0], references[1], call_barrier, (int)paths,
-                    (int)steps, gx, dt, rate, dividends[0], dividends[1]};
+  const float lo[2] = {ls_axis[0], ls_axis[gx]}, hi[2] = {ls_axis[gx - 1], ls_axis[2 * gx - 1]};
+  const LvPropagateArgs a = lv_args((const float*)dn.p, (const float*)dr.p, (float*)dout.p, (int*)dko.p, lo, hi, gx,
+                                    paths, steps, dt, s0, references, call_barrier, rate, dividends);
   HIP_TRY(launch_lv_propagate(a, ctx->stream), "lv launch");
   HIP_TRY(hipMemcpyAsync(out_terminal, dout.p, (size_t)paths * 8, hipMemcpyDeviceToHost, ctx->stream), "D2H out");
   HIP_TRY(hipMemcpyAsync(out_ko, dko.p, (size_t)paths * 4, hipMemcpyDeviceToHost, ctx->stream), "D2H ko");
@@ -2089,13 +2238,14 @@ int riga_resident_lv_create(riga_ctx* ctx, const float* normals, const float* ro
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
   HIP_TRY(h->normals.ensure(std::max<size_t>(nn, 1) * 4), "alloc");
   HIP_TRY(h->rows.ensure(std::max<size_t>(nr, 1) * 4), "alloc");
-  HIP_TRY(h->ls.ensure((size_t)2 * gx * 4), "alloc");
-  HIP_TRY(h->s0.ensure(8), "alloc");
   HIP_TRY(h->out.ensure(std::max<size_t>((size_t)paths, 1) * 8), "alloc");
   HIP_TRY(h->ko.ensure(std::max<size_t>((size_t)paths, 1) * 4), "alloc");
+  for (int f = 0; f < 2; ++f) {
+    h->ls_lo[f] = ls_axis[(size_t)f * gx];
+    h->ls_hi[f] = ls_axis[(size_t)f * gx + gx - 1];
+  }
   if (nn) HIP_TRY(hipMemcpyAsync(h->normals.p, normals, nn * 4, hipMemcpyHostToDevice, ctx->stream), "H2D normals");
   if (nr) HIP_TRY(hipMemcpyAsync(h->rows.p, rows, nr * 4, hipMemcpyHostToDevice, ctx->stream), "H2D rows");
-  HIP_TRY(hipMemcpyAsync(h->ls.p, ls_axis, (size_t)2 * gx * 4, hipMemcpyHostToDevice, ctx->stream), "H2D ls");
   HIP_TRY(hipStreamSynchronize(ctx->stream), "upload");
   *out = h.release();
   return 0;
@@ -2110,12 +2260,10 @@ int riga_resident_lv_run(riga_resident_lv* h, const float* s0, const float* refe
   }
   std::lock_guard<std::mutex> lock(h->ctx->mu);
   HIP_TRY(hipSetDevice(h->ctx->device), "hipSetDevice");
-  // ordered on the stream after the previous run's kernel (which reads s0);
-  // a pageable source is staged before the call returns
-  HIP_TRY(hipMemcpyAsync(h->s0.p, s0, 8, hipMemcpyHostToDevice, h->ctx->stream), "H2D s0");
-  LvPropagateArgs a{(const float*)h->normals.p, (const float*)h->rows.p, (const float*)h->ls.p,
-                    (const float*)h->s0.p, (float*)h->out.p, (int*)h->ko.p, references[0], references[1],
-                    call_barrier, h->paths, h->steps, h->gx, h->dt, rate, dividends[0], dividends[1]};
+  // the spots enter as kernel arguments (their logs), so a run needs no upload
+  const LvPropagateArgs a = lv_args((const float*)h->normals.p, (const float*)h->rows.p, (float*)h->out.p,
+                                    (int*)h->ko.p, h->ls_lo, h->ls_hi, h->gx, h->paths, h->steps, h->dt, s0,
+                                    references, call_barrier, rate, dividends);
   HIP_TRY(launch_lv_propagate(a, h->ctx->stream), "lv launch");
   return 0;
 }

@@ -14,26 +14,32 @@ struct PhoenixLaunch {
   int F;       // factors (1..kMaxFactors)
   int smax;    // scenario bucket (1, 4 or 10)
   int mode;    // kLadderDirect, kLadderTable or kLadderMonteCarlo
-  int blocks;  // grid size
+  int blocks;  // grid size: one workgroup per unit
   int threads; // lanes per workgroup: 64, 128 or 256
   size_t lds;  // dynamic LDS bytes (bridge stack and local-vol rows)
   const ItemDesc* items;
-  const int32_t* block_item;  // (item, chunk) per block
+  const int32_t* block_item;  // (item, chunk) units, longest items first
+  int n_units;                // entries of block_item
+  uint32_t* queue;            // the launch's unit counter, zero at launch
   const unsigned char* arena;
   Partial* partials;
   PathRecordDev* records;
 };
 
 hipError_t launch_phoenix_f64(const PhoenixLaunch& l, hipStream_t stream);
-hipError_t launch_debug_sobol_f32(const uint32_t* dirs, int ladder_dims, const uint16_t* owen_tab, uint64_t seed,
+// Zero n unit counters spaced kQueueStride words apart (a kernel, so a captured
+// graph records no memset node).
+constexpr int kQueueStride = 16;
+hipError_t launch_queue_reset(uint32_t* queues, int n, hipStream_t stream);
+hipError_t launch_debug_sobol_f32(const uint32_t* dirs, int ladder_dims, const OwenEntry* owen_tab, uint64_t seed,
                                   uint64_t first_point, int n_points, int n_nodes, int factors, int nbits,
-                                  uint32_t* xs_out, float* z_out, hipStream_t stream);
+                                  bool full_ladder, uint32_t* xs_out, float* z_out, hipStream_t stream);
 hipError_t launch_phoenix_f32(const PhoenixLaunch& l, hipStream_t stream);
 
 // Shared normal table z[dim][path] (Real = double for FP64, float for FP32).
-hipError_t launch_normal_table_f64(const uint32_t* dirs, const uint16_t* owen_tab, uint64_t seed,
+hipError_t launch_normal_table_f64(const uint32_t* dirs, const OwenEntry* owen_tab, uint64_t seed,
                                    int dims, int n_paths, int nbits, double* out, hipStream_t stream);
-hipError_t launch_normal_table_f32(const uint32_t* dirs, const uint16_t* owen_tab, uint64_t seed,
+hipError_t launch_normal_table_f32(const uint32_t* dirs, const OwenEntry* owen_tab, uint64_t seed,
                                    int dims, int n_paths, int nbits, float* out, hipStream_t stream);
 
 // Monte Carlo lane stream parse (kernels_mc.hip) for one (seed, draws per path).
@@ -83,23 +89,30 @@ struct CubeArgs {
 };
 hipError_t launch_cube(const CubeArgs& a, int factors, hipStream_t stream);
 
-// The reference device module's kernels (gpu.rs:27-75), FP32.
+// The reference device module's kernels (gpu.rs:27-75), FP32.  The local-vol
+// propagation's constants are formed on the host per factor: the knot position
+// of a log-spot x is x * knot_scale + knot_offset on evenly spaced knots, the
+// Euler drift is (rate - q) dt, and the KO level ln(call_barrier * reference).
 struct LvPropagateArgs {
   const float* normals;  // [paths][steps][2]
   const float* rows;     // [2][steps][gx]
-  const float* ls_axis;  // [2][gx]
-  const float* s0;       // [2]
   float* out;            // [paths][2] terminal spots
   int* ko_out;           // [paths] first KO step or -1
-  float ref0, ref1, ko_bar;
+  float x0[2];           // ln s0
+  float knot_scale[2], knot_offset[2];
+  float drift[2];        // (rate - q) dt
+  float ln_ko[2];        // ln(call_barrier * reference)
+  float sqrt_dt, neg_half_dt;
   int paths, steps, gx;
-  float dt, rate, q0, q1;
 };
+// Largest knot count the propagation stages in LDS.
+constexpr int kLvMaxKnots = 3072;
 hipError_t launch_worst_of(const float* a, const float* b, float* o, uint64_t n, hipStream_t stream);
 hipError_t launch_lv_propagate(const LvPropagateArgs& a, hipStream_t stream);
 
-// Owen prefix table: out[dim][v] = scrambled top 12 bits of v << 20, dims [0, dims).
-hipError_t launch_owen_table(uint64_t seed, int dims, uint16_t* out, hipStream_t stream);
+// Owen prefix table: out[dim][owen_slot(v)] = scrambled top kOwenPrefixBits bits of
+// v << (32 - kOwenPrefixBits), dims [0, dims).
+hipError_t launch_owen_table(uint64_t seed, int dims, OwenEntry* out, hipStream_t stream);
 
 // Coupon strips [path][period] of a replayed item from its worst-of series.
 hipError_t launch_ledger(const ItemDesc* items, int item, uint64_t paths, const unsigned char* arena,
@@ -114,6 +127,15 @@ hipError_t launch_debug_sobol(const uint32_t* dirs, int ladder_dims, uint64_t se
                               uint64_t first_point, int n_points, int first_dim, int n_dims,
                               int nbits, uint32_t* unscrambled, uint32_t* scrambled,
                               double* normal, hipStream_t stream);
+
+
+// The FP64 tier's exp and log (exp_log.h) elementwise, for the parity tests.
+hipError_t launch_debug_exp_log(const double* x, uint64_t n, double* e, double* l, hipStream_t stream);
+
+// Halton diagnostics: permuted digits [cell][32] (0xFFFFFFFF after the last),
+// radical inverses and FP64 normals for points first_point .. and dims first_dim ..
+hipError_t launch_debug_halton(const uint32_t* tables, int rows, uint64_t first_point, int n_points, int first_dim,
+                               int n_dims, uint32_t* digits, double* uniform, double* normal, hipStream_t stream);
 
 
 }  // namespace riga

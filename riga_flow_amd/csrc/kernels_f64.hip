@@ -5,47 +5,58 @@
 
 namespace riga {
 
-template <int F, int SMAX>
-static hipError_t launch_one(const PhoenixLaunch& l, hipStream_t stream) {
+// The kernel of a launch (factor count, scenario bucket, ladder mode), passed to `op`.
+template <int F, int SMAX, typename Op>
+static hipError_t with_kernel(const PhoenixLaunch& l, Op op) {
   if (l.threads < 64 || l.threads > kBlock || l.threads % 64 != 0) return hipErrorInvalidValue;
   switch (l.mode) {
-    case kLadderDirect:
-      hipLaunchKernelGGL((phoenix_kernel<double, F, SMAX, kLadderDirect>), dim3(l.blocks), dim3(l.threads), l.lds,
-                         stream, l.items, l.block_item, l.arena, l.partials, l.records);
-      break;
-    case kLadderTable:
-      hipLaunchKernelGGL((phoenix_kernel<double, F, SMAX, kLadderTable>), dim3(l.blocks), dim3(l.threads), l.lds,
-                         stream, l.items, l.block_item, l.arena, l.partials, l.records);
-      break;
-    case kLadderMonteCarlo:
-      hipLaunchKernelGGL((phoenix_kernel<double, F, SMAX, kLadderMonteCarlo>), dim3(l.blocks), dim3(l.threads),
-                         l.lds, stream, l.items, l.block_item, l.arena, l.partials, l.records);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-template <int F>
-static hipError_t launch_f(const PhoenixLaunch& l, hipStream_t stream) {
-  switch (l.smax) {
-    case 1: return launch_one<F, 1>(l, stream);
-    case 4: return launch_one<F, 4>(l, stream);
-    case 10: return launch_one<F, 10>(l, stream);
+    case kLadderDirect: return op(phoenix_kernel<double, F, SMAX, kLadderDirect>);
+    case kLadderTable: return op(phoenix_kernel<double, F, SMAX, kLadderTable>);
+    case kLadderMonteCarlo: return op(phoenix_kernel<double, F, SMAX, kLadderMonteCarlo>);
     default: return hipErrorInvalidValue;
   }
+}
+
+template <typename Op>
+static hipError_t with_kernel(const PhoenixLaunch& l, Op op) {
+#define RIGA_F(FF)                                            \
+  case FF:                                                    \
+    switch (l.smax) {                                         \
+      case 1: return with_kernel<FF, 1>(l, op);               \
+      case 4: return with_kernel<FF, 4>(l, op);               \
+      case 10: return with_kernel<FF, 10>(l, op);             \
+      default: return hipErrorInvalidValue;                   \
+    }
+  switch (l.F) {
+    RIGA_F(1)
+    RIGA_F(2)
+    RIGA_F(3)
+    RIGA_F(4)
+    RIGA_F(5)
+    default: return hipErrorInvalidValue;
+  }
+#undef RIGA_F
 }
 
 hipError_t launch_phoenix_f64(const PhoenixLaunch& l, hipStream_t stream) {
-  switch (l.F) {
-    case 1: return launch_f<1>(l, stream);
-    case 2: return launch_f<2>(l, stream);
-    case 3: return launch_f<3>(l, stream);
-    case 4: return launch_f<4>(l, stream);
-    case 5: return launch_f<5>(l, stream);
-    default: return hipErrorInvalidValue;
-  }
+  if (l.blocks <= 0 || l.blocks > l.n_units || !l.queue) return hipErrorInvalidValue;
+  return with_kernel(l, [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(l.blocks), dim3(l.threads), l.lds, stream, l.items, l.block_item, l.arena,
+                       l.partials, l.records, l.queue, l.n_units);
+    return hipGetLastError();
+  });
+}
+
+
+__global__ void queue_reset_kernel(uint32_t* __restrict__ queues, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) queues[(size_t)i * kQueueStride] = 0u;
+}
+
+hipError_t launch_queue_reset(uint32_t* queues, int n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(queue_reset_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, queues, n);
+  return hipGetLastError();
 }
 
 // Merge the block partials of every (item, scenario) in block order.
@@ -145,19 +156,72 @@ hipError_t launch_debug_sobol(const uint32_t* dirs, int ladder_dims, uint64_t se
   return hipGetLastError();
 }
 
-// tab[dim][v] = top kOwenTableBits bits of owen_scramble(v << (32-L), key(dim)).
-__global__ __launch_bounds__(256) void owen_table_kernel(uint64_t seed, int dims, uint16_t* __restrict__ out) {
-  const uint32_t v = blockIdx.x * 256 + threadIdx.x;
+// The FP64 tier's exp and log (exp_log.h) of given arguments.
+__global__ void debug_exp_log_kernel(const double* __restrict__ x, uint64_t n, double* __restrict__ e,
+                                     double* __restrict__ l) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  e[i] = riga_exp(x[i]);
+  l[i] = riga_log(x[i]);
+}
+
+hipError_t launch_debug_exp_log(const double* x, uint64_t n, double* e, double* l, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(debug_exp_log_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, x, n, e, l);
+  return hipGetLastError();
+}
+
+// Halton diagnostics (qmc.rs:143-153 as the path kernels evaluate it): for
+// point n = first_point + p and dim = first_dim + d, the permuted digits
+// perm[digit] of n in the dimension's base, least significant first (0xFFFFFFFF
+// after the last), the radical inverse from halton_u and its FP64 inverse normal.
+// `tables` is the planner's Halton layout: bases[rows], offsets[rows], then the
+// permutations.
+__global__ void debug_halton_kernel(const uint32_t* __restrict__ tables, int rows, uint64_t first_point,
+                                    int n_points, int first_dim, int n_dims, uint32_t* __restrict__ digits,
+                                    double* __restrict__ uniform, double* __restrict__ normal) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = blockIdx.y;
+  if (p >= n_points || d >= n_dims) return;
+  const uint32_t n = (uint32_t)(first_point + (uint64_t)p);
+  const int di = (int)((uint32_t)(first_dim + d) % (uint32_t)rows);
+  const uint32_t base = tables[di];
+  const uint32_t* perm = tables + tables[rows + di];
+  const size_t o = (size_t)p * n_dims + d;
+  uint32_t m = n;
+  for (int k = 0; k < 32; ++k) {
+    digits[o * 32 + k] = m > 0 ? perm[m % base] : 0xFFFFFFFFu;
+    m /= base;
+  }
+  const double u = halton_u(n, base, perm);
+  uniform[o] = u;
+  normal[o] = probit_f64(u);
+}
+
+hipError_t launch_debug_halton(const uint32_t* tables, int rows, uint64_t first_point, int n_points, int first_dim,
+                               int n_dims, uint32_t* digits, double* uniform, double* normal, hipStream_t stream) {
+  dim3 grid((n_points + 255) / 256, n_dims);
+  hipLaunchKernelGGL(debug_halton_kernel, grid, dim3(256), 0, stream, tables, rows, first_point, n_points,
+                     first_dim, n_dims, digits, uniform, normal);
+  return hipGetLastError();
+}
+
+// tab[dim][owen_slot(v)] = top kOwenTableBits bits of owen_scramble(v << (32-L), key(dim)).
+// Lanes run over slots (coalesced stores); the prefix of a slot is its rotation back.
+__global__ __launch_bounds__(256) void owen_table_kernel(uint64_t seed, int dims, OwenEntry* __restrict__ out) {
+  constexpr uint32_t mask = (1u << kOwenTableBits) - 1u;
+  const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
   const uint32_t dim = blockIdx.y;
-  if (v >= (1u << kOwenTableBits) || (int)dim >= dims) return;
+  if (slot > mask || (int)dim >= dims) return;
+  const uint32_t v = ((slot >> 6) | (slot << (kOwenTableBits - 6))) & mask;
   const uint64_t key = seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
   const uint32_t x = v << (32 - kOwenTableBits);
   uint64_t P = 0;
   for (int bit = 31; bit >= 32 - kOwenTableBits; --bit) owen_level(x, key, bit, P);
-  out[(size_t)dim * (1u << kOwenTableBits) + v] = (uint16_t)owen_prefix(P);
+  out[(size_t)dim * (1u << kOwenTableBits) + slot] = (OwenEntry)owen_prefix(P);
 }
 
-hipError_t launch_owen_table(uint64_t seed, int dims, uint16_t* out, hipStream_t stream) {
+hipError_t launch_owen_table(uint64_t seed, int dims, OwenEntry* out, hipStream_t stream) {
   dim3 grid((1u << kOwenTableBits) / 256, dims);
   hipLaunchKernelGGL(owen_table_kernel, grid, dim3(256), 0, stream, seed, dims, out);
   return hipGetLastError();
@@ -240,7 +304,7 @@ hipError_t launch_ledger(const ItemDesc* items, int item, uint64_t paths, const 
   return hipGetLastError();
 }
 
-hipError_t launch_normal_table_f64(const uint32_t* dirs, const uint16_t* owen_tab, uint64_t seed,
+hipError_t launch_normal_table_f64(const uint32_t* dirs, const OwenEntry* owen_tab, uint64_t seed,
                                    int dims, int n_paths, int nbits, double* out, hipStream_t stream) {
   dim3 grid((n_paths + 255) / 256, dims);
   hipLaunchKernelGGL((normal_table_kernel<double>), grid, dim3(256), 0, stream, dirs, owen_tab, seed,

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "exp_log.h"
 #include "phoenix_layout.h"
 
 namespace riga {
@@ -118,7 +119,7 @@ struct ZigSlow {
     }
     if (state == 1) {
       const double v = (double)(w >> 11) * (1.0 / 9007199254740992.0);  // StandardUniform f64
-      if (zf[i + 1] + (zf[i] - zf[i + 1]) * v < exp(-x * x / 2.0)) {
+      if (zf[i + 1] + (zf[i] - zf[i + 1]) * v < riga_exp(-x * x / 2.0)) {
         *out = x;
         return true;
       }
@@ -128,11 +129,11 @@ struct ZigSlow {
     // Open01: a double in [1, 2) minus (1 - eps/2)
     const double o = bits_f64((w >> 12) | 0x3FF0000000000000ull) - (1.0 - 2.220446049250313e-16 / 2.0);
     if (state == 2) {
-      tx = log(o) / kZigR;
+      tx = riga_log(o) / kZigR;
       state = 3;
       return false;
     }
-    const double ty = log(o);
+    const double ty = riga_log(o);
     if (-2.0 * ty < tx * tx) {
       state = 2;
       return false;

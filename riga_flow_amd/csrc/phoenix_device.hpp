@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "exp_log.h"
 #include "mc_rng.hpp"
 #include "phoenix_layout.h"
 
@@ -47,6 +48,19 @@ constexpr uint32_t kOwenC3InvLo = 0xF69349B7u;  // (C3 mod 2^32)^-1 mod 2^32
 static_assert((uint32_t)kOwenC3 * kOwenC3InvLo == 1u, "C3 inverse");
 
 __device__ __forceinline__ uint32_t owen_prefix(uint64_t P) { return (uint32_t)P * kOwenC3InvLo; }
+
+// Position of a coordinate's prefix-table entry within its dimension's row.
+// The 64 lanes of a wave hold consecutive Sobol points; their gray codes differ
+// only in the low 6 bits, which select direction numbers v[0..5], and v[k]
+// occupies bits 31-k..31 only -- so across a wave the top 6 bits of x take all
+// 64 values while the rest of the prefix is the same.  The row keeps entry
+// p = x >> (32 - L) at slot rotl_L(p, 6): the wave's 64 entries are adjacent,
+// one or two 128-byte lines per gather instead of 64.
+__host__ __device__ __forceinline__ uint32_t owen_slot(uint32_t prefix) {
+  constexpr uint32_t mask = (1u << kOwenTableBits) - 1u;
+  return ((prefix << 6) | (prefix >> (kOwenTableBits - 6))) & mask;
+}
+__device__ __forceinline__ uint32_t owen_slot_of(uint32_t x) { return owen_slot(x >> (32 - kOwenTableBits)); }
 
 __device__ __forceinline__ void owen_level(uint32_t x, uint64_t key, int bit, uint64_t& P) {
   constexpr uint64_t C4 = 0xBF58476D1CE4E5B9ull;
@@ -129,8 +143,8 @@ __device__ __forceinline__ uint32_t owen_low8(uint32_t x, uint64_t key, uint32_t
   return owen_prefix(P);
 }
 
-__device__ __forceinline__ uint32_t owen_f32(uint32_t x, uint64_t key, const uint16_t* __restrict__ tab_row) {
-  const uint32_t p = owen_top24(x, key, tab_row != nullptr, tab_row ? tab_row[x >> (32 - kOwenTableBits)] : 0u);
+__device__ __forceinline__ uint32_t owen_f32(uint32_t x, uint64_t key, const OwenEntry* __restrict__ tab_row) {
+  const uint32_t p = owen_top24(x, key, tab_row != nullptr, tab_row ? tab_row[owen_slot_of(x)] : 0u);
   return owen_f32_tail(p) ? owen_low8(x, key, p) : ((p << 8) | 0x80u);
 }
 
@@ -204,7 +218,7 @@ __device__ __forceinline__ double probit_f64_tail(double p) {
   if (p < 1.0e-12) p = 1.0e-12;
   if (p > 1.0 - 1.0e-12) p = 1.0 - 1.0e-12;
   const bool lower = p < 0.02425;
-  const double q = sqrt(-2.0 * log(lower ? p : 1.0 - p));
+  const double q = sqrt(-2.0 * riga_log(lower ? p : 1.0 - p));
   const double num = ((((C0 * q + C1) * q + C2) * q + C3) * q + C4) * q + C5;
   const double den = (((D0 * q + D1) * q + D2) * q + D3) * q + 1.0;
   return lower ? num / den : -num / den;
@@ -229,8 +243,8 @@ __device__ __forceinline__ float probit_f32_central(double p) {
                    B4 = -1.328068155288572e+01;
   const double q = p - 0.5;
   const double r = q * q;
-  const double num = (((((A0 * r + A1) * r + A2) * r + A3) * r + A4) * r + A5) * q;
-  const double den = ((((B0 * r + B1) * r + B2) * r + B3) * r + B4) * r + 1.0;
+  const double num = fma(fma(fma(fma(fma(A0, r, A1), r, A2), r, A3), r, A4), r, A5) * q;
+  const double den = fma(fma(fma(fma(fma(B0, r, B1), r, B2), r, B3), r, B4), r, 1.0);
   return __fdividef((float)num, (float)den);
 }
 
@@ -244,9 +258,12 @@ __device__ __forceinline__ float probit_f32_tail(double p) {
   if (p > 1.0 - 1.0e-12) p = 1.0 - 1.0e-12;
   const bool lower = p < 0.5;
   const float t = (float)(lower ? p : 1.0 - p);
-  const float q = __fsqrt_rn(-2.0f * __logf(t));
-  const float num = ((((C0 * q + C1) * q + C2) * q + C3) * q + C4) * q + C5;
-  const float den = (((D0 * q + D1) * q + D2) * q + D3) * q + 1.0f;
+  // the logarithm from exp_log.h in FP64, rounded once: the device's float log
+  // intrinsic is lowered differently from one kernel variant to the next, and a
+  // path's normals must not depend on the launch it is priced in
+  const float q = __fsqrt_rn(-2.0f * (float)riga_log((double)t));
+  const float num = fmaf(fmaf(fmaf(fmaf(fmaf(C0, q, C1), q, C2), q, C3), q, C4), q, C5);
+  const float den = fmaf(fmaf(fmaf(fmaf(D0, q, D1), q, D2), q, D3), q, 1.0f);
   const float z = __fdividef(num, den);
   return lower ? z : -z;
 }
@@ -370,13 +387,21 @@ __device__ __forceinline__ void table_normals(const ItemDesc& it, int node, uint
   for (int f = 0; f < F; ++f) z[f] = zt[(size_t)((uint32_t)node * F + f) * (size_t)it.zt_stride + (n - 1)];
 }
 
+// The tiers' multiply-adds, written out: both tiers are compiled without
+// contraction, so the FP64 tier rounds the product and the sum apart as the
+// reference does, and the FP32 tier fuses exactly where it says so -- the same
+// instructions in every kernel variant, so a result does not depend on the
+// launch it was priced in.
+__device__ __forceinline__ double tier_madd(double a, double b, double c) { return a * b + c; }
+__device__ __forceinline__ float tier_madd(float a, float b, float c) { return fmaf(a, b, c); }
+
 template <typename Real, int F>
 __device__ __forceinline__ void correlate(const ItemDesc& it, const Real (&z)[F], Real (&y)[F]) {
 #pragma unroll
   for (int i = 0; i < F; ++i) {
     Real acc = Real(0);
 #pragma unroll
-    for (int j = 0; j <= i; ++j) acc += chol_of(it, i * kMaxFactors + j, Real(0)) * z[j];
+    for (int j = 0; j <= i; ++j) acc = tier_madd(chol_of(it, i * kMaxFactors + j, Real(0)), z[j], acc);
     y[i] = acc;
   }
 }
@@ -401,7 +426,7 @@ __device__ __forceinline__ void sobol_fetch(const ItemDesc& it, const unsigned c
     const uint32_t d = dim % (uint32_t)it.ladder_dims;
     const uint32_t* row = at<uint32_t>(arena, it.off_dirs) + (size_t)d * 32;
     sf.x[f] = sobol_x(gray, row, it.nbits);
-    sf.top[f] = tab ? (uint32_t)it.owen_tab[((size_t)dim << kOwenTableBits) + (sf.x[f] >> (32 - kOwenTableBits))] : 0u;
+    sf.top[f] = tab ? (uint32_t)it.owen_tab[((size_t)dim << kOwenTableBits) + owen_slot_of(sf.x[f])] : 0u;
   }
 }
 
@@ -413,8 +438,10 @@ template <int F>
 __device__ __forceinline__ void sobol_ints_f32(const ItemDesc& it, int node, const SobolFetch<F>& sf,
                                                uint32_t (&xs)[F]) {
   const bool tab = (it.flags & kOwenTable) != 0;
+  // the full-ladder option sends every coordinate through the last 8 levels
+  const uint32_t all = (it.flags & kFullLadder) ? (1u << F) - 1u : 0u;
   uint64_t keys[F];
-  uint32_t pend = 0;
+  uint32_t pend = all;
 #pragma unroll
   for (int f = 0; f < F; ++f) {
     keys[f] = it.seed ^ ((uint64_t)((uint32_t)node * F + f) * 0x9E3779B97F4A7C15ull);
@@ -494,10 +521,11 @@ __device__ __forceinline__ void node_normals(const ItemDesc& it, const unsigned 
 }
 
 
-// exp/log per tier: FP64 uses the correctly rounded-to-1ulp device libm.
-__device__ __forceinline__ double tier_exp(double x) { return exp(x); }
+// exp/log per tier: FP64 uses exp_log.h's functions (the same bits as the
+// oracle's host build of them); FP32 the device's fast intrinsics.
+__device__ __forceinline__ double tier_exp(double x) { return riga_exp(x); }
 __device__ __forceinline__ float tier_exp(float x) { return __expf(x); }
-__device__ __forceinline__ double tier_log(double x) { return log(x); }
+__device__ __forceinline__ double tier_log(double x) { return riga_log(x); }
 __device__ __forceinline__ float tier_log(float x) { return __logf(x); }
 __device__ __forceinline__ double tier_div(double a, double b) { return a / b; }
 __device__ __forceinline__ float tier_div(float a, float b) { return __fdividef(a, b); }
@@ -568,7 +596,7 @@ __device__ __noinline__ float lv_sigma_f32_general(const float* __restrict__ ls,
 // dimension, so the direction row, the key and the Owen table row are uniform.
 template <typename Real>
 __global__ __launch_bounds__(256) void normal_table_kernel(const uint32_t* __restrict__ dirs,
-                                                           const uint16_t* __restrict__ owen_tab,
+                                                           const OwenEntry* __restrict__ owen_tab,
                                                            uint64_t seed, int n_paths, int nbits,
                                                            Real* __restrict__ out) {
   const int path = blockIdx.x * 256 + threadIdx.x;
@@ -581,7 +609,7 @@ __global__ __launch_bounds__(256) void normal_table_kernel(const uint32_t* __res
   Real z;
   if constexpr (sizeof(Real) == 8) {
     const uint32_t xs = owen_tab ? owen_scramble_below<true>(x, key, owen_tab[((size_t)dim << kOwenTableBits) +
-                                                                     (x >> (32 - kOwenTableBits))])
+                                                                     owen_slot_of(x)])
                                  : owen_scramble(x, key);
     z = probit_f64(((double)xs + 0.5) / 4294967296.0);
   } else {
@@ -601,606 +629,625 @@ typedef __attribute__((address_space(3))) const F32x4 LdsF32x4;
 // kMode: how the items of the launch get their normals (kLadderDirect: Sobol
 // or Halton per lane; kLadderTable: the shared normal table; kLadderMonteCarlo:
 // the ChaCha8 stream).  Each mode compiles only its own ladder code.
+//
+// Work queue: each workgroup claims its (item, chunk) unit from the launch's
+// device counter -- the next index of the block table, which lists the longest
+// items first -- so units start longest first in the order workgroups get a
+// CU, whatever the dispatcher's order.  A unit's results (its block partial)
+// do not depend on which workgroup ran it.  `queue` is zero when the launch
+// starts; the grid holds at least as many workgroups as units.  (A looping
+// form, in which resident workgroups claim units until the table is empty,
+// holds 14-33 more VGPRs across its back edge -- the FP32 table-mode kernel
+// goes from 147 to 180, three waves per SIMD to two -- for nothing: a unit
+// runs milliseconds, its dispatch microseconds.)
 template <typename Real, int F, int SMAX, int kMode>
 __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restrict__ items,
                                                           const int32_t* __restrict__ block_item,
                                                           const unsigned char* __restrict__ arena,
                                                           Partial* __restrict__ partials,
-                                                          PathRecordDev* __restrict__ records) {
+                                                          PathRecordDev* __restrict__ records,
+                                                          uint32_t* __restrict__ queue, int n_units) {
   extern __shared__ __align__(16) unsigned char lds_dyn[];
   __shared__ BlockShared sh[SMAX];
   __shared__ SfF32 sf_lds[sizeof(Real) == 4 ? SMAX * F : 1];  // FP32 Euler constants, read per step
+  __shared__ int unit_sh;
 
-  const int item = block_item[2 * blockIdx.x];  // block table: (item, chunk) pairs
-  const ItemDesc& it = items[item];
-  const int S = it.S;
-  const int steps = it.steps;
-  const int tid = threadIdx.x;
-  const int nthreads = blockDim.x;  // 64, 128 or 256: the launch shape the autotuner picked
-  const int nwaves = nthreads >> 6;
-  const int wave = tid >> 6;
-  const int lane = tid & 63;
-  const int chunk = block_item[2 * blockIdx.x + 1];
-  const bool local_vol = (it.flags & kLocalVol) != 0;
-  const bool trace = (it.flags & kTrace) != 0;
-  const bool want_records = (it.flags & kRecords) != 0 && records != nullptr;
-  constexpr bool kLogState = sizeof(Real) == 4;  // FP32 tier carries X = ln(S / S0)
-  constexpr bool kMC = kMode == kLadderMonteCarlo;
-  constexpr bool kTab = kMode == kLadderTable;
-  constexpr int kSurfWords = kLogState ? 4 : 3;   // LDS words per knot and surface
-  const bool lv_uniform = kLogState && (it.flags & kLvUniform) != 0;
-  // dynamic LDS: QMC lanes [bridge stack: (depth-1) x F x lanes][local-vol rows:
-  // n_surf x kSurfWords x max_nx]; the Monte Carlo lane [stream ring: slots x
-  // lanes][ziggurat x_i, f(x_i)][local-vol rows]
-  uint64_t* mc_ring = reinterpret_cast<uint64_t*>(lds_dyn);
-  constexpr int kRingSlots = mc_ring_slots(SMAX);
-  double* zx = reinterpret_cast<double*>(mc_ring + (kMC ? kRingSlots * nthreads : 0));
-  double* zf = zx + kZigLen;
-  Real* lds_stack = reinterpret_cast<Real*>(lds_dyn);
-  Real* lds_w = kMC ? reinterpret_cast<Real*>(zf + kZigLen) : lds_stack + (size_t)it.stack_levels * F * nthreads;
-  const Real* lds = lds_w;
-  const int nxs = it.max_nx;
-  const StepInfo* stepinfo = at<StepInfo>(arena, it.off_steps);
-  const BridgeNode* bnodes = at<BridgeNode>(arena, it.off_nodes);
-  const LvBracket* lvb = at<LvBracket>(arena, it.off_lvb);
-  const ScenarioDesc* scen = at<ScenarioDesc>(arena, it.off_scen);
-  const LvSurface* surf = at<LvSurface>(arena, it.off_surf);
+  {
+    if (threadIdx.x == 0) unit_sh = (int)atomicAdd(queue, 1u);
+    __syncthreads();
+    const int unit = unit_sh;
+    if (unit >= n_units) return;  // the same value in every lane: the whole block leaves
+    const int item = block_item[2 * unit];  // block table: (item, chunk) pairs
+    const ItemDesc& it = items[item];
+    const int S = it.S;
+    const int steps = it.steps;
+    const int tid = threadIdx.x;
+    const int nthreads = blockDim.x;  // 64, 128 or 256: the launch shape the autotuner picked
+    const int nwaves = nthreads >> 6;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const int chunk = block_item[2 * unit + 1];
+    const bool local_vol = (it.flags & kLocalVol) != 0;
+    const bool trace = (it.flags & kTrace) != 0;
+    const bool want_records = (it.flags & kRecords) != 0 && records != nullptr;
+    constexpr bool kLogState = sizeof(Real) == 4;  // FP32 tier carries X = ln(S / S0)
+    constexpr bool kMC = kMode == kLadderMonteCarlo;
+    constexpr bool kTab = kMode == kLadderTable;
+    constexpr int kSurfWords = kLogState ? 4 : 3;   // LDS words per knot and surface
+    const bool lv_uniform = kLogState && (it.flags & kLvUniform) != 0;
+    // dynamic LDS: QMC lanes [bridge stack: (depth-1) x F x lanes][local-vol rows:
+    // n_surf x kSurfWords x max_nx]; the Monte Carlo lane [stream ring: slots x
+    // lanes][ziggurat x_i, f(x_i)][local-vol rows]
+    uint64_t* mc_ring = reinterpret_cast<uint64_t*>(lds_dyn);
+    constexpr int kRingSlots = mc_ring_slots(SMAX);
+    double* zx = reinterpret_cast<double*>(mc_ring + (kMC ? kRingSlots * nthreads : 0));
+    double* zf = zx + kZigLen;
+    Real* lds_stack = reinterpret_cast<Real*>(lds_dyn);
+    Real* lds_w = kMC ? reinterpret_cast<Real*>(zf + kZigLen) : lds_stack + (size_t)it.stack_levels * F * nthreads;
+    const Real* lds = lds_w;
+    const int nxs = it.max_nx;
+    const StepInfo* stepinfo = at<StepInfo>(arena, it.off_steps);
+    const BridgeNode* bnodes = at<BridgeNode>(arena, it.off_nodes);
+    const LvBracket* lvb = at<LvBracket>(arena, it.off_lvb);
+    const ScenarioDesc* scen = at<ScenarioDesc>(arena, it.off_scen);
+    const LvSurface* surf = at<LvSurface>(arena, it.off_surf);
 
-  for (int s = 0; s < SMAX; ++s) {
-    for (int i = tid; i < kWaves * kNumQ; i += nthreads) sh[s].acc[i / kNumQ][i % kNumQ] = DD{0.0, 0.0};
-    for (int i = tid; i < kWaves * kNumC; i += nthreads) sh[s].cnt[i / kNumC][i % kNumC] = 0;
-    for (int i = tid; i < kWaves; i += nthreads) sh[s].wave_ko[i] = 0;
-    if (tid == 0) sh[s].ev_n = 0;
-  }
-  if constexpr (sizeof(Real) == 4) {
-    const SfF32* sfg = at<SfF32>(arena, it.off_sf32);
-    // surface offsets are staged as absolute LDS addresses, so a slope-row read
-    // is one multiply-add of the knot index away from its address
-    const int32_t lv_base = (int32_t)(uint32_t)(uintptr_t)((const LdsBytes*)lds_w);
-    for (int i = tid; i < S * F; i += nthreads) {
-      SfF32 c = sfg[i];
-      c.lds_off += lv_base;
-      sf_lds[i] = c;
-    }
-  }
-  if constexpr (kMC) {
-    const double* zg = at<double>(arena, it.off_zig);
-    for (int i = tid; i < 2 * kZigLen; i += nthreads) zx[i] = zg[i];
-  }
-  __syncthreads();
-  const Real sqrt_dt_r = (Real)it.sqrt_dt;
-  // -dt/2 for the FP32 step, held in a vector register so the step loop does
-  // not reload it through the scalar cache (whose wait also stalls LDS reads)
-  float nhdt = it.neg_half_dt;
-  asm volatile("" : "+v"(nhdt));
-  // the FP32 scan's barriers and coupon rate, held the same way: reloading them
-  // per scenario through the scalar cache would stall on every in-flight LDS read
-  float ln_ko32 = it.ln_ko32, ln_cb32 = it.ln_cb32, cr32 = (float)it.coupon_rate;
-  asm volatile("" : "+v"(ln_ko32), "+v"(ln_cb32), "+v"(cr32));
-  // and the FP64 step's constants (its kernels have registers to spare: their
-  // bridge stack in LDS holds them at two workgroups per CU)
-  double dt64 = 0.0, sqdt64 = 0.0, ko64 = 0.0, cb64 = 0.0, cr64 = 0.0;
-  double s0_64[F];
-#pragma unroll
-  for (int f = 0; f < F; ++f) s0_64[f] = 1.0;
-  if constexpr (!kLogState) {
-    dt64 = it.dt;
-    sqdt64 = it.sqrt_dt;
-    ko64 = it.ko;
-    cb64 = it.cb;
-    cr64 = it.coupon_rate;
-    asm volatile("" : "+v"(dt64), "+v"(sqdt64), "+v"(ko64), "+v"(cb64), "+v"(cr64));
-#pragma unroll
-    for (int f = 0; f < F; ++f) {
-      s0_64[f] = it.initial_spot[f];
-      asm volatile("" : "+v"(s0_64[f]));
-    }
-  }
-
-  const int ppb = it.paths_per_block;
-  const int batches = ppb / nthreads;
-  for (int b = 0; b < batches; ++b) {
-    const uint64_t path_local = (uint64_t)chunk * ppb + (uint64_t)b * nthreads + tid;
-    const bool valid = path_local < it.paths;
-    const uint64_t path = it.path_list ? it.path_list[valid ? path_local : 0] : it.path_begin + path_local;
-    const uint32_t n = (uint32_t)(path + 1);  // Sobol/Halton point index (qmc.rs:286, 121)
-    const uint32_t gray = n ^ (n >> 1);
-
-    // ---- path state
-    BridgeStack<Real, F> bs;
-    bs.col = lds_stack + tid;
-    bs.stride = nthreads;
-    bs.sp = 0;
-    Real wlo[F];
-    Real spot[SMAX][F];
-    int ko_step[SMAX];
-    Real accrual[SMAX], unpaid[SMAX], cpv[SMAX], mpv[SMAX];
-    bool received[SMAX];
-    // FP32 tier: KO step + 1 (bits 0-19, 0 = not knocked out), unpaid memory
-    // coupons as a count (bits 20-30) and coupon received (bit 31) in one word
-    uint32_t pk[SMAX];
-    Real worst_last[SMAX];
-    McRing<kRingSlots> ring;
-    McCursor cur;
-    uint64_t wpos = 0;  // stream index of the lane's next word
-    if constexpr (kMC) {
-      // the path's words start where the parse found its first draw
-      wpos = it.mc_start[valid ? path : it.path_begin];
-      ring.base = mc_ring + tid;
-      ring.stride = nthreads;
-      ring.seek(it.mc_key, wpos);
-      cur.slow = it.mc_slow;
-      cur.val = it.mc_val;
-      cur.chunks = it.mc_chunks;
-      cur.n_chunks = it.mc_n_chunks;
-      cur.C = it.mc_chunk_words;
-      cur.cap = it.mc_slow_cap;
-      cur.seek(wpos);
-    } else {
-      Real y[F];
-      if constexpr (kTab) {
-        Real z[F];
-        table_normals<Real, F>(it, 0, n, z);
-        correlate<Real, F>(it, z, y);
-      } else {
-        node_normals<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, arena, n, gray, 0, y);
-      }
-#pragma unroll
-      for (int f = 0; f < F; ++f) {
-        bs.top[f] = y[f] * root_sd_of(it, Real(0));  // W[T] = y_0 sqrt(T dt), qmc.rs:165
-        wlo[f] = Real(0);
-      }
-    }
-#pragma unroll
     for (int s = 0; s < SMAX; ++s) {
-#pragma unroll
-      for (int f = 0; f < F; ++f)
-        spot[s][f] = kLogState ? (Real)scen[s < S ? s : 0].ln_rel[f] : (Real)scen[s < S ? s : 0].spot[f];
-      ko_step[s] = -1;
-      accrual[s] = unpaid[s] = cpv[s] = mpv[s] = Real(0);
-      received[s] = false;
-      pk[s] = 0u;
-      worst_last[s] = kLogState ? Real(0) : Real(1);  // final_worst starts at 1.0 (note.rs:115)
+      for (int i = tid; i < kWaves * kNumQ; i += nthreads) sh[s].acc[i / kNumQ][i % kNumQ] = DD{0.0, 0.0};
+      for (int i = tid; i < kWaves * kNumC; i += nthreads) sh[s].cnt[i / kNumC][i % kNumC] = 0;
+      for (int i = tid; i < kWaves; i += nthreads) sh[s].wave_ko[i] = 0;
+      if (tid == 0) sh[s].ev_n = 0;
+    }
+    if constexpr (sizeof(Real) == 4) {
+      const SfF32* sfg = at<SfF32>(arena, it.off_sf32);
+      // surface offsets are staged as absolute LDS addresses, so a slope-row read
+      // is one multiply-add of the knot index away from its address
+      const int32_t lv_base = (int32_t)(uint32_t)(uintptr_t)((const LdsBytes*)lds_w);
+      for (int i = tid; i < S * F; i += nthreads) {
+        SfF32 c = sfg[i];
+        c.lds_off += lv_base;
+        sf_lds[i] = c;
+      }
+    }
+    if constexpr (kMC) {
+      const double* zg = at<double>(arena, it.off_zig);
+      for (int i = tid; i < 2 * kZigLen; i += nthreads) zx[i] = zg[i];
+    }
+    __syncthreads();
+    const Real sqrt_dt_r = (Real)it.sqrt_dt;
+    // -dt/2 for the FP32 step, held in a vector register so the step loop does
+    // not reload it through the scalar cache (whose wait also stalls LDS reads)
+    float nhdt = it.neg_half_dt;
+    asm volatile("" : "+v"(nhdt));
+    // the FP32 scan's barriers and coupon rate, held the same way: reloading them
+    // per scenario through the scalar cache would stall on every in-flight LDS read
+    float ln_ko32 = it.ln_ko32, ln_cb32 = it.ln_cb32, cr32 = (float)it.coupon_rate;
+    asm volatile("" : "+v"(ln_ko32), "+v"(ln_cb32), "+v"(cr32));
+    // and the FP64 step's constants (its kernels have registers to spare: their
+    // bridge stack in LDS holds them at two workgroups per CU)
+    double dt64 = 0.0, sqdt64 = 0.0, ko64 = 0.0, cb64 = 0.0, cr64 = 0.0;
+    double s0_64[F];
+  #pragma unroll
+    for (int f = 0; f < F; ++f) s0_64[f] = 1.0;
+    if constexpr (!kLogState) {
+      dt64 = it.dt;
+      sqdt64 = it.sqrt_dt;
+      ko64 = it.ko;
+      cb64 = it.cb;
+      cr64 = it.coupon_rate;
+      asm volatile("" : "+v"(dt64), "+v"(sqdt64), "+v"(ko64), "+v"(cb64), "+v"(cr64));
+  #pragma unroll
+      for (int f = 0; f < F; ++f) {
+        s0_64[f] = it.initial_spot[f];
+        asm volatile("" : "+v"(s0_64[f]));
+      }
     }
 
-    int node_ptr = 0;
-    // The next node's loads are issued one descent ahead, so their latency
-    // overlaps the current step's scenario work: its normals from the shared
-    // table, or its Sobol coordinates and Owen prefix-table entries.
-    const bool sobol = kMode == kLadderDirect && (it.flags & (kNormalTable | kHalton)) == 0;
-    Real zn[F];
-    SobolFetch<F> sf;
-#pragma unroll
-    for (int f = 0; f < F; ++f) {
-      zn[f] = Real(0);
-      sf.x[f] = sf.top[f] = 0u;
-    }
-    if constexpr (kTab) {
-      if (steps > 1) table_normals<Real, F>(it, bnodes[0].node, n, zn);
-    } else {
-      if (sobol && steps > 1) sobol_fetch<F>(it, arena, gray, bnodes[0].node, sf);
-    }
-    int loaded_i0[F];
-#pragma unroll
-    for (int f = 0; f < F; ++f) loaded_i0[f] = -1;
+    const int ppb = it.paths_per_block;
+    const int batches = ppb / nthreads;
+    for (int b = 0; b < batches; ++b) {
+      const uint64_t path_local = (uint64_t)chunk * ppb + (uint64_t)b * nthreads + tid;
+      const bool valid = path_local < it.paths;
+      const uint64_t path = it.path_list ? it.path_list[valid ? path_local : 0] : it.path_begin + path_local;
+      const uint32_t n = (uint32_t)(path + 1);  // Sobol/Halton point index (qmc.rs:286, 121)
+      const uint32_t gray = n ^ (n >> 1);
 
-    for (int step = 0; step < steps; ++step) {
-      const StepInfo si = stepinfo[step];
-      Real shock[F];
-      Real dw[F];  // Brownian increment W[t+1] - W[t]
+      // ---- path state
+      BridgeStack<Real, F> bs;
+      bs.col = lds_stack + tid;
+      bs.stride = nthreads;
+      bs.sp = 0;
+      Real wlo[F];
+      Real spot[SMAX][F];
+      int ko_step[SMAX];
+      Real accrual[SMAX], unpaid[SMAX], cpv[SMAX], mpv[SMAX];
+      bool received[SMAX];
+      // FP32 tier: KO step + 1 (bits 0-19, 0 = not knocked out), unpaid memory
+      // coupons as a count (bits 20-30) and coupon received (bit 31) in one word
+      uint32_t pk[SMAX];
+      Real worst_last[SMAX];
+      McRing<kRingSlots> ring;
+      McCursor cur;
+      uint64_t wpos = 0;  // stream index of the lane's next word
       if constexpr (kMC) {
-        // ---- Monte Carlo lane (note.rs:123-126): F StandardNormals in stream
-        // order, shocks = L z.  A wave tops up every lane's ring together when
-        // any lane is short of F words, so the ChaCha blocks run converged.
-        if (__ballot(ring.fill < F) != 0ull && ring.fill <= kRingSlots - 8) ring.append(it.mc_key);
-        Real z[F];
-#pragma unroll
-        for (int f = 0; f < F; ++f) z[f] = (Real)mc_draw(ring, cur, wpos, it.mc_key, zx);
-        correlate<Real, F>(it, z, shock);
-#pragma unroll
-        for (int f = 0; f < F; ++f) dw[f] = shock[f] * sqrt_dt_r;
+        // the path's words start where the parse found its first draw
+        wpos = it.mc_start[valid ? path : it.path_begin];
+        ring.base = mc_ring + tid;
+        ring.stride = nthreads;
+        ring.seek(it.mc_key, wpos);
+        cur.slow = it.mc_slow;
+        cur.val = it.mc_val;
+        cur.chunks = it.mc_chunks;
+        cur.n_chunks = it.mc_n_chunks;
+        cur.C = it.mc_chunk_words;
+        cur.cap = it.mc_slow_cap;
+        cur.seek(wpos);
       } else {
-      // ---- streaming bridge: descend to the interval (step, step+1)
-      Real wnext[F];
-      if (si.n_desc == 0) {
-#pragma unroll
-        for (int f = 0; f < F; ++f) wnext[f] = bs.top[f];
-        bs.pop();
-      } else {
-        for (int d = 0; d < si.n_desc; ++d) {
-          const BridgeNode bn = bnodes[node_ptr++];
-          Real y[F], wmid[F];
-          if constexpr (kTab) {  // prefetched a step ahead; fetch the following node's now
-            correlate<Real, F>(it, zn, y);
-            if (node_ptr < steps - 1) table_normals<Real, F>(it, bnodes[node_ptr].node, n, zn);
-          } else if (sobol) {
-            Real z[F];
-            sobol_finish<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, bn.node, sf, z);
-            correlate<Real, F>(it, z, y);
-            if (node_ptr < steps - 1) sobol_fetch<F>(it, arena, gray, bnodes[node_ptr].node, sf);
-          } else {
-            node_normals<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, arena, n, gray, bn.node, y);
-          }
-#pragma unroll
-          for (int f = 0; f < F; ++f)
-            wmid[f] = Real(0.5) * (wlo[f] + bs.top[f]) + y[f] * sd_of(bn, Real(0));  // qmc.rs:185-187
-          if (d + 1 < si.n_desc) {
-            bs.push(wmid);
-          } else {
-#pragma unroll
-            for (int f = 0; f < F; ++f) wnext[f] = wmid[f];
-          }
+        Real y[F];
+        if constexpr (kTab) {
+          Real z[F];
+          table_normals<Real, F>(it, 0, n, z);
+          correlate<Real, F>(it, z, y);
+        } else {
+          node_normals<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, arena, n, gray, 0, y);
+        }
+  #pragma unroll
+        for (int f = 0; f < F; ++f) {
+          bs.top[f] = y[f] * root_sd_of(it, Real(0));  // W[T] = y_0 sqrt(T dt), qmc.rs:165
+          wlo[f] = Real(0);
         }
       }
-#pragma unroll
-      for (int f = 0; f < F; ++f) {
-        dw[f] = wnext[f] - wlo[f];
-        shock[f] = dw[f] * (Real)it.inv_sqrt_dt;  // qmc.rs:169, note.rs:101-105
-        wlo[f] = wnext[f];
-      }
+  #pragma unroll
+      for (int s = 0; s < SMAX; ++s) {
+  #pragma unroll
+        for (int f = 0; f < F; ++f)
+          spot[s][f] = kLogState ? (Real)scen[s < S ? s : 0].ln_rel[f] : (Real)scen[s < S ? s : 0].spot[f];
+        ko_step[s] = -1;
+        accrual[s] = unpaid[s] = cpv[s] = mpv[s] = Real(0);
+        received[s] = false;
+        pk[s] = 0u;
+        worst_last[s] = kLogState ? Real(0) : Real(1);  // final_worst starts at 1.0 (note.rs:115)
       }
 
-      // ---- local-vol rows for this step (uniform across the block)
-      if (local_vol) {
-        bool reload = false;
-#pragma unroll
-        for (int f = 0; f < F; ++f) reload |= lvb[step * F + f].i0 != loaded_i0[f];
-        if (reload) {
-          __syncthreads();
-          for (int u = 0; u < it.n_surf; ++u) {
-            const LvSurface su = surf[u];
-            const LvBracket br = lvb[step * F + su.factor];
-            const double* xs = su.dev_x ? su.dev_x : at<double>(arena, su.off_x);
-            const double* vv = su.dev_v ? su.dev_v : at<double>(arena, su.off_v);
-            const double* v0 = vv + (size_t)br.i0 * su.n_x;
-            const double* v1 = vv + (size_t)br.i1 * su.n_x;
-            Real* dst = lds_w + (size_t)u * kSurfWords * nxs;
-            if (lv_uniform) {
-              float4* d4 = reinterpret_cast<float4*>(dst);
-              for (int i = tid; i < su.n_x; i += nthreads) {
-                const int k = i + 1 < su.n_x ? i + 1 : i;
-                d4[i] = make_float4((float)v0[i], (float)(v0[k] - v0[i]), (float)v1[i],
-                                    (float)(v1[k] - v1[i]));
-              }
+      int node_ptr = 0;
+      // The next node's loads are issued one descent ahead, so their latency
+      // overlaps the current step's scenario work: its normals from the shared
+      // table, or its Sobol coordinates and Owen prefix-table entries.
+      const bool sobol = kMode == kLadderDirect && (it.flags & (kNormalTable | kHalton)) == 0;
+      Real zn[F];
+      SobolFetch<F> sf;
+  #pragma unroll
+      for (int f = 0; f < F; ++f) {
+        zn[f] = Real(0);
+        sf.x[f] = sf.top[f] = 0u;
+      }
+      if constexpr (kTab) {
+        if (steps > 1) table_normals<Real, F>(it, bnodes[0].node, n, zn);
+      } else {
+        if (sobol && steps > 1) sobol_fetch<F>(it, arena, gray, bnodes[0].node, sf);
+      }
+      int loaded_i0[F];
+  #pragma unroll
+      for (int f = 0; f < F; ++f) loaded_i0[f] = -1;
+
+      for (int step = 0; step < steps; ++step) {
+        const StepInfo si = stepinfo[step];
+        Real shock[F];
+        Real dw[F];  // Brownian increment W[t+1] - W[t]
+        if constexpr (kMC) {
+          // ---- Monte Carlo lane (note.rs:123-126): F StandardNormals in stream
+          // order, shocks = L z.  A wave tops up every lane's ring together when
+          // any lane is short of F words, so the ChaCha blocks run converged.
+          if (__ballot(ring.fill < F) != 0ull && ring.fill <= kRingSlots - 8) ring.append(it.mc_key);
+          Real z[F];
+  #pragma unroll
+          for (int f = 0; f < F; ++f) z[f] = (Real)mc_draw(ring, cur, wpos, it.mc_key, zx);
+          correlate<Real, F>(it, z, shock);
+  #pragma unroll
+          for (int f = 0; f < F; ++f) dw[f] = shock[f] * sqrt_dt_r;
+        } else {
+        // ---- streaming bridge: descend to the interval (step, step+1)
+        Real wnext[F];
+        if (si.n_desc == 0) {
+  #pragma unroll
+          for (int f = 0; f < F; ++f) wnext[f] = bs.top[f];
+          bs.pop();
+        } else {
+          for (int d = 0; d < si.n_desc; ++d) {
+            const BridgeNode bn = bnodes[node_ptr++];
+            Real y[F], wmid[F];
+            if constexpr (kTab) {  // prefetched a step ahead; fetch the following node's now
+              correlate<Real, F>(it, zn, y);
+              if (node_ptr < steps - 1) table_normals<Real, F>(it, bnodes[node_ptr].node, n, zn);
+            } else if (sobol) {
+              Real z[F];
+              sobol_finish<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, bn.node, sf, z);
+              correlate<Real, F>(it, z, y);
+              if (node_ptr < steps - 1) sobol_fetch<F>(it, arena, gray, bnodes[node_ptr].node, sf);
             } else {
-              for (int i = tid; i < su.n_x; i += nthreads) {
-                dst[i] = (Real)xs[i];
-                dst[nxs + i] = (Real)v0[i];
-                dst[2 * nxs + i] = (Real)v1[i];
-              }
+              node_normals<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, arena, n, gray, bn.node, y);
+            }
+  #pragma unroll
+            for (int f = 0; f < F; ++f)
+              wmid[f] = tier_madd(y[f], sd_of(bn, Real(0)), Real(0.5) * (wlo[f] + bs.top[f]));  // qmc.rs:185-187
+            if (d + 1 < si.n_desc) {
+              bs.push(wmid);
+            } else {
+  #pragma unroll
+              for (int f = 0; f < F; ++f) wnext[f] = wmid[f];
             }
           }
-#pragma unroll
-          for (int f = 0; f < F; ++f) loaded_i0[f] = lvb[step * F + f].i0;
-          __syncthreads();
         }
-      }
+  #pragma unroll
+        for (int f = 0; f < F; ++f) {
+          dw[f] = wnext[f] - wlo[f];
+          shock[f] = dw[f] * (Real)it.inv_sqrt_dt;  // qmc.rs:169, note.rs:101-105
+          wlo[f] = wnext[f];
+        }
+        }
 
-      const bool call_active = (si.flags & 2) != 0;
-      const bool period_end = (si.flags & 1) != 0;
-      const bool new_period = (si.flags & 4) != 0;
-      const bool fixed_now = si.period == 0 && (it.flags & kFixedFirst) != 0;
-      if constexpr (kLogState) {
-        // ---- FP32 tier: X = ln(S/S0) per (scenario, factor), constants from
-        // SfF32, the local-vol knot found by one FMA on evenly spaced knots.
-        // The Phoenix scan below is note.rs:143-218 with the KO'd-path skip
-        // turned into masks (values of a KO'd scenario are never read again).
-        // staged in LDS and read per use; the opaque zero keeps the compiler from
-        // hoisting these reads out of the step loop (which would pin S*F records
-        // in registers for the whole path)
-        int opaque = 0;
-        asm volatile("" : "+s"(opaque));
-        const SfF32* sf = sf_lds + opaque;
-        float wt[F];
+        // ---- local-vol rows for this step (uniform across the block)
         if (local_vol) {
-#pragma unroll
-          for (int f = 0; f < F; ++f) wt[f] = at<float>(arena, it.off_wt32)[step * F + f];
+          bool reload = false;
+  #pragma unroll
+          for (int f = 0; f < F; ++f) reload |= lvb[step * F + f].i0 != loaded_i0[f];
+          if (reload) {
+            __syncthreads();
+            for (int u = 0; u < it.n_surf; ++u) {
+              const LvSurface su = surf[u];
+              const LvBracket br = lvb[step * F + su.factor];
+              const double* xs = su.dev_x ? su.dev_x : at<double>(arena, su.off_x);
+              const double* vv = su.dev_v ? su.dev_v : at<double>(arena, su.off_v);
+              const double* v0 = vv + (size_t)br.i0 * su.n_x;
+              const double* v1 = vv + (size_t)br.i1 * su.n_x;
+              Real* dst = lds_w + (size_t)u * kSurfWords * nxs;
+              if (lv_uniform) {
+                float4* d4 = reinterpret_cast<float4*>(dst);
+                for (int i = tid; i < su.n_x; i += nthreads) {
+                  const int k = i + 1 < su.n_x ? i + 1 : i;
+                  d4[i] = make_float4((float)v0[i], (float)(v0[k] - v0[i]), (float)v1[i],
+                                      (float)(v1[k] - v1[i]));
+                }
+              } else {
+                for (int i = tid; i < su.n_x; i += nthreads) {
+                  dst[i] = (Real)xs[i];
+                  dst[nxs + i] = (Real)v0[i];
+                  dst[2 * nxs + i] = (Real)v1[i];
+                }
+              }
+            }
+  #pragma unroll
+            for (int f = 0; f < F; ++f) loaded_i0[f] = lvb[step * F + f].i0;
+            __syncthreads();
+          }
         }
-        const float inc = fixed_now ? 0.0f : (float)si.accrual_inc;
-        // Phase 1: the Euler step of every (scenario, factor).  The chains are
-        // independent and the local-vol mode is chosen once per step, so the
-        // factors of a scenario form one straight-line block the scheduler can
-        // overlap (the LDS reads of one chain behind the arithmetic of another).
-        if (lv_uniform) {
-          // the Euler step of scenario slot s; slots past S (when S < SMAX) reread
-          // scenario S-1's constants (in bounds) and their state is never read
-#define RIGA_LV_EULER(s)                                                                   \
-  {                                                                                        \
-    const int ss_ = (s) < S ? (s) : S - 1;                                                 \
-    _Pragma("unroll") for (int f = 0; f < F; ++f) {                                        \
-      const SfF32 c = sf[ss_ * F + f];                                                     \
-      /* knot position clamped to [0, tmax] (one med3), its index and weight */            \
-      const float t = __builtin_amdgcn_fmed3f(fmaf(spot[s][f], c.a, c.b), 0.0f, c.tmax);   \
-      const int j = (int)t;                                                                \
-      const float wx = __builtin_amdgcn_fractf(t); /* = t - j exactly, t >= 0 */           \
-      const F32x4 e = *(const LdsF32x4*)(uintptr_t)(uint32_t)(c.lds_off + j * 16);     \
-      const float v0 = fmaf(wx, e.y, e.x);                                                 \
-      const float v1 = fmaf(wx, e.w, e.z);                                                 \
-      const float vol = fmaf(wt[f], v1 - v0, v0);                                          \
-      /* X += (r - q) dt + vol dW - vol^2 dt / 2 */                                        \
-      spot[s][f] += fmaf(vol, fmaf(vol, nhdt, dw[f]), c.c);                                \
-    }                                                                                      \
-  }
-          if (F * SMAX <= 30 && S == SMAX) {
-            // a full bucket: one straight-line block over every scenario, so the LDS
-            // reads of one (scenario, factor) chain overlap the arithmetic of others
-            // (up to 30 chains; more hold too many registers)
-#pragma unroll
-            for (int s = 0; s < SMAX; ++s) RIGA_LV_EULER(s)
-          } else {
-            // a partly filled bucket computes only its scenarios
-#pragma unroll
+
+        const bool call_active = (si.flags & 2) != 0;
+        const bool period_end = (si.flags & 1) != 0;
+        const bool new_period = (si.flags & 4) != 0;
+        const bool fixed_now = si.period == 0 && (it.flags & kFixedFirst) != 0;
+        if constexpr (kLogState) {
+          // ---- FP32 tier: X = ln(S/S0) per (scenario, factor), constants from
+          // SfF32, the local-vol knot found by one FMA on evenly spaced knots.
+          // The Phoenix scan below is note.rs:143-218 with the KO'd-path skip
+          // turned into masks (values of a KO'd scenario are never read again).
+          // staged in LDS and read per use; the opaque zero keeps the compiler from
+          // hoisting these reads out of the step loop (which would pin S*F records
+          // in registers for the whole path)
+          int opaque = 0;
+          asm volatile("" : "+s"(opaque));
+          const SfF32* sf = sf_lds + opaque;
+          float wt[F];
+          if (local_vol) {
+  #pragma unroll
+            for (int f = 0; f < F; ++f) wt[f] = at<float>(arena, it.off_wt32)[step * F + f];
+          }
+          const float inc = fixed_now ? 0.0f : (float)si.accrual_inc;
+          // Phase 1: the Euler step of every (scenario, factor).  The chains are
+          // independent and the local-vol mode is chosen once per step, so the
+          // factors of a scenario form one straight-line block the scheduler can
+          // overlap (the LDS reads of one chain behind the arithmetic of another).
+          if (lv_uniform) {
+            // the Euler step of scenario slot s; slots past S (when S < SMAX) reread
+            // scenario S-1's constants (in bounds) and their state is never read
+  #define RIGA_LV_EULER(s)                                                                   \
+    {                                                                                        \
+      const int ss_ = (s) < S ? (s) : S - 1;                                                 \
+      _Pragma("unroll") for (int f = 0; f < F; ++f) {                                        \
+        const SfF32 c = sf[ss_ * F + f];                                                     \
+        /* knot position clamped to [0, tmax] (one med3), its index and weight */            \
+        const float t = __builtin_amdgcn_fmed3f(fmaf(spot[s][f], c.a, c.b), 0.0f, c.tmax);   \
+        const int j = (int)t;                                                                \
+        const float wx = __builtin_amdgcn_fractf(t); /* = t - j exactly, t >= 0 */           \
+        const F32x4 e = *(const LdsF32x4*)(uintptr_t)(uint32_t)(c.lds_off + j * 16);     \
+        const float v0 = fmaf(wx, e.y, e.x);                                                 \
+        const float v1 = fmaf(wx, e.w, e.z);                                                 \
+        const float vol = fmaf(wt[f], v1 - v0, v0);                                          \
+        /* X += (r - q) dt + vol dW - vol^2 dt / 2 */                                        \
+        spot[s][f] += fmaf(vol, fmaf(vol, nhdt, dw[f]), c.c);                                \
+      }                                                                                      \
+    }
+            if (F * SMAX <= 30 && S == SMAX) {
+              // a full bucket: one straight-line block over every scenario, so the LDS
+              // reads of one (scenario, factor) chain overlap the arithmetic of others
+              // (up to 30 chains; more hold too many registers)
+  #pragma unroll
+              for (int s = 0; s < SMAX; ++s) RIGA_LV_EULER(s)
+            } else {
+              // a partly filled bucket computes only its scenarios
+  #pragma unroll
+              for (int s = 0; s < SMAX; ++s) {
+                if (s >= S) continue;
+                RIGA_LV_EULER(s)
+              }
+            }
+  #undef RIGA_LV_EULER
+          } else if (local_vol) {  // knots not evenly spaced: bracketing search
+  #pragma unroll
             for (int s = 0; s < SMAX; ++s) {
               if (s >= S) continue;
-              RIGA_LV_EULER(s)
+              const int ss = s;
+  #pragma unroll
+              for (int f = 0; f < F; ++f) {
+                const SfF32 c = sf[ss * F + f];
+                const int u = scen[ss].lv[f];
+                const LvSurface& su = surf[u];
+                const float vol = lv_sigma_f32_general(lds + (size_t)u * kSurfWords * nxs, su.n_x, (float)su.inv_dx,
+                                                       wt[f], spot[s][f] + (float)it.ln_s0[f]);
+                spot[s][f] += fmaf(vol, dw[f], fmaf(vol * vol, nhdt, c.c));
+              }
             }
-          }
-#undef RIGA_LV_EULER
-        } else if (local_vol) {  // knots not evenly spaced: bracketing search
-#pragma unroll
-          for (int s = 0; s < SMAX; ++s) {
-            if (s >= S) continue;
-            const int ss = s;
-#pragma unroll
-            for (int f = 0; f < F; ++f) {
-              const SfF32 c = sf[ss * F + f];
-              const int u = scen[ss].lv[f];
-              const LvSurface& su = surf[u];
-              const float vol = lv_sigma_f32_general(lds + (size_t)u * kSurfWords * nxs, su.n_x, (float)su.inv_dx,
-                                                     wt[f], spot[s][f] + (float)it.ln_s0[f]);
-              spot[s][f] += fmaf(vol, dw[f], fmaf(vol * vol, nhdt, c.c));
-            }
-          }
-        } else {
-#pragma unroll
-          for (int s = 0; s < SMAX; ++s) {
-            if (s >= S) continue;
-            const int ss = s;
-#pragma unroll
-            for (int f = 0; f < F; ++f) {
-              const SfF32 c = sf[ss * F + f];
-              spot[s][f] += fmaf(c.a, dw[f], c.c);
-            }
-          }
-        }
-        // Phase 2: worst-of, accrual and the daily KO with its payment as
-        // selects (note.rs:143-195).  The payment runs only in waves where some
-        // lane of the scenario knocks out at this step (a few percent of wave
-        // steps; a wave-uniform branch): for every other lane it changes nothing.
-        const float cr = cr32;
-        const float df = (float)si.df;
-        float w_first = 0.0f;  // scenario 0's worst-of, for replay series
-#pragma unroll
-        for (int s = 0; s < SMAX; ++s) {
-          float w = spot[s][0];
-#pragma unroll
-          for (int f = 1; f < F; ++f) w = fminf(w, spot[s][f]);
-          if (s == 0) w_first = w;
-          worst_last[s] = w;
-          const bool alive = (pk[s] & 0xFFFFFu) == 0u;
-          accrual[s] = (new_period ? 0.0f : accrual[s]) + (w >= ln_cb32 ? inc : 0.0f);
-          const bool ko_now = alive & call_active & (w >= ln_ko32);  // no short-circuit branch
-          if (__ballot(ko_now && s < S) != 0ull) {
-            const float due = fixed_now ? cr : accrual[s];
-            const float carry = due > 0.0f ? (float)((pk[s] >> 20) & 0x7FFu) * cr : 0.0f;
-            const bool pay = ko_now && due + carry > 0.0f;
-            cpv[s] += pay ? due * df : 0.0f;
-            mpv[s] += pay ? carry * df : 0.0f;
-            const uint32_t recv = pay ? 0x80000000u : (pk[s] & 0x80000000u);
-            pk[s] = ko_now ? ((pk[s] & 0x7FF00000u) | recv | (uint32_t)(step + 1)) : pk[s];
-          }
-        }
-        if (it.series != nullptr && valid)  // replay: the worst-of for the coupon strip
-          it.series[(size_t)step * it.paths + path_local] = exp((double)w_first);
-        // Phase 3: period end (note.rs:197-218), a step-uniform branch.
-        if (period_end) {
-          const bool memory = (it.flags & kMemoryCoupon) != 0;
-#pragma unroll
-          for (int s = 0; s < SMAX; ++s) {
-            // lanes knocked out at this very step took their coupon above
-            const bool open = (pk[s] & 0xFFFFFu) == 0u;
-            const float due = fixed_now ? cr : accrual[s];
-            const bool pays = open && due > 0.0f;
-            cpv[s] += pays ? due * df : 0.0f;
-            mpv[s] += pays ? (float)((pk[s] >> 20) & 0x7FFu) * cr * df : 0.0f;
-            uint32_t q = pk[s];
-            q = pays ? ((q & 0x000FFFFFu) | 0x80000000u) : q;          // paid: unpaid count back to 0
-            q = (open && !pays && memory) ? q + (1u << 20) : q;       // missed: one more unpaid coupon
-            pk[s] = q;
-          }
-        }
-        continue;
-      }
-#pragma unroll
-      for (int s = 0; s < SMAX; ++s) {
-        if (s >= S) continue;
-        const ScenarioDesc& sc = scen[s];
-        // ---- Euler step (note.rs:134-142).  FP64: S *= exp(incr), as the
-        // reference rounds it.  FP32: the state is X = ln S and X += incr, the
-        // same step without an exp, a log and two divides per factor.
-#pragma unroll
-        for (int f = 0; f < F; ++f) {
-          Real incr;
-          if (local_vol) {
-            const int u = sc.lv[f];
-            const LvSurface& su = surf[u];
-            Real vol;
-            if constexpr (kLogState) {
-              if (lv_uniform)
-                vol = lv_sigma_uniform(reinterpret_cast<const float4*>(lds + (size_t)u * kSurfWords * nxs),
-                                       su.n_x, (float)su.inv_dx, (float)su.rel_off,
-                                       (float)lvb[step * F + f].wt, spot[s][f]);
-              else
-                vol = lv_sigma<Real, true>(lds + (size_t)u * kSurfWords * nxs, su.n_x, (Real)su.inv_dx,
-                                           (Real)lvb[step * F + f].wt, spot[s][f] + (Real)it.ln_s0[f]);
-            } else {
-              vol = lv_sigma<Real, false>(lds + (size_t)u * kSurfWords * nxs, su.n_x, (Real)su.inv_dx,
-                                          (Real)lvb[step * F + f].wt, spot[s][f]);
-            }
-            const Real drift = ((Real)sc.rq[f] - Real(0.5) * vol * vol) * (Real)dt64;
-            incr = drift + vol * (Real)sqdt64 * shock[f];
           } else {
-            incr = (Real)sc.drift[f] + (Real)sc.vsd[f] * shock[f];
+  #pragma unroll
+            for (int s = 0; s < SMAX; ++s) {
+              if (s >= S) continue;
+              const int ss = s;
+  #pragma unroll
+              for (int f = 0; f < F; ++f) {
+                const SfF32 c = sf[ss * F + f];
+                spot[s][f] += fmaf(c.a, dw[f], c.c);
+              }
+            }
           }
-          if constexpr (kLogState)
-            spot[s][f] += incr;
-          else
-            spot[s][f] *= tier_exp(incr);
-        }
-        // ---- worst-of (note.rs:143-145); the FP32 tier keeps ln(worst)
-        Real worst;
-        if constexpr (kLogState) {
-          worst = spot[s][0];
-#pragma unroll
-          for (int f = 1; f < F; ++f) worst = fminf(worst, spot[s][f]);
-        } else {
-          worst = spot[s][0] / (Real)s0_64[0];
-#pragma unroll
-          for (int f = 1; f < F; ++f) worst = fmin(worst, spot[s][f] / (Real)s0_64[f]);
-        }
-        worst_last[s] = worst;
-        if (ko_step[s] >= 0) continue;  // note.rs:146-148: cash flows stop after KO
-        // ---- coupon accrual (note.rs:151-163)
-        if (new_period) accrual[s] = Real(0);
-        const bool above_coupon = kLogState ? worst >= (Real)it.ln_cb : (double)worst >= cb64;
-        if (!fixed_now && above_coupon) accrual[s] += (Real)si.accrual_inc;
-        // ---- daily global KO (note.rs:166-195): every name at or above the
-        // barrier, i.e. the worst-of at or above it
-        bool all_above = call_active;
-        if constexpr (kLogState) {
-          all_above = all_above && worst >= (Real)it.ln_ko;
-        } else {
-#pragma unroll
-          for (int f = 0; f < F; ++f)
-            all_above = all_above && ((double)(spot[s][f] / (Real)s0_64[f]) >= ko64);
-        }
-        if (all_above) {
-          ko_step[s] = step;
-          const Real due = fixed_now ? (Real)cr64 : accrual[s];
-          const Real carry = due > Real(0) ? unpaid[s] : Real(0);
-          if (due + carry > Real(0)) {
-            cpv[s] += due * (Real)si.df;
-            mpv[s] += carry * (Real)si.df;
-            received[s] = true;
+          // Phase 2: worst-of, accrual and the daily KO with its payment as
+          // selects (note.rs:143-195).  The payment runs only in waves where some
+          // lane of the scenario knocks out at this step (a few percent of wave
+          // steps; a wave-uniform branch): for every other lane it changes nothing.
+          const float cr = cr32;
+          const float df = (float)si.df;
+          float w_first = 0.0f;  // scenario 0's worst-of, for replay series
+  #pragma unroll
+          for (int s = 0; s < SMAX; ++s) {
+            float w = spot[s][0];
+  #pragma unroll
+            for (int f = 1; f < F; ++f) w = fminf(w, spot[s][f]);
+            if (s == 0) w_first = w;
+            worst_last[s] = w;
+            const bool alive = (pk[s] & 0xFFFFFu) == 0u;
+            accrual[s] = (new_period ? 0.0f : accrual[s]) + (w >= ln_cb32 ? inc : 0.0f);
+            const bool ko_now = alive & call_active & (w >= ln_ko32);  // no short-circuit branch
+            if (__ballot(ko_now && s < S) != 0ull) {
+              const float due = fixed_now ? cr : accrual[s];
+              const float carry = due > 0.0f ? (float)((pk[s] >> 20) & 0x7FFu) * cr : 0.0f;
+              const bool pay = ko_now && due + carry > 0.0f;
+              cpv[s] += pay ? due * df : 0.0f;
+              mpv[s] += pay ? carry * df : 0.0f;
+              const uint32_t recv = pay ? 0x80000000u : (pk[s] & 0x80000000u);
+              pk[s] = ko_now ? ((pk[s] & 0x7FF00000u) | recv | (uint32_t)(step + 1)) : pk[s];
+            }
+          }
+          if (it.series != nullptr && valid)  // replay: the worst-of for the coupon strip
+            it.series[(size_t)step * it.paths + path_local] = riga_exp((double)w_first);
+          // Phase 3: period end (note.rs:197-218), a step-uniform branch.
+          if (period_end) {
+            const bool memory = (it.flags & kMemoryCoupon) != 0;
+  #pragma unroll
+            for (int s = 0; s < SMAX; ++s) {
+              // lanes knocked out at this very step took their coupon above
+              const bool open = (pk[s] & 0xFFFFFu) == 0u;
+              const float due = fixed_now ? cr : accrual[s];
+              const bool pays = open && due > 0.0f;
+              cpv[s] += pays ? due * df : 0.0f;
+              mpv[s] += pays ? (float)((pk[s] >> 20) & 0x7FFu) * cr * df : 0.0f;
+              uint32_t q = pk[s];
+              q = pays ? ((q & 0x000FFFFFu) | 0x80000000u) : q;          // paid: unpaid count back to 0
+              q = (open && !pays && memory) ? q + (1u << 20) : q;       // missed: one more unpaid coupon
+              pk[s] = q;
+            }
           }
           continue;
         }
-        // ---- period end (note.rs:197-218)
-        if (period_end) {
-          const Real due = fixed_now ? (Real)cr64 : accrual[s];
-          if (due > Real(0)) {
-            cpv[s] += due * (Real)si.df;
-            mpv[s] += unpaid[s] * (Real)si.df;
-            received[s] = true;
-            unpaid[s] = Real(0);
-          } else if (it.flags & kMemoryCoupon) {
-            unpaid[s] += (Real)cr64;
+  #pragma unroll
+        for (int s = 0; s < SMAX; ++s) {
+          if (s >= S) continue;
+          const ScenarioDesc& sc = scen[s];
+          // ---- Euler step (note.rs:134-142).  FP64: S *= exp(incr), as the
+          // reference rounds it.  FP32: the state is X = ln S and X += incr, the
+          // same step without an exp, a log and two divides per factor.
+  #pragma unroll
+          for (int f = 0; f < F; ++f) {
+            Real incr;
+            if (local_vol) {
+              const int u = sc.lv[f];
+              const LvSurface& su = surf[u];
+              Real vol;
+              if constexpr (kLogState) {
+                if (lv_uniform)
+                  vol = lv_sigma_uniform(reinterpret_cast<const float4*>(lds + (size_t)u * kSurfWords * nxs),
+                                         su.n_x, (float)su.inv_dx, (float)su.rel_off,
+                                         (float)lvb[step * F + f].wt, spot[s][f]);
+                else
+                  vol = lv_sigma<Real, true>(lds + (size_t)u * kSurfWords * nxs, su.n_x, (Real)su.inv_dx,
+                                             (Real)lvb[step * F + f].wt, spot[s][f] + (Real)it.ln_s0[f]);
+              } else {
+                vol = lv_sigma<Real, false>(lds + (size_t)u * kSurfWords * nxs, su.n_x, (Real)su.inv_dx,
+                                            (Real)lvb[step * F + f].wt, spot[s][f]);
+              }
+              const Real drift = ((Real)sc.rq[f] - Real(0.5) * vol * vol) * (Real)dt64;
+              incr = drift + vol * (Real)sqdt64 * shock[f];
+            } else {
+              incr = (Real)sc.drift[f] + (Real)sc.vsd[f] * shock[f];
+            }
+            if constexpr (kLogState)
+              spot[s][f] += incr;
+            else
+              spot[s][f] *= tier_exp(incr);
+          }
+          // ---- worst-of (note.rs:143-145); the FP32 tier keeps ln(worst)
+          Real worst;
+          if constexpr (kLogState) {
+            worst = spot[s][0];
+  #pragma unroll
+            for (int f = 1; f < F; ++f) worst = fminf(worst, spot[s][f]);
+          } else {
+            worst = spot[s][0] / (Real)s0_64[0];
+  #pragma unroll
+            for (int f = 1; f < F; ++f) worst = fmin(worst, spot[s][f] / (Real)s0_64[f]);
+          }
+          worst_last[s] = worst;
+          if (ko_step[s] >= 0) continue;  // note.rs:146-148: cash flows stop after KO
+          // ---- coupon accrual (note.rs:151-163)
+          if (new_period) accrual[s] = Real(0);
+          const bool above_coupon = kLogState ? worst >= (Real)it.ln_cb : (double)worst >= cb64;
+          if (!fixed_now && above_coupon) accrual[s] += (Real)si.accrual_inc;
+          // ---- daily global KO (note.rs:166-195): every name at or above the
+          // barrier, i.e. the worst-of at or above it
+          bool all_above = call_active;
+          if constexpr (kLogState) {
+            all_above = all_above && worst >= (Real)it.ln_ko;
+          } else {
+  #pragma unroll
+            for (int f = 0; f < F; ++f)
+              all_above = all_above && ((double)(spot[s][f] / (Real)s0_64[f]) >= ko64);
+          }
+          if (all_above) {
+            ko_step[s] = step;
+            const Real due = fixed_now ? (Real)cr64 : accrual[s];
+            const Real carry = due > Real(0) ? unpaid[s] : Real(0);
+            if (due + carry > Real(0)) {
+              cpv[s] += due * (Real)si.df;
+              mpv[s] += carry * (Real)si.df;
+              received[s] = true;
+            }
+            continue;
+          }
+          // ---- period end (note.rs:197-218)
+          if (period_end) {
+            const Real due = fixed_now ? (Real)cr64 : accrual[s];
+            if (due > Real(0)) {
+              cpv[s] += due * (Real)si.df;
+              mpv[s] += unpaid[s] * (Real)si.df;
+              received[s] = true;
+              unpaid[s] = Real(0);
+            } else if (it.flags & kMemoryCoupon) {
+              unpaid[s] += (Real)cr64;
+            }
           }
         }
+        if (it.series != nullptr && valid)  // replay: the worst-of for the coupon strip
+          it.series[(size_t)step * it.paths + path_local] = (double)worst_last[0];
       }
-      if (it.series != nullptr && valid)  // replay: the worst-of for the coupon strip
-        it.series[(size_t)step * it.paths + path_local] = (double)worst_last[0];
-    }
 
-    // ---- path end (note.rs:221-258) and reductions
-#pragma unroll
-    for (int s = 0; s < SMAX; ++s) {
-      if (s >= S) continue;
-      if constexpr (kLogState) {  // unpack the FP32 tier's state word
-        ko_step[s] = (int)(pk[s] & 0xFFFFFu) - 1;
-        received[s] = (pk[s] >> 31) != 0u;
-      }
-      const bool ko = ko_step[s] >= 0;
-      const double funding = ko ? stepinfo[ko_step[s]].df : it.put_df;
-      const double fw = kLogState ? exp((double)worst_last[s]) : (double)worst_last[s];
-      const bool hit_ki = fw <= it.ki;
-      const bool eligible = hit_ki && !ko;
-      const double put = eligible ? fmax(it.strike - fw, 0.0) * it.put_df : 0.0;
-      const double nominal = (double)cpv[s] + (double)mpv[s];
-      const double coupon_quote = nominal * 10.0;
-      const double memory_quote = (double)mpv[s] * 10.0;
-      const double pv = funding + nominal - put;
-      if (want_records && valid) {
-        PathRecordDev r;
-        r.ko_step = ko_step[s];
-        r.hit_ki = hit_ki;
-        r.final_worst = fw;
-        r.funding = funding;
-        r.put = put;
-        r.coupon_pv = (double)cpv[s];
-        r.memory_pv = (double)mpv[s];
-        r.pv = pv;
-        records[it.record_base + (int64_t)s * (int64_t)it.paths + (int64_t)path_local] = r;
-      }
-      // plain sums (funding, put, coupon and memory quotes) and (x, x^2)
-      // pairs for the four moment sets; invalid lanes contribute zero.
-      DD qd[kNumQ];
-      qd[kQFunding] = dd_from(valid ? funding : 0.0);
-      qd[kQPut] = dd_from(valid ? put : 0.0);
-      qd[kQCq] = dd_from(valid ? coupon_quote - memory_quote : 0.0);
-      qd[kQMq] = dd_from(valid ? memory_quote : 0.0);
-      const double mx[4] = {pv, ko ? funding : 0.0, hit_ki ? put : 0.0, received[s] ? nominal : 0.0};
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const double x = valid ? mx[k] : 0.0;
-        qd[kQPv1 + 2 * k] = dd_from(x);
-        qd[kQPv2 + 2 * k] = dd_sq(x);
-      }
-#pragma unroll
-      for (int k = 0; k < kNumQ; ++k) qd[k] = wave_sum(qd[k]);
-      const uint64_t b_valid = __ballot(valid);
-      const uint64_t b_ko = __ballot(valid && ko);
-      const uint64_t b_ki = __ballot(valid && hit_ki);
-      const uint64_t b_el = __ballot(valid && eligible);
-      const uint64_t b_cp = __ballot(valid && received[s]);
-      if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < kNumQ; ++k) sh[s].acc[wave][k] = dd_add(sh[s].acc[wave][k], qd[k]);
-        sh[s].cnt[wave][kCN] += __popcll(b_valid);
-        sh[s].cnt[wave][kCKo] += __popcll(b_ko);
-        sh[s].cnt[wave][kCKi] += __popcll(b_ki);
-        sh[s].cnt[wave][kCPutEligible] += __popcll(b_el);
-        sh[s].cnt[wave][kCCouponPaths] += __popcll(b_cp);
-      }
-      if (trace) {
-        // first 64 KO events of this block in path order (note.rs:174-180)
-        if (lane == 0) sh[s].wave_ko[wave] = __popcll(b_ko);
-        __syncthreads();
-        int base = sh[s].ev_n;
-        for (int w = 0; w < wave; ++w) base += sh[s].wave_ko[w];
-        const int rank = base + (int)__popcll(b_ko & ((1ull << lane) - 1ull));
-        if (valid && ko && rank < kTraceCap)  // paths_per_block <= 2048, steps < 2^20
-          sh[s].ev[rank] = ((uint32_t)(b * nthreads + tid) << 20) | (uint32_t)ko_step[s];
-        __syncthreads();
-        if (tid == 0) {
-          int tot = sh[s].ev_n;
-          for (int w = 0; w < nwaves; ++w) tot += sh[s].wave_ko[w];
-          sh[s].ev_n = tot < kTraceCap ? tot : kTraceCap;
+      // ---- path end (note.rs:221-258) and reductions
+  #pragma unroll
+      for (int s = 0; s < SMAX; ++s) {
+        if (s >= S) continue;
+        if constexpr (kLogState) {  // unpack the FP32 tier's state word
+          ko_step[s] = (int)(pk[s] & 0xFFFFFu) - 1;
+          received[s] = (pk[s] >> 31) != 0u;
         }
-        __syncthreads();
+        const bool ko = ko_step[s] >= 0;
+        const double funding = ko ? stepinfo[ko_step[s]].df : it.put_df;
+        const double fw = kLogState ? riga_exp((double)worst_last[s]) : (double)worst_last[s];
+        const bool hit_ki = fw <= it.ki;
+        const bool eligible = hit_ki && !ko;
+        const double put = eligible ? fmax(it.strike - fw, 0.0) * it.put_df : 0.0;
+        const double nominal = (double)cpv[s] + (double)mpv[s];
+        const double coupon_quote = nominal * 10.0;
+        const double memory_quote = (double)mpv[s] * 10.0;
+        const double pv = funding + nominal - put;
+        if (want_records && valid) {
+          PathRecordDev r;
+          r.ko_step = ko_step[s];
+          r.hit_ki = hit_ki;
+          r.final_worst = fw;
+          r.funding = funding;
+          r.put = put;
+          r.coupon_pv = (double)cpv[s];
+          r.memory_pv = (double)mpv[s];
+          r.pv = pv;
+          records[it.record_base + (int64_t)s * (int64_t)it.paths + (int64_t)path_local] = r;
+        }
+        // plain sums (funding, put, coupon and memory quotes) and (x, x^2)
+        // pairs for the four moment sets; invalid lanes contribute zero.
+        DD qd[kNumQ];
+        qd[kQFunding] = dd_from(valid ? funding : 0.0);
+        qd[kQPut] = dd_from(valid ? put : 0.0);
+        qd[kQCq] = dd_from(valid ? coupon_quote - memory_quote : 0.0);
+        qd[kQMq] = dd_from(valid ? memory_quote : 0.0);
+        const double mx[4] = {pv, ko ? funding : 0.0, hit_ki ? put : 0.0, received[s] ? nominal : 0.0};
+  #pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double x = valid ? mx[k] : 0.0;
+          qd[kQPv1 + 2 * k] = dd_from(x);
+          qd[kQPv2 + 2 * k] = dd_sq(x);
+        }
+  #pragma unroll
+        for (int k = 0; k < kNumQ; ++k) qd[k] = wave_sum(qd[k]);
+        const uint64_t b_valid = __ballot(valid);
+        const uint64_t b_ko = __ballot(valid && ko);
+        const uint64_t b_ki = __ballot(valid && hit_ki);
+        const uint64_t b_el = __ballot(valid && eligible);
+        const uint64_t b_cp = __ballot(valid && received[s]);
+        if (lane == 0) {
+  #pragma unroll
+          for (int k = 0; k < kNumQ; ++k) sh[s].acc[wave][k] = dd_add(sh[s].acc[wave][k], qd[k]);
+          sh[s].cnt[wave][kCN] += __popcll(b_valid);
+          sh[s].cnt[wave][kCKo] += __popcll(b_ko);
+          sh[s].cnt[wave][kCKi] += __popcll(b_ki);
+          sh[s].cnt[wave][kCPutEligible] += __popcll(b_el);
+          sh[s].cnt[wave][kCCouponPaths] += __popcll(b_cp);
+        }
+        if (trace) {
+          // first 64 KO events of this block in path order (note.rs:174-180)
+          if (lane == 0) sh[s].wave_ko[wave] = __popcll(b_ko);
+          __syncthreads();
+          int base = sh[s].ev_n;
+          for (int w = 0; w < wave; ++w) base += sh[s].wave_ko[w];
+          const int rank = base + (int)__popcll(b_ko & ((1ull << lane) - 1ull));
+          if (valid && ko && rank < kTraceCap)  // paths_per_block <= 2048, steps < 2^20
+            sh[s].ev[rank] = ((uint32_t)(b * nthreads + tid) << 20) | (uint32_t)ko_step[s];
+          __syncthreads();
+          if (tid == 0) {
+            int tot = sh[s].ev_n;
+            for (int w = 0; w < nwaves; ++w) tot += sh[s].wave_ko[w];
+            sh[s].ev_n = tot < kTraceCap ? tot : kTraceCap;
+          }
+          __syncthreads();
+        }
       }
     }
-  }
 
-  __syncthreads();
-  // ---- block partials, waves merged in order
-  for (int s = 0; s < S && s < SMAX; ++s) {
-    Partial* out = partials + it.partial_base + (int64_t)s * it.n_blocks + chunk;
-    if (tid < kNumQ) {
-      DD a = sh[s].acc[0][tid];
-      for (int w = 1; w < nwaves; ++w) a = dd_add(a, sh[s].acc[w][tid]);
-      out->q[tid] = a;
-    } else if (tid >= 32 && tid < 32 + kNumC) {
-      const int c = tid - 32;
-      uint64_t a = 0;
-      for (int w = 0; w < nwaves; ++w) a += sh[s].cnt[w][c];
-      out->c[c] = a;
-    } else if (tid == 48) {  // within the first wave: every launch shape has it
-      const int ne = trace ? sh[s].ev_n : 0;
-      out->n_events = ne;
-      for (int e = 0; e < ne; ++e) {
-        out->ev_path_lo[e] = (int32_t)((int64_t)chunk * it.paths_per_block + (int64_t)(sh[s].ev[e] >> 20));
-        out->ev_step[e] = (int32_t)(sh[s].ev[e] & 0xFFFFFu);
+    __syncthreads();
+    // ---- block partials, waves merged in order
+    for (int s = 0; s < S && s < SMAX; ++s) {
+      Partial* out = partials + it.partial_base + (int64_t)s * it.n_blocks + chunk;
+      if (tid < kNumQ) {
+        DD a = sh[s].acc[0][tid];
+        for (int w = 1; w < nwaves; ++w) a = dd_add(a, sh[s].acc[w][tid]);
+        out->q[tid] = a;
+      } else if (tid >= 32 && tid < 32 + kNumC) {
+        const int c = tid - 32;
+        uint64_t a = 0;
+        for (int w = 0; w < nwaves; ++w) a += sh[s].cnt[w][c];
+        out->c[c] = a;
+      } else if (tid == 48) {  // within the first wave: every launch shape has it
+        const int ne = trace ? sh[s].ev_n : 0;
+        out->n_events = ne;
+        for (int e = 0; e < ne; ++e) {
+          out->ev_path_lo[e] = (int32_t)((int64_t)chunk * it.paths_per_block + (int64_t)(sh[s].ev[e] >> 20));
+          out->ev_step[e] = (int32_t)(sh[s].ev[e] & 0xFFFFFu);
+        }
       }
     }
   }

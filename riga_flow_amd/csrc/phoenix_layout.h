@@ -23,7 +23,13 @@ constexpr int kMaxLvX = 256;        // log-spot points per local-vol row staged 
 #define RIGA_OWEN_PREFIX_BITS 16
 #endif
 constexpr int kOwenPrefixBits = RIGA_OWEN_PREFIX_BITS;  // Owen levels tabulated per dimension
-static_assert(kOwenPrefixBits >= 8 && kOwenPrefixBits <= 16, "prefix table width out of range");
+static_assert(kOwenPrefixBits >= 8 && kOwenPrefixBits <= 24, "prefix table width out of range");
+// an entry holds the scrambled top kOwenPrefixBits bits of its prefix
+#if RIGA_OWEN_PREFIX_BITS <= 16
+typedef uint16_t OwenEntry;
+#else
+typedef uint32_t OwenEntry;
+#endif
 constexpr int kBlock = 256;         // threads per workgroup (4 waves of 64)
 
 enum ItemFlags : uint32_t {
@@ -37,6 +43,7 @@ enum ItemFlags : uint32_t {
   kNormalTable = 1u << 7,    // ladder normals read from the batch's shared [dim][path] table
   kLvUniform = 1u << 8,      // FP32 tier: every surface has evenly spaced knots (slope rows)
   kMonteCarlo = 1u << 9,     // VarianceReduction::None: ChaCha8 + ziggurat normals (note.rs:124-126)
+  kFullLadder = 1u << 10,    // FP32 tier: all 32 Owen levels for every coordinate (the FP64 tier's integers)
 };
 
 // Monte Carlo lane (ChaCha8 stream + StandardNormal ziggurat).
@@ -157,7 +164,7 @@ struct ItemDesc {
   int32_t n_blocks;
   int32_t paths_per_block;
   int32_t ladder_rows;   // ladder rows uploaded (min(ladder_dims, F*steps))
-  const uint16_t* owen_tab; // device table [dims][2^kOwenPrefixBits]: scrambled top bits per raw dimension
+  const OwenEntry* owen_tab; // device table [dims][2^kOwenPrefixBits]: scrambled top bits per raw dimension
   // FP32 tier: constants pre-rounded on the host, so the path loop converts nothing
   float chol32[kMaxFactors * kMaxFactors];
   float neg_half_dt, ln_ko32, ln_cb32, root_sd32;

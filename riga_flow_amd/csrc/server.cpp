@@ -145,6 +145,7 @@ struct riga_server {
   std::thread worker;
   riga_server_stats stats{};
   double t0 = 0;
+  int waiters = 0;  // callers inside riga_server_wait (destroy waits for them to leave)
 
   riga_launch_config launch_config() const {
     riga_launch_config lc{};
@@ -432,10 +433,15 @@ int riga_server_wait(riga_server* s, uint64_t ticket, double timeout_s, riga_res
   }
   Ticket* tk = it->second.get();
   auto finished = [tk] { return tk->state >= kDone; };
+  s->waiters += 1;
+  bool got = true;
   if (timeout_s < 0)
     s->cv_done.wait(lock, finished);
-  else if (!s->cv_done.wait_for(lock, std::chrono::duration<double>(timeout_s), finished))
-    return 1;  // still pending
+  else
+    got = s->cv_done.wait_for(lock, std::chrono::duration<double>(timeout_s), finished);
+  s->waiters -= 1;
+  if (s->waiters == 0) s->cv_done.notify_all();  // destroy may be waiting for the last waiter
+  if (!got) return 1;  // still pending
   if (info) {
     info->state = tk->state;
     info->n_requests = (int32_t)tk->reqs.size();
@@ -477,12 +483,21 @@ void riga_server_destroy(riga_server* s) {
     for (Ticket* tk : s->queue) {
       tk->state = kShed;
       tk->t_end = now_s();
+      snprintf(tk->err.code, sizeof tk->err.code, "shed");
+      snprintf(tk->err.message, sizeof tk->err.message, "the server was destroyed before the ticket ran");
+      s->stats.shed_tickets += 1;
     }
     s->queue.clear();
   }
   s->cv_work.notify_all();
   if (s->worker.joinable()) s->worker.join();
-  s->cv_done.notify_all();
+  {
+    // every ticket is finished now: wake the waiters and let them leave before
+    // the server (and its mutex) is freed
+    std::unique_lock<std::mutex> lock(s->mu);
+    s->cv_done.notify_all();
+    s->cv_done.wait(lock, [s] { return s->waiters == 0; });
+  }
   delete s;
 }
 

@@ -233,11 +233,31 @@ def poisson_load(server: Server, requests: List[List[PriceRequest]], tiers: List
     ok = [d for d in done if d.state == DONE]
     lat = sorted(d.latency_s for d in ok)
     ps = sum(d.path_steps for d in ok)
+    tier_of = [tiers[i % len(requests)] for i in range(len(tickets))]
+
+    def censored(ds):
+        # shed and failed tickets never finish: they count as later than every
+        # completed one, so a percentile that lands on them is reported as None
+        xs = sorted(d.latency_s if d.state == DONE else math.inf for d in ds)
+        out = {}
+        for q in (50, 99):
+            if not xs:
+                out[f"p{q}_s"] = None
+                continue
+            k = min(len(xs) - 1, int(math.ceil(q / 100.0 * len(xs))) - 1)
+            out[f"p{q}_s"] = xs[k] if xs[k] < math.inf else None
+        return out
+
+    per_tier = {}
+    for tier in sorted(set(tier_of)):
+        ds = [d for d, t in zip(done, tier_of) if t == tier]
+        per_tier[tier] = {"submitted": len(ds), "completed": sum(d.state == DONE for d in ds), **censored(ds)}
     return {
         "qps": qps, "submitted": len(tickets), "completed": len(ok),
         "shed": sum(d.state == SHED for d in done), "failed": sum(d.state == FAILED for d in done),
         "p50_s": float(np.percentile(lat, 50)) if lat else None,
         "p99_s": float(np.percentile(lat, 99)) if lat else None,
+        "censored": censored(done), "per_tier": per_tier,
         "throughput_path_steps_per_s": ps / wall, "wall_s": wall, "arrival_window_s": submitted,
         "mean_batch_tickets": float(np.mean([d.batch_tickets for d in ok])) if ok else None,
     }

@@ -27,6 +27,10 @@ def test_device_is_gfx950_and_named():
     (1, 756, 1, 1024, 0, 24),
     (1, 512, 60000, 256, 500, 40),   # reference cap: dims >= 512 wrap their direction row
     (7, 2268, 262100, 44, 2240, 28),
+    (3, 756, (1 << 21) + 5, 300, 0, 12),           # 22-bit indices: the 24-bit gray-code sweep
+    (3, 2268, (1 << 24) + 77, 300, 2250, 18),      # 25-bit indices: the 32-bit sweep
+    (5, 756, (1 << 31) - 150, 300, 100, 10),       # across bit 31
+    (9, 756, (1 << 32) - 301, 300, 0, 6),          # the last 32-bit points
 ])
 def test_sobol_integers_bit_exact(seed, ladder_dims, first_point, n_points, first_dim, n_dims):
     u, s, z = api.debug_sobol(seed, ladder_dims, first_point, n_points, first_dim, n_dims)
@@ -37,7 +41,27 @@ def test_sobol_integers_bit_exact(seed, ladder_dims, first_point, n_points, firs
             dim = first_dim + d
             assert int(u[p, d]) == sob.unscrambled(n, dim)
             assert int(s[p, d]) == sob.scrambled(n, dim)
-            assert z[p, d] == pytest.approx(sob.normal(n - 1, dim), rel=1e-15, abs=1e-15)
+            # the inverse normal in FP64 (tails: exp_log.h's log, IEEE sqrt and divide) bit for bit
+            assert z[p, d] == sob.normal(n - 1, dim), (n, dim, z[p, d], sob.normal(n - 1, dim))
+
+
+def test_fp64_exp_and_log_on_the_device_equal_their_host_build_bit_for_bit():
+    """exp_log.h compiled for gfx950 and by gcc into the oracle: the same bits for
+    Euler increments, spots, inverse-normal tail arguments, the whole range,
+    subnormals and special values."""
+    rng = np.random.default_rng(31)
+    x = np.concatenate([rng.normal(0.0, 0.05, 100_000), rng.uniform(-746.0, 710.0, 30_000),
+                        100.0 * np.exp(rng.normal(0.0, 0.4, 50_000)), rng.uniform(1e-12, 0.025, 30_000),
+                        2.0 ** rng.uniform(-1074.0, 1023.9, 30_000),
+                        [0.0, -0.0, 1.0, -1.0, 5e-324, 2.2250738585072014e-308, 1.7976931348623157e308,
+                         709.78, 709.79, -745.0, -745.2, np.inf, -np.inf, np.nan]])
+    e, lg = api.debug_exp_log(x)
+    we = np.array([O.shared_exp(float(v)) for v in x])
+    wl = np.array([O.shared_log(float(v)) for v in x])
+    assert np.array_equal(e.view(np.uint64)[~np.isnan(we)], we.view(np.uint64)[~np.isnan(we)])
+    assert np.array_equal(np.isnan(e), np.isnan(we))
+    assert np.array_equal(lg.view(np.uint64)[~np.isnan(wl)], wl.view(np.uint64)[~np.isnan(wl)])
+    assert np.array_equal(np.isnan(lg), np.isnan(wl))
 
 
 @pytest.mark.parametrize("factors,owen_table", [(3, True), (5, True), (3, False)])
@@ -67,6 +91,95 @@ def test_fp32_tier_ladder_keeps_24_owen_levels_and_full_tails(factors, owen_tabl
         assert int(full[p, d]) == sob.scrambled(int(p) + 1, int(d))
     zr = ndtri((want.astype(np.float64) + 0.5) / 2.0 ** 32)  # Acklam is within 1.2e-9 of it
     assert np.max(np.abs(z - zr) / np.maximum(1.0, np.abs(zr))) < 5e-6
+
+
+def test_fp32_ladder_normals_do_not_depend_on_the_kernel_variant():
+    """The same point and dimension give the same FP32 normal in the one-, three-
+    and five-factor ladders (tail coordinates included), so a trade's paths do not
+    depend on the launch -- factor count, scenario bucket, shared table or not --
+    it is priced in."""
+    seed, ladder, n_pts = 1, 720, 8192
+    x1, z1 = api.debug_sobol_f32(seed, ladder, 1, n_pts, 720, 1, True)
+    x3, z3 = api.debug_sobol_f32(seed, ladder, 1, n_pts, 240, 3, True)
+    x5, z5 = api.debug_sobol_f32(seed, ladder, 1, n_pts, 144, 5, True)
+    assert np.array_equal(x1, x3) and np.array_equal(x1, x5)
+    assert np.array_equal(z1.view(np.uint32), z3.view(np.uint32))
+    assert np.array_equal(z1.view(np.uint32), z5.view(np.uint32))
+    assert (np.abs(z1) > 1.97).sum() > 10_000  # the inverse-normal tails are exercised
+
+
+def test_fp32_full_ladder_option_hashes_all_32_levels():
+    """The fp32_full_ladder option gives the FP32 tier the FP64 tier's integers."""
+    seed, ladder, nodes, factors, n_pts = 99, 2268, 5, 3, 8192
+    xs, z = api.debug_sobol_f32(seed, ladder, 1, n_pts, nodes, factors, True, full_ladder=True)
+    _, full, _ = api.debug_sobol(seed, ladder, 1, n_pts, 0, nodes * factors)
+    assert np.array_equal(xs, full)
+    from scipy.special import ndtri
+    zr = ndtri((full.astype(np.float64) + 0.5) / 2.0 ** 32)
+    assert np.max(np.abs(z - zr) / np.maximum(1.0, np.abs(zr))) < 5e-6
+    # priced: every scenario of a CRN batch within the FP32 bar of the FP64 tier
+    names, reqs = W.crn_scenarios(paths=2048, steps=126, years=0.5)
+    md = max(W.max_dimension_for(r) for r in reqs)
+    f64 = api.price_batch(reqs, "fp64", max_dimension=md)
+    full32 = api.price_batch(reqs, "fp32", max_dimension=md, fp32_full_ladder=True)
+    for a, b in zip(full32, f64):
+        assert rel(a.pv, b.pv) <= FP32_REL
+
+
+@pytest.mark.parametrize("first_point", [(1 << 21) + 3, (1 << 24) + 11])
+def test_fp32_tier_ladder_beyond_21_bit_point_indices(first_point):
+    """The FP32 tier's ladder (prefix table, 24 hashed levels, tail passes) at point
+    indices that take the 24-bit and the 32-bit gray-code sweeps, against the rule
+    applied to the FP64 ladder's full 32-level integers at the same points."""
+    seed, ladder, nodes, factors, n_pts = 4_242, 2268, 4, 3, 4096
+    xs, _ = api.debug_sobol_f32(seed, ladder, first_point, n_pts, nodes, factors, True)
+    _, full, _ = api.debug_sobol(seed, ladder, first_point, n_pts, 0, nodes * factors)
+    p24 = full >> np.uint32(8)
+    tail = (p24 < (1 << 14)) | (p24 >= (1 << 24) - (1 << 14))
+    want = np.where(tail, full, (p24 << np.uint32(8)) | np.uint32(0x80)).astype(np.uint32)
+    assert np.array_equal(xs, want)
+    sob = O.Sobol(ladder, seed, max_dim=ladder)
+    for p in (0, 1, 63, 64, 2047, n_pts - 1):
+        for d in range(nodes * factors):
+            assert int(full[p, d]) == sob.scrambled(first_point + p, d)
+
+
+@pytest.mark.parametrize("seed,ladder_dims,first_point,n_points,first_dim,n_dims", [
+    (1729, 64, 1, 200, 0, 64),            # the reference's term-sheet lane shape (bases 2..311)
+    (7, 512, 60_000, 100, 400, 112),      # large bases, 5-digit indices
+    (11, 700, (1 << 31) + 12_345, 64, 400, 300),  # past the reference cap of 512 (bases to 5,279)
+])
+def test_halton_digits_and_radical_inverses_bit_exact(seed, ladder_dims, first_point, n_points, first_dim, n_dims):
+    """The device's Halton ladder (qmc.rs:94-153): the permuted base-b digits of every
+    point, its radical inverse bit for bit and its inverse normal, against the
+    oracle's bases and Fisher-Yates permutations (qmc.rs:129-140)."""
+    dig, u, z = api.debug_halton(seed, ladder_dims, first_point, n_points, first_dim, n_dims)
+    hal = O.Halton(ladder_dims, seed, max_dim=max(512, ladder_dims))
+    perms = {}
+    for d in range(n_dims):
+        dim = first_dim + d
+        di = dim % hal.dimension
+        if di not in perms:
+            perms[di] = (hal.base(di), hal.perm(di))
+        b, perm = perms[di]
+        for p in range(n_points):
+            n = first_point + p
+            want, m = [], n
+            while m > 0:
+                want.append(perm[m % b])
+                m //= b
+            k = len(want)
+            assert list(dig[p, d, :k]) == want and np.all(dig[p, d, k:] == 0xFFFFFFFF), (n, dim)
+            assert u[p, d] == hal.uniform(n, dim), (n, dim)
+            assert z[p, d] == pytest.approx(hal.normal(n - 1, dim), rel=1e-15, abs=1e-15)
+
+
+def test_path_counts_beyond_the_device_point_index_are_refused():
+    from riga_flow_amd.types import RigaError
+    req = W.flat_book(paths=1 << 31, steps=4)
+    with pytest.raises(RigaError) as e:
+        api.price_put_leg(req, "fp64")
+    assert e.value.code == "invalid_paths"
 
 
 def _flat(paths, steps=252, **kw):
