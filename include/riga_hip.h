@@ -198,6 +198,22 @@ int riga_validate(const riga_request *req, riga_error *err);
 int riga_price_put_leg(riga_ctx *ctx, const riga_request *req, int precision, riga_result *out,
                        riga_error *err);
 
+/* ---- pricing without allocation ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): riga_price_put_leg's result for one request, with the
+ * request's tables, block partials, merged records and (Monte Carlo lane) stream
+ * parse laid out in a fixed per-GPU scratch array of 32 MiB set up by
+ * riga_hip_init.  No hipMalloc, hipFree or other allocating routine runs (the
+ * host-side planning uses the C++ heap only).  The Owen prefix tables and shared
+ * normal tables are not used (the ladder hashes all 32 levels; results are the
+ * same integers, so the same prices), nor path records.  A request whose work
+ * area exceeds the array fails with "scratch_exceeded" (about 11 million paths
+ * for one scenario).  Calls through one context run one at a time; calls through
+ * several contexts on one GPU share its array and race. */
+int riga_price_put_leg_noalloc(riga_ctx *ctx, const riga_request *req, int precision, riga_result *out,
+                               riga_error *err);
+
 /* ---- batched pricing: n requests in one launch; results[i] for reqs[i] ---- */
 int riga_price_batch(riga_ctx *ctx, const riga_request *reqs, int n, int precision,
                      const riga_launch_config *cfg, riga_result *results, riga_error *err);

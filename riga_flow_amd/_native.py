@@ -161,7 +161,8 @@ class ServerStats(C.Structure):
 
 
 EXPORTS = ("riga_hip_init", "riga_hip_destroy", "riga_hip_device_name", "riga_hip_arch",
-           "riga_validate", "riga_price_put_leg", "riga_price_batch", "riga_replay_paths",
+           "riga_validate", "riga_price_put_leg", "riga_price_put_leg_noalloc", "riga_price_batch",
+           "riga_replay_paths",
            "riga_plan_create", "riga_plan_estimate", "riga_plan_launch", "riga_plan_ready",
            "riga_plan_wait", "riga_plan_run", "riga_plan_query", "riga_plan_destroy",
            "riga_server_create", "riga_server_submit", "riga_server_wait",
@@ -202,6 +203,8 @@ def lib():
             L.riga_validate.argtypes = [C.POINTER(Request), C.POINTER(Error)]
             L.riga_price_put_leg.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int,
                                              C.POINTER(Result), C.POINTER(Error)]
+            L.riga_price_put_leg_noalloc.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int,
+                                                     C.POINTER(Result), C.POINTER(Error)]
             L.riga_price_batch.argtypes = [C.c_void_p, C.POINTER(Request), C.c_int, C.c_int,
                                            C.POINTER(LaunchConfig), C.POINTER(Result),
                                            C.POINTER(Error)]

@@ -50,21 +50,41 @@ constexpr int RIGA_ENODEV = -3;
 // stream-ordered pool (hipMallocAsync / hipFreeAsync): no device-wide
 // synchronisation, so a serving loop can create and retire plans while other
 // plans run.
+// A fixed device work area handed out front to back (the allocation-free call's
+// scratch array): 256-byte aligned pieces, nothing returned.
+struct Scratch {
+  unsigned char* base = nullptr;
+  size_t cap = 0, used = 0;
+  void* take(size_t bytes) {
+    const size_t at = (used + 255) & ~(size_t)255;
+    if (at > cap || bytes > cap - at) return nullptr;
+    used = at + bytes;
+    return base + at;
+  }
+};
+
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
   hipStream_t stream = nullptr;
+  Scratch* from = nullptr;  // set: memory is taken from a scratch area, never allocated or freed
   hipError_t ensure(size_t bytes) {
     if (bytes <= cap) return hipSuccess;
     release();
     size_t want = std::max(bytes, (size_t)4096);
+    if (from) {
+      p = from->take(want);
+      if (!p) return hipErrorOutOfMemory;
+      cap = want;
+      return hipSuccess;
+    }
     hipError_t e = stream ? hipMallocAsync(&p, want, stream) : hipMalloc(&p, want);
     if (e == hipSuccess) cap = want;
     else p = nullptr;
     return e;
   }
   void release() {
-    if (p) (void)(stream ? hipFreeAsync(p, stream) : hipFree(p));
+    if (p && !from) (void)(stream ? hipFreeAsync(p, stream) : hipFree(p));
     p = nullptr;
     cap = 0;
   }
@@ -107,9 +127,19 @@ struct HostBuf {
   void* p = nullptr;
   size_t cap = 0;
   PinnedPool* pool = nullptr;  // plans borrow from the context's pool
+  // set: the buffer is a fixed pinned block of the context (the allocation-free
+  // call's download area), used as it is and never freed here
+  void* fixed = nullptr;
+  size_t fixed_cap = 0;
   hipError_t ensure(size_t bytes) {
     if (bytes <= cap) return hipSuccess;
     drop();
+    if (fixed) {
+      if (bytes > fixed_cap) return hipErrorOutOfMemory;
+      p = fixed;
+      cap = fixed_cap;
+      return hipSuccess;
+    }
     if (pool) return pool->get(std::max(bytes, (size_t)4096), &p, &cap);
     size_t want = std::max(bytes, (size_t)4096);
     hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
@@ -117,7 +147,7 @@ struct HostBuf {
     return e;
   }
   void drop() {
-    if (p) {
+    if (p && !fixed) {
       if (pool)
         pool->put(p, cap);
       else
@@ -158,7 +188,16 @@ struct riga_ctx {
   DevBuf arena, items, block_item, partials, finals, final_base, records;
   HostBuf staging, finals_host;
   PinnedPool pinned;  // plans' download buffers (plans end before their context)
+  // the allocation-free call: this device's scratch array (its address from
+  // hipGetSymbolAddress at init), its stream, events and pinned download block,
+  // all made once here; calls through this context take noalloc_mu
+  void* scratch = nullptr;
+  hipStream_t noalloc_stream = nullptr;
+  hipEvent_t noalloc_e0 = nullptr, noalloc_e1 = nullptr;
+  void* noalloc_host = nullptr;
+  std::mutex noalloc_mu;
 };
+constexpr size_t kNoallocHostBytes = (size_t)1 << 20;  // merged records of one request (S <= 10) + status
 
 // A calibrated grid kept in device memory: [log_s (n_x)][v (n_t x n_x)] doubles.
 // The host copy serves the tenor brackets and the LDS sizing of each plan.
@@ -643,12 +682,17 @@ struct riga_plan {
   int64_t off_dirs = 0, off_zig = 0;
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool own_stream = true;  // false: the context's allocation-free stream and events, borrowed
   bool serial = false;     // launches one after another on `stream` (no fork)
   hipGraph_t g = nullptr;
   hipGraphExec_t gexec = nullptr;
   ~riga_plan() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (g) (void)hipGraphDestroy(g);
+    if (!own_stream) {
+      e0 = e1 = nullptr;
+      stream = nullptr;
+    }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     for (hipEvent_t ev : fork_join) (void)hipEventDestroy(ev);
@@ -666,6 +710,11 @@ struct riga_plan {
     for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d, &mc_status,
                       &queues})
       b->stream = stream;
+  }
+  void bind_scratch(Scratch* sc) {
+    for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d, &path_list,
+                      &series, &ledger_d, &mc_status, &queues})
+      b->from = sc;
   }
 };
 
@@ -841,9 +890,12 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
 
 // With `estimate` set, only the device bytes the plan would hold are computed
 // (no stream, no allocation): the serving loop admits requests with it.
+// With `scratch` set, every device buffer is taken from that fixed area and the
+// context's allocation-free stream, events and pinned block are used: nothing
+// is allocated (no shared normal tables, no Owen prefix tables, no graph).
 int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, const riga_launch_config* cfg,
                 bool records, riga_plan** out, riga_error* err, int64_t* estimate = nullptr,
-                const ReplayOpts* replay = nullptr) {
+                const ReplayOpts* replay = nullptr, Scratch* scratch = nullptr) {
   clear_err(err);
   *out = nullptr;
   if (!ctx) {
@@ -868,12 +920,21 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     set_err(err, "invalid_launch_config", "waves_per_block must be 1, 2, 3 or 4");
     return RIGA_EINVAL;
   }
+  if (scratch && (records || replay)) {
+    set_err(err, "invalid_argument", "the allocation-free call prices without path records");
+    return RIGA_EINVAL;
+  }
   std::unique_ptr<riga_plan> pl(new riga_plan());
   pl->ctx = ctx;
   pl->finals_host.pool = &ctx->pinned;
+  if (scratch) {
+    pl->finals_host.fixed = ctx->noalloc_host;
+    pl->finals_host.fixed_cap = kNoallocHostBytes;
+    pl->bind_scratch(scratch);
+  }
   pl->precision = precision;
   pl->records = records;
-  pl->graph = cfg && cfg->use_graph != 0 && !records;
+  pl->graph = cfg && cfg->use_graph != 0 && !records && !scratch;
   pl->n_requests = n;
   std::vector<Prep> preps(n);
   for (int i = 0; i < n; ++i) {
@@ -959,7 +1020,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   // normals than it holds (an uneven book), instead of hashing them per trade.
   // Decided here, since the items reading a table launch as their own kernel mode.
   const bool full_ladder = precision == RIGA_FP32 && cfg && cfg->fp32_full_ladder != 0;
-  const int zmode = full_ladder ? -1 : (cfg ? cfg->normal_table : 0);
+  const int zmode = scratch || full_ladder ? -1 : (cfg ? cfg->normal_table : 0);
   if (full_ladder)
     for (Item& it : items)
       if (!(it.d.flags & (kHalton | kMonteCarlo))) it.d.flags |= kFullLadder;
@@ -1025,7 +1086,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   const int64_t off_zig = any_mc ? arena.add(ziggurat_tables().data(), 2 * kZigLen) : 0;
   // one Owen prefix table per seed (built on the device at creation), covering
   // the largest raw dimension used
-  const bool owen_table = !(cfg && cfg->owen_table < 0);
+  const bool owen_table = !(cfg && cfg->owen_table < 0) && !scratch;
   std::map<uint64_t, int> owen_dims;
   if (owen_table) {
     for (Item& it : items)
@@ -1152,8 +1213,15 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
                    [](const LaunchPlan& a, const LaunchPlan& b) { return a.cost > b.cost; });
   pl->off_dirs = off_dirs;
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
-  if (!estimate) HIP_TRY(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "stream");
-  const bool stream_ordered = cfg && cfg->stream_ordered_alloc != 0;
+  if (scratch) {
+    pl->own_stream = false;
+    pl->stream = ctx->noalloc_stream;
+    pl->e0 = ctx->noalloc_e0;
+    pl->e1 = ctx->noalloc_e1;
+  } else if (!estimate) {
+    HIP_TRY(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "stream");
+  }
+  const bool stream_ordered = cfg && cfg->stream_ordered_alloc != 0 && !scratch;
   if (stream_ordered) pl->bind_stream();
   double ztable_bytes = 0.0;
   std::map<uint64_t, const OwenEntry*> owen_ptr;
@@ -1213,6 +1281,9 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       mp->off_key = arena.add(key, 8);
       mc_bytes += mc_parse_bytes(*mp);
       if (stream_ordered) mp->each([](DevBuf& b, hipStream_t st) { b.stream = st; }, pl->stream);
+      if (scratch)
+        for (DevBuf* b : {&mp->chunks, &mp->slow, &mp->slow_val, &mp->entry, &mp->cnt, &mp->block_off, &mp->starts})
+          b->from = scratch;
       if (!estimate) HIP_TRY(mc_parse_alloc(mp.get()), "stream parse alloc");
       for (int i : kv.second) {
         ItemDesc& d = items[i].d;
@@ -1269,9 +1340,23 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     for (auto& mp : pl->mc) mp->each([](DevBuf& b, hipStream_t) { b.stream = nullptr; }, nullptr);
     return 0;
   }
+  if (scratch) {
+    // every piece is at least 4 KiB and 256-byte aligned: 20 pieces at most
+    const double need = device_bytes + 20 * (4096.0 + 256.0);
+    const double host_need = (double)pl->final_slots * sizeof(Partial) + 8.0 * pl->mc.size();
+    if (need > (double)(scratch->cap - scratch->used) || host_need > (double)kNoallocHostBytes) {
+      char buf[200];
+      snprintf(buf, sizeof buf, "the request needs %.0f bytes of device work area; the allocation-free call has %zu",
+               need, scratch->cap - scratch->used);
+      set_err(err, "scratch_exceeded", buf);
+      return RIGA_EINVAL;
+    }
+  }
 
-  HIP_TRY(hipEventCreate(&pl->e0), "event");
-  HIP_TRY(hipEventCreate(&pl->e1), "event");
+  if (!scratch) {
+    HIP_TRY(hipEventCreate(&pl->e0), "event");
+    HIP_TRY(hipEventCreate(&pl->e1), "event");
+  }
   HIP_TRY(pl->arena.ensure(arena.bytes.size()), "arena alloc");
   HIP_TRY(pl->items_d.ensure(descs.size() * sizeof(ItemDesc)), "items alloc");
   HIP_TRY(pl->block_item.ensure(all_blocks.size() * sizeof(int32_t)), "block table alloc");
@@ -1465,6 +1550,20 @@ int riga_hip_init(int device, riga_ctx** out, riga_error* err) {
     set_err(err, "gpu_error", "stream/event creation failed");
     return RIGA_EGPU;
   }
+  // the allocation-free call's set-up, once per context with its device current:
+  // this device's address of the scratch array, a stream, two events and the
+  // pinned block its merged records come back through
+  e = scratch_address(&ctx->scratch);
+  if (e != hipSuccess || !ctx->scratch) {
+    set_err(err, "gpu_error", std::string("scratch array address: ") + hipGetErrorString(e));
+    return RIGA_EGPU;
+  }
+  if (hipStreamCreateWithFlags(&ctx->noalloc_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ctx->noalloc_e0) != hipSuccess || hipEventCreate(&ctx->noalloc_e1) != hipSuccess ||
+      hipHostMalloc(&ctx->noalloc_host, kNoallocHostBytes, hipHostMallocDefault) != hipSuccess) {
+    set_err(err, "gpu_error", "allocation-free call set-up failed");
+    return RIGA_EGPU;
+  }
   *out = ctx.release();
   return 0;
 }
@@ -1473,9 +1572,14 @@ void riga_hip_destroy(riga_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->noalloc_stream) (void)hipStreamSynchronize(ctx->noalloc_stream);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->noalloc_e0) (void)hipEventDestroy(ctx->noalloc_e0);
+  if (ctx->noalloc_e1) (void)hipEventDestroy(ctx->noalloc_e1);
+  if (ctx->noalloc_stream) (void)hipStreamDestroy(ctx->noalloc_stream);
+  if (ctx->noalloc_host) (void)hipHostFree(ctx->noalloc_host);
   delete ctx;
 }
 
@@ -1579,6 +1683,32 @@ void riga_plan_destroy(riga_plan* plan) {
 int riga_price_put_leg(riga_ctx* ctx, const riga_request* req, int precision, riga_result* out,
                        riga_error* err) {
   return run_batch(ctx, req, 1, precision, nullptr, out, nullptr, err);
+}
+
+int riga_price_put_leg_noalloc(riga_ctx* ctx, const riga_request* req, int precision, riga_result* out,
+                               riga_error* err) {
+  clear_err(err);
+  if (!ctx) {
+    set_err(err, "gpu_unavailable", "no device context");
+    return RIGA_ENODEV;
+  }
+  if (!req || !out) {
+    set_err(err, "invalid_argument", "request and result are required");
+    return RIGA_EINVAL;
+  }
+  // calls through one context share its scratch array: one at a time
+  std::lock_guard<std::mutex> g(ctx->noalloc_mu);
+  Scratch sc;
+  sc.base = (unsigned char*)ctx->scratch;
+  sc.cap = kScratchBytes;
+  riga_plan* pl = nullptr;
+  int rc = plan_create(ctx, req, 1, precision, nullptr, false, &pl, err, nullptr, nullptr, &sc);
+  if (rc) return rc;
+  rc = plan_launch(pl, err);
+  if (!rc) rc = plan_wait(pl, out, nullptr, err);
+  if (rc) (void)hipStreamSynchronize(ctx->noalloc_stream);  // nothing of this call still runs
+  delete pl;
+  return rc;
 }
 
 int riga_price_batch(riga_ctx* ctx, const riga_request* reqs, int n, int precision,

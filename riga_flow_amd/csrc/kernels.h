@@ -129,6 +129,12 @@ hipError_t launch_debug_sobol(const uint32_t* dirs, int ladder_dims, uint64_t se
                               double* normal, hipStream_t stream);
 
 
+// The allocation-free pricing call's device work area (scratch.hip): a
+// file-scope array of kScratchBytes per GPU; scratch_address gives the current
+// device's address of it.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+hipError_t scratch_address(void** ptr);
+
 // The FP64 tier's exp and log (exp_log.h) elementwise, for the parity tests.
 hipError_t launch_debug_exp_log(const double* x, uint64_t n, double* e, double* l, hipStream_t stream);
 
