@@ -144,6 +144,9 @@ typedef struct riga_launch_config {
   int32_t fp32_full_ladder; /* FP32 tier: 1 hashes all 32 Owen levels of every coordinate (the FP64
                                tier's integers) instead of the top 24 plus the cell middle; no
                                shared normal table then.  For pricing what the 24-level rule costs */
+  int32_t stack_split;      /* bridge stack levels nearest the root kept in private memory instead of
+                               LDS: 0 = the fewest that let each launch reach the occupancy its
+                               registers allow; -1 = none; k > 0 = k (at most 8) */
 } riga_launch_config;
 
 /* A batch laid out in device memory once (surfaces, factors, schedules, Sobol

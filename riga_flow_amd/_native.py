@@ -118,7 +118,8 @@ class LaunchConfig(C.Structure):
     _fields_ = [("paths_per_thread", C.c_int32), ("use_graph", C.c_int32),
                 ("owen_table", C.c_int32), ("normal_table", C.c_int32),
                 ("stream_ordered_alloc", C.c_int32), ("waves_per_block", C.c_int32),
-                ("serial_launches", C.c_int32), ("fp32_full_ladder", C.c_int32)]
+                ("serial_launches", C.c_int32), ("fp32_full_ladder", C.c_int32),
+                ("stack_split", C.c_int32)]
 
 
 class PlanInfo(C.Structure):

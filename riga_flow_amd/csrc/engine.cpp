@@ -180,6 +180,7 @@ struct riga_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string name, arch;
   size_t lds_limit = 65536;
+  int n_cus = 256;  // compute units of the device
   std::mutex mu;
   std::mutex tune_mu;
   std::map<std::string, std::pair<int, int>> tuned;  // shape signature -> (paths/thread, waves)
@@ -596,6 +597,10 @@ struct LaunchPlan {
   int mode = kLadderDirect;  // how the launch's items get their normals (kernel template mode)
   int threads = kBlock;  // lanes per workgroup
   size_t lds = 0;
+  int pooled = 0;           // bottom bridge levels each workgroup keeps in an HBM slot
+  int pool_n = 0;           // slots: the workgroups of this launch resident at once, at most
+  size_t pool_off = 0;      // byte offset of this launch's slots in the plan's stack pool
+  size_t flags_off = 0;     // word offset of its claim words (pool_n flags, one failure count)
   size_t block_off = 0;     // first (item, chunk) pair of this launch in the block table
   std::vector<int> items;   // item indices, longest first
 };
@@ -646,6 +651,8 @@ struct riga_plan {
   size_t lds_max = 0;
   DevBuf arena, items_d, block_item, partials, finals, final_base_d, records_d;
   DevBuf queues;  // one unit counter per launch, kQueueStride words apart
+  DevBuf stack_pool, pool_words;  // launches' HBM slots for bottom bridge levels and their claim words
+  size_t pool_bytes = 0, pool_nwords = 0;
   // launches after the first run on their own streams, forked from and joined
   // back into the plan's stream, so the persistent kernels of an uneven book
   // share the device instead of draining one after another
@@ -699,7 +706,7 @@ struct riga_plan {
     for (hipStream_t st : aux) (void)hipStreamDestroy(st);
     // buffers go back to the pool on this stream before it is destroyed
     for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d, &path_list,
-                      &series, &ledger_d, &queues})
+                      &series, &ledger_d, &queues, &stack_pool, &pool_words})
       b->release();
     for (auto& zt : ztables) zt->buf.release();
     for (auto& mp : mc) mp->each([](DevBuf& b, hipStream_t) { b.release(); }, nullptr);
@@ -708,7 +715,7 @@ struct riga_plan {
   }
   void bind_stream() {
     for (DevBuf* b : {&arena, &items_d, &block_item, &partials, &finals, &final_base_d, &records_d, &mc_status,
-                      &queues})
+                      &queues, &stack_pool, &pool_words})
       b->stream = stream;
   }
   void bind_scratch(Scratch* sc) {
@@ -883,9 +890,58 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
   if (e != hipSuccess) return e;
   const size_t fin = (size_t)pl->final_slots * sizeof(Partial);
   e = hipMemcpyAsync(pl->finals_host.p, pl->finals.p, fin, hipMemcpyDeviceToHost, pl->stream);
-  if (e != hipSuccess || pl->mc.empty()) return e;
-  return hipMemcpyAsync((unsigned char*)pl->finals_host.p + fin, pl->mc_status.p, 8 * pl->mc.size(),
-                        hipMemcpyDeviceToHost, pl->stream);
+  if (e != hipSuccess) return e;
+  if (!pl->mc.empty()) {
+    e = hipMemcpyAsync((unsigned char*)pl->finals_host.p + fin, pl->mc_status.p, 8 * pl->mc.size(),
+                       hipMemcpyDeviceToHost, pl->stream);
+    if (e != hipSuccess) return e;
+  }
+  // each pooled launch's count of workgroups that found no free slot (zero)
+  unsigned char* failed = (unsigned char*)pl->finals_host.p + fin + 8 * pl->mc.size();
+  for (size_t k = 0; k < pl->launches.size(); ++k) {
+    const LaunchPlan& lp = pl->launches[k];
+    if (lp.pooled <= 0) continue;
+    e = hipMemcpyAsync(failed + 4 * k, (uint32_t*)pl->pool_words.p + lp.flags_off + lp.pool_n, 4,
+                       hipMemcpyDeviceToHost, pl->stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Static LDS of a path kernel, bounded above: the scenario-sum slots, the
+// per-scenario trace counters, the FP32 tier's Euler constants and the unit id.
+constexpr size_t kStaticLdsBound = sizeof(BlockShared) + 10 * 4 + 10 * kMaxFactors * sizeof(SfF32) + 64;
+
+// Bridge stack levels nearest the root (the least used) that a launch keeps in
+// HBM slots rather than LDS.  `split` < 0: none; > 0: that many; 0: the
+// fewest that give the launch the most resident workgroups per CU -- a deep
+// stack in LDS (levels x factors x lanes x 8 bytes in FP64) can otherwise hold
+// occupancy below what the kernel's registers allow.
+static int stack_pooled_levels(int precision, const LaunchPlan& lp, int levels, size_t lv_bytes, int real_size,
+                                int split) {
+  const int most = std::min(levels, kStackPooled);
+  if (split < 0 || most <= 0) return 0;
+  if (split > 0) return std::min(split, most);
+  PhoenixLaunch L{};
+  L.F = lp.F;
+  L.smax = lp.smax;
+  L.mode = lp.mode;
+  L.threads = lp.threads;
+  auto blocks_per_cu = [&](int p) {
+    L.lds = (size_t)(levels - p) * lp.F * lp.threads * real_size + lv_bytes;
+    int nb = 0;
+    const hipError_t e = precision == RIGA_FP64 ? phoenix_blocks_per_cu_f64(L, &nb) : phoenix_blocks_per_cu_f32(L, &nb);
+    return e == hipSuccess ? nb : 0;
+  };
+  int best = blocks_per_cu(0), best_p = 0;
+  for (int p = 1; p <= most; ++p) {
+    const int nb = blocks_per_cu(p);
+    if (nb > best) {
+      best = nb;
+      best_p = p;
+    }
+  }
+  return best_p;
 }
 
 // With `estimate` set, only the device bytes the plan would hold are computed
@@ -1117,11 +1173,15 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       lv_words = std::max(lv_words, (int)items[i].surfaces.size() * (real_size == 4 ? 4 : 3) * max_nx);
     }
     lp.threads = threads;
-    if (lp.mode == kLadderMonteCarlo)  // stream ring and ziggurat tables instead of the bridge stack
+    if (lp.mode == kLadderMonteCarlo) {  // stream ring and ziggurat tables instead of the bridge stack
       lp.lds = (size_t)mc_ring_slots(lp.smax) * threads * 8 + 2 * kZigLen * 8 + (size_t)lv_words * real_size;
-    else
-      lp.lds = (size_t)stack_levels * lp.F * threads * real_size + (size_t)lv_words * real_size;
-    if (lp.lds + (size_t)lp.smax * sizeof(BlockShared) > ctx->lds_limit) {
+    } else {
+      // the allocation-free call keeps the whole stack in LDS (no slot pool)
+      lp.pooled = stack_pooled_levels(precision, lp, stack_levels, (size_t)lv_words * real_size, real_size,
+                                      scratch ? -1 : (cfg ? cfg->stack_split : 0));
+      lp.lds = (size_t)(stack_levels - lp.pooled) * lp.F * threads * real_size + (size_t)lv_words * real_size;
+    }
+    if (lp.lds + kStaticLdsBound > ctx->lds_limit) {
       set_err(err, "invalid_local_vol",
               "the batch needs " + std::to_string(lp.lds) + " bytes of LDS per workgroup (limit " +
                   std::to_string(ctx->lds_limit) + "); use fewer distinct surfaces or knots");
@@ -1132,7 +1192,8 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     for (int i : idx) {
       Item& it = items[i];
       ItemDesc& d = it.d;
-      d.stack_levels = lp.mode == kLadderMonteCarlo ? 0 : stack_levels;
+      d.stack_levels = lp.mode == kLadderMonteCarlo ? 0 : stack_levels - lp.pooled;
+      d.stack_pooled = lp.pooled;
       d.paths_per_block = threads * ppt;
       d.n_blocks = (int32_t)((d.paths + d.paths_per_block - 1) / d.paths_per_block);
       blocks += d.n_blocks;
@@ -1204,6 +1265,26 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     }
     lp.blocks = blocks;
     lp.cost = items[idx.front()].cost;
+    if (lp.pooled > 0) {  // one slot per workgroup that can be resident at once
+      PhoenixLaunch L{};
+      L.F = lp.F;
+      L.smax = lp.smax;
+      L.mode = lp.mode;
+      L.threads = lp.threads;
+      L.lds = lp.lds;
+      int per_cu = 0;
+      const hipError_t e = precision == RIGA_FP64 ? phoenix_blocks_per_cu_f64(L, &per_cu)
+                                                  : phoenix_blocks_per_cu_f32(L, &per_cu);
+      if (e != hipSuccess || per_cu <= 0) {
+        set_err(err, "gpu_error", "occupancy of the path kernel could not be found");
+        return RIGA_EGPU;
+      }
+      lp.pool_n = std::min(blocks, per_cu * ctx->n_cus);
+      lp.pool_off = pl->pool_bytes;
+      pl->pool_bytes += (size_t)lp.pool_n * lp.pooled * lp.F * lp.threads * real_size;
+      lp.flags_off = pl->pool_nwords;
+      pl->pool_nwords += (size_t)lp.pool_n + 1;
+    }
     pl->blocks += blocks;
     pl->launches.push_back(lp);
   }
@@ -1333,7 +1414,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
                               descs.size() * sizeof(ItemDesc) + all_blocks.size() * 4.0 +
                               (double)pl->partial_slots * sizeof(Partial) + (double)pl->final_slots * sizeof(Partial) +
                               pl->final_base.size() * 8.0 + (double)pl->record_slots * sizeof(PathRecordDev) +
-                              8 * 4096.0;
+                              (double)pl->pool_bytes + 4.0 * pl->pool_nwords + 8 * 4096.0;
   if (estimate) {
     *estimate = (int64_t)device_bytes;
     for (auto& zt : pl->ztables) zt->buf.stream = nullptr;
@@ -1364,6 +1445,19 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   HIP_TRY(pl->finals.ensure((size_t)pl->final_slots * sizeof(Partial)), "finals alloc");
   HIP_TRY(pl->final_base_d.ensure(pl->final_base.size() * sizeof(int64_t)), "final index alloc");
   HIP_TRY(pl->queues.ensure(pl->launches.size() * kQueueStride * sizeof(uint32_t)), "unit queue alloc");
+  if (pl->pool_nwords > 0) {
+    HIP_TRY(pl->stack_pool.ensure(pl->pool_bytes), "bridge stack pool alloc");
+    HIP_TRY(pl->pool_words.ensure(pl->pool_nwords * sizeof(uint32_t)), "stack pool flags alloc");
+    // every slot free; each workgroup hands its slot back when it ends
+    HIP_TRY(hipMemsetAsync(pl->pool_words.p, 0, pl->pool_nwords * sizeof(uint32_t), pl->stream), "pool flags");
+    for (const LaunchPlan& lp : pl->launches)
+      if (lp.pooled > 0)
+        for (int i : lp.items) {
+          descs[i].stack_pool = (unsigned char*)pl->stack_pool.p + lp.pool_off;
+          descs[i].pool_flags = (uint32_t*)pl->pool_words.p + lp.flags_off;
+          descs[i].pool_n = lp.pool_n;
+        }
+  }
   // one workgroup per unit (each claims its unit from the launch's counter)
   for (LaunchPlan& lp : pl->launches) lp.grid = lp.blocks;
   pl->serial = cfg && cfg->serial_launches != 0;
@@ -1373,7 +1467,8 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     pl->aux.assign(pl->launches.size() - 1, nullptr);
     for (hipStream_t& st : pl->aux) HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
   }
-  HIP_TRY(pl->finals_host.ensure((size_t)pl->final_slots * sizeof(Partial) + 8 * pl->mc.size()),
+  HIP_TRY(pl->finals_host.ensure((size_t)pl->final_slots * sizeof(Partial) + 8 * pl->mc.size() +
+                                     4 * pl->launches.size()),
           "host finals alloc");
   if (records) {
     HIP_TRY(pl->records_d.ensure((size_t)pl->record_slots * sizeof(PathRecordDev)), "records alloc");
@@ -1456,6 +1551,13 @@ int plan_wait(riga_plan* pl, riga_result* results, riga_path_record* records, ri
       set_err(err, "mc_stream_error",
               "Monte Carlo stream parse: " + std::to_string(mc_status[2 * k]) + " of " +
                   std::to_string(pl->mc[k]->paths) + " path starts, flags " + std::to_string(mc_status[2 * k + 1]));
+      return RIGA_EGPU;
+    }
+  const uint32_t* pool_failed = mc_status + 2 * pl->mc.size();
+  for (size_t k = 0; k < pl->launches.size(); ++k)
+    if (pl->launches[k].pooled > 0 && pool_failed[k] != 0) {
+      set_err(err, "gpu_error", "bridge stack pool: " + std::to_string(pool_failed[k]) +
+                                    " workgroups found no free slot (results not valid)");
       return RIGA_EGPU;
     }
   for (size_t i = 0; i < pl->items.size(); ++i) {
@@ -1545,6 +1647,7 @@ int riga_hip_init(int device, riga_ctx** out, riga_error* err) {
   if (ctx->name.empty()) ctx->name = "AMD Instinct (" + arch + ")";
   ctx->arch = arch;
   ctx->lds_limit = prop.sharedMemPerBlock;
+  ctx->n_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
     set_err(err, "gpu_error", "stream/event creation failed");
@@ -1562,6 +1665,17 @@ int riga_hip_init(int device, riga_ctx** out, riga_error* err) {
       hipEventCreate(&ctx->noalloc_e0) != hipSuccess || hipEventCreate(&ctx->noalloc_e1) != hipSuccess ||
       hipHostMalloc(&ctx->noalloc_host, kNoallocHostBytes, hipHostMallocDefault) != hipSuccess) {
     set_err(err, "gpu_error", "allocation-free call set-up failed");
+    return RIGA_EGPU;
+  }
+  // the path kernels keep the bridge levels nearest the root in private memory:
+  // dispatch each once now (no work; the unit counter is the scratch array's
+  // first word), so the runtime's private-memory set-up for them happens here
+  // and not inside an allocation-free call
+  e = warm_path_kernels_f64((uint32_t*)ctx->scratch, ctx->noalloc_stream);
+  if (e == hipSuccess) e = warm_path_kernels_f32((uint32_t*)ctx->scratch, ctx->noalloc_stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->noalloc_stream);
+  if (e != hipSuccess) {
+    set_err(err, "gpu_error", std::string("path kernel set-up: ") + hipGetErrorString(e));
     return RIGA_EGPU;
   }
   *out = ctx.release();

@@ -27,6 +27,12 @@ struct PhoenixLaunch {
 };
 
 hipError_t launch_phoenix_f64(const PhoenixLaunch& l, hipStream_t stream);
+// Resident workgroups per CU of the launch's kernel at its shape and dynamic LDS.
+hipError_t phoenix_blocks_per_cu_f64(const PhoenixLaunch& l, int* blocks);
+hipError_t phoenix_blocks_per_cu_f32(const PhoenixLaunch& l, int* blocks);
+// Every path-kernel variant dispatched once with no work (private memory set up).
+hipError_t warm_path_kernels_f64(uint32_t* queue, hipStream_t stream);
+hipError_t warm_path_kernels_f32(uint32_t* queue, hipStream_t stream);
 // Zero n unit counters spaced kQueueStride words apart (a kernel, so a captured
 // graph records no memset node).
 constexpr int kQueueStride = 16;

@@ -38,6 +38,37 @@ static hipError_t with_kernel(const PhoenixLaunch& l, Op op) {
 #undef RIGA_F
 }
 
+// Workgroups of the launch's kernel that fit one CU with l.threads lanes and
+// l.lds bytes of dynamic LDS (registers, LDS and wave slots all counted).
+hipError_t phoenix_blocks_per_cu_f32(const PhoenixLaunch& l, int* blocks) {
+  return with_kernel(l, [&](auto kernel) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, kernel, l.threads, l.lds);
+  });
+}
+
+// Dispatches every path-kernel variant of the tier once with no units (each
+// workgroup claims past the end of an empty table and leaves): the runtime sets
+// up the private memory the kernels' bridge levels use on `stream`'s queue now,
+// not at a later call's first dispatch.  `queue` is one device word.
+hipError_t warm_path_kernels_f32(uint32_t* queue, hipStream_t stream) {
+  for (int F = 1; F <= kMaxFactors; ++F)
+    for (int smax : kScenarioBuckets)
+      for (int mode : {kLadderDirect, kLadderTable, kLadderMonteCarlo}) {
+        PhoenixLaunch l{};
+        l.F = F;
+        l.smax = smax;
+        l.mode = mode;
+        l.threads = 64;
+        const hipError_t e = with_kernel(l, [&](auto kernel) {
+          hipLaunchKernelGGL(kernel, dim3(1), dim3(64), 0, stream, nullptr, nullptr, nullptr, nullptr, nullptr,
+                             queue, 0);
+          return hipGetLastError();
+        });
+        if (e != hipSuccess) return e;
+      }
+  return hipSuccess;
+}
+
 hipError_t launch_phoenix_f32(const PhoenixLaunch& l, hipStream_t stream) {
   if (l.blocks <= 0 || l.blocks > l.n_units || !l.queue) return hipErrorInvalidValue;
   return with_kernel(l, [&](auto kernel) {

@@ -341,25 +341,49 @@ __device__ __forceinline__ DD wave_sum(DD v) {
 // consecutive words: no bank conflicts and no barrier).  The depth is uniform
 // across the block.  A step that needs c new nodes pushes c-1 of them (the
 // last one is emitted at once) and a step that needs none pops one.
+//
+// The bottom `low_levels` levels (those nearest the root: level k is pushed
+// and popped about 2^k times per path) may live in the workgroup's HBM slot
+// instead, so that a deep stack does not hold occupancy below what registers
+// allow; the value a pop reads there is first needed a step later.
+template <typename Real>
+using LdsPtr = __attribute__((address_space(3))) Real*;     // an LDS address: ds_* accesses only
+template <typename Real>
+using GlobalPtr = __attribute__((address_space(1))) Real*;  // an HBM address: global_* accesses only
+
 template <typename Real, int F>
 struct BridgeStack {
   Real top[F];
-  Real* col;   // this lane's column: col[(level*F + f) * stride]
-  int stride;  // lanes per workgroup
-  int sp;      // levels held in LDS
+  LdsPtr<Real> col;  // this lane's LDS column: col[((sp - low_levels)*F + f) * stride]
+  GlobalPtr<Real> low;  // this lane's HBM column: low[(sp*F + f) * stride]
+  int stride;        // lanes per workgroup
+  int sp;            // levels held below the top
+  int low_levels;    // of which the bottom low_levels are in `low`
 
   __device__ __forceinline__ void push(const Real (&v)[F]) {
+    if (sp < low_levels) {
 #pragma unroll
-    for (int f = 0; f < F; ++f) {
-      col[(sp * F + f) * stride] = top[f];
-      top[f] = v[f];
+      for (int f = 0; f < F; ++f) low[(sp * F + f) * stride] = top[f];
+    } else {
+#pragma unroll
+      for (int f = 0; f < F; ++f) col[((sp - low_levels) * F + f) * stride] = top[f];
     }
+#pragma unroll
+    for (int f = 0; f < F; ++f) top[f] = v[f];
     ++sp;
   }
+  // The last step's pop finds the stack empty (its top, W(T), was the last value
+  // needed): nothing is read then.
   __device__ __forceinline__ void pop() {
+    if (sp == 0) return;
     --sp;
+    if (sp < low_levels) {
 #pragma unroll
-    for (int f = 0; f < F; ++f) top[f] = col[(sp * F + f) * stride];
+      for (int f = 0; f < F; ++f) top[f] = low[(sp * F + f) * stride];
+    } else {
+#pragma unroll
+      for (int f = 0; f < F; ++f) top[f] = col[((sp - low_levels) * F + f) * stride];
+    }
   }
 };
 
@@ -648,15 +672,16 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
                                                           PathRecordDev* __restrict__ records,
                                                           uint32_t* __restrict__ queue, int n_units) {
   extern __shared__ __align__(16) unsigned char lds_dyn[];
-  __shared__ BlockShared sh[SMAX];
+  __shared__ BlockShared sh;      // one scenario's wave sums at a time
+  __shared__ int ev_n[SMAX];      // KO events this block has recorded per scenario (trace)
   __shared__ SfF32 sf_lds[sizeof(Real) == 4 ? SMAX * F : 1];  // FP32 Euler constants, read per step
-  __shared__ int unit_sh;
+  __shared__ uint32_t unit_sh, slot_sh;
 
   {
-    if (threadIdx.x == 0) unit_sh = (int)atomicAdd(queue, 1u);
+    if (threadIdx.x == 0) unit_sh = atomicAdd(queue, 1u);
     __syncthreads();
-    const int unit = unit_sh;
-    if (unit >= n_units) return;  // the same value in every lane: the whole block leaves
+    const uint32_t unit = unit_sh;
+    if (unit >= (uint32_t)n_units) return;  // the same value in every lane: the whole block leaves
     const int item = block_item[2 * unit];  // block table: (item, chunk) pairs
     const ItemDesc& it = items[item];
     const int S = it.S;
@@ -683,6 +708,28 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     double* zx = reinterpret_cast<double*>(mc_ring + (kMC ? kRingSlots * nthreads : 0));
     double* zf = zx + kZigLen;
     Real* lds_stack = reinterpret_cast<Real*>(lds_dyn);
+    // the bottom bridge levels' HBM slot: claimed for this workgroup's lifetime
+    // from the launch's pool, which holds a slot for every workgroup that can be
+    // resident at once (the claim scans the pool twice at most, so it ends)
+    Real* stack_low = nullptr;
+    if (it.stack_pooled > 0) {
+      if (tid == 0) {
+        const uint32_t n = (uint32_t)it.pool_n;
+        uint32_t slot = blockIdx.x % n, got = 0xFFFFFFFFu;
+        for (uint32_t probe = 0; probe < 2 * n; ++probe) {
+          if (atomicCAS(it.pool_flags + slot, 0u, 1u) == 0u) {
+            got = slot;
+            break;
+          }
+          slot = slot + 1 == n ? 0 : slot + 1;
+        }
+        if (got == 0xFFFFFFFFu) atomicAdd(it.pool_flags + n, 1u);  // the host reports the plan failed
+        slot_sh = got;
+      }
+      __syncthreads();
+      if (slot_sh == 0xFFFFFFFFu) return;  // uniform: the whole block leaves
+      stack_low = reinterpret_cast<Real*>(it.stack_pool) + (size_t)slot_sh * it.stack_pooled * F * nthreads;
+    }
     Real* lds_w = kMC ? reinterpret_cast<Real*>(zf + kZigLen) : lds_stack + (size_t)it.stack_levels * F * nthreads;
     const Real* lds = lds_w;
     const int nxs = it.max_nx;
@@ -692,12 +739,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
     const ScenarioDesc* scen = at<ScenarioDesc>(arena, it.off_scen);
     const LvSurface* surf = at<LvSurface>(arena, it.off_surf);
 
-    for (int s = 0; s < SMAX; ++s) {
-      for (int i = tid; i < kWaves * kNumQ; i += nthreads) sh[s].acc[i / kNumQ][i % kNumQ] = DD{0.0, 0.0};
-      for (int i = tid; i < kWaves * kNumC; i += nthreads) sh[s].cnt[i / kNumC][i % kNumC] = 0;
-      for (int i = tid; i < kWaves; i += nthreads) sh[s].wave_ko[i] = 0;
-      if (tid == 0) sh[s].ev_n = 0;
-    }
+    for (int s = tid; s < SMAX; s += nthreads) ev_n[s] = 0;
     if constexpr (sizeof(Real) == 4) {
       const SfF32* sfg = at<SfF32>(arena, it.off_sf32);
       // surface offsets are staged as absolute LDS addresses, so a slope-row read
@@ -754,9 +796,11 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
 
       // ---- path state
       BridgeStack<Real, F> bs;
-      bs.col = lds_stack + tid;
+      bs.col = (LdsPtr<Real>)(lds_stack + tid);
       bs.stride = nthreads;
       bs.sp = 0;
+      bs.low_levels = it.stack_pooled;
+      bs.low = it.stack_pooled > 0 ? (GlobalPtr<Real>)(stack_low + tid) : nullptr;
       Real wlo[F];
       Real spot[SMAX][F];
       int ko_step[SMAX];
@@ -1201,53 +1245,54 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         const uint64_t b_cp = __ballot(valid && received[s]);
         if (lane == 0) {
   #pragma unroll
-          for (int k = 0; k < kNumQ; ++k) sh[s].acc[wave][k] = dd_add(sh[s].acc[wave][k], qd[k]);
-          sh[s].cnt[wave][kCN] += __popcll(b_valid);
-          sh[s].cnt[wave][kCKo] += __popcll(b_ko);
-          sh[s].cnt[wave][kCKi] += __popcll(b_ki);
-          sh[s].cnt[wave][kCPutEligible] += __popcll(b_el);
-          sh[s].cnt[wave][kCCouponPaths] += __popcll(b_cp);
+          for (int k = 0; k < kNumQ; ++k) sh.acc[wave][k] = qd[k];
+          sh.cnt[wave][kCN] = __popcll(b_valid);
+          sh.cnt[wave][kCKo] = __popcll(b_ko);
+          sh.cnt[wave][kCKi] = __popcll(b_ki);
+          sh.cnt[wave][kCPutEligible] = __popcll(b_el);
+          sh.cnt[wave][kCCouponPaths] = __popcll(b_cp);
+          sh.wave_ko[wave] = __popcll(b_ko);
         }
+        __syncthreads();
+        // the block's Partial for this scenario: waves merged in order, then added
+        // to the earlier batches' sums; KO events (trace) go straight to their
+        // slots in path order (note.rs:174-180: the first 64)
+        Partial* out = partials + it.partial_base + (int64_t)s * it.n_blocks + chunk;
+        int ev_total = 0;
         if (trace) {
-          // first 64 KO events of this block in path order (note.rs:174-180)
-          if (lane == 0) sh[s].wave_ko[wave] = __popcll(b_ko);
-          __syncthreads();
-          int base = sh[s].ev_n;
-          for (int w = 0; w < wave; ++w) base += sh[s].wave_ko[w];
+          int base = ev_n[s];
+          for (int w = 0; w < wave; ++w) base += sh.wave_ko[w];
           const int rank = base + (int)__popcll(b_ko & ((1ull << lane) - 1ull));
-          if (valid && ko && rank < kTraceCap)  // paths_per_block <= 2048, steps < 2^20
-            sh[s].ev[rank] = ((uint32_t)(b * nthreads + tid) << 20) | (uint32_t)ko_step[s];
-          __syncthreads();
-          if (tid == 0) {
-            int tot = sh[s].ev_n;
-            for (int w = 0; w < nwaves; ++w) tot += sh[s].wave_ko[w];
-            sh[s].ev_n = tot < kTraceCap ? tot : kTraceCap;
+          if (valid && ko && rank < kTraceCap) {  // paths_per_block <= 2048, steps < 2^20
+            out->ev_path_lo[rank] = (int32_t)((int64_t)chunk * it.paths_per_block + (int64_t)(b * nthreads + tid));
+            out->ev_step[rank] = (int32_t)ko_step[s];
           }
-          __syncthreads();
+          ev_total = ev_n[s];
+          for (int w = 0; w < nwaves; ++w) ev_total += sh.wave_ko[w];
+          ev_total = ev_total < kTraceCap ? ev_total : kTraceCap;
+        }
+        if (tid < kNumQ) {
+          DD a = sh.acc[0][tid];
+          for (int w = 1; w < nwaves; ++w) a = dd_add(a, sh.acc[w][tid]);
+          out->q[tid] = b == 0 ? a : dd_add(out->q[tid], a);
+        } else if (tid >= 32 && tid < 32 + kNumC) {
+          const int c = tid - 32;
+          uint64_t a = 0;
+          for (int w = 0; w < nwaves; ++w) a += sh.cnt[w][c];
+          out->c[c] = b == 0 ? a : out->c[c] + a;
+        }
+        __syncthreads();  // every lane has read sh and ev_n[s]: the next scenario may reuse them
+        if (tid == 48) {  // within the first wave: every launch shape has it
+          ev_n[s] = ev_total;
+          out->n_events = ev_total;
         }
       }
     }
-
-    __syncthreads();
-    // ---- block partials, waves merged in order
-    for (int s = 0; s < S && s < SMAX; ++s) {
-      Partial* out = partials + it.partial_base + (int64_t)s * it.n_blocks + chunk;
-      if (tid < kNumQ) {
-        DD a = sh[s].acc[0][tid];
-        for (int w = 1; w < nwaves; ++w) a = dd_add(a, sh[s].acc[w][tid]);
-        out->q[tid] = a;
-      } else if (tid >= 32 && tid < 32 + kNumC) {
-        const int c = tid - 32;
-        uint64_t a = 0;
-        for (int w = 0; w < nwaves; ++w) a += sh[s].cnt[w][c];
-        out->c[c] = a;
-      } else if (tid == 48) {  // within the first wave: every launch shape has it
-        const int ne = trace ? sh[s].ev_n : 0;
-        out->n_events = ne;
-        for (int e = 0; e < ne; ++e) {
-          out->ev_path_lo[e] = (int32_t)((int64_t)chunk * it.paths_per_block + (int64_t)(sh[s].ev[e] >> 20));
-          out->ev_step[e] = (int32_t)(sh[s].ev[e] & 0xFFFFFu);
-        }
+    if (it.stack_pooled > 0) {  // every lane is done with the slot: hand it back
+      __syncthreads();
+      if (tid == 0) {
+        __threadfence();
+        atomicExch(it.pool_flags + slot_sh, 0u);
       }
     }
   }

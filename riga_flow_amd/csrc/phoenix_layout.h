@@ -13,6 +13,7 @@ namespace riga {
 
 constexpr int kMaxFactors = 5;      // per-trade underlyings handled by the kernels
 constexpr int kBridgeStack = 14;    // pending bridge intervals (host-checked per trade)
+constexpr int kStackPooled = 8;     // bottom bridge levels a lane may keep in its workgroup's HBM slot
 constexpr int kTraceCap = 64;       // note.rs:57 TRACE_EVENT_CAP
 // How a launch's items get their normals; the path kernel is compiled once per mode.
 constexpr int kLadderDirect = 0;      // Sobol (Owen) or Halton points generated per lane
@@ -157,7 +158,7 @@ struct ItemDesc {
   int32_t n_surf;
   int32_t max_nx;
   int32_t stack_levels;  // bridge levels below the top held in LDS (launch-wide)
-  int32_t pad2;
+  int32_t stack_pooled;  // further levels below those, the bottom of the stack, in an HBM slot
   int64_t partial_base;  // first Partial slot: [scenario][block_in_item]
   int64_t record_base;   // first PathRecord slot: [scenario][path]
   int32_t pad3;
@@ -186,6 +187,13 @@ struct ItemDesc {
   // per (step, path) for the coupon strips (null: not written)
   const uint64_t* path_list;
   double* series;        // [step][path]
+  // bottom bridge levels (stack_pooled > 0): the launch's pool of HBM slots, one
+  // per workgroup resident at a time, [slot][level][factor][lane] of Real, and
+  // its claim words (pool_n flags, then a count of claims that found no slot)
+  unsigned char* stack_pool;
+  uint32_t* pool_flags;
+  int32_t pool_n;
+  int32_t pool_pad;
 };
 
 // Double-double accumulator (hi + lo).
@@ -213,14 +221,13 @@ struct Partial {
   int32_t ev_step[kTraceCap];
 };
 
-// Static LDS of the kernel, one per scenario the lane carries.
+// Static LDS of the kernel: the wave sums of one scenario at a time (the block
+// merges them into its Partial before the next scenario reuses the slots).
 constexpr int kWavesPerBlock = kBlock / 64;
 struct BlockShared {
-  DD acc[kWavesPerBlock][kNumQ];     // per-wave running sums (lane 0 of each wave writes its slot)
+  DD acc[kWavesPerBlock][kNumQ];     // lane 0 of each wave writes its wave's sums
   uint64_t cnt[kWavesPerBlock][kNumC];
-  int32_t ev_n;                      // KO events collected so far in this block
-  int32_t wave_ko[kWavesPerBlock];
-  uint32_t ev[kTraceCap];             // (path within the block << 20) | KO step
+  int32_t wave_ko[kWavesPerBlock];   // KO events per wave (trace)
 };
 
 // One period of a path's coupon strip (explain.rs:35-55 CouponPeriodRecord);
