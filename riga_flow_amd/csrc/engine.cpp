@@ -1667,10 +1667,11 @@ int riga_hip_init(int device, riga_ctx** out, riga_error* err) {
     set_err(err, "gpu_error", "allocation-free call set-up failed");
     return RIGA_EGPU;
   }
-  // the path kernels keep the bridge levels nearest the root in private memory:
-  // dispatch each once now (no work; the unit counter is the scratch array's
-  // first word), so the runtime's private-memory set-up for them happens here
-  // and not inside an allocation-free call
+  // a path kernel that spills registers to private memory (the FP64 5-factor
+  // 10-scenario kernels do) needs the runtime's private-memory set-up on its
+  // first dispatch: dispatch each once now (no work; the unit counter is the
+  // scratch array's first word), so that happens here and not inside an
+  // allocation-free call
   e = warm_path_kernels_f64((uint32_t*)ctx->scratch, ctx->noalloc_stream);
   if (e == hipSuccess) e = warm_path_kernels_f32((uint32_t*)ctx->scratch, ctx->noalloc_stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->noalloc_stream);

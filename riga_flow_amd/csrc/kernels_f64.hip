@@ -48,7 +48,7 @@ hipError_t phoenix_blocks_per_cu_f64(const PhoenixLaunch& l, int* blocks) {
 
 // Dispatches every path-kernel variant of the tier once with no units (each
 // workgroup claims past the end of an empty table and leaves): the runtime sets
-// up the private memory the kernels' bridge levels use on `stream`'s queue now,
+// up the private memory of kernels that spill registers on `stream`'s queue now,
 // not at a later call's first dispatch.  `queue` is one device word.
 hipError_t warm_path_kernels_f64(uint32_t* queue, hipStream_t stream) {
   for (int F = 1; F <= kMaxFactors; ++F)
