@@ -228,25 +228,51 @@ __device__ __forceinline__ double probit_f64(double p) {
   return probit_is_tail(p) ? probit_f64_tail(p) : probit_f64_central(p);
 }
 
-// FP32 inverse normal for the fast tier.  The uniform arrives in FP64 so the
-// tail argument (p or 1-p) and the central offset p-0.5 are exact before the
-// rounding to float.  The central rational's two polynomials are evaluated in
-// FP64 (the same issue rate as FP32 here): in FP32 their mixed-sign terms
-// cancel to a relative error of 1.9e-4 near |z| = 1.95; this way the quotient
-// of their float roundings is within 4e-7 of the exact inverse normal.
-__device__ __forceinline__ float probit_f32_central(double p) {
-  constexpr double A0 = -3.969683028665376e+01, A1 = 2.209460984245205e+02,
-                   A2 = -2.759285104469687e+02, A3 = 1.383577518672690e+02,
-                   A4 = -3.066479806614716e+01, A5 = 2.506628277459239e+00;
-  constexpr double B0 = -5.447609879822406e+01, B1 = 1.615858368580409e+02,
-                   B2 = -1.556989798598866e+02, B3 = 6.680131188771972e+01,
-                   B4 = -1.328068155288572e+01;
-  const double q = p - 0.5;
-  const double r = q * q;
-  const double num = fma(fma(fma(fma(fma(A0, r, A1), r, A2), r, A3), r, A4), r, A5) * q;
-  const double den = fma(fma(fma(fma(fma(B0, r, B1), r, B2), r, B3), r, B4), r, 1.0);
-  return __fdividef((float)num, (float)den);
+// FP32 inverse normal for the fast tier: z = sqrt(2) erfinv(2u - 1) with
+// M. Giles' single-precision erfinv ("Approximating the erfinv function", GPU
+// Computing Gems Jade Edition, 2011): w = -ln((1 - x)(1 + x)), a polynomial in
+// w - 2.5 for w < 5 and in sqrt(w) - 3 beyond.  (1 - x)(1 + x) = 4u(1 - u) and
+// x = 2u - 1 are formed in FP64 from the FP64 uniform, so neither cancels; the
+// logarithm is the hardware's log2 (the same instruction in every kernel).  Its
+// error stays below 1.5e-6 in z for 1e-7 <= u <= 1 - 1e-7; the 2e-7 of uniforms
+// outside that take the Acklam tail below, as the FP64 tier's tail.  w >= 5
+// (u within 0.00168 of 0 or 1) is the rare case: the central form is
+// evaluated branch-free for every coordinate and the rare ones redone after.
+struct GilesArg {
+  float w, x;
+};
+__device__ __forceinline__ GilesArg giles_arg(double p) {
+  const float t = (float)(4.0 * p * (1.0 - p));
+  return GilesArg{-0.693147181f * __builtin_amdgcn_logf(t), (float)(2.0 * p - 1.0)};
 }
+__device__ __forceinline__ bool giles_rare(GilesArg a) { return !(a.w < 5.0f); }
+__device__ __forceinline__ float giles_central(GilesArg a) {
+  const float w = a.w - 2.5f;
+  float q = 2.81022636e-08f;
+  q = fmaf(q, w, 3.43273939e-07f);
+  q = fmaf(q, w, -3.5233877e-06f);
+  q = fmaf(q, w, -4.39150654e-06f);
+  q = fmaf(q, w, 0.00021858087f);
+  q = fmaf(q, w, -0.00125372503f);
+  q = fmaf(q, w, -0.00417768164f);
+  q = fmaf(q, w, 0.246640727f);
+  q = fmaf(q, w, 1.50140941f);
+  return 1.41421356f * (q * a.x);
+}
+__device__ __forceinline__ float giles_outer(GilesArg a) {
+  const float w = __builtin_amdgcn_sqrtf(a.w) - 3.0f;
+  float q = -0.000200214257f;
+  q = fmaf(q, w, 0.000100950558f);
+  q = fmaf(q, w, 0.00134934322f);
+  q = fmaf(q, w, -0.00367342844f);
+  q = fmaf(q, w, 0.00573950773f);
+  q = fmaf(q, w, -0.0076224613f);
+  q = fmaf(q, w, 0.00943887047f);
+  q = fmaf(q, w, 1.00167406f);
+  q = fmaf(q, w, 2.83297682f);
+  return 1.41421356f * (q * a.x);
+}
+__device__ __forceinline__ bool probit_f32_far(double p) { return p < 1e-7 || p > 1.0 - 1e-7; }
 
 __device__ __forceinline__ float probit_f32_tail(double p) {
   constexpr float C0 = -7.784894002430293e-03f, C1 = -3.223964580411365e-01f,
@@ -258,37 +284,50 @@ __device__ __forceinline__ float probit_f32_tail(double p) {
   if (p > 1.0 - 1.0e-12) p = 1.0 - 1.0e-12;
   const bool lower = p < 0.5;
   const float t = (float)(lower ? p : 1.0 - p);
-  // the logarithm from exp_log.h in FP64, rounded once: the device's float log
-  // intrinsic is lowered differently from one kernel variant to the next, and a
-  // path's normals must not depend on the launch it is priced in
-  const float q = __fsqrt_rn(-2.0f * (float)riga_log((double)t));
+  // the hardware's log2, square root and reciprocal: one instruction each, the
+  // same in every kernel variant (the compiler lowers the library forms by
+  // context, and a path's normals must not depend on the launch it is priced in)
+  const float q = __builtin_amdgcn_sqrtf(-1.386294361f * __builtin_amdgcn_logf(t));
   const float num = fmaf(fmaf(fmaf(fmaf(fmaf(C0, q, C1), q, C2), q, C3), q, C4), q, C5);
   const float den = fmaf(fmaf(fmaf(fmaf(D0, q, D1), q, D2), q, D3), q, 1.0f);
-  const float z = __fdividef(num, den);
+  const float z = num * __builtin_amdgcn_rcpf(den);
   return lower ? z : -z;
 }
 
+// The rare coordinates (w >= 5): the outer polynomial, or the Acklam tail
+// beyond 1e-7 of 0 or 1.
+__device__ __forceinline__ float probit_f32_rare(double p) {
+  return probit_f32_far(p) ? probit_f32_tail(p) : giles_outer(giles_arg(p));
+}
+
 __device__ __forceinline__ float probit_f32(double p) {
-  return probit_is_tail(p) ? probit_f32_tail(p) : probit_f32_central(p);
+  const GilesArg a = giles_arg(p);
+  return giles_rare(a) ? probit_f32_rare(p) : giles_central(a);
 }
 
 __device__ __forceinline__ double probit_tail_tier(double p, double) { return probit_f64_tail(p); }
-__device__ __forceinline__ float probit_tail_tier(double p, float) { return probit_f32_tail(p); }
-__device__ __forceinline__ double probit_central_tier(double p, double) { return probit_f64_central(p); }
-__device__ __forceinline__ float probit_central_tier(double p, float) { return probit_f32_central(p); }
+__device__ __forceinline__ float probit_tail_tier(double p, float) { return probit_f32_rare(p); }
 
-// Inverse normals of one node's F uniforms: the central rational for every
-// coordinate, then the tail coordinates one per lane per pass while any lane of
-// the wave has one.  A branch per coordinate would run both regions for nearly
-// every wave (a wave of 64 lanes almost always holds a tail uniform); this way
-// the tails cost about one pass per node.  Each z is the same value either way.
+// Inverse normals of one node's F uniforms: the main form for every coordinate,
+// branch-free (FP64: the central rational; FP32: the central erfinv polynomial),
+// then the rare coordinates (FP64: the 4.85% of uniforms in the tails; FP32: the
+// 0.34% with w >= 5) one per lane per pass while any lane of the wave has one.  A
+// branch per coordinate would run both forms in most waves and serialize the F
+// coordinates' dependent chains; this way they interleave and the rare forms
+// cost about one pass per node.  Each z is the same value either way.
 template <typename Real, int F>
 __device__ __forceinline__ void probit_node(const double (&u)[F], Real (&z)[F]) {
   uint32_t pend = 0;
 #pragma unroll
   for (int f = 0; f < F; ++f) {
-    z[f] = probit_central_tier(u[f], Real(0));
-    pend |= (uint32_t)probit_is_tail(u[f]) << f;
+    if constexpr (sizeof(Real) == 8) {
+      z[f] = probit_f64_central(u[f]);
+      pend |= (uint32_t)probit_is_tail(u[f]) << f;
+    } else {
+      const GilesArg a = giles_arg(u[f]);
+      z[f] = giles_central(a);
+      pend |= (uint32_t)giles_rare(a) << f;
+    }
   }
   while (__ballot(pend != 0u) != 0ull) {
     if (pend) {
@@ -545,14 +584,20 @@ __device__ __forceinline__ void node_normals(const ItemDesc& it, const unsigned 
 }
 
 
-// exp/log per tier: FP64 uses exp_log.h's functions (the same bits as the
-// oracle's host build of them); FP32 the device's fast intrinsics.
+// The FP32 tier's exp of a log-state value: the hardware's exp2 (one
+// instruction, the same in every kernel variant), within 2 ulp of float.
+__device__ __forceinline__ float exp_f32(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
+
+// exp/log/divide per tier: FP64 uses exp_log.h's functions (the same bits as the
+// oracle's host build of them) and IEEE division; FP32 the hardware's exp2, log2
+// and reciprocal, one instruction each (the library forms are lowered
+// differently from one kernel variant to the next).
 __device__ __forceinline__ double tier_exp(double x) { return riga_exp(x); }
-__device__ __forceinline__ float tier_exp(float x) { return __expf(x); }
+__device__ __forceinline__ float tier_exp(float x) { return exp_f32(x); }
 __device__ __forceinline__ double tier_log(double x) { return riga_log(x); }
-__device__ __forceinline__ float tier_log(float x) { return __logf(x); }
+__device__ __forceinline__ float tier_log(float x) { return 0.693147181f * __builtin_amdgcn_logf(x); }
 __device__ __forceinline__ double tier_div(double a, double b) { return a / b; }
-__device__ __forceinline__ float tier_div(float a, float b) { return __fdividef(a, b); }
+__device__ __forceinline__ float tier_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // Local-vol lookup sigma(t, S) (riga-market/src/lib.rs:305-343) against a
 // surface whose knots and two bracketing tenor rows sit in LDS.
@@ -664,8 +709,30 @@ typedef __attribute__((address_space(3))) const F32x4 LdsF32x4;
 // holds 14-33 more VGPRs across its back edge -- the FP32 table-mode kernel
 // goes from 147 to 180, three waves per SIMD to two -- for nothing: a unit
 // runs milliseconds, its dispatch microseconds.)
+// Resident waves per SIMD each path kernel is compiled for (its register
+// budget: 512 / waves VGPRs per lane), chosen per variant from the compiler's
+// spill counts and timings: more waves wherever the kernel's live state fits the
+// smaller budget with its step loop unspilled.  The ten-scenario lanes keep their
+// log-spots and scan state in registers at two waves; the FP64 one-scenario
+// direct kernel of up to three factors runs four (its spills sit outside the
+// hot arithmetic: 16.4 -> 14.1 ms on the 262,144-path local-vol trade).
 template <typename Real, int F, int SMAX, int kMode>
-__global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restrict__ items,
+constexpr int path_kernel_waves() {
+  if constexpr (sizeof(Real) == 8) {
+    if (SMAX >= 10) return 1;
+    if (SMAX >= 4) return 2;
+    if (kMode == kLadderDirect) return F <= 3 ? 4 : 3;
+    return 3;
+  } else {
+    if (SMAX >= 10) return 2;
+    if (SMAX >= 4) return 3;
+    return 4;
+  }
+}
+
+template <typename Real, int F, int SMAX, int kMode>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(path_kernel_waves<Real, F, SMAX, kMode>(), 8)))
+void phoenix_kernel(const ItemDesc* __restrict__ items,
                                                           const int32_t* __restrict__ block_item,
                                                           const unsigned char* __restrict__ arena,
                                                           Partial* __restrict__ partials,
@@ -1066,7 +1133,6 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
   #pragma unroll
             for (int f = 1; f < F; ++f) w = fminf(w, spot[s][f]);
             if (s == 0) w_first = w;
-            worst_last[s] = w;
             const bool alive = (pk[s] & 0xFFFFFu) == 0u;
             accrual[s] = (new_period ? 0.0f : accrual[s]) + (w >= ln_cb32 ? inc : 0.0f);
             const bool ko_now = alive & call_active & (w >= ln_ko32);  // no short-circuit branch
@@ -1081,7 +1147,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
             }
           }
           if (it.series != nullptr && valid)  // replay: the worst-of for the coupon strip
-            it.series[(size_t)step * it.paths + path_local] = riga_exp((double)w_first);
+            it.series[(size_t)step * it.paths + path_local] = (double)exp_f32(w_first);
           // Phase 3: period end (note.rs:197-218), a step-uniform branch.
           if (period_end) {
             const bool memory = (it.flags & kMemoryCoupon) != 0;
@@ -1193,6 +1259,17 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
       }
 
       // ---- path end (note.rs:221-258) and reductions
+      if constexpr (kLogState) {
+        // the FP32 tier's final worst-of, from the log spots the last step left
+        // (not carried through the step loop; every path takes at least one step)
+  #pragma unroll
+        for (int s = 0; s < SMAX; ++s) {
+          float w = spot[s][0];
+  #pragma unroll
+          for (int f = 1; f < F; ++f) w = fminf(w, spot[s][f]);
+          worst_last[s] = w;
+        }
+      }
   #pragma unroll
       for (int s = 0; s < SMAX; ++s) {
         if (s >= S) continue;
@@ -1202,7 +1279,7 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
         }
         const bool ko = ko_step[s] >= 0;
         const double funding = ko ? stepinfo[ko_step[s]].df : it.put_df;
-        const double fw = kLogState ? riga_exp((double)worst_last[s]) : (double)worst_last[s];
+        const double fw = kLogState ? (double)exp_f32(worst_last[s]) : (double)worst_last[s];
         const bool hit_ki = fw <= it.ki;
         const bool eligible = hit_ki && !ko;
         const double put = eligible ? fmax(it.strike - fw, 0.0) * it.put_df : 0.0;
@@ -1223,29 +1300,25 @@ __global__ __launch_bounds__(kBlock) void phoenix_kernel(const ItemDesc* __restr
           records[it.record_base + (int64_t)s * (int64_t)it.paths + (int64_t)path_local] = r;
         }
         // plain sums (funding, put, coupon and memory quotes) and (x, x^2)
-        // pairs for the four moment sets; invalid lanes contribute zero.
-        DD qd[kNumQ];
-        qd[kQFunding] = dd_from(valid ? funding : 0.0);
-        qd[kQPut] = dd_from(valid ? put : 0.0);
-        qd[kQCq] = dd_from(valid ? coupon_quote - memory_quote : 0.0);
-        qd[kQMq] = dd_from(valid ? memory_quote : 0.0);
-        const double mx[4] = {pv, ko ? funding : 0.0, hit_ki ? put : 0.0, received[s] ? nominal : 0.0};
+        // pairs for the four moment sets, each summed over the wave in turn (the
+        // scheduling barrier keeps one quantity's shuffles live at a time);
+        // invalid lanes contribute zero.
+        const double mx[8] = {funding, put, coupon_quote - memory_quote, memory_quote, pv,
+                              ko ? funding : 0.0, hit_ki ? put : 0.0, received[s] ? nominal : 0.0};
   #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const double x = valid ? mx[k] : 0.0;
-          qd[kQPv1 + 2 * k] = dd_from(x);
-          qd[kQPv2 + 2 * k] = dd_sq(x);
+        for (int k = 0; k < kNumQ; ++k) {
+          const double x = valid ? mx[k < kQPv1 ? k : kQPv1 + (k - kQPv1) / 2] : 0.0;
+          DD v = (k >= kQPv1 && (k - kQPv1) % 2 == 1) ? dd_sq(x) : dd_from(x);
+          v = wave_sum(v);
+          if (lane == 0) sh.acc[wave][k] = v;
+          __builtin_amdgcn_sched_barrier(0);
         }
-  #pragma unroll
-        for (int k = 0; k < kNumQ; ++k) qd[k] = wave_sum(qd[k]);
         const uint64_t b_valid = __ballot(valid);
         const uint64_t b_ko = __ballot(valid && ko);
         const uint64_t b_ki = __ballot(valid && hit_ki);
         const uint64_t b_el = __ballot(valid && eligible);
         const uint64_t b_cp = __ballot(valid && received[s]);
         if (lane == 0) {
-  #pragma unroll
-          for (int k = 0; k < kNumQ; ++k) sh.acc[wave][k] = qd[k];
           sh.cnt[wave][kCN] = __popcll(b_valid);
           sh.cnt[wave][kCKo] = __popcll(b_ko);
           sh.cnt[wave][kCKi] = __popcll(b_ki);
