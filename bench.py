@@ -61,21 +61,25 @@ def ops_per_path_step(F: int, S: int, nbits: int, local_vol: bool, tier: str) ->
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "phoenix_kernel_lv3_crn10_fp32_traffic.txt")
+BOOK_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "phoenix_kernel_uneven_book_traffic.txt")
 
 
-def measured_traffic():
-    """HBM bytes per launch of the FP32 batch kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated), or None."""
+def measured_traffic(path: str = TRAFFIC_FILE):
+    """HBM bytes per launch (or per book) from a committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-kernel calibrated) and the commit it was profiled
+    at, or (None, None)."""
     try:
-        vals = {}
-        for line in open(TRAFFIC_FILE):
-            if "=" in line and not line.startswith("#"):
+        vals, commit = {}, None
+        for line in open(path):
+            if line.startswith("commit="):
+                commit = line.strip().split("=", 1)[1]
+            elif "=" in line and not line.startswith("#"):
                 k, v = line.strip().split("=", 1)
                 vals[k] = float(v)
-        return vals["fetch_kib_per_dispatch"] * 1024 * vals["fetch_scale"] + \
-            vals["write_kib_per_dispatch"] * 1024 * vals["write_scale"]
+        return (vals["fetch_kib_per_dispatch"] * 1024 * vals["fetch_scale"] +
+                vals["write_kib_per_dispatch"] * 1024 * vals["write_scale"]), commit
     except (OSError, KeyError, ValueError):
-        return None
+        return None, None
 
 
 # ---------------------------------------------------------------- CPU baseline
@@ -171,6 +175,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (shape 1 x 4 waves)")
     ap.add_argument("--book-only", action="store_true",
                     help="run only the uneven-book leg and print its JSON (for profiling)")
+    ap.add_argument("--legs-only", action="store_true",
+                    help="time only the FP32 batch and FP64 plain legs (for profiling: no error "
+                         "checks or other legs launch the same kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the hot path shards by contiguous path range "
@@ -210,6 +217,13 @@ def main():
     log(f"fp32 batched: {wall32 / args.steps * 1e3:.1f} ms/step")
     wall64, dev64, res64 = timed(fp64, args.steps, args.warmup)
     log(f"fp64 plain: {wall64 / args.steps * 1e3:.1f} ms/step")
+    if args.legs_only:
+        print(json.dumps({"fp32_batched_ms": wall32 / args.steps * 1e3, "fp64_plain_ms": wall64 / args.steps * 1e3,
+                          "fp32_device_ms": sum(dev32) / len(dev32), "fp64_device_ms": sum(dev64) / len(dev64)}),
+              flush=True)
+        fp32.close()
+        fp64.close()
+        return
     ps32 = fp32.path_steps
     ps64 = fp64.path_steps
     value = ps32 * args.steps / wall32
@@ -258,7 +272,7 @@ def main():
     ops = ops_per_path_step(3, 10, nbits, True, "fp32") / 10.0  # per (scenario) path-step
     dev_s = sum(dev32) / len(dev32) / 1e3
     achieved = ops * ps32 / dev_s
-    traffic = measured_traffic()
+    traffic, traffic_commit = measured_traffic()
     ops64 = ops_per_path_step(3, 1, plain_paths.bit_length(), True, "fp64")
     dev64_s = sum(dev64) / len(dev64) / 1e3
     achieved64 = ops64 * ps64 / dev64_s
@@ -308,9 +322,11 @@ def main():
                      "hbm_frac": traffic / dev_s / HBM_PEAK if traffic else None,
                      "kernel": "phoenix_kernel<float,3,10>", "device_ms": dev_s * 1e3,
                      "ops_per_path_step": ops,
+                     "traffic_profiled_at": traffic_commit,
                      "note": "VALU-bound: no MFMA (hashing, scalar FP); traffic = rocprofv3 "
                              "FETCH_SIZE+WRITE_SIZE bytes per launch, copy-calibrated, from "
-                             "profiles/phoenix_kernel_lv3_crn10_fp32_traffic.txt"},
+                             "profiles/phoenix_kernel_lv3_crn10_fp32_traffic.txt (profiled at "
+                             "commit traffic_profiled_at)"},
         "autotune": tuning,
     }
     fp32.close()
@@ -460,12 +476,18 @@ def book_roofline(book, tier: str, device_ms: float) -> dict:
     sec = device_ms / 1e3
     hbm = (reads + writes) / sec
     valu = ops / sec
+    traffic, commit = measured_traffic(BOOK_TRAFFIC_FILE) if tier == "fp32" else (None, None)
+    if tier == "fp64":
+        traffic, commit = measured_traffic(BOOK_TRAFFIC_FILE.replace("_traffic", "_fp64_traffic"))
     return {"bound": "hbm" if hbm / HBM_PEAK >= valu / VALU_PEAK else "valu",
             "achieved": hbm / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": hbm / HBM_PEAK,
             "algorithmic_bytes": reads + writes, "table_bytes_written": writes, "normals_bytes_read": reads,
+            "traffic": traffic, "traffic_profiled_at": commit,
             "valu_tops": valu / 1e12, "valu_frac": valu / VALU_PEAK, "device_ms": device_ms,
             "note": "device_ms spans the normal-table kernel and every path launch (events on the "
-                    "plan's stream); HBM traffic by rocprofv3 in profiles/phoenix_kernel_uneven_book_traffic.md"}
+                    "plan's stream); traffic = rocprofv3 FETCH_SIZE+WRITE_SIZE bytes of one book run "
+                    "(normal table + path launches), copy-calibrated, from "
+                    "profiles/phoenix_kernel_uneven_book_traffic.txt / _fp64_traffic.txt"}
 
 
 def book_leg(quick: bool) -> dict:

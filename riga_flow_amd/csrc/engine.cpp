@@ -352,6 +352,7 @@ struct Item {
   std::vector<int> surface_factor;
   std::vector<uint32_t> halton;
   int depth = 1;
+  int stack_need = 0;  // bridge levels the kernel holds below its top at most
   double cost = 0.0;
   uint64_t span = 0;  // paths [0, span) of the request's stream that the item reads
 };
@@ -388,6 +389,19 @@ void plan_bridge(int T, double dt, Item* it) {
     ++lo;
   }
   it->depth = max_depth;
+  // the kernel's stack (top in registers): a step with c > 0 nodes pushes c - 1
+  // of them, a step with none pops one; the levels it needs are the most pushed
+  int sp = 0, need = 0;
+  for (int t = 0; t < T; ++t) {
+    const int c = it->steps[t].n_desc;
+    if (c == 0) {
+      sp = sp > 0 ? sp - 1 : 0;
+    } else {
+      sp += c - 1;
+      need = std::max(need, sp);
+    }
+  }
+  it->stack_need = need;
 }
 
 void plan_schedule(const riga_request& r, double dt, Item* it) {
@@ -912,8 +926,8 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
 // per-scenario trace counters, the FP32 tier's Euler constants and the unit id.
 constexpr size_t kStaticLdsBound = sizeof(BlockShared) + 10 * 4 + 10 * kMaxFactors * sizeof(SfF32) + 64;
 
-// Bridge stack levels nearest the root (the least used) that a launch keeps in
-// HBM slots rather than LDS.  `split` < 0: none; > 0: that many; 0: the
+// Bridge stack levels (the least used positions, from both ends of the stack)
+// that a launch keeps in HBM slots rather than LDS.  `split` < 0: none; > 0: that many; 0: the
 // fewest that give the launch the most resident workgroups per CU -- a deep
 // stack in LDS (levels x factors x lanes x 8 bytes in FP64) can otherwise hold
 // occupancy below what the kernel's registers allow.
@@ -1167,7 +1181,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     lp.items = idx;
     int stack_levels = 0, lv_words = 0;
     for (int i : idx) {
-      stack_levels = std::max(stack_levels, items[i].depth - 1);
+      stack_levels = std::max(stack_levels, items[i].stack_need);
       int max_nx = 0;
       for (const LvGrid* g : items[i].surfaces) max_nx = std::max(max_nx, g->n_x());
       lv_words = std::max(lv_words, (int)items[i].surfaces.size() * (real_size == 4 ? 4 : 3) * max_nx);
@@ -1193,7 +1207,11 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       Item& it = items[i];
       ItemDesc& d = it.d;
       d.stack_levels = lp.mode == kLadderMonteCarlo ? 0 : stack_levels - lp.pooled;
+      // the pooled positions are the least used: a stack position's pushes grow
+      // binomially towards the middle (position k is the k-th left turn on the
+      // way to a leaf), so the pool takes the top and the bottom in turn
       d.stack_pooled = lp.pooled;
+      d.stack_pooled_low = lp.pooled / 2;
       d.paths_per_block = threads * ppt;
       d.n_blocks = (int32_t)((d.paths + d.paths_per_block - 1) / d.paths_per_block);
       blocks += d.n_blocks;

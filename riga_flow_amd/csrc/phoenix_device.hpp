@@ -381,10 +381,13 @@ __device__ __forceinline__ DD wave_sum(DD v) {
 // across the block.  A step that needs c new nodes pushes c-1 of them (the
 // last one is emitted at once) and a step that needs none pops one.
 //
-// The bottom `low_levels` levels (those nearest the root: level k is pushed
-// and popped about 2^k times per path) may live in the workgroup's HBM slot
-// instead, so that a deep stack does not hold occupancy below what registers
-// allow; the value a pop reads there is first needed a step later.
+// A launch may keep some stack positions in the workgroup's HBM slot instead,
+// so that a deep stack does not hold occupancy below what registers allow: the
+// least used ones.  Position k is the k-th left turn on the way to a leaf, so
+// its pushes grow binomially towards the middle (756 steps: 16, 56, 112, 140,
+// 112, 56, 16, 2 accesses per path): the bottom `low_levels` positions and the
+// top `hbm_levels - low_levels` go to HBM, the middle stays in LDS.  A value a
+// pop reads is first needed a step later.
 template <typename Real>
 using LdsPtr = __attribute__((address_space(3))) Real*;     // an LDS address: ds_* accesses only
 template <typename Real>
@@ -393,19 +396,33 @@ using GlobalPtr = __attribute__((address_space(1))) Real*;  // an HBM address: g
 template <typename Real, int F>
 struct BridgeStack {
   Real top[F];
-  LdsPtr<Real> col;  // this lane's LDS column: col[((sp - low_levels)*F + f) * stride]
-  GlobalPtr<Real> low;  // this lane's HBM column: low[(sp*F + f) * stride]
-  int stride;        // lanes per workgroup
-  int sp;            // levels held below the top
-  int low_levels;    // of which the bottom low_levels are in `low`
+  LdsPtr<Real> col;     // this lane's LDS column: col[((sp - low_levels)*F + f) * stride]
+  GlobalPtr<Real> hbm;  // this lane's HBM column: hbm[(slot*F + f) * stride]
+  int stride;           // lanes per workgroup
+  int sp;               // levels held below the top
+  int low_levels;       // positions [0, low_levels) in HBM slots 0..
+  int mid_end;          // positions [low_levels, mid_end) in LDS, [mid_end, ...) in HBM after the low ones
 
+  // where position p lives: an LDS address, or (returned false) HBM slot *h
+  __device__ __forceinline__ bool in_lds(int p, int* h) const {
+    if (p < low_levels) {
+      *h = p;
+      return false;
+    }
+    if (p >= mid_end) {
+      *h = low_levels + (p - mid_end);
+      return false;
+    }
+    return true;
+  }
   __device__ __forceinline__ void push(const Real (&v)[F]) {
-    if (sp < low_levels) {
-#pragma unroll
-      for (int f = 0; f < F; ++f) low[(sp * F + f) * stride] = top[f];
-    } else {
+    int h;
+    if (in_lds(sp, &h)) {
 #pragma unroll
       for (int f = 0; f < F; ++f) col[((sp - low_levels) * F + f) * stride] = top[f];
+    } else {
+#pragma unroll
+      for (int f = 0; f < F; ++f) hbm[(h * F + f) * stride] = top[f];
     }
 #pragma unroll
     for (int f = 0; f < F; ++f) top[f] = v[f];
@@ -416,12 +433,13 @@ struct BridgeStack {
   __device__ __forceinline__ void pop() {
     if (sp == 0) return;
     --sp;
-    if (sp < low_levels) {
-#pragma unroll
-      for (int f = 0; f < F; ++f) top[f] = low[(sp * F + f) * stride];
-    } else {
+    int h;
+    if (in_lds(sp, &h)) {
 #pragma unroll
       for (int f = 0; f < F; ++f) top[f] = col[((sp - low_levels) * F + f) * stride];
+    } else {
+#pragma unroll
+      for (int f = 0; f < F; ++f) top[f] = hbm[(h * F + f) * stride];
     }
   }
 };
@@ -866,8 +884,9 @@ void phoenix_kernel(const ItemDesc* __restrict__ items,
       bs.col = (LdsPtr<Real>)(lds_stack + tid);
       bs.stride = nthreads;
       bs.sp = 0;
-      bs.low_levels = it.stack_pooled;
-      bs.low = it.stack_pooled > 0 ? (GlobalPtr<Real>)(stack_low + tid) : nullptr;
+      bs.low_levels = it.stack_pooled > 0 ? it.stack_pooled_low : 0;
+      bs.mid_end = bs.low_levels + it.stack_levels;
+      bs.hbm = it.stack_pooled > 0 ? (GlobalPtr<Real>)(stack_low + tid) : nullptr;
       Real wlo[F];
       Real spot[SMAX][F];
       int ko_step[SMAX];

@@ -13,7 +13,7 @@ namespace riga {
 
 constexpr int kMaxFactors = 5;      // per-trade underlyings handled by the kernels
 constexpr int kBridgeStack = 14;    // pending bridge intervals (host-checked per trade)
-constexpr int kStackPooled = 8;     // bottom bridge levels a lane may keep in its workgroup's HBM slot
+constexpr int kStackPooled = 8;     // bridge stack positions a lane may keep in its workgroup's HBM slot
 constexpr int kTraceCap = 64;       // note.rs:57 TRACE_EVENT_CAP
 // How a launch's items get their normals; the path kernel is compiled once per mode.
 constexpr int kLadderDirect = 0;      // Sobol (Owen) or Halton points generated per lane
@@ -158,7 +158,7 @@ struct ItemDesc {
   int32_t n_surf;
   int32_t max_nx;
   int32_t stack_levels;  // bridge levels below the top held in LDS (launch-wide)
-  int32_t stack_pooled;  // further levels below those, the bottom of the stack, in an HBM slot
+  int32_t stack_pooled;  // further stack positions (the bottom and top ones) in an HBM slot
   int64_t partial_base;  // first Partial slot: [scenario][block_in_item]
   int64_t record_base;   // first PathRecord slot: [scenario][path]
   int32_t pad3;
@@ -193,7 +193,7 @@ struct ItemDesc {
   unsigned char* stack_pool;
   uint32_t* pool_flags;
   int32_t pool_n;
-  int32_t pool_pad;
+  int32_t stack_pooled_low;  // of the stack_pooled positions, how many are the bottom ones (the rest the top)
 };
 
 // Double-double accumulator (hi + lo).
