@@ -730,10 +730,15 @@ typedef __attribute__((address_space(3))) const F32x4 LdsF32x4;
 // Resident waves per SIMD each path kernel is compiled for (its register
 // budget: 512 / waves VGPRs per lane), chosen per variant from the compiler's
 // spill counts and timings: more waves wherever the kernel's live state fits the
-// smaller budget with its step loop unspilled.  The ten-scenario lanes keep their
-// log-spots and scan state in registers at two waves; the FP64 one-scenario
-// direct kernel of up to three factors runs four (its spills sit outside the
-// hot arithmetic: 16.4 -> 14.1 ms on the 262,144-path local-vol trade).
+// smaller budget or spills only what the step loop rarely touches.  The
+// ten-scenario FP32 lane of up to three factors runs three (its spills are the
+// coupon legs, read and written at period ends and knock-outs: 49.6 -> 39.2 ms
+// for the ten-scenario 3-asset batch, with its LDS, the exactly sized bridge
+// stack and twelve surfaces' slope rows, admitting three workgroups per CU); the
+// FP64 one-scenario direct kernel of up to three factors runs four (16.0 ->
+// 14.1 ms on the 262,144-path local-vol trade).  The Monte Carlo lane's word
+// ring holds its ten-scenario kernels at two workgroups per CU whatever the
+// registers.
 template <typename Real, int F, int SMAX, int kMode>
 constexpr int path_kernel_waves() {
   if constexpr (sizeof(Real) == 8) {
@@ -742,7 +747,7 @@ constexpr int path_kernel_waves() {
     if (kMode == kLadderDirect) return F <= 3 ? 4 : 3;
     return 3;
   } else {
-    if (SMAX >= 10) return 2;
+    if (SMAX >= 10) return (F <= 3 && kMode != kLadderMonteCarlo) ? 3 : 2;
     if (SMAX >= 4) return 3;
     return 4;
   }
