@@ -175,6 +175,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (shape 1 x 4 waves)")
     ap.add_argument("--book-only", action="store_true",
                     help="run only the uneven-book leg and print its JSON (for profiling)")
+    ap.add_argument("--mc-only", action="store_true",
+                    help="run only the Monte Carlo lane leg and print its JSON (for profiling)")
     ap.add_argument("--legs-only", action="store_true",
                     help="time only the FP32 batch and FP64 plain legs (for profiling: no error "
                          "checks or other legs launch the same kernels)")
@@ -189,6 +191,9 @@ def main():
     api.gpu_backend()
     if args.book_only:
         print(json.dumps({"uneven_book": book_leg(args.quick)}), flush=True)
+        return
+    if args.mc_only:
+        print(json.dumps({"mc_lane": mc_leg(args)}), flush=True)
         return
     big = 1_048_576 if not args.quick else 65_536
     plain_paths = 262_144 if not args.quick else 16_384

@@ -736,18 +736,21 @@ typedef __attribute__((address_space(3))) const F32x4 LdsF32x4;
 // for the ten-scenario 3-asset batch, with its LDS, the exactly sized bridge
 // stack and twelve surfaces' slope rows, admitting three workgroups per CU); the
 // FP64 one-scenario direct kernel of up to three factors runs four (16.0 ->
-// 14.1 ms on the 262,144-path local-vol trade).  The Monte Carlo lane's word
-// ring holds its ten-scenario kernels at two workgroups per CU whatever the
-// registers.
+// 14.1 ms on the 262,144-path local-vol trade).  The Monte Carlo kernels keep
+// the compiler's own budget (112 VGPRs for the ten-scenario FP32 one; its word
+// ring's LDS admits two workgroups per CU anyway): built for two waves, that
+// kernel ran 38.8 ms instead of 34.2 on ten 1,048,576-path scenarios.
 template <typename Real, int F, int SMAX, int kMode>
 constexpr int path_kernel_waves() {
-  if constexpr (sizeof(Real) == 8) {
+  if constexpr (kMode == kLadderMonteCarlo) {
+    return 1;
+  } else if constexpr (sizeof(Real) == 8) {
     if (SMAX >= 10) return 1;
     if (SMAX >= 4) return 2;
     if (kMode == kLadderDirect) return F <= 3 ? 4 : 3;
     return 3;
   } else {
-    if (SMAX >= 10) return (F <= 3 && kMode != kLadderMonteCarlo) ? 3 : 2;
+    if (SMAX >= 10) return F <= 3 ? 3 : 2;
     if (SMAX >= 4) return 3;
     return 4;
   }
