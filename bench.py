@@ -173,6 +173,9 @@ def main():
     ap.add_argument("--ppt64", type=int, default=0, help="FP64 plain: paths per lane (0 = autotuned)")
     ap.add_argument("--waves64", type=int, default=0, help="FP64 plain: waves per workgroup (0 = autotuned)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (shape 1 x 4 waves)")
+    ap.add_argument("--dispatched", action="store_true",
+                    help="one workgroup per unit instead of persistent workgroups (as many as the "
+                         "device holds, each claiming units until the queue is empty)")
     ap.add_argument("--book-only", action="store_true",
                     help="run only the uneven-book leg and print its JSON (for profiling)")
     ap.add_argument("--mc-only", action="store_true",
@@ -190,7 +193,7 @@ def main():
 
     api.gpu_backend()
     if args.book_only:
-        print(json.dumps({"uneven_book": book_leg(args.quick)}), flush=True)
+        print(json.dumps({"uneven_book": book_leg(args.quick, not args.dispatched)}), flush=True)
         return
     if args.mc_only:
         print(json.dumps({"mc_lane": mc_leg(args)}), flush=True)
@@ -208,7 +211,7 @@ def main():
             f"{tuning['fp32_batched']['waves_per_block']} waves")
     ppt, waves = args.ppt, args.waves
     fp32 = api.Plan(crn, "fp32", paths_per_thread=ppt, waves_per_block=waves, use_graph=True,
-                    max_dimension=md)
+                    max_dimension=md, persistent=not args.dispatched)
     plain = W.lv_book(paths=plain_paths, steps=756)
     if not args.no_tune and not (args.ppt64 and args.waves64):
         tuning["fp64_plain"] = api.autotune([plain], "fp64", use_graph=True, max_dimension=md,
@@ -216,7 +219,7 @@ def main():
         log(f"autotune fp64: {tuning['fp64_plain']['paths_per_thread']} paths/lane x "
             f"{tuning['fp64_plain']['waves_per_block']} waves")
     fp64 = api.Plan([plain], "fp64", paths_per_thread=args.ppt64, waves_per_block=args.waves64,
-                    use_graph=True, max_dimension=md)
+                    use_graph=True, max_dimension=md, persistent=not args.dispatched)
 
     wall32, dev32, res32 = timed(fp32, args.steps, args.warmup)
     log(f"fp32 batched: {wall32 / args.steps * 1e3:.1f} ms/step")
@@ -229,6 +232,18 @@ def main():
         fp32.close()
         fp64.close()
         return
+    # the same two batches in the other launch mode (same shapes), for the record
+    modes = {}
+    for key, plan, reqs, tier in (("fp32_batched", fp32, crn, "fp32"), ("fp64_plain", fp64, [plain], "fp64")):
+        other = api.Plan(reqs, tier, paths_per_thread=plan.info.paths_per_thread,
+                         waves_per_block=plan.info.waves_per_block, use_graph=True, max_dimension=md,
+                         persistent=args.dispatched)
+        w, d, _ = timed(other, max(3, args.steps // 2), 1)
+        other.close()
+        this_ms = (sum(dev32) / len(dev32)) if key == "fp32_batched" else (sum(dev64) / len(dev64))
+        modes[key] = {"persistent_device_ms": this_ms if not args.dispatched else sum(d) / len(d),
+                      "dispatched_device_ms": sum(d) / len(d) if not args.dispatched else this_ms}
+    log(f"launch modes: {modes}")
     ps32 = fp32.path_steps
     ps64 = fp64.path_steps
     value = ps32 * args.steps / wall32
@@ -333,6 +348,8 @@ def main():
                              "profiles/phoenix_kernel_lv3_crn10_fp32_traffic.txt (profiled at "
                              "commit traffic_profiled_at)"},
         "autotune": tuning,
+        "launch_mode": "dispatched" if args.dispatched else "persistent",
+        "launch_modes": modes,
     }
     fp32.close()
     fp64.close()
@@ -342,7 +359,7 @@ def main():
     if not args.no_mc:
         out["mc_lane"] = mc_leg(args)
     if not args.no_book:
-        out["uneven_book"] = book_leg(args.quick)
+        out["uneven_book"] = book_leg(args.quick, not args.dispatched)
     if not args.no_serving:
         out["serving"] = serving_leg(args.quick)
     if not args.no_cpu:
@@ -495,16 +512,17 @@ def book_roofline(book, tier: str, device_ms: float) -> dict:
                     "profiles/phoenix_kernel_uneven_book_traffic.txt / _fp64_traffic.txt"}
 
 
-def book_leg(quick: bool) -> dict:
+def book_leg(quick: bool, persistent: bool = False) -> dict:
     """2,048 uneven Phoenix trades (1-5 of 20 assets, 0.5-5y daily, 65,536 paths
     each) in one hipGraph: a shared normal table, then one launch per factor count,
-    the launches running concurrently on streams of their own."""
+    the launches running concurrently on streams of their own (persistent
+    workgroups claiming the trades' path chunks longest first)."""
     n = 2048 if not quick else 128
     book = W.uneven_book(n_trades=n, paths=65_536)
     out = {"trades": n, "paths": 65_536}
     for tier in ("fp32", "fp64"):
         t0 = time.perf_counter()
-        plan = api.Plan(book, tier, use_graph=True, max_dimension=7000)
+        plan = api.Plan(book, tier, use_graph=True, max_dimension=7000, persistent=persistent)
         setup = time.perf_counter() - t0
         plan.run(raw=True)
         t0 = time.perf_counter()
@@ -517,7 +535,7 @@ def book_leg(quick: bool) -> dict:
         out[tier] = {"path_steps": plan.path_steps, "ms_per_book": wall * 1e3,
                      "path_steps_per_s": plan.path_steps / wall,
                      "normal_tables": plan.info.normal_tables, "launches": plan.info.n_launches,
-                     "concurrent_launches": True,
+                     "concurrent_launches": True, "persistent_workgroups": persistent,
                      "plan_setup_s": setup, "device_gb": plan.info.device_bytes / 1e9,
                      "roofline": book_roofline(book, tier, min(dev))}
         log(f"book {tier}: {wall * 1e3:.1f} ms/book")

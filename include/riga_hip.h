@@ -147,6 +147,10 @@ typedef struct riga_launch_config {
   int32_t stack_split;      /* bridge stack levels nearest the root kept in private memory instead of
                                LDS: 0 = the fewest that let each launch reach the occupancy its
                                registers allow; -1 = none; k > 0 = k (at most 8) */
+  int32_t persistent;       /* 0: one workgroup per (item, path-chunk) unit, each claiming the next
+                               unit of the launch's longest-first queue as the hardware dispatches
+                               it into a free slot; 1: persistent workgroups, as many as the device
+                               holds at once, each claiming units until the queue is empty */
 } riga_launch_config;
 
 /* A batch laid out in device memory once (surfaces, factors, schedules, Sobol
@@ -232,6 +236,11 @@ int riga_plan_ready(riga_plan *plan);  /* 1: finished, 0: running, < 0: device e
 int riga_plan_wait(riga_plan *plan, riga_result *results, riga_error *err);
 int riga_plan_run(riga_plan *plan, riga_result *results, riga_error *err); /* launch + wait */
 int riga_plan_query(const riga_plan *plan, riga_plan_info *info);
+/* Device clock of the plan's last finished launch: when its stream reached the
+ * first kernel and when the path kernels were done, in seconds since the context
+ * was created.  No reference counterpart (the reference runs one blocking call at
+ * a time); the serving loop uses it to show plans overlapping on the device. */
+int riga_plan_device_times(riga_plan *plan, double *start_s, double *end_s);
 void riga_plan_destroy(riga_plan *plan);
 
 /* ---- serving: continuous batching on one device ----
@@ -260,13 +269,15 @@ typedef struct riga_ticket_info {
   double submit_s, start_s, end_s, latency_s; /* seconds since the server started */
   int64_t device_bytes;
   double path_steps;
+  double device_start_s, device_end_s; /* its plan's device span (riga_plan_device_times); -1: none */
 } riga_ticket_info;
 
 typedef struct riga_server_stats {
   int64_t submitted_tickets, completed_tickets, completed_requests, shed_tickets, failed_tickets;
   int64_t batches, peak_bytes, bytes_in_use;
   double path_steps;
-  int32_t queued_tickets, running_plans, max_concurrent, reserved;
+  int32_t queued_tickets, running_plans, max_concurrent;
+  int32_t overlapped_plans; /* plans whose device span met that of a plan finished before them */
 } riga_server_stats;
 
 int riga_server_create(riga_ctx *ctx, const riga_server_config *cfg, riga_server **out,

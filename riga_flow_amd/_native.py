@@ -119,7 +119,7 @@ class LaunchConfig(C.Structure):
                 ("owen_table", C.c_int32), ("normal_table", C.c_int32),
                 ("stream_ordered_alloc", C.c_int32), ("waves_per_block", C.c_int32),
                 ("serial_launches", C.c_int32), ("fp32_full_ladder", C.c_int32),
-                ("stack_split", C.c_int32)]
+                ("stack_split", C.c_int32), ("persistent", C.c_int32)]
 
 
 class PlanInfo(C.Structure):
@@ -149,7 +149,8 @@ class TicketInfo(C.Structure):
     _fields_ = [("state", C.c_int32), ("n_requests", C.c_int32), ("batch_tickets", C.c_int32),
                 ("reserved", C.c_int32), ("submit_s", C.c_double), ("start_s", C.c_double),
                 ("end_s", C.c_double), ("latency_s", C.c_double), ("device_bytes", C.c_int64),
-                ("path_steps", C.c_double)]
+                ("path_steps", C.c_double), ("device_start_s", C.c_double),
+                ("device_end_s", C.c_double)]
 
 
 class ServerStats(C.Structure):
@@ -158,14 +159,15 @@ class ServerStats(C.Structure):
                 ("failed_tickets", C.c_int64), ("batches", C.c_int64), ("peak_bytes", C.c_int64),
                 ("bytes_in_use", C.c_int64), ("path_steps", C.c_double),
                 ("queued_tickets", C.c_int32), ("running_plans", C.c_int32),
-                ("max_concurrent", C.c_int32), ("reserved", C.c_int32)]
+                ("max_concurrent", C.c_int32), ("overlapped_plans", C.c_int32)]
 
 
 EXPORTS = ("riga_hip_init", "riga_hip_destroy", "riga_hip_device_name", "riga_hip_arch",
            "riga_validate", "riga_price_put_leg", "riga_price_put_leg_noalloc", "riga_price_batch",
            "riga_replay_paths",
            "riga_plan_create", "riga_plan_estimate", "riga_plan_launch", "riga_plan_ready",
-           "riga_plan_wait", "riga_plan_run", "riga_plan_query", "riga_plan_destroy",
+           "riga_plan_wait", "riga_plan_run", "riga_plan_query", "riga_plan_device_times",
+           "riga_plan_destroy",
            "riga_server_create", "riga_server_submit", "riga_server_wait",
            "riga_server_stats_get", "riga_server_destroy",
            "riga_surface_create", "riga_surface_destroy", "riga_autotune",
@@ -232,6 +234,8 @@ def lib():
             L.riga_plan_wait.argtypes = [C.c_void_p, C.POINTER(Result), C.POINTER(Error)]
             L.riga_plan_run.argtypes = [C.c_void_p, C.POINTER(Result), C.POINTER(Error)]
             L.riga_plan_query.argtypes = [C.c_void_p, C.POINTER(PlanInfo)]
+            L.riga_plan_device_times.argtypes = [C.c_void_p, C.POINTER(C.c_double),
+                                                 C.POINTER(C.c_double)]
             L.riga_plan_destroy.argtypes = [C.c_void_p]
             L.riga_plan_destroy.restype = None
             L.riga_lv_calibrate.argtypes = [C.POINTER(ImpliedGrid), C.c_double, C.c_double,

@@ -279,7 +279,8 @@ def result_to_json(res: PriceResult) -> dict:
 def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",
                 paths_per_thread: int = 0, max_dimension: int = 512,
                 raw: bool = False, owen_table: bool = True, normal_table: int = 0,
-                waves_per_block: int = 0, fp32_full_ladder: bool = False, stack_split: int = 0):
+                waves_per_block: int = 0, fp32_full_ladder: bool = False, stack_split: int = 0,
+                persistent: bool = False):
     """Price many requests in one launch; CRN scenarios share their Sobol points."""
     gpu_backend()
     for q in requests:
@@ -289,7 +290,8 @@ def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",
     arr = (N.Request * n)(*[p.r for p in packed])
     res = (N.Result * n)()
     cfg = N.LaunchConfig(paths_per_thread, 0, 0 if owen_table else -1, normal_table, 0,
-                         waves_per_block, 0, 1 if fp32_full_ladder else 0, stack_split)
+                         waves_per_block, 0, 1 if fp32_full_ladder else 0, stack_split,
+                         1 if persistent else 0)
     err = N.Error()
     dev = N.Device.get()
     N.check(N.lib().riga_price_batch(dev.handle, arr, n, PRECISIONS[precision], C.byref(cfg), res,
@@ -328,7 +330,7 @@ class Plan:
                  paths_per_thread: int = 0, use_graph: bool = False, max_dimension: int = 512,
                  owen_table: bool = True, normal_table: int = 0, waves_per_block: int = 0,
                  serial_launches: bool = False, fp32_full_ladder: bool = False,
-                 stack_split: int = 0):
+                 stack_split: int = 0, persistent: bool = False):
         gpu_backend()
         for q in requests:
             _check_as_of(q)
@@ -338,7 +340,7 @@ class Plan:
         arr = (N.Request * n)(*[p.r for p in packed])
         cfg = N.LaunchConfig(paths_per_thread, 1 if use_graph else 0, 0 if owen_table else -1,
                              normal_table, 0, waves_per_block, 1 if serial_launches else 0,
-                             1 if fp32_full_ladder else 0, stack_split)
+                             1 if fp32_full_ladder else 0, stack_split, 1 if persistent else 0)
         err = N.Error()
         h = C.c_void_p()
         N.check(N.lib().riga_plan_create(N.Device.get().handle, arr, n, PRECISIONS[precision],

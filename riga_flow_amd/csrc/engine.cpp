@@ -178,6 +178,7 @@ struct riga_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t epoch = nullptr;  // recorded once at set-up: the origin of plans' device times
   std::string name, arch;
   size_t lds_limit = 65536;
   int n_cus = 256;  // compute units of the device
@@ -1283,7 +1284,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
     }
     lp.blocks = blocks;
     lp.cost = items[idx.front()].cost;
-    if (lp.pooled > 0) {  // one slot per workgroup that can be resident at once
+    if (lp.pooled > 0) {  // a slot per workgroup that can be resident at once, and spares
       PhoenixLaunch L{};
       L.F = lp.F;
       L.smax = lp.smax;
@@ -1297,7 +1298,9 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
         set_err(err, "gpu_error", "occupancy of the path kernel could not be found");
         return RIGA_EGPU;
       }
-      lp.pool_n = std::min(blocks, per_cu * ctx->n_cus);
+      // plus one spare per CU: a workgroup the dispatcher starts as another ends
+      // never waits on the ending one's hand-back
+      lp.pool_n = std::min(blocks, (per_cu + 1) * ctx->n_cus);
       lp.pool_off = pl->pool_bytes;
       pl->pool_bytes += (size_t)lp.pool_n * lp.pooled * lp.F * lp.threads * real_size;
       lp.flags_off = pl->pool_nwords;
@@ -1476,8 +1479,31 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
           descs[i].pool_n = lp.pool_n;
         }
   }
-  // one workgroup per unit (each claims its unit from the launch's counter)
-  for (LaunchPlan& lp : pl->launches) lp.grid = lp.blocks;
+  // Each workgroup claims its unit from the launch's counter.  By default the grid
+  // is one workgroup per unit and the hardware dispatcher starts each in a slot as
+  // one frees up: the last units spread over idle CUs, and a plan launched while
+  // another runs gets slots as the running one's workgroups end.  Persistent
+  // launches hold as many workgroups as the device holds at once (at most one per
+  // unit), each claiming units until none is left.
+  const bool persistent = cfg && cfg->persistent != 0;
+  for (LaunchPlan& lp : pl->launches) {
+    lp.grid = lp.blocks;
+    if (!persistent) continue;
+    PhoenixLaunch L{};
+    L.F = lp.F;
+    L.smax = lp.smax;
+    L.mode = lp.mode;
+    L.threads = lp.threads;
+    L.lds = lp.lds;
+    int per_cu = 0;
+    const hipError_t e = precision == RIGA_FP64 ? phoenix_blocks_per_cu_f64(L, &per_cu)
+                                                : phoenix_blocks_per_cu_f32(L, &per_cu);
+    if (e != hipSuccess || per_cu <= 0) {
+      set_err(err, "gpu_error", "occupancy of the path kernel could not be found");
+      return RIGA_EGPU;
+    }
+    lp.grid = std::min(lp.blocks, per_cu * ctx->n_cus);
+  }
   pl->serial = cfg && cfg->serial_launches != 0;
   if (pl->launches.size() > 1 && !pl->serial) {
     pl->fork_join.assign(pl->launches.size(), nullptr);
@@ -1697,6 +1723,13 @@ int riga_hip_init(int device, riga_ctx** out, riga_error* err) {
     set_err(err, "gpu_error", std::string("path kernel set-up: ") + hipGetErrorString(e));
     return RIGA_EGPU;
   }
+  e = hipEventCreate(&ctx->epoch);
+  if (e == hipSuccess) e = hipEventRecord(ctx->epoch, ctx->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(ctx->epoch);
+  if (e != hipSuccess) {
+    set_err(err, "gpu_error", std::string("device clock origin: ") + hipGetErrorString(e));
+    return RIGA_EGPU;
+  }
   *out = ctx.release();
   return 0;
 }
@@ -1708,6 +1741,7 @@ void riga_hip_destroy(riga_ctx* ctx) {
   if (ctx->noalloc_stream) (void)hipStreamSynchronize(ctx->noalloc_stream);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->epoch) (void)hipEventDestroy(ctx->epoch);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->noalloc_e0) (void)hipEventDestroy(ctx->noalloc_e0);
   if (ctx->noalloc_e1) (void)hipEventDestroy(ctx->noalloc_e1);
@@ -1785,6 +1819,18 @@ int riga_plan_wait(riga_plan* plan, riga_result* results, riga_error* err) {
 int riga_plan_run(riga_plan* plan, riga_result* results, riga_error* err) {
   int rc = riga_plan_launch(plan, err);
   return rc ? rc : riga_plan_wait(plan, results, err);
+}
+
+int riga_plan_device_times(riga_plan* plan, double* start_s, double* end_s) {
+  if (!plan || !start_s || !end_s || !plan->e0 || !plan->e1 || !plan->ctx->epoch) return RIGA_EINVAL;
+  if (hipEventQuery(plan->e1) != hipSuccess) return RIGA_EINVAL;  // not launched, or still running
+  float a = 0.0f, b = 0.0f;
+  if (hipEventElapsedTime(&a, plan->ctx->epoch, plan->e0) != hipSuccess ||
+      hipEventElapsedTime(&b, plan->ctx->epoch, plan->e1) != hipSuccess)
+    return RIGA_EGPU;
+  *start_s = 1e-3 * (double)a;
+  *end_s = 1e-3 * (double)b;
+  return 0;
 }
 
 int riga_plan_query(const riga_plan* plan, riga_plan_info* info) {

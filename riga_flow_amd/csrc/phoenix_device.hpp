@@ -770,7 +770,13 @@ void phoenix_kernel(const ItemDesc* __restrict__ items,
   __shared__ SfF32 sf_lds[sizeof(Real) == 4 ? SMAX * F : 1];  // FP32 Euler constants, read per step
   __shared__ uint32_t unit_sh, slot_sh;
 
-  {
+  // Each workgroup claims (item, chunk) units from the launch's counter, longest
+  // items first.  With one workgroup per unit it claims once; a persistent grid
+  // (at most the workgroups the device holds at once) claims until the table is
+  // used up.  The counter only grows, so every workgroup reaches the end of the
+  // table and leaves.
+  for (;;) {
+    __syncthreads();  // every lane is done with the previous unit's LDS (unit id, constants, sums)
     if (threadIdx.x == 0) unit_sh = atomicAdd(queue, 1u);
     __syncthreads();
     const uint32_t unit = unit_sh;
@@ -779,7 +785,12 @@ void phoenix_kernel(const ItemDesc* __restrict__ items,
     const ItemDesc& it = items[item];
     const int S = it.S;
     const int steps = it.steps;
-    const int tid = threadIdx.x;
+    // the lane id through an opaque copy per unit: values derived from it are then
+    // formed where they are used, not hoisted out of the unit loop and carried in
+    // registers through the path loops
+    int tid_unit = threadIdx.x;
+    asm volatile("" : "+v"(tid_unit));
+    const int tid = tid_unit;
     const int nthreads = blockDim.x;  // 64, 128 or 256: the launch shape the autotuner picked
     const int nwaves = nthreads >> 6;
     const int wave = tid >> 6;
@@ -801,20 +812,23 @@ void phoenix_kernel(const ItemDesc* __restrict__ items,
     double* zx = reinterpret_cast<double*>(mc_ring + (kMC ? kRingSlots * nthreads : 0));
     double* zf = zx + kZigLen;
     Real* lds_stack = reinterpret_cast<Real*>(lds_dyn);
-    // the bottom bridge levels' HBM slot: claimed for this workgroup's lifetime
-    // from the launch's pool, which holds a slot for every workgroup that can be
-    // resident at once (the claim scans the pool twice at most, so it ends)
+    // the bottom bridge levels' HBM slot: claimed for the unit from the launch's
+    // pool, which holds a slot for every workgroup that can be resident at once
+    // plus one spare per CU.  The claim scans the pool at most eight times, with a
+    // short sleep between passes, so it ends.
     Real* stack_low = nullptr;
     if (it.stack_pooled > 0) {
       if (tid == 0) {
         const uint32_t n = (uint32_t)it.pool_n;
-        uint32_t slot = blockIdx.x % n, got = 0xFFFFFFFFu;
-        for (uint32_t probe = 0; probe < 2 * n; ++probe) {
+        const uint32_t first = blockIdx.x % n;
+        uint32_t slot = first, got = 0xFFFFFFFFu;
+        for (uint32_t probe = 0; probe < 8 * n; ++probe) {
           if (atomicCAS(it.pool_flags + slot, 0u, 1u) == 0u) {
             got = slot;
             break;
           }
           slot = slot + 1 == n ? 0 : slot + 1;
+          if (slot == first) __builtin_amdgcn_s_sleep(32);
         }
         if (got == 0xFFFFFFFFu) atomicAdd(it.pool_flags + n, 1u);  // the host reports the plan failed
         slot_sh = got;
@@ -1392,9 +1406,15 @@ void phoenix_kernel(const ItemDesc* __restrict__ items,
       __syncthreads();
       if (tid == 0) {
         __threadfence();
-        atomicExch(it.pool_flags + slot_sh, 0u);
+        // a returning exchange: the workgroup waits for it, so the slot is free
+        // before the workgroup ends and the dispatcher starts another in its place
+        // (a slot found free here was never held: counted as a failure)
+        if (atomicExch(it.pool_flags + slot_sh, 0u) != 1u) atomicAdd(it.pool_flags + it.pool_n, 1u);
       }
     }
+    // one workgroup per unit: this one is done (the dispatcher starts the next);
+    // a persistent grid (fewer workgroups than units) claims again
+    if (gridDim.x >= (uint32_t)n_units) return;
   }
 }
 

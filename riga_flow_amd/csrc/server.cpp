@@ -11,7 +11,12 @@
 //     (estimated exactly by a dry-run plan) and the in-flight limit allow;
 //     everything admitted together becomes one plan (CRN scenarios share points);
 //   * each plan runs on its own stream with stream-ordered memory, so a plan
-//     admitted while others run overlaps with them on the device;
+//     admitted while others run overlaps with them on the device: its launches
+//     are one workgroup per (item, path-chunk) unit (not persistent workgroups,
+//     which would hold every slot until their queue is empty), so the hardware
+//     dispatcher gives the new plan's workgroups slots as the running plans'
+//     workgroups end -- the new work joins the running work within one unit's
+//     time, and each plan's device span is recorded to show it;
 //   * finished plans are reaped by polling their streams, results go back to
 //     their tickets and the memory returns to the budget;
 //   * tickets that wait longer than the queue limit are shed.
@@ -117,6 +122,7 @@ struct Ticket {
   int64_t bytes = 0;
   int batch_tickets = 0;
   double path_steps = 0;
+  double dev_start = -1.0, dev_end = -1.0;  // its plan's device span (seconds, context clock)
   std::vector<riga_result> results;
   riga_error err{};
 };
@@ -146,12 +152,14 @@ struct riga_server {
   riga_server_stats stats{};
   double t0 = 0;
   int waiters = 0;  // callers inside riga_server_wait (destroy waits for them to leave)
+  std::deque<std::pair<double, double>> spans;  // device spans of the latest finished plans
 
   riga_launch_config launch_config() const {
     riga_launch_config lc{};
     lc.paths_per_thread = cfg.paths_per_thread;  // 0: the autotuned shape, if any
     lc.normal_table = cfg.normal_table;
     lc.stream_ordered_alloc = 1;
+    lc.persistent = 0;  // one workgroup per unit: plans share the device's slots
     return lc;
   }
 
@@ -169,9 +177,22 @@ struct riga_server {
       riga_error err{};
       const int rc = riga_plan_wait(f.plan, f.results.data(), &err);
       const double t = now_s();
+      double ds = -1.0, de = -1.0;
+      if (rc == 0 && riga_plan_device_times(f.plan, &ds, &de) == 0) {
+        // did this plan run on the device while another one did?
+        bool met = false;
+        for (const auto& sp : spans) met = met || (ds < sp.second && sp.first < de);
+        if (met) stats.overlapped_plans += 1;
+        spans.emplace_back(ds, de);
+        if (spans.size() > 64) spans.pop_front();
+      } else {
+        ds = de = -1.0;
+      }
       for (size_t k = 0; k < f.tickets.size(); ++k) {
         Ticket* tk = f.tickets[k];
         tk->t_end = t;
+        tk->dev_start = ds;
+        tk->dev_end = de;
         if (rc == 0) {
           tk->results.assign(f.results.begin() + f.offsets[k], f.results.begin() + f.offsets[k] + tk->reqs.size());
           tk->state = kDone;
@@ -452,6 +473,8 @@ int riga_server_wait(riga_server* s, uint64_t ticket, double timeout_s, riga_res
     info->latency_s = tk->t_end - tk->t_submit;
     info->device_bytes = tk->bytes;
     info->path_steps = tk->path_steps;
+    info->device_start_s = tk->dev_start;
+    info->device_end_s = tk->dev_end;
   }
   int rc = 0;
   if (tk->state == kDone) {

@@ -41,6 +41,8 @@ class TicketResult:
     n_requests: int
     batch_tickets: int
     path_steps: float
+    device_start_s: float = -1.0  # its plan's span on the device clock (s since the context began)
+    device_end_s: float = -1.0
     results: list = field(default_factory=list)
     error: Optional[RigaError] = None
 
@@ -85,7 +87,8 @@ class Server:
             return None
         del self._pending[ticket]
         out = TicketResult(ticket, info.state, info.latency_s, info.submit_s, info.start_s,
-                           info.end_s, info.n_requests, info.batch_tickets, info.path_steps)
+                           info.end_s, info.n_requests, info.batch_tickets, info.path_steps,
+                           info.device_start_s, info.device_end_s)
         if rc == 0:
             out.results = [api._assemble(q, res[i]) for i, q in enumerate(reqs)] if assemble \
                 else list(res)
@@ -96,7 +99,7 @@ class Server:
     def stats(self) -> dict:
         s = N.ServerStats()
         N.lib().riga_server_stats_get(self.handle, C.byref(s))
-        return {k: getattr(s, k) for k, _ in N.ServerStats._fields_ if k != "reserved"}
+        return {k: getattr(s, k) for k, _ in N.ServerStats._fields_}
 
     def close(self) -> None:
         if getattr(self, "handle", None):

@@ -90,3 +90,26 @@ def test_pricing_without_a_device_fails_loudly():
     with pytest.raises(RigaError) as e:
         api.price_put_leg(W.flat_book(paths=16))
     assert e.value.code == "gpu_unavailable"
+
+
+def test_ctypes_mirrors_match_the_header_compiled_by_gcc(tmp_path):
+    # the C compiler's sizes and offsets of the structs that gained fields,
+    # against the ctypes mirror the Python side passes through the ABI
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "riga_hip.h"\n'
+        "int main(void) {\n"
+        '  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(riga_launch_config),\n'
+        "         offsetof(riga_launch_config, persistent), sizeof(riga_ticket_info),\n"
+        "         offsetof(riga_ticket_info, device_end_s), sizeof(riga_server_stats),\n"
+        "         offsetof(riga_server_stats, overlapped_plans));\n"
+        "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    want = [C.sizeof(N.LaunchConfig), N.LaunchConfig.persistent.offset, C.sizeof(N.TicketInfo),
+            N.TicketInfo.device_end_s.offset, C.sizeof(N.ServerStats),
+            N.ServerStats.overlapped_plans.offset]
+    assert got == want

@@ -332,6 +332,29 @@ def test_launch_shape_changes_nothing_but_the_shape(ppt, waves):
     assert rel(b.pv, a.pv) <= 1e-14
 
 
+@pytest.mark.parametrize("tier", ["fp64", "fp32"])
+def test_persistent_workgroups_price_as_one_workgroup_per_unit(tier):
+    # persistent launches (as many workgroups as the device holds, each claiming
+    # units until the queue is empty) against one workgroup per unit: a CRN batch
+    # of more units than the device holds workgroups, and a book of 1-5 factor
+    # trades (one launch per factor count, FP64 ones with pooled bridge stacks)
+    names, crn = W.crn_scenarios(paths=262_144, steps=126, years=0.5)
+    book = W.uneven_book(n_trades=10, paths=4096, seed=21)
+    for q in book:
+        q.trade.steps = min(q.trade.steps, 300)
+        q.trade.maturity_years = q.trade.steps / 252.0
+        q.trade.coupon_periods = max(1, round(12 * q.trade.maturity_years))
+        q.config.variance_reduction = W.VarianceReduction("sobol", q.trade.steps * q.factors)
+    for reqs in (crn, book):
+        md = max(W.max_dimension_for(r) for r in reqs)
+        a = api.price_batch(reqs, tier, max_dimension=md)
+        b = api.price_batch(reqs, tier, max_dimension=md, persistent=True)
+        for x, y in zip(a, b):
+            assert x.pv == y.pv and x.legs.down_and_in_put == y.legs.down_and_in_put
+            assert x.stats[0].output_count == y.stats[0].output_count
+            assert x.trace_events == y.trace_events
+
+
 def test_autotuner_picks_a_shape_that_plans_then_use():
     names, reqs = W.crn_scenarios(paths=8192, steps=126, years=0.5)
     md = max(W.max_dimension_for(r) for r in reqs)

@@ -54,6 +54,36 @@ def test_plans_overlap_on_the_device():
     assert rel(done[0].results[0].pv, done[2].results[0].pv) == 0.0
 
 
+def test_a_ticket_arriving_mid_run_joins_the_running_device_work():
+    # A long FP64 ticket (ten CRN scenarios on 1,048,576 paths of the 3y trade,
+    # about a second of device work) and, once it runs, a small FP32 ticket: the
+    # small ticket's workgroups take slots as the long launch's workgroups end, so
+    # on the device clock it starts after the long one and finishes before it,
+    # and its result equals the one it gets alone.
+    _, crn = W.crn_scenarios(paths=1_048_576, steps=756, years=3.0)
+    small = [W.flat_book(paths=4096, steps=252)]
+    srv = serving.Server(max_inflight=3, max_batch_requests=64)
+    long_id = srv.submit(crn, "fp64")
+    t0 = time.time()
+    while srv.stats()["running_plans"] < 1 and time.time() - t0 < 60.0:
+        time.sleep(0.005)
+    time.sleep(0.05)
+    small_id = srv.submit(small, "fp32")
+    s = srv.wait(small_id)
+    long_r = srv.wait(long_id)
+    stats = srv.stats()
+    srv.close()
+    assert s.state == serving.DONE and long_r.state == serving.DONE
+    print(f"long ticket device {long_r.device_start_s:.4f}-{long_r.device_end_s:.4f} s, "
+          f"small ticket device {s.device_start_s:.4f}-{s.device_end_s:.4f} s", flush=True)
+    assert long_r.device_start_s >= 0.0 and s.device_start_s >= 0.0
+    assert long_r.device_start_s < s.device_start_s
+    assert s.device_end_s < long_r.device_end_s
+    assert stats["overlapped_plans"] >= 1
+    alone = api.price_batch(small, "fp32", max_dimension=MD)
+    assert s.results[0].pv == alone[0].pv
+
+
 def test_stale_tickets_are_shed_and_budget_is_enforced():
     big = [W.lv_book(paths=262_144, steps=756)]
     srv = serving.Server(max_inflight=1, max_batch_requests=1, max_queue_seconds=0.002)
