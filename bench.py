@@ -31,10 +31,12 @@ from riga_flow_amd import api, workloads as W  # noqa: E402
 OWEN_ROUND = 21     # hash input xor 2 (three-input), xorshift 2, 64-bit multiply 4,
                     # xorshift 3, top bit of the 2nd multiply 5, scrambled bit 2, prefix*C3 update 3
                     # (the prefix itself is one multiply of P by C3^-1 after the last level)
-# hashed levels: the top 16 of the 32 come from the per-dimension prefix table; the FP32
-# tier hashes levels 15..8 (a float's 24 bits) and levels 7..0 only for coordinates
-# within 2^-10 of 0 or 1 (a fraction 2^-9 of them)
-OWEN_ROUNDS = {"fp64": 16, "fp32": 8 + 8 * 2.0 ** -9}
+# hashed levels: the top OWEN_TABLE_BITS of the 32 come from the per-dimension prefix
+# table (RIGA_OWEN_PREFIX_BITS in riga_flow_amd/csrc/phoenix_layout.h); the FP32 tier
+# hashes the rest of a float's 24 bits and levels 7..0 only for coordinates within
+# 2^-10 of 0 or 1 (a fraction 2^-9 of them)
+OWEN_TABLE_BITS = 20
+OWEN_ROUNDS = {"fp64": 32 - OWEN_TABLE_BITS, "fp32": 24 - OWEN_TABLE_BITS + 8 * 2.0 ** -9}
 OWEN_TABLE = 3      # prefix table address and load
 PROBIT = 22         # rational (12) + divide (8) + tails (5% of log + sqrt + rational)
 UNIFORM = 4         # int->float, +0.5, scale, clamp

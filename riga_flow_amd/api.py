@@ -276,9 +276,16 @@ def result_to_json(res: PriceResult) -> dict:
 
 
 # ------------------------------------------------------------------ pricing
+def _owen_mode(owen_table: Optional[bool]) -> int:
+    """riga_launch_config.owen_table: None lets the library decide per seed (a table it
+    holds already, or one worth building for the plan's points), True forces the
+    prefix table, False hashes all 32 Owen levels (the same integers)."""
+    return 0 if owen_table is None else (1 if owen_table else -1)
+
+
 def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",
                 paths_per_thread: int = 0, max_dimension: int = 512,
-                raw: bool = False, owen_table: bool = True, normal_table: int = 0,
+                raw: bool = False, owen_table: Optional[bool] = None, normal_table: int = 0,
                 waves_per_block: int = 0, fp32_full_ladder: bool = False, stack_split: int = 0,
                 persistent: bool = False):
     """Price many requests in one launch; CRN scenarios share their Sobol points."""
@@ -289,7 +296,7 @@ def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",
     n = len(packed)
     arr = (N.Request * n)(*[p.r for p in packed])
     res = (N.Result * n)()
-    cfg = N.LaunchConfig(paths_per_thread, 0, 0 if owen_table else -1, normal_table, 0,
+    cfg = N.LaunchConfig(paths_per_thread, 0, _owen_mode(owen_table), normal_table, 0,
                          waves_per_block, 0, 1 if fp32_full_ladder else 0, stack_split,
                          1 if persistent else 0)
     err = N.Error()
@@ -302,7 +309,7 @@ def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",
 
 
 def autotune(requests: Sequence[PriceRequest], precision: str = "fp64", use_graph: bool = True,
-             max_dimension: int = 512, owen_table: bool = True, normal_table: int = 0,
+             max_dimension: int = 512, owen_table: Optional[bool] = None, normal_table: int = 0,
              reps: int = 2) -> dict:
     """Time the batch at every launch shape (paths per lane x waves per workgroup) and
     keep the fastest for batches of its shape: plans and batches created afterwards
@@ -311,7 +318,7 @@ def autotune(requests: Sequence[PriceRequest], precision: str = "fp64", use_grap
     packed = [_Packed(q, max_dimension) for q in requests]
     n = len(packed)
     arr = (N.Request * n)(*[p.r for p in packed])
-    cfg = N.LaunchConfig(0, 1 if use_graph else 0, 0 if owen_table else -1, normal_table)
+    cfg = N.LaunchConfig(0, 1 if use_graph else 0, _owen_mode(owen_table), normal_table)
     rep = N.TuneReport()
     err = N.Error()
     N.check(N.lib().riga_autotune(N.Device.get().handle, arr, n, PRECISIONS[precision],
@@ -328,7 +335,7 @@ class Plan:
 
     def __init__(self, requests: Sequence[PriceRequest], precision: str = "fp64",
                  paths_per_thread: int = 0, use_graph: bool = False, max_dimension: int = 512,
-                 owen_table: bool = True, normal_table: int = 0, waves_per_block: int = 0,
+                 owen_table: Optional[bool] = None, normal_table: int = 0, waves_per_block: int = 0,
                  serial_launches: bool = False, fp32_full_ladder: bool = False,
                  stack_split: int = 0, persistent: bool = False):
         gpu_backend()
@@ -338,7 +345,7 @@ class Plan:
         packed = [_Packed(q, max_dimension) for q in requests]
         n = len(packed)
         arr = (N.Request * n)(*[p.r for p in packed])
-        cfg = N.LaunchConfig(paths_per_thread, 1 if use_graph else 0, 0 if owen_table else -1,
+        cfg = N.LaunchConfig(paths_per_thread, 1 if use_graph else 0, _owen_mode(owen_table),
                              normal_table, 0, waves_per_block, 1 if serial_launches else 0,
                              1 if fp32_full_ladder else 0, stack_split, 1 if persistent else 0)
         err = N.Error()

@@ -167,6 +167,7 @@ struct HostBuf {
 struct OwenTable {
   uint64_t seed = 0;
   int dims = 0;
+  uint64_t last_use = 0;  // the context's request count when a plan last took it
   DevBuf buf;
   hipEvent_t ready = nullptr;  // recorded after the build
   ~OwenTable() {
@@ -187,6 +188,7 @@ struct riga_ctx {
   std::map<std::string, std::pair<int, int>> tuned;  // shape signature -> (paths/thread, waves)
   std::mutex owen_mu;
   std::map<uint64_t, std::shared_ptr<struct OwenTable>> owen;  // per seed, grown on demand
+  uint64_t owen_uses = 0;  // requests for a table so far (least recently used ones go first)
   DevBuf arena, items, block_item, partials, finals, final_base, records;
   HostBuf staging, finals_host;
   PinnedPool pinned;  // plans' download buffers (plans end before their context)
@@ -745,11 +747,17 @@ namespace {
 // The context's table for `seed` covering at least `dims` dimensions, building a
 // larger one (at least twice the old size, so growth is rare) on `stream` when
 // needed; callers wait on table->ready before their kernels read it.
+// The context keeps at most this many bytes of Owen tables; beyond it the least
+// recently used tables of other seeds are dropped (plans still holding one keep it).
+constexpr size_t kOwenTableCap = (size_t)64 << 30;
+
 hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t stream,
                           std::shared_ptr<OwenTable>* out) {
   std::lock_guard<std::mutex> g(ctx->owen_mu);
+  const uint64_t use = ++ctx->owen_uses;
   auto f = ctx->owen.find(seed);
   if (f != ctx->owen.end() && f->second->dims >= dims) {
+    f->second->last_use = use;
     *out = f->second;
     return hipSuccess;
   }
@@ -762,9 +770,60 @@ hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t st
   if (e == hipSuccess) e = launch_owen_table(seed, want, (OwenEntry*)t->buf.p, stream);
   if (e == hipSuccess) e = hipEventRecord(t->ready, stream);
   if (e != hipSuccess) return e;
+  t->last_use = use;
   ctx->owen[seed] = t;
+  size_t held = 0;
+  for (const auto& kv : ctx->owen) held += kv.second->buf.cap;
+  while (held > kOwenTableCap && ctx->owen.size() > 1) {
+    auto lru = ctx->owen.end();
+    for (auto it = ctx->owen.begin(); it != ctx->owen.end(); ++it)
+      if (it->first != seed && (lru == ctx->owen.end() || it->second->last_use < lru->second->last_use)) lru = it;
+    if (lru == ctx->owen.end()) break;
+    held -= lru->second->buf.cap;
+    ctx->owen.erase(lru);
+  }
   *out = t;
   return hipSuccess;
+}
+
+// Which seeds of a plan read an Owen prefix table (the others hash all 32 levels:
+// the same integers).  mode 1: every seed.  mode 0: a seed whose table the context
+// holds already, wide enough; otherwise a new table is built when the plan reads at
+// least 2^18 points per dimension of that seed (a table costs 2^20 hashed prefixes
+// per dimension and saves 20 hashed levels per coordinate read through it), largest
+// users first, while the new tables stay within kOwenNewPerPlan and the context's
+// tables that plans still hold plus the new ones within kOwenTableCap.  `dims`
+// (seed -> dimensions) keeps the picked seeds.
+constexpr size_t kOwenNewPerPlan = (size_t)32 << 30;
+
+void owen_table_seeds(riga_ctx* ctx, int mode, const std::map<uint64_t, double>& coords,
+                      std::map<uint64_t, int>* dims) {
+  if (mode >= 1) return;
+  std::vector<std::pair<double, uint64_t>> order;
+  for (const auto& kv : *dims) order.emplace_back(coords.at(kv.first) / std::max(1, kv.second), kv.first);
+  std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  std::lock_guard<std::mutex> g(ctx->owen_mu);
+  size_t pinned = 0;  // tables some plan still holds
+  for (const auto& kv : ctx->owen)
+    if (kv.second.use_count() > 1) pinned += kv.second->buf.cap;
+  size_t fresh = 0;
+  std::map<uint64_t, int> keep;
+  for (const auto& pr : order) {
+    const uint64_t seed = pr.second;
+    const int want = dims->at(seed);
+    auto f = ctx->owen.find(seed);
+    if (f != ctx->owen.end() && f->second->dims >= want) {
+      keep[seed] = want;
+      continue;
+    }
+    const int build = f == ctx->owen.end() ? want : std::max(want, 2 * f->second->dims);  // as owen_table_for grows it
+    const size_t bytes = ((size_t)build << kOwenPrefixBits) * sizeof(OwenEntry);
+    if (pr.first < (double)(1 << 18) || fresh + bytes > kOwenNewPerPlan || pinned + fresh + bytes > kOwenTableCap)
+      continue;
+    fresh += bytes;
+    keep[seed] = want;
+  }
+  dims->swap(keep);
 }
 
 // Stream parse sizing for paths [0, mp->paths) at mp->dpp draws per path:
@@ -1156,15 +1215,19 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   for (Item& it : items) any_mc = any_mc || (it.d.flags & kMonteCarlo) != 0;
   const int64_t off_zig = any_mc ? arena.add(ziggurat_tables().data(), 2 * kZigLen) : 0;
   // one Owen prefix table per seed (built on the device at creation), covering
-  // the largest raw dimension used
-  const bool owen_table = !(cfg && cfg->owen_table < 0) && !scratch;
+  // the largest raw dimension used -- for the seeds owen_table_seeds picks
+  const int owen_mode = scratch ? -1 : (cfg ? cfg->owen_table : 0);
+  const bool owen_table = owen_mode >= 0;
   std::map<uint64_t, int> owen_dims;
   if (owen_table) {
+    std::map<uint64_t, double> coords;
     for (Item& it : items)
       if (!(it.d.flags & (kHalton | kMonteCarlo))) {
         int& m = owen_dims[it.d.seed];
         m = std::max(m, it.d.F * it.d.steps);
+        coords[it.d.seed] += (double)it.d.paths * (double)it.d.F * (double)it.d.steps;
       }
+    owen_table_seeds(ctx, owen_mode, coords, &owen_dims);
   }
   pl->final_base.resize(items.size());
   for (size_t i = 0; i < items.size(); ++i) {
@@ -1221,7 +1284,8 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       d.record_base = pl->record_slots;
       if (d.flags & kRecords) pl->record_slots += (int64_t)d.S * (int64_t)d.paths;
       d.off_dirs = off_dirs;
-      if (owen_table && !(d.flags & (kHalton | kMonteCarlo))) d.flags |= kOwenTable;  // table address set below
+      if (owen_dims.count(d.seed) && !(d.flags & (kHalton | kMonteCarlo)))
+        d.flags |= kOwenTable;  // table address set below
       d.off_zig = off_zig;
       d.off_nodes = arena.add(it.nodes.data(), it.nodes.size());
       d.off_steps = arena.add(it.steps.data(), it.steps.size());

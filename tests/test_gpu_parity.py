@@ -310,14 +310,18 @@ def test_reference_dimension_cap_wraps_like_the_reference():
     assert_fp64_parity(res, ora)
 
 
-def test_owen_table_and_full_hash_agree():
-    # the tabulated top 12 levels and the 32-level hash give identical integers
-    req = _flat(2048, steps=252)
-    md = W.max_dimension_for(req)
-    a = api.price_batch([req], "fp64", max_dimension=md, owen_table=True)[0]
-    b = api.price_batch([req], "fp64", max_dimension=md, owen_table=False)[0]
-    assert a.pv == b.pv and a.legs.down_and_in_put == b.legs.down_and_in_put
-    assert a.stats[0].output_count == b.stats[0].output_count
+@pytest.mark.parametrize("tier", ["fp64", "fp32"])
+def test_owen_table_and_full_hash_agree(tier):
+    # the tabulated top 20 levels and the hashed ones give identical integers, so
+    # identical prices: a flat 1y book and the 3y local-vol trade (2,268 dimensions)
+    for req in (_flat(2048, steps=252), W.lv_book(paths=2048, steps=756)):
+        md = W.max_dimension_for(req)
+        a = api.price_batch([req], tier, max_dimension=md, owen_table=True)[0]
+        b = api.price_batch([req], tier, max_dimension=md, owen_table=False)[0]
+        c = api.price_batch([req], tier, max_dimension=md)[0]  # the library's choice
+        assert a.pv == b.pv == c.pv and a.legs.down_and_in_put == b.legs.down_and_in_put
+        assert a.stats[0].output_count == b.stats[0].output_count
+        assert a.trace_events == b.trace_events
 
 
 @pytest.mark.parametrize("ppt,waves", [(2, 4), (4, 4), (1, 1), (8, 2), (1, 3)])
