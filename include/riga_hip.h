@@ -170,6 +170,8 @@ typedef struct riga_plan_info {
   int32_t graph;
   int32_t normal_tables; /* shared [dim][path] normal tables the plan fills per launch */
   int32_t paths_per_thread, waves_per_block; /* the launch shape in use */
+  int32_t owen_tables;   /* seeds whose coordinates read an Owen prefix table (the rest hash) */
+  int32_t reserved;
 } riga_plan_info;
 
 /* One autotuner trial: a launch shape and its best device time (< 0: the shape

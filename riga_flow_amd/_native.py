@@ -126,7 +126,8 @@ class PlanInfo(C.Structure):
     _fields_ = [("n_requests", C.c_int32), ("n_items", C.c_int32), ("n_launches", C.c_int32),
                 ("blocks", C.c_int32), ("path_steps", C.c_double), ("lds_bytes", C.c_int64),
                 ("device_bytes", C.c_int64), ("graph", C.c_int32), ("normal_tables", C.c_int32),
-                ("paths_per_thread", C.c_int32), ("waves_per_block", C.c_int32)]
+                ("paths_per_thread", C.c_int32), ("waves_per_block", C.c_int32),
+                ("owen_tables", C.c_int32), ("reserved", C.c_int32)]
 
 
 class TuneTrial(C.Structure):

@@ -1913,6 +1913,7 @@ int riga_plan_query(const riga_plan* plan, riga_plan_info* info) {
   info->graph = plan->graph ? 1 : 0;
   info->paths_per_thread = plan->ppt;
   info->waves_per_block = plan->waves;
+  info->owen_tables = (int32_t)plan->owen_tabs.size();
   return 0;
 }
 

@@ -99,10 +99,11 @@ def test_ctypes_mirrors_match_the_header_compiled_by_gcc(tmp_path):
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "riga_hip.h"\n'
         "int main(void) {\n"
-        '  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(riga_launch_config),\n'
+        '  printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(riga_launch_config),\n'
         "         offsetof(riga_launch_config, persistent), sizeof(riga_ticket_info),\n"
         "         offsetof(riga_ticket_info, device_end_s), sizeof(riga_server_stats),\n"
-        "         offsetof(riga_server_stats, overlapped_plans));\n"
+        "         offsetof(riga_server_stats, overlapped_plans), sizeof(riga_plan_info),\n"
+        "         offsetof(riga_plan_info, owen_tables));\n"
         "  return 0;\n}\n")
     exe = tmp_path / "layout"
     import subprocess
@@ -111,7 +112,7 @@ def test_ctypes_mirrors_match_the_header_compiled_by_gcc(tmp_path):
                                           text=True).stdout.split()]
     want = [C.sizeof(N.LaunchConfig), N.LaunchConfig.persistent.offset, C.sizeof(N.TicketInfo),
             N.TicketInfo.device_end_s.offset, C.sizeof(N.ServerStats),
-            N.ServerStats.overlapped_plans.offset]
+            N.ServerStats.overlapped_plans.offset, C.sizeof(N.PlanInfo), N.PlanInfo.owen_tables.offset]
     assert got == want
 
 

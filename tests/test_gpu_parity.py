@@ -324,6 +324,37 @@ def test_owen_table_and_full_hash_agree(tier):
         assert a.trace_events == b.trace_events
 
 
+def test_owen_tables_are_built_for_plans_that_read_enough_points():
+    # a seed no other test uses: a plan reading fewer than 2^18 points per dimension
+    # hashes every level; one reading 2^18 builds the seed's prefix table; a later
+    # small plan of the seed reads that table; a forced table and none at all price
+    # identically
+    def req(paths):
+        q = _flat(paths, steps=21)
+        q.config.seed = 987_651
+        return q
+    md = W.max_dimension_for(req(1024))
+    small = api.Plan([req(4096)], "fp64", max_dimension=md)
+    assert small.info.owen_tables == 0
+    a = small.run()[0]
+    small.close()
+    big = api.Plan([req(1 << 18)], "fp64", max_dimension=md)
+    assert big.info.owen_tables == 1
+    big.close()
+    again = api.Plan([req(4096)], "fp64", max_dimension=md)
+    assert again.info.owen_tables == 1
+    b = again.run()[0]
+    again.close()
+    forced = api.Plan([req(4096)], "fp32", max_dimension=md, owen_table=True)
+    off = api.Plan([req(4096)], "fp32", max_dimension=md, owen_table=False)
+    assert (forced.info.owen_tables, off.info.owen_tables) == (1, 0)
+    c, d = forced.run()[0], off.run()[0]
+    forced.close()
+    off.close()
+    assert a.pv == b.pv and a.stats[0].output_count == b.stats[0].output_count
+    assert c.pv == d.pv and c.stats[0].output_count == d.stats[0].output_count
+
+
 @pytest.mark.parametrize("ppt,waves", [(2, 4), (4, 4), (1, 1), (8, 2), (1, 3)])
 def test_launch_shape_changes_nothing_but_the_shape(ppt, waves):
     req = W.lv_book(paths=3000, steps=126, years=0.5)
