@@ -84,6 +84,18 @@ def measured_traffic(path: str = TRAFFIC_FILE):
         return None, None
 
 
+def profiled_issue(path: str = TRAFFIC_FILE):
+    """VALU lane-operations the kernel issued per (scenario) path-step, from the same
+    committed rocprofv3 summary (SQ_INSTS_VALU x 64 / path-steps), or None."""
+    try:
+        for line in open(path):
+            if line.startswith("issued_lane_ops_per_path_step="):
+                return float(line.strip().split("=", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 # ---------------------------------------------------------------- CPU baseline
 def cpu_baseline(seconds_target: float = 12.0):
     """The oracle (single thread, then all host threads) on a bounded sample of the
@@ -295,6 +307,7 @@ def main():
     dev_s = sum(dev32) / len(dev32) / 1e3
     achieved = ops * ps32 / dev_s
     traffic, traffic_commit = measured_traffic()
+    issued = profiled_issue()
     ops64 = ops_per_path_step(3, 1, plain_paths.bit_length(), True, "fp64")
     dev64_s = sum(dev64) / len(dev64) / 1e3
     achieved64 = ops64 * ps64 / dev64_s
@@ -345,8 +358,13 @@ def main():
                      "kernel": "phoenix_kernel<float,3,10>", "device_ms": dev_s * 1e3,
                      "ops_per_path_step": ops,
                      "traffic_profiled_at": traffic_commit,
-                     "note": "VALU-bound: no MFMA (hashing, scalar FP); traffic = rocprofv3 "
-                             "FETCH_SIZE+WRITE_SIZE bytes per launch, copy-calibrated, from "
+                     "issued_ops_per_path_step": issued,
+                     "issue_frac": issued * ps32 / dev_s / VALU_PEAK if issued else None,
+                     "note": "VALU-bound: no MFMA (hashing, scalar FP); achieved counts the "
+                             "algorithmic lane-ops of the op model, issue_frac the lane-ops the "
+                             "kernel issued (rocprofv3 SQ_INSTS_VALU x 64 per scenario path-step); "
+                             "traffic = rocprofv3 FETCH_SIZE+WRITE_SIZE bytes per launch, "
+                             "copy-calibrated; both from "
                              "profiles/phoenix_kernel_lv3_crn10_fp32_traffic.txt (profiled at "
                              "commit traffic_profiled_at)"},
         "autotune": tuning,
