@@ -507,7 +507,11 @@ __device__ __forceinline__ void sobol_fetch(const ItemDesc& it, const unsigned c
     const uint32_t d = dim % (uint32_t)it.ladder_dims;
     const uint32_t* row = at<uint32_t>(arena, it.off_dirs) + (size_t)d * 32;
     sf.x[f] = sobol_x(gray, row, it.nbits);
-    sf.top[f] = tab ? (uint32_t)it.owen_tab[((size_t)dim << kOwenTableBits) + owen_slot_of(sf.x[f])] : 0u;
+    // a streaming gather (each entry is read once per launch): non-temporal, so it
+    // does not push the shared tables out of the caches
+    sf.top[f] = tab ? (uint32_t)__builtin_nontemporal_load(&it.owen_tab[((size_t)dim << kOwenTableBits) +
+                                                                       owen_slot_of(sf.x[f])])
+                    : 0u;
   }
 }
 
