@@ -189,6 +189,8 @@ struct riga_ctx {
   std::mutex owen_mu;
   std::map<uint64_t, std::shared_ptr<struct OwenTable>> owen;  // per seed, grown on demand
   uint64_t owen_uses = 0;  // requests for a table so far (least recently used ones go first)
+  size_t owen_cap = (size_t)64 << 30;       // Owen table bytes the context keeps: a third of the device
+  size_t owen_plan_cap = (size_t)48 << 30;  // new table bytes one plan may build: a quarter of it
   DevBuf arena, items, block_item, partials, finals, final_base, records;
   HostBuf staging, finals_host;
   PinnedPool pinned;  // plans' download buffers (plans end before their context)
@@ -747,9 +749,8 @@ namespace {
 // The context's table for `seed` covering at least `dims` dimensions, building a
 // larger one (at least twice the old size, so growth is rare) on `stream` when
 // needed; callers wait on table->ready before their kernels read it.
-// The context keeps at most this many bytes of Owen tables; beyond it the least
+// The context keeps at most ctx->owen_cap bytes of Owen tables; beyond it the least
 // recently used tables of other seeds are dropped (plans still holding one keep it).
-constexpr size_t kOwenTableCap = (size_t)64 << 30;
 
 hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t stream,
                           std::shared_ptr<OwenTable>* out) {
@@ -774,7 +775,7 @@ hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t st
   ctx->owen[seed] = t;
   size_t held = 0;
   for (const auto& kv : ctx->owen) held += kv.second->buf.cap;
-  while (held > kOwenTableCap && ctx->owen.size() > 1) {
+  while (held > ctx->owen_cap && ctx->owen.size() > 1) {
     auto lru = ctx->owen.end();
     for (auto it = ctx->owen.begin(); it != ctx->owen.end(); ++it)
       if (it->first != seed && (lru == ctx->owen.end() || it->second->last_use < lru->second->last_use)) lru = it;
@@ -789,12 +790,11 @@ hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t st
 // Which seeds of a plan read an Owen prefix table (the others hash all 32 levels:
 // the same integers).  mode 1: every seed.  mode 0: a seed whose table the context
 // holds already, wide enough; otherwise a new table is built when the plan reads at
-// least 2^18 points per dimension of that seed (a table costs 2^20 hashed prefixes
-// per dimension and saves 20 hashed levels per coordinate read through it), largest
-// users first, while the new tables stay within kOwenNewPerPlan and the context's
-// tables that plans still hold plus the new ones within kOwenTableCap.  `dims`
+// least 2^18 points per dimension of that seed (a table costs 2^21 hashed prefixes
+// per dimension and saves 21 hashed levels per coordinate read through it), largest
+// users first, while the new tables stay within ctx->owen_plan_cap and the context's
+// tables that plans still hold plus the new ones within ctx->owen_cap.  `dims`
 // (seed -> dimensions) keeps the picked seeds.
-constexpr size_t kOwenNewPerPlan = (size_t)32 << 30;
 
 void owen_table_seeds(riga_ctx* ctx, int mode, const std::map<uint64_t, double>& coords,
                       std::map<uint64_t, int>* dims) {
@@ -818,7 +818,7 @@ void owen_table_seeds(riga_ctx* ctx, int mode, const std::map<uint64_t, double>&
     }
     const int build = f == ctx->owen.end() ? want : std::max(want, 2 * f->second->dims);  // as owen_table_for grows it
     const size_t bytes = ((size_t)build << kOwenPrefixBits) * sizeof(OwenEntry);
-    if (pr.first < (double)(1 << 18) || fresh + bytes > kOwenNewPerPlan || pinned + fresh + bytes > kOwenTableCap)
+    if (pr.first < (double)(1 << 18) || fresh + bytes > ctx->owen_plan_cap || pinned + fresh + bytes > ctx->owen_cap)
       continue;
     fresh += bytes;
     keep[seed] = want;
@@ -1756,6 +1756,10 @@ int riga_hip_init(int device, riga_ctx** out, riga_error* err) {
   ctx->arch = arch;
   ctx->lds_limit = prop.sharedMemPerBlock;
   ctx->n_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  if (prop.totalGlobalMem > 0) {  // Owen table budgets scale with the device's memory
+    ctx->owen_cap = (size_t)prop.totalGlobalMem / 3;
+    ctx->owen_plan_cap = (size_t)prop.totalGlobalMem / 4;
+  }
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
     set_err(err, "gpu_error", "stream/event creation failed");

@@ -20,11 +20,12 @@ constexpr int kLadderDirect = 0;      // Sobol (Owen) or Halton points generated
 constexpr int kLadderTable = 1;       // read from the batch's shared [dim][path] normal table
 constexpr int kLadderMonteCarlo = 2;  // ChaCha8 + ziggurat stream (VarianceReduction::None)
 constexpr int kMaxLvX = 256;        // log-spot points per local-vol row staged in LDS
-// 20 levels: 4 MiB per raw dimension (u32 entries).  Measured on the 3-asset 756-step
-// batches against 16 and 18 bits: FP32 ten-scenario 38.96 / 37.06 / 35.55 ms, FP64 plain
-// 14.26 / 13.74 / 13.01 ms for 16 / 18 / 20 bits.
+// 21 levels: 8 MiB per raw dimension (u32 entries).  Measured on the 3-asset 756-step
+// batches: FP32 ten-scenario 38.96 / 37.06 / 35.55 ms, FP64 plain 14.26 / 13.74 / 13.01 ms
+// for 16 / 18 / 20 bits; with the gathers non-temporal, 34.00 / 12.77 ms at 20 bits and
+// 32.70 / 12.49 ms at 21.
 #ifndef RIGA_OWEN_PREFIX_BITS
-#define RIGA_OWEN_PREFIX_BITS 20
+#define RIGA_OWEN_PREFIX_BITS 21
 #endif
 constexpr int kOwenPrefixBits = RIGA_OWEN_PREFIX_BITS;  // Owen levels tabulated per dimension
 static_assert(kOwenPrefixBits >= 8 && kOwenPrefixBits <= 24, "prefix table width out of range");

@@ -312,7 +312,7 @@ def test_reference_dimension_cap_wraps_like_the_reference():
 
 @pytest.mark.parametrize("tier", ["fp64", "fp32"])
 def test_owen_table_and_full_hash_agree(tier):
-    # the tabulated top 20 levels and the hashed ones give identical integers, so
+    # the tabulated top 21 levels and the hashed ones give identical integers, so
     # identical prices: a flat 1y book and the 3y local-vol trade (2,268 dimensions)
     for req in (_flat(2048, steps=252), W.lv_book(paths=2048, steps=756)):
         md = W.max_dimension_for(req)
