@@ -35,7 +35,7 @@ OWEN_ROUND = 21     # hash input xor 2 (three-input), xorshift 2, 64-bit multipl
 # table (RIGA_OWEN_PREFIX_BITS in riga_flow_amd/csrc/phoenix_layout.h); the FP32 tier
 # hashes the rest of a float's 24 bits and levels 7..0 only for coordinates within
 # 2^-10 of 0 or 1 (a fraction 2^-9 of them)
-OWEN_TABLE_BITS = 21
+OWEN_TABLE_BITS = 22
 OWEN_ROUNDS = {"fp64": 32 - OWEN_TABLE_BITS, "fp32": 24 - OWEN_TABLE_BITS + 8 * 2.0 ** -9}
 OWEN_TABLE = 3      # prefix table address and load
 PROBIT = 22         # rational (12) + divide (8) + tails (5% of log + sqrt + rational)

@@ -133,11 +133,13 @@ typedef struct riga_launch_config {
   int32_t paths_per_thread; /* 1, 2, 4 or 8 paths per lane; 0: the autotuned choice (riga_autotune)
                                for the batch's shape, else 1 */
   int32_t use_graph;        /* 1: capture the launch sequence in a hipGraph at plan creation */
-  int32_t owen_table;       /* top 21 Owen levels of a seed's coordinates from a per-dimension table
-                               the context keeps (8 MiB per dimension; the same integers as
-                               hashing them): 0 = a table the context holds already, or a new
-                               one when the plan reads at least 2^18 points per dimension of the
-                               seed and memory allows; 1 = always; -1 = hash all 32 levels */
+  int32_t owen_table;       /* top 22 Owen levels of a seed's coordinates from a per-dimension table
+                               the context keeps (16 MiB per dimension, covering a seed's first
+                               dimensions; the same integers as hashing them): 0 = a table the
+                               context holds already, or a new one when the plan reads at least
+                               2^18 points per dimension of the seed and memory allows; 1 =
+                               always; k > 1 = always, covering at most the first k dimensions;
+                               -1 = hash all 32 levels */
   int32_t normal_table;     /* 0: auto (share one [dim][path] normal table among same-seed trades
                                when they need 4x more normals than it holds); 1: force; -1: off */
   int32_t stream_ordered_alloc; /* 1: plan memory from the stream-ordered pool (serving) */

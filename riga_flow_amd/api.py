@@ -276,11 +276,16 @@ def result_to_json(res: PriceResult) -> dict:
 
 
 # ------------------------------------------------------------------ pricing
-def _owen_mode(owen_table: Optional[bool]) -> int:
+def _owen_mode(owen_table) -> int:
     """riga_launch_config.owen_table: None lets the library decide per seed (a table it
     holds already, or one worth building for the plan's points), True forces the
-    prefix table, False hashes all 32 Owen levels (the same integers)."""
-    return 0 if owen_table is None else (1 if owen_table else -1)
+    prefix table, an int k > 1 forces one covering at most the first k dimensions,
+    False hashes all 32 Owen levels (the same integers)."""
+    if owen_table is None:
+        return 0
+    if owen_table is True or owen_table is False:
+        return 1 if owen_table else -1
+    return max(2, int(owen_table))
 
 
 def price_batch(requests: Sequence[PriceRequest], precision: str = "fp64",

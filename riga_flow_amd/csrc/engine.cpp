@@ -688,6 +688,7 @@ struct riga_plan {
     int dims = 0, nbits = 0;
     int64_t paths = 0;
     const OwenEntry* owen = nullptr;
+    int owen_dims = 0;  // dimensions the prefix table covers
     DevBuf buf;
   };
   std::vector<std::unique_ptr<NormalTable>> ztables;
@@ -762,7 +763,7 @@ hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t st
     *out = f->second;
     return hipSuccess;
   }
-  const int want = f == ctx->owen.end() ? dims : std::max(dims, 2 * f->second->dims);
+  const int want = dims;  // the caller decides how far a table grows
   std::shared_ptr<OwenTable> t(new OwenTable());
   t->seed = seed;
   t->dims = want;
@@ -787,18 +788,23 @@ hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t st
   return hipSuccess;
 }
 
-// Which seeds of a plan read an Owen prefix table (the others hash all 32 levels:
-// the same integers).  mode 1: every seed.  mode 0: a seed whose table the context
-// holds already, wide enough; otherwise a new table is built when the plan reads at
-// least 2^18 points per dimension of that seed (a table costs 2^21 hashed prefixes
-// per dimension and saves 21 hashed levels per coordinate read through it), largest
-// users first, while the new tables stay within ctx->owen_plan_cap and the context's
-// tables that plans still hold plus the new ones within ctx->owen_cap.  `dims`
-// (seed -> dimensions) keeps the picked seeds.
-
+// Which seeds of a plan read an Owen prefix table, and how many dimensions the
+// table to build for each covers (a table covers the first dimensions of its seed;
+// higher ones, and the seeds left out, hash every level -- the same integers).
+// A table grows to at least twice its dimensions when it grows, and never past the
+// bytes one plan may build (ctx->owen_plan_cap).  mode 1: every seed; mode k > 1:
+// every seed, at most k dimensions.  mode 0: a seed whose table the context holds
+// already, wide enough or as wide as it can be; otherwise a new table is built when
+// the plan reads at least 2^18 points per dimension of that seed (a table costs
+// 2^22 hashed prefixes per dimension and saves 22 hashed levels per coordinate read
+// through it), largest users first, while the new tables stay within
+// ctx->owen_plan_cap and the context's tables that plans still hold plus the new
+// ones within ctx->owen_cap.  `dims` (seed -> dimensions wanted) comes back as
+// seed -> dimensions to request from owen_table_for.
 void owen_table_seeds(riga_ctx* ctx, int mode, const std::map<uint64_t, double>& coords,
                       std::map<uint64_t, int>* dims) {
-  if (mode >= 1) return;
+  const size_t row = ((size_t)1 << kOwenPrefixBits) * sizeof(OwenEntry);
+  const int most = (int)std::min<size_t>(ctx->owen_plan_cap / row, (size_t)1 << 24);
   std::vector<std::pair<double, uint64_t>> order;
   for (const auto& kv : *dims) order.emplace_back(coords.at(kv.first) / std::max(1, kv.second), kv.first);
   std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
@@ -810,18 +816,21 @@ void owen_table_seeds(riga_ctx* ctx, int mode, const std::map<uint64_t, double>&
   std::map<uint64_t, int> keep;
   for (const auto& pr : order) {
     const uint64_t seed = pr.second;
-    const int want = dims->at(seed);
+    const int want = std::min(dims->at(seed), mode > 1 ? std::min(mode, most) : most);
+    if (want <= 0) continue;
     auto f = ctx->owen.find(seed);
-    if (f != ctx->owen.end() && f->second->dims >= want) {
-      keep[seed] = want;
+    const int have = f == ctx->owen.end() ? 0 : f->second->dims;
+    if (have >= want) {
+      keep[seed] = have;
       continue;
     }
-    const int build = f == ctx->owen.end() ? want : std::max(want, 2 * f->second->dims);  // as owen_table_for grows it
-    const size_t bytes = ((size_t)build << kOwenPrefixBits) * sizeof(OwenEntry);
-    if (pr.first < (double)(1 << 18) || fresh + bytes > ctx->owen_plan_cap || pinned + fresh + bytes > ctx->owen_cap)
+    const int build = mode > 1 ? want : std::min(std::max(want, 2 * have), most);
+    const size_t bytes = (size_t)build * row;
+    if (mode == 0 && (pr.first < (double)(1 << 18) || fresh + bytes > ctx->owen_plan_cap ||
+                      pinned + fresh + bytes > ctx->owen_cap))
       continue;
     fresh += bytes;
-    keep[seed] = want;
+    keep[seed] = build;
   }
   dims->swap(keep);
 }
@@ -930,10 +939,10 @@ hipError_t enqueue(riga_plan* pl, bool timed) {
     const uint32_t* dirs = (const uint32_t*)(a + pl->off_dirs);
     const OwenEntry* ot = zt->owen;
     e = pl->precision == RIGA_FP64
-            ? launch_normal_table_f64(dirs, ot, zt->seed, zt->dims, (int)zt->paths, zt->nbits, (double*)zt->buf.p,
-                                      pl->stream)
-            : launch_normal_table_f32(dirs, ot, zt->seed, zt->dims, (int)zt->paths, zt->nbits, (float*)zt->buf.p,
-                                      pl->stream);
+            ? launch_normal_table_f64(dirs, ot, zt->owen_dims, zt->seed, zt->dims, (int)zt->paths, zt->nbits,
+                                      (double*)zt->buf.p, pl->stream)
+            : launch_normal_table_f32(dirs, ot, zt->owen_dims, zt->seed, zt->dims, (int)zt->paths, zt->nbits,
+                                      (float*)zt->buf.p, pl->stream);
     if (e != hipSuccess) return e;
   }
   e = launch_queue_reset((uint32_t*)pl->queues.p, (int)pl->launches.size(), pl->stream);
@@ -1391,16 +1400,28 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   if (stream_ordered) pl->bind_stream();
   double ztable_bytes = 0.0;
   std::map<uint64_t, const OwenEntry*> owen_ptr;
+  std::map<uint64_t, int> owen_cover;  // dimensions each seed's table covers
   if (!estimate)
     for (auto& kv : owen_dims) {
       std::shared_ptr<OwenTable> ot;
       HIP_TRY(owen_table_for(ctx, kv.first, kv.second, pl->stream, &ot), "owen table");
       HIP_TRY(hipStreamWaitEvent(pl->stream, ot->ready, 0), "owen table wait");
       owen_ptr[kv.first] = (const OwenEntry*)ot->buf.p;
+      owen_cover[kv.first] = ot->dims;
       pl->owen_tabs.push_back(std::move(ot));
     }
+  // a path kernel's item reads the table only if it covers all the item's
+  // dimensions (the decision stays out of the step loop); the others hash
   for (Item& it : items)
-    if (it.d.flags & kOwenTable) it.d.owen_tab = owen_ptr[it.d.seed];
+    if (it.d.flags & kOwenTable) {
+      const auto c = owen_cover.find(it.d.seed);
+      if (c != owen_cover.end() && c->second >= it.d.F * it.d.steps) {
+        it.d.owen_tab = owen_ptr[it.d.seed];
+        it.d.owen_dims = c->second;
+      } else {
+        it.d.flags &= ~kOwenTable;
+      }
+    }
   // the shared normal tables decided above
   {
     const size_t real = precision == RIGA_FP64 ? 8 : 4;
@@ -1418,6 +1439,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       zt->paths = (int64_t)paths;
       zt->nbits = 64 - __builtin_clzll(paths);
       zt->owen = owen_table ? owen_ptr[kv.first] : nullptr;
+      zt->owen_dims = zt->owen ? owen_cover[kv.first] : 0;
       if (stream_ordered) zt->buf.stream = pl->stream;
       ztable_bytes += bytes;
       if (!estimate) HIP_TRY(zt->buf.ensure((size_t)bytes), "normal table alloc");

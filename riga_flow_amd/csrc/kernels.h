@@ -42,10 +42,11 @@ hipError_t launch_debug_sobol_f32(const uint32_t* dirs, int ladder_dims, const O
                                   bool full_ladder, uint32_t* xs_out, float* z_out, hipStream_t stream);
 hipError_t launch_phoenix_f32(const PhoenixLaunch& l, hipStream_t stream);
 
-// Shared normal table z[dim][path] (Real = double for FP64, float for FP32).
-hipError_t launch_normal_table_f64(const uint32_t* dirs, const OwenEntry* owen_tab, uint64_t seed,
+// Shared normal table z[dim][path] (Real = double for FP64, float for FP32); the
+// Owen prefix table (may be null) covers dimensions [0, owen_dims).
+hipError_t launch_normal_table_f64(const uint32_t* dirs, const OwenEntry* owen_tab, int owen_dims, uint64_t seed,
                                    int dims, int n_paths, int nbits, double* out, hipStream_t stream);
-hipError_t launch_normal_table_f32(const uint32_t* dirs, const OwenEntry* owen_tab, uint64_t seed,
+hipError_t launch_normal_table_f32(const uint32_t* dirs, const OwenEntry* owen_tab, int owen_dims, uint64_t seed,
                                    int dims, int n_paths, int nbits, float* out, hipStream_t stream);
 
 // Monte Carlo lane stream parse (kernels_mc.hip) for one (seed, draws per path).

@@ -97,6 +97,7 @@ __global__ __launch_bounds__(256) void debug_sobol_f32_kernel(const uint32_t* __
   it.off_dirs = 0;
   it.nbits = nbits;
   it.owen_tab = owen_tab;
+  it.owen_dims = n_nodes * F;  // the host's table covers every dimension read here
   it.seed = seed;
   const uint32_t n = (uint32_t)(first_point + (uint64_t)p);
   const uint32_t gray = n ^ (n >> 1);
@@ -251,10 +252,10 @@ hipError_t launch_lv_propagate(const LvPropagateArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_normal_table_f32(const uint32_t* dirs, const OwenEntry* owen_tab, uint64_t seed,
+hipError_t launch_normal_table_f32(const uint32_t* dirs, const OwenEntry* owen_tab, int owen_dims, uint64_t seed,
                                    int dims, int n_paths, int nbits, float* out, hipStream_t stream) {
   dim3 grid((n_paths + 255) / 256, dims);
-  hipLaunchKernelGGL((normal_table_kernel<float>), grid, dim3(256), 0, stream, dirs, owen_tab, seed,
+  hipLaunchKernelGGL((normal_table_kernel<float>), grid, dim3(256), 0, stream, dirs, owen_tab, owen_dims, seed,
                      n_paths, nbits, out);
   return hipGetLastError();
 }

@@ -335,10 +335,10 @@ hipError_t launch_ledger(const ItemDesc* items, int item, uint64_t paths, const 
   return hipGetLastError();
 }
 
-hipError_t launch_normal_table_f64(const uint32_t* dirs, const OwenEntry* owen_tab, uint64_t seed,
+hipError_t launch_normal_table_f64(const uint32_t* dirs, const OwenEntry* owen_tab, int owen_dims, uint64_t seed,
                                    int dims, int n_paths, int nbits, double* out, hipStream_t stream) {
   dim3 grid((n_paths + 255) / 256, dims);
-  hipLaunchKernelGGL((normal_table_kernel<double>), grid, dim3(256), 0, stream, dirs, owen_tab, seed,
+  hipLaunchKernelGGL((normal_table_kernel<double>), grid, dim3(256), 0, stream, dirs, owen_tab, owen_dims, seed,
                      n_paths, nbits, out);
   return hipGetLastError();
 }

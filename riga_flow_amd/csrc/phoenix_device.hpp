@@ -497,6 +497,10 @@ struct SobolFetch {
   uint32_t top[F];  // prefix-table entries (unused without a table)
 };
 
+// An item reads its coordinates' top Owen levels from the prefix table when the
+// host found a table covering all its dimensions (kOwenTable); otherwise it hashes
+// every level -- the same integers.  The decision is per item, so the step loop
+// holds one path or the other.
 template <int F>
 __device__ __forceinline__ void sobol_fetch(const ItemDesc& it, const unsigned char* arena, uint32_t gray, int node,
                                             SobolFetch<F>& sf) {
@@ -687,12 +691,14 @@ __device__ __noinline__ float lv_sigma_f32_general(const float* __restrict__ ls,
 // dimension, so the direction row, the key and the Owen table row are uniform.
 template <typename Real>
 __global__ __launch_bounds__(256) void normal_table_kernel(const uint32_t* __restrict__ dirs,
-                                                           const OwenEntry* __restrict__ owen_tab,
-                                                           uint64_t seed, int n_paths, int nbits,
+                                                           const OwenEntry* __restrict__ owen_tab_in,
+                                                           int owen_dims, uint64_t seed, int n_paths, int nbits,
                                                            Real* __restrict__ out) {
   const int path = blockIdx.x * 256 + threadIdx.x;
   const uint32_t dim = blockIdx.y;
   if (path >= n_paths) return;
+  // the prefix table covers the first owen_dims dimensions; the others hash every level
+  const OwenEntry* owen_tab = (int)dim < owen_dims ? owen_tab_in : nullptr;
   const uint32_t n = (uint32_t)path + 1;
   const uint32_t gray = n ^ (n >> 1);
   const uint32_t x = sobol_x(gray, dirs + (size_t)dim * 32, nbits);
@@ -726,11 +732,11 @@ typedef __attribute__((address_space(3))) const F32x4 LdsF32x4;
 // items first -- so units start longest first in the order workgroups get a
 // CU, whatever the dispatcher's order.  A unit's results (its block partial)
 // do not depend on which workgroup ran it.  `queue` is zero when the launch
-// starts; the grid holds at least as many workgroups as units.  (A looping
-// form, in which resident workgroups claim units until the table is empty,
-// holds 14-33 more VGPRs across its back edge -- the FP32 table-mode kernel
-// goes from 147 to 180, three waves per SIMD to two -- for nothing: a unit
-// runs milliseconds, its dispatch microseconds.)
+// starts.  The grid is either one workgroup per unit (each claims once) or a
+// persistent grid of at most the workgroups the device holds at once (each
+// claims until the table is empty).  The unit loop carries no lane-derived
+// values across its back edge (the lane id enters each unit through an opaque
+// copy), so it costs the kernels no registers in their hot loops.
 // Resident waves per SIMD each path kernel is compiled for (its register
 // budget: 512 / waves VGPRs per lane), chosen per variant from the compiler's
 // spill counts and timings: more waves wherever the kernel's live state fits the

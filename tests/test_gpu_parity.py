@@ -312,7 +312,7 @@ def test_reference_dimension_cap_wraps_like_the_reference():
 
 @pytest.mark.parametrize("tier", ["fp64", "fp32"])
 def test_owen_table_and_full_hash_agree(tier):
-    # the tabulated top 21 levels and the hashed ones give identical integers, so
+    # the tabulated top 22 levels and the hashed ones give identical integers, so
     # identical prices: a flat 1y book and the 3y local-vol trade (2,268 dimensions)
     for req in (_flat(2048, steps=252), W.lv_book(paths=2048, steps=756)):
         md = W.max_dimension_for(req)
@@ -353,6 +353,21 @@ def test_owen_tables_are_built_for_plans_that_read_enough_points():
     off.close()
     assert a.pv == b.pv and a.stats[0].output_count == b.stats[0].output_count
     assert c.pv == d.pv and c.stats[0].output_count == d.stats[0].output_count
+    # a table covering only the first 40 of the 63 dimensions (another fresh seed):
+    # the shared normal table reads it for those dimensions and hashes every level
+    # of the others, and a path kernel hashes every level of an item it does not
+    # cover whole -- the integers are the same either way
+    for tier in ("fp64", "fp32"):
+        q = req(4096)
+        q.config.seed = 987_652 if tier == "fp64" else 987_653
+        f = api.price_batch([q], tier, max_dimension=md, owen_table=False, normal_table=-1)[0]
+        for table in (1, -1):
+            part = api.Plan([q], tier, max_dimension=md, owen_table=40, normal_table=table)
+            assert part.info.owen_tables == 1 and part.info.normal_tables == (1 if table == 1 else 0)
+            e = part.run()[0]
+            part.close()
+            assert e.pv == f.pv and e.stats[0].output_count == f.stats[0].output_count
+            assert e.trace_events == f.trace_events
 
 
 @pytest.mark.parametrize("ppt,waves", [(2, 4), (4, 4), (1, 1), (8, 2), (1, 3)])
