@@ -796,8 +796,9 @@ hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t st
 // every seed, at most k dimensions.  mode 0: a seed whose table the context holds
 // already, wide enough or as wide as it can be; otherwise a new table is built when
 // the plan reads at least 2^18 points per dimension of that seed (a table costs
-// 2^22 hashed prefixes per dimension and saves 22 hashed levels per coordinate read
-// through it), largest users first, while the new tables stay within
+// 2^22 x 32 hashed levels per dimension; it saves 22 hashed levels per coordinate read
+// through it, and all 32 for points below 2^22, whose entries are the coordinates),
+// largest users first, while the new tables stay within
 // ctx->owen_plan_cap and the context's tables that plans still hold plus the new
 // ones within ctx->owen_cap.  `dims` (seed -> dimensions wanted) comes back as
 // seed -> dimensions to request from owen_table_for.

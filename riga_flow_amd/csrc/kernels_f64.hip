@@ -237,7 +237,8 @@ hipError_t launch_debug_halton(const uint32_t* tables, int rows, uint64_t first_
   return hipGetLastError();
 }
 
-// tab[dim][owen_slot(v)] = top kOwenTableBits bits of owen_scramble(v << (32-L), key(dim)).
+// tab[dim][owen_slot(v)] = owen_scramble(v << (32-L), key(dim)) with 32-bit entries (all
+// 32 levels: the coordinate itself for points below 2^L), its top L bits with 16-bit ones.
 // Lanes run over slots (coalesced stores); the prefix of a slot is its rotation back.
 __global__ __launch_bounds__(256) void owen_table_kernel(uint64_t seed, int dims, OwenEntry* __restrict__ out) {
   constexpr uint32_t mask = (1u << kOwenTableBits) - 1u;
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(256) void owen_table_kernel(uint64_t seed, int dims
   const uint64_t key = seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
   const uint32_t x = v << (32 - kOwenTableBits);
   uint64_t P = 0;
-  for (int bit = 31; bit >= 32 - kOwenTableBits; --bit) owen_level(x, key, bit, P);
+  for (int bit = 31; bit >= (kOwenEntryWhole ? 0 : 32 - kOwenTableBits); --bit) owen_level(x, key, bit, P);
   out[(size_t)dim * (1u << kOwenTableBits) + slot] = (OwenEntry)owen_prefix(P);
 }
 

@@ -87,13 +87,27 @@ __device__ __forceinline__ uint32_t owen_scramble(uint32_t x, uint64_t key) {
   return owen_prefix(P);
 }
 
-// The same scramble with its first kOwenTableBits levels read from a table:
-// the top L output bits depend only on the top L bits of x (each level hashes
-// the already scrambled prefix), so tab[x >> (32-L)] = owen(x)[31:32-L] for the
-// dimension, and the remaining levels continue from P = that prefix * C3.
+// Prefix-table entries.  The top L output bits depend only on the top L bits of x
+// (each level hashes the already scrambled prefix).  A 32-bit entry for prefix p is
+// the whole 32-level scramble of x = p << (32-L): its top L bits are the scrambled
+// prefix of every x starting with p, and it is the scrambled coordinate itself when
+// the low 32-L bits of x are zero -- for every point below 2^L, since v[k] occupies
+// bits 31-k..31 only and a gray code below 2^L selects v[0..L-1].  A 16-bit entry
+// holds the scrambled prefix alone.
+constexpr bool kOwenEntryWhole = sizeof(OwenEntry) == 4;
+__device__ __forceinline__ uint32_t owen_entry_prefix(uint32_t e) {
+  return kOwenEntryWhole ? e >> (32 - kOwenTableBits) : e;
+}
+// An item whose point indices need at most L bits reads its coordinates whole.
+__host__ __device__ __forceinline__ bool owen_entry_is_coordinate(int nbits) {
+  return kOwenEntryWhole && nbits <= kOwenTableBits;
+}
+
+// The same scramble with its first kOwenTableBits levels read from the table
+// entry for x: the remaining levels continue from P = the scrambled prefix * C3.
 template <bool kFullUnroll>
-__device__ __forceinline__ uint32_t owen_scramble_below(uint32_t x, uint64_t key, uint32_t top) {
-  uint64_t P = (uint64_t)top * kOwenC3;
+__device__ __forceinline__ uint32_t owen_scramble_below(uint32_t x, uint64_t key, uint32_t entry) {
+  uint64_t P = (uint64_t)owen_entry_prefix(entry) * kOwenC3;
   if constexpr (kFullUnroll) {
 #pragma unroll
     for (int bit = 31 - kOwenTableBits; bit >= 0; --bit) owen_level(x, key, bit, P);
@@ -120,10 +134,10 @@ __device__ __forceinline__ void owen_levels(uint32_t x, uint64_t key, uint64_t& 
 // 2^-25 / phi(z(2^-10)) = 9.1e-6.
 constexpr uint32_t kF32TailCells = 1u << 14;  // 24-bit cells below u = 2^-10
 
-// Top 24 levels; `top` is the prefix-table entry for x when `has_table`.
-__device__ __forceinline__ uint32_t owen_top24(uint32_t x, uint64_t key, bool has_table, uint32_t top) {
+// Top 24 levels; `entry` is the prefix-table entry for x when `has_table`.
+__device__ __forceinline__ uint32_t owen_top24(uint32_t x, uint64_t key, bool has_table, uint32_t entry) {
   if (has_table) {
-    uint64_t P = (uint64_t)top * kOwenC3;
+    uint64_t P = (uint64_t)owen_entry_prefix(entry) * kOwenC3;
     owen_levels<31 - kOwenTableBits, 8>(x, key, P);
     return owen_prefix(P);
   }
@@ -143,7 +157,15 @@ __device__ __forceinline__ uint32_t owen_low8(uint32_t x, uint64_t key, uint32_t
   return owen_prefix(P);
 }
 
-__device__ __forceinline__ uint32_t owen_f32(uint32_t x, uint64_t key, const OwenEntry* __restrict__ tab_row) {
+// The FP32 rule applied to a whole scrambled coordinate.
+__device__ __forceinline__ uint32_t owen_f32_of_whole(uint32_t xs, bool full_ladder) {
+  return full_ladder || owen_f32_tail(xs >> 8) ? xs : ((xs & 0xFFFFFF00u) | 0x80u);
+}
+
+// `whole`: the table entry of x is its scrambled coordinate (owen_entry_is_coordinate)
+__device__ __forceinline__ uint32_t owen_f32(uint32_t x, uint64_t key, const OwenEntry* __restrict__ tab_row,
+                                             bool whole) {
+  if (tab_row && whole) return owen_f32_of_whole(tab_row[owen_slot_of(x)], false);
   const uint32_t p = owen_top24(x, key, tab_row != nullptr, tab_row ? tab_row[owen_slot_of(x)] : 0u);
   return owen_f32_tail(p) ? owen_low8(x, key, p) : ((p << 8) | 0x80u);
 }
@@ -527,6 +549,12 @@ template <int F>
 __device__ __forceinline__ void sobol_ints_f32(const ItemDesc& it, int node, const SobolFetch<F>& sf,
                                                uint32_t (&xs)[F]) {
   const bool tab = (it.flags & kOwenTable) != 0;
+  if (tab && owen_entry_is_coordinate(it.nbits)) {
+    // the entries are the scrambled coordinates: nothing left to hash
+#pragma unroll
+    for (int f = 0; f < F; ++f) xs[f] = owen_f32_of_whole(sf.top[f], (it.flags & kFullLadder) != 0);
+    return;
+  }
   // the full-ladder option sends every coordinate through the last 8 levels
   const uint32_t all = (it.flags & kFullLadder) ? (1u << F) - 1u : 0u;
   uint64_t keys[F];
@@ -562,6 +590,7 @@ __device__ __forceinline__ void sobol_ints_f32(const ItemDesc& it, int node, con
 template <typename Real, int F, bool kUnrollOwen>
 __device__ __forceinline__ void sobol_finish(const ItemDesc& it, int node, const SobolFetch<F>& sf, Real (&z)[F]) {
   const bool tab = (it.flags & kOwenTable) != 0;
+  const bool whole = tab && owen_entry_is_coordinate(it.nbits);
   if constexpr (sizeof(Real) == 4) {
     uint32_t xs[F];
     sobol_ints_f32<F>(it, node, sf, xs);
@@ -575,7 +604,9 @@ __device__ __forceinline__ void sobol_finish(const ItemDesc& it, int node, const
     for (int f = 0; f < F; ++f) {
       const uint64_t key = it.seed ^ ((uint64_t)((uint32_t)node * F + f) * 0x9E3779B97F4A7C15ull);
       // full unroll where registers allow (the FP64 10-scenario lane keeps the loop)
-      const uint32_t xs = tab ? owen_scramble_below<kUnrollOwen>(sf.x[f], key, sf.top[f]) : owen_scramble(sf.x[f], key);
+      const uint32_t xs = whole ? sf.top[f]
+                          : tab ? owen_scramble_below<kUnrollOwen>(sf.x[f], key, sf.top[f])
+                                : owen_scramble(sf.x[f], key);
       u[f] = ((double)xs + 0.5) / 4294967296.0;
     }
     probit_node<Real, F>(u, z);
@@ -704,14 +735,19 @@ __global__ __launch_bounds__(256) void normal_table_kernel(const uint32_t* __res
   const uint32_t x = sobol_x(gray, dirs + (size_t)dim * 32, nbits);
   const uint64_t key = seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
   Real z;
+  const bool whole = owen_entry_is_coordinate(nbits);
   if constexpr (sizeof(Real) == 8) {
-    const uint32_t xs = owen_tab ? owen_scramble_below<true>(x, key, owen_tab[((size_t)dim << kOwenTableBits) +
-                                                                     owen_slot_of(x)])
-                                 : owen_scramble(x, key);
+    uint32_t xs;
+    if (owen_tab) {
+      const uint32_t e = owen_tab[((size_t)dim << kOwenTableBits) + owen_slot_of(x)];
+      xs = whole ? e : owen_scramble_below<true>(x, key, e);
+    } else {
+      xs = owen_scramble(x, key);
+    }
     z = probit_f64(((double)xs + 0.5) / 4294967296.0);
   } else {
     // the FP32 tier's 24-level scramble, as the inline ladder computes it
-    const uint32_t xs = owen_f32(x, key, owen_tab ? owen_tab + ((size_t)dim << kOwenTableBits) : nullptr);
+    const uint32_t xs = owen_f32(x, key, owen_tab ? owen_tab + ((size_t)dim << kOwenTableBits) : nullptr, whole);
     z = probit_f32(((double)xs + 0.5) / 4294967296.0);
   }
   out[(size_t)dim * n_paths + path] = z;

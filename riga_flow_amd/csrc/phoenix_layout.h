@@ -23,13 +23,15 @@ constexpr int kMaxLvX = 256;        // log-spot points per local-vol row staged 
 // 22 levels: 16 MiB per raw dimension (u32 entries).  Measured on the 3-asset 756-step
 // batches: FP32 ten-scenario 38.96 / 37.06 / 35.55 ms, FP64 plain 14.26 / 13.74 / 13.01 ms
 // for 16 / 18 / 20 bits; with the gathers non-temporal, 34.00 / 12.77 ms at 20 bits,
-// 32.70 / 12.49 ms at 21 and 31.68 / 12.16 ms at 22.
+// 32.70 / 12.49 ms at 21 and 31.68 / 12.16 ms at 22; with whole-coordinate entries
+// (no level hashed below 2^22 points), 28.65 / 8.73 ms.
 #ifndef RIGA_OWEN_PREFIX_BITS
 #define RIGA_OWEN_PREFIX_BITS 22
 #endif
 constexpr int kOwenPrefixBits = RIGA_OWEN_PREFIX_BITS;  // Owen levels tabulated per dimension
 static_assert(kOwenPrefixBits >= 8 && kOwenPrefixBits <= 24, "prefix table width out of range");
-// an entry holds the scrambled top kOwenPrefixBits bits of its prefix
+// a 32-bit entry holds the whole scramble of its prefix followed by zero bits (whose top
+// kOwenPrefixBits bits are the scrambled prefix), a 16-bit one the scrambled prefix alone
 #if RIGA_OWEN_PREFIX_BITS <= 16
 typedef uint16_t OwenEntry;
 #else
@@ -169,7 +171,7 @@ struct ItemDesc {
   int32_t n_blocks;
   int32_t paths_per_block;
   int32_t ladder_rows;   // ladder rows uploaded (min(ladder_dims, F*steps))
-  const OwenEntry* owen_tab; // device table [dims][2^kOwenPrefixBits]: scrambled top bits per raw dimension
+  const OwenEntry* owen_tab; // device table [dims][2^kOwenPrefixBits]: entries per raw dimension
   // FP32 tier: constants pre-rounded on the host, so the path loop converts nothing
   float chol32[kMaxFactors * kMaxFactors];
   float neg_half_dt, ln_ko32, ln_cb32, root_sd32;

@@ -324,6 +324,21 @@ def test_owen_table_and_full_hash_agree(tier):
         assert a.trace_events == b.trace_events
 
 
+@pytest.mark.parametrize("tier", ["fp64", "fp32"])
+def test_owen_table_entries_past_22_bit_point_indices(tier):
+    # below 2^22 points a table entry is the scrambled coordinate itself; an item
+    # whose point indices need 23 bits hashes the low levels after its entries' top
+    # 22: a book of 2^22 + 1,000 paths prices identically with the table and with
+    # every level hashed, inline and through the shared normal table
+    req = _flat((1 << 22) + 1000, steps=12)
+    md = W.max_dimension_for(req)
+    a = api.price_batch([req], tier, max_dimension=md, owen_table=True, normal_table=-1)[0]
+    b = api.price_batch([req], tier, max_dimension=md, owen_table=False, normal_table=-1)[0]
+    c = api.price_batch([req], tier, max_dimension=md, owen_table=True, normal_table=1)[0]
+    assert a.pv == b.pv == c.pv and a.legs.down_and_in_put == b.legs.down_and_in_put
+    assert a.stats[0].output_count == b.stats[0].output_count == c.stats[0].output_count
+
+
 def test_owen_tables_are_built_for_plans_that_read_enough_points():
     # a seed no other test uses: a plan reading fewer than 2^18 points per dimension
     # hashes every level; one reading 2^18 builds the seed's prefix table; a later
