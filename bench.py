@@ -31,21 +31,20 @@ from riga_flow_amd import api, workloads as W  # noqa: E402
 OWEN_ROUND = 21     # hash input xor 2 (three-input), xorshift 2, 64-bit multiply 4,
                     # xorshift 3, top bit of the 2nd multiply 5, scrambled bit 2, prefix*C3 update 3
                     # (the prefix itself is one multiply of P by C3^-1 after the last level)
-# hashed levels: the top OWEN_TABLE_BITS of the 32 come from the per-dimension prefix
-# table (RIGA_OWEN_PREFIX_BITS in riga_flow_amd/csrc/phoenix_layout.h), whose entry is
-# the whole scrambled coordinate for point indices of at most OWEN_TABLE_BITS bits; past
-# them the FP64 tier hashes the remaining levels, the FP32 tier the rest of a float's 24
-# bits and levels 7..0 only for coordinates within 2^-10 of 0 or 1 (a fraction 2^-9)
+# Points below 2^OWEN_TABLE_BITS (RIGA_OWEN_PREFIX_BITS in riga_flow_amd/csrc/phoenix_layout.h)
+# read their scrambled coordinates from the seed's per-dimension table; past them a
+# coordinate is the gray-code XOR and hashed levels: all 32 in the FP64 tier, a float's 24
+# in the FP32 tier plus levels 7..0 for coordinates within 2^-10 of 0 or 1 (a fraction 2^-9)
 OWEN_TABLE_BITS = 22
 
 
 def owen_rounds(tier: str, nbits: int) -> float:
     if nbits <= OWEN_TABLE_BITS:
         return 0.0
-    return 32 - OWEN_TABLE_BITS if tier == "fp64" else 24 - OWEN_TABLE_BITS + 8 * 2.0 ** -9
+    return 32 if tier == "fp64" else 24 + 8 * 2.0 ** -9
 
 
-OWEN_TABLE = 3      # prefix table address and load
+OWEN_TABLE = 3      # coordinate table address and load
 PROBIT = 22         # rational (12) + divide (8) + tails (5% of log + sqrt + rational)
 UNIFORM = 4         # int->float, +0.5, scale, clamp
 # Per scenario and factor:
@@ -64,7 +63,8 @@ HBM_PEAK = 8.0e12
 
 def ops_per_path_step(F: int, S: int, nbits: int, local_vol: bool, tier: str) -> float:
     """Lane operations of one path advanced one step for all S scenarios."""
-    coord = owen_rounds(tier, nbits) * OWEN_ROUND + OWEN_TABLE + 2 * nbits + UNIFORM + PROBIT
+    source = OWEN_TABLE if nbits <= OWEN_TABLE_BITS else 2 * nbits + owen_rounds(tier, nbits) * OWEN_ROUND
+    coord = source + UNIFORM + PROBIT
     node = F * coord + F * (F + 1) + 3 * F + 2 * F  # coordinates, L z, bridge node, shock
     scen = F * ((LV_LOOKUP[tier] if local_vol else 0) + EULER[tier]) + SCAN[tier](F)
     return node + S * scen

@@ -161,7 +161,9 @@ struct HostBuf {
 
 }  // namespace
 
-// Owen prefix table of one seed: OwenEntry[dims][2^kOwenPrefixBits], built on the
+// Owen coordinate table of one seed: OwenEntry[dims][2^kOwenPrefixBits], the scrambled
+// Sobol coordinate of every point below 2^kOwenPrefixBits in each dimension (with the
+// dimension's own direction row), built on the
 // device, shared by every plan of the context (plans keep a reference, so a
 // table replaced by a larger one stays valid until its last plan is gone).
 struct OwenTable {
@@ -169,6 +171,8 @@ struct OwenTable {
   int dims = 0;
   uint64_t last_use = 0;  // the context's request count when a plan last took it
   DevBuf buf;
+  DevBuf dirs;  // the direction rows it was built from
+  std::vector<uint32_t> host_dirs;
   hipEvent_t ready = nullptr;  // recorded after the build
   ~OwenTable() {
     if (ready) (void)hipEventDestroy(ready);
@@ -768,8 +772,17 @@ hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t st
   t->seed = seed;
   t->dims = want;
   hipError_t e = t->buf.ensure(((size_t)want << kOwenPrefixBits) * sizeof(OwenEntry));  // 2^bits entries per dimension
+  // every dimension's own direction row (a table serves items whose dimensions do not wrap)
+  {
+    const std::vector<uint32_t>& all = sobol_directions(want);
+    t->host_dirs.assign(all.begin(), all.begin() + (size_t)want * 32);  // lives as long as the copy may
+  }
+  if (e == hipSuccess) e = t->dirs.ensure((size_t)want * 32 * sizeof(uint32_t));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(t->dirs.p, t->host_dirs.data(), (size_t)want * 32 * sizeof(uint32_t), hipMemcpyHostToDevice,
+                       stream);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&t->ready, hipEventDisableTiming);
-  if (e == hipSuccess) e = launch_owen_table(seed, want, (OwenEntry*)t->buf.p, stream);
+  if (e == hipSuccess) e = launch_owen_table(seed, want, (const uint32_t*)t->dirs.p, (OwenEntry*)t->buf.p, stream);
   if (e == hipSuccess) e = hipEventRecord(t->ready, stream);
   if (e != hipSuccess) return e;
   t->last_use = use;
@@ -1416,7 +1429,9 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   for (Item& it : items)
     if (it.d.flags & kOwenTable) {
       const auto c = owen_cover.find(it.d.seed);
-      if (c != owen_cover.end() && c->second >= it.d.F * it.d.steps) {
+      // it holds the item's points and dimensions, and the item's rows do not wrap
+      if (c != owen_cover.end() && c->second >= it.d.F * it.d.steps && it.d.nbits <= kOwenPrefixBits &&
+          it.d.F * it.d.steps <= it.d.ladder_dims) {
         it.d.owen_tab = owen_ptr[it.d.seed];
         it.d.owen_dims = c->second;
       } else {
@@ -1440,6 +1455,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
       zt->paths = (int64_t)paths;
       zt->nbits = 64 - __builtin_clzll(paths);
       zt->owen = owen_table ? owen_ptr[kv.first] : nullptr;
+      if (zt->nbits > kOwenPrefixBits) zt->owen = nullptr;  // its points pass the table's
       zt->owen_dims = zt->owen ? owen_cover[kv.first] : 0;
       if (stream_ordered) zt->buf.stream = pl->stream;
       ztable_bytes += bytes;
@@ -2298,7 +2314,7 @@ int riga_debug_sobol_f32(riga_ctx* ctx, uint64_t seed, int32_t ladder_dims, uint
   std::lock_guard<std::mutex> lock(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
   std::shared_ptr<OwenTable> ot;
-  if (owen_table & 1) {
+  if ((owen_table & 1) && nbits <= kOwenPrefixBits && dims <= ladder_dims) {
     HIP_TRY(owen_table_for(ctx, seed, dims, ctx->stream, &ot), "owen table");
     HIP_TRY(hipStreamWaitEvent(ctx->stream, ot->ready, 0), "owen table wait");
   }

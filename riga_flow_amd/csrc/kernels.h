@@ -119,7 +119,7 @@ hipError_t launch_lv_propagate(const LvPropagateArgs& a, hipStream_t stream);
 
 // Owen prefix table: out[dim][owen_slot(v)] = scrambled top kOwenPrefixBits bits of
 // v << (32 - kOwenPrefixBits), dims [0, dims).
-hipError_t launch_owen_table(uint64_t seed, int dims, OwenEntry* out, hipStream_t stream);
+hipError_t launch_owen_table(uint64_t seed, int dims, const uint32_t* dirs, OwenEntry* out, hipStream_t stream);
 
 // Coupon strips [path][period] of a replayed item from its worst-of series.
 hipError_t launch_ledger(const ItemDesc* items, int item, uint64_t paths, const unsigned char* arena,

@@ -79,7 +79,7 @@ hipError_t launch_phoenix_f32(const PhoenixLaunch& l, hipStream_t stream) {
 }
 
 
-// The FP32 tier's Sobol ladder as the path kernel runs it (prefix-table gather,
+// The FP32 tier's Sobol ladder as the path kernel runs it (coordinate-table read or
 // 24 hashed levels, the wave's tail passes, the node's FP32 inverse normals) for points
 // first_point + p and dims node*F + f: xs_out/z_out[p][node*F + f].
 template <int F>
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void debug_sobol_f32_kernel(const uint32_t* __
   const size_t row = (size_t)p * n_nodes * F;
   for (int node = 0; node < n_nodes; ++node) {
     SobolFetch<F> sf;
-    sobol_fetch<F>(it, reinterpret_cast<const unsigned char*>(dirs), gray, node, sf);
+    sobol_fetch<F>(it, reinterpret_cast<const unsigned char*>(dirs), n, gray, node, sf);
     uint32_t xs[F];
     sobol_ints_f32<F>(it, node, sf, xs);
     double u[F];

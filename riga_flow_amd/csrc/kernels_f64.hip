@@ -237,25 +237,20 @@ hipError_t launch_debug_halton(const uint32_t* tables, int rows, uint64_t first_
   return hipGetLastError();
 }
 
-// tab[dim][owen_slot(v)] = owen_scramble(v << (32-L), key(dim)) with 32-bit entries (all
-// 32 levels: the coordinate itself for points below 2^L), its top L bits with 16-bit ones.
-// Lanes run over slots (coalesced stores); the prefix of a slot is its rotation back.
-__global__ __launch_bounds__(256) void owen_table_kernel(uint64_t seed, int dims, OwenEntry* __restrict__ out) {
-  constexpr uint32_t mask = (1u << kOwenTableBits) - 1u;
-  const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+// tab[dim][n] = owen_scramble(sobol_x(gray(n), row(dim)), key(dim)) for the points n below 2^L:
+// the scrambled Sobol coordinates of a seed, one lane per entry (coalesced stores).
+__global__ __launch_bounds__(256) void owen_table_kernel(uint64_t seed, int dims, const uint32_t* __restrict__ dirs,
+                                                         OwenEntry* __restrict__ out) {
+  const uint32_t n = blockIdx.x * 256 + threadIdx.x;
   const uint32_t dim = blockIdx.y;
-  if (slot > mask || (int)dim >= dims) return;
-  const uint32_t v = ((slot >> 6) | (slot << (kOwenTableBits - 6))) & mask;
-  const uint64_t key = seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
-  const uint32_t x = v << (32 - kOwenTableBits);
-  uint64_t P = 0;
-  for (int bit = 31; bit >= (kOwenEntryWhole ? 0 : 32 - kOwenTableBits); --bit) owen_level(x, key, bit, P);
-  out[(size_t)dim * (1u << kOwenTableBits) + slot] = (OwenEntry)owen_prefix(P);
+  if (n >= (1u << kOwenTableBits) || (int)dim >= dims) return;
+  const uint32_t x = sobol_x(n ^ (n >> 1), dirs + (size_t)dim * 32, kOwenTableBits);
+  out[((size_t)dim << kOwenTableBits) + n] = owen_scramble(x, seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull));
 }
 
-hipError_t launch_owen_table(uint64_t seed, int dims, OwenEntry* out, hipStream_t stream) {
+hipError_t launch_owen_table(uint64_t seed, int dims, const uint32_t* dirs, OwenEntry* out, hipStream_t stream) {
   dim3 grid((1u << kOwenTableBits) / 256, dims);
-  hipLaunchKernelGGL(owen_table_kernel, grid, dim3(256), 0, stream, seed, dims, out);
+  hipLaunchKernelGGL(owen_table_kernel, grid, dim3(256), 0, stream, seed, dims, dirs, out);
   return hipGetLastError();
 }
 

@@ -39,7 +39,7 @@ constexpr int kWaves = kWavesPerBlock;
 // final xorshift of mix64 leaves bit 63 unchanged.
 constexpr uint64_t kOwenC2 = 0xD6E8FEB86659FD93ull;
 constexpr uint64_t kOwenC3 = 0xA24BAED4963EE407ull;
-constexpr int kOwenTableBits = kOwenPrefixBits;  // levels read from the per-dimension prefix table
+constexpr int kOwenTableBits = kOwenPrefixBits;  // log2 of the points per dimension of a coordinate table
 
 // The scrambled prefix is not carried level by level: C3 is odd, so
 // prefix = P * C3^-1 (mod 2^64), and its low 32 bits -- all of a prefix -- are
@@ -48,19 +48,6 @@ constexpr uint32_t kOwenC3InvLo = 0xF69349B7u;  // (C3 mod 2^32)^-1 mod 2^32
 static_assert((uint32_t)kOwenC3 * kOwenC3InvLo == 1u, "C3 inverse");
 
 __device__ __forceinline__ uint32_t owen_prefix(uint64_t P) { return (uint32_t)P * kOwenC3InvLo; }
-
-// Position of a coordinate's prefix-table entry within its dimension's row.
-// The 64 lanes of a wave hold consecutive Sobol points; their gray codes differ
-// only in the low 6 bits, which select direction numbers v[0..5], and v[k]
-// occupies bits 31-k..31 only -- so across a wave the top 6 bits of x take all
-// 64 values while the rest of the prefix is the same.  The row keeps entry
-// p = x >> (32 - L) at slot rotl_L(p, 6): the wave's 64 entries are adjacent,
-// one or two 128-byte lines per gather instead of 64.
-__host__ __device__ __forceinline__ uint32_t owen_slot(uint32_t prefix) {
-  constexpr uint32_t mask = (1u << kOwenTableBits) - 1u;
-  return ((prefix << 6) | (prefix >> (kOwenTableBits - 6))) & mask;
-}
-__device__ __forceinline__ uint32_t owen_slot_of(uint32_t x) { return owen_slot(x >> (32 - kOwenTableBits)); }
 
 __device__ __forceinline__ void owen_level(uint32_t x, uint64_t key, int bit, uint64_t& P) {
   constexpr uint64_t C4 = 0xBF58476D1CE4E5B9ull;
@@ -87,37 +74,6 @@ __device__ __forceinline__ uint32_t owen_scramble(uint32_t x, uint64_t key) {
   return owen_prefix(P);
 }
 
-// Prefix-table entries.  The top L output bits depend only on the top L bits of x
-// (each level hashes the already scrambled prefix).  A 32-bit entry for prefix p is
-// the whole 32-level scramble of x = p << (32-L): its top L bits are the scrambled
-// prefix of every x starting with p, and it is the scrambled coordinate itself when
-// the low 32-L bits of x are zero -- for every point below 2^L, since v[k] occupies
-// bits 31-k..31 only and a gray code below 2^L selects v[0..L-1].  A 16-bit entry
-// holds the scrambled prefix alone.
-constexpr bool kOwenEntryWhole = sizeof(OwenEntry) == 4;
-__device__ __forceinline__ uint32_t owen_entry_prefix(uint32_t e) {
-  return kOwenEntryWhole ? e >> (32 - kOwenTableBits) : e;
-}
-// An item whose point indices need at most L bits reads its coordinates whole.
-__host__ __device__ __forceinline__ bool owen_entry_is_coordinate(int nbits) {
-  return kOwenEntryWhole && nbits <= kOwenTableBits;
-}
-
-// The same scramble with its first kOwenTableBits levels read from the table
-// entry for x: the remaining levels continue from P = the scrambled prefix * C3.
-template <bool kFullUnroll>
-__device__ __forceinline__ uint32_t owen_scramble_below(uint32_t x, uint64_t key, uint32_t entry) {
-  uint64_t P = (uint64_t)owen_entry_prefix(entry) * kOwenC3;
-  if constexpr (kFullUnroll) {
-#pragma unroll
-    for (int bit = 31 - kOwenTableBits; bit >= 0; --bit) owen_level(x, key, bit, P);
-  } else {
-#pragma unroll 4
-    for (int bit = 31 - kOwenTableBits; bit >= 0; --bit) owen_level(x, key, bit, P);
-  }
-  return owen_prefix(P);
-}
-
 // Levels kHi..kLo of the scramble, continuing from P.
 template <int kHi, int kLo>
 __device__ __forceinline__ void owen_levels(uint32_t x, uint64_t key, uint64_t& P) {
@@ -134,13 +90,8 @@ __device__ __forceinline__ void owen_levels(uint32_t x, uint64_t key, uint64_t& 
 // 2^-25 / phi(z(2^-10)) = 9.1e-6.
 constexpr uint32_t kF32TailCells = 1u << 14;  // 24-bit cells below u = 2^-10
 
-// Top 24 levels; `entry` is the prefix-table entry for x when `has_table`.
-__device__ __forceinline__ uint32_t owen_top24(uint32_t x, uint64_t key, bool has_table, uint32_t entry) {
-  if (has_table) {
-    uint64_t P = (uint64_t)owen_entry_prefix(entry) * kOwenC3;
-    owen_levels<31 - kOwenTableBits, 8>(x, key, P);
-    return owen_prefix(P);
-  }
+// Top 24 levels.
+__device__ __forceinline__ uint32_t owen_top24(uint32_t x, uint64_t key) {
   uint64_t P = 0;
 #pragma unroll 4
   for (int bit = 31; bit >= 8; --bit) owen_level(x, key, bit, P);
@@ -162,11 +113,8 @@ __device__ __forceinline__ uint32_t owen_f32_of_whole(uint32_t xs, bool full_lad
   return full_ladder || owen_f32_tail(xs >> 8) ? xs : ((xs & 0xFFFFFF00u) | 0x80u);
 }
 
-// `whole`: the table entry of x is its scrambled coordinate (owen_entry_is_coordinate)
-__device__ __forceinline__ uint32_t owen_f32(uint32_t x, uint64_t key, const OwenEntry* __restrict__ tab_row,
-                                             bool whole) {
-  if (tab_row && whole) return owen_f32_of_whole(tab_row[owen_slot_of(x)], false);
-  const uint32_t p = owen_top24(x, key, tab_row != nullptr, tab_row ? tab_row[owen_slot_of(x)] : 0u);
+__device__ __forceinline__ uint32_t owen_f32(uint32_t x, uint64_t key) {
+  const uint32_t p = owen_top24(x, key);
   return owen_f32_tail(p) ? owen_low8(x, key, p) : ((p << 8) | 0x80u);
 }
 
@@ -519,25 +467,32 @@ struct SobolFetch {
   uint32_t top[F];  // prefix-table entries (unused without a table)
 };
 
-// An item reads its coordinates' top Owen levels from the prefix table when the
-// host found a table covering all its dimensions (kOwenTable); otherwise it hashes
-// every level -- the same integers.  The decision is per item, so the step loop
-// holds one path or the other.
+// An item reads its scrambled coordinates from the seed's coordinate table when the
+// host found one covering all its dimensions and points (kOwenTable); otherwise it
+// forms the Sobol integer and hashes every Owen level -- the same integers.  The
+// decision is per item, so the step loop holds one path or the other.
 template <int F>
-__device__ __forceinline__ void sobol_fetch(const ItemDesc& it, const unsigned char* arena, uint32_t gray, int node,
-                                            SobolFetch<F>& sf) {
-  const bool tab = (it.flags & kOwenTable) != 0;
+__device__ __forceinline__ void sobol_fetch(const ItemDesc& it, const unsigned char* arena, uint32_t n, uint32_t gray,
+                                            int node, SobolFetch<F>& sf) {
+  // the table holds points below 2^kOwenTableBits (the host sets the flag only then)
+  const bool tab = (it.flags & kOwenTable) != 0 && it.nbits <= kOwenTableBits;
 #pragma unroll
   for (int f = 0; f < F; ++f) {
     const uint32_t dim = (uint32_t)node * F + f;
-    const uint32_t d = dim % (uint32_t)it.ladder_dims;
-    const uint32_t* row = at<uint32_t>(arena, it.off_dirs) + (size_t)d * 32;
-    sf.x[f] = sobol_x(gray, row, it.nbits);
-    // a streaming gather (each entry is read once per launch): non-temporal, so it
-    // does not push the shared tables out of the caches
-    sf.top[f] = tab ? (uint32_t)__builtin_nontemporal_load(&it.owen_tab[((size_t)dim << kOwenTableBits) +
-                                                                       owen_slot_of(sf.x[f])])
-                    : 0u;
+    if (tab) {
+      // a streaming read (each entry once per launch; the wave's 64 entries are
+      // adjacent): non-temporal, so it does not push the shared tables out of the caches
+      // lanes past the item's last path (results unused) may hold points past the
+      // table: the mask keeps their reads inside the row
+      sf.x[f] = 0u;
+      sf.top[f] = (uint32_t)__builtin_nontemporal_load(
+          &it.owen_tab[((size_t)dim << kOwenTableBits) + (n & ((1u << kOwenTableBits) - 1u))]);
+    } else {
+      const uint32_t d = dim % (uint32_t)it.ladder_dims;
+      const uint32_t* row = at<uint32_t>(arena, it.off_dirs) + (size_t)d * 32;
+      sf.x[f] = sobol_x(gray, row, it.nbits);
+      sf.top[f] = 0u;
+    }
   }
 }
 
@@ -548,8 +503,8 @@ __device__ __forceinline__ void sobol_fetch(const ItemDesc& it, const unsigned c
 template <int F>
 __device__ __forceinline__ void sobol_ints_f32(const ItemDesc& it, int node, const SobolFetch<F>& sf,
                                                uint32_t (&xs)[F]) {
-  const bool tab = (it.flags & kOwenTable) != 0;
-  if (tab && owen_entry_is_coordinate(it.nbits)) {
+  const bool tab = (it.flags & kOwenTable) != 0 && it.nbits <= kOwenTableBits;
+  if (tab) {
     // the entries are the scrambled coordinates: nothing left to hash
 #pragma unroll
     for (int f = 0; f < F; ++f) xs[f] = owen_f32_of_whole(sf.top[f], (it.flags & kFullLadder) != 0);
@@ -562,7 +517,7 @@ __device__ __forceinline__ void sobol_ints_f32(const ItemDesc& it, int node, con
 #pragma unroll
   for (int f = 0; f < F; ++f) {
     keys[f] = it.seed ^ ((uint64_t)((uint32_t)node * F + f) * 0x9E3779B97F4A7C15ull);
-    const uint32_t p = owen_top24(sf.x[f], keys[f], tab, sf.top[f]);
+    const uint32_t p = owen_top24(sf.x[f], keys[f]);
     xs[f] = (p << 8) | 0x80u;
     pend |= (uint32_t)owen_f32_tail(p) << f;
   }
@@ -589,8 +544,7 @@ __device__ __forceinline__ void sobol_ints_f32(const ItemDesc& it, int node, con
 
 template <typename Real, int F, bool kUnrollOwen>
 __device__ __forceinline__ void sobol_finish(const ItemDesc& it, int node, const SobolFetch<F>& sf, Real (&z)[F]) {
-  const bool tab = (it.flags & kOwenTable) != 0;
-  const bool whole = tab && owen_entry_is_coordinate(it.nbits);
+  const bool tab = (it.flags & kOwenTable) != 0 && it.nbits <= kOwenTableBits;
   if constexpr (sizeof(Real) == 4) {
     uint32_t xs[F];
     sobol_ints_f32<F>(it, node, sf, xs);
@@ -604,9 +558,7 @@ __device__ __forceinline__ void sobol_finish(const ItemDesc& it, int node, const
     for (int f = 0; f < F; ++f) {
       const uint64_t key = it.seed ^ ((uint64_t)((uint32_t)node * F + f) * 0x9E3779B97F4A7C15ull);
       // full unroll where registers allow (the FP64 10-scenario lane keeps the loop)
-      const uint32_t xs = whole ? sf.top[f]
-                          : tab ? owen_scramble_below<kUnrollOwen>(sf.x[f], key, sf.top[f])
-                                : owen_scramble(sf.x[f], key);
+      const uint32_t xs = tab ? sf.top[f] : owen_scramble(sf.x[f], key);
       u[f] = ((double)xs + 0.5) / 4294967296.0;
     }
     probit_node<Real, F>(u, z);
@@ -634,7 +586,7 @@ __device__ __forceinline__ void node_normals(const ItemDesc& it, const unsigned 
     probit_node<Real, F>(u, z);
   } else {
     SobolFetch<F> sf;
-    sobol_fetch<F>(it, arena, gray, node, sf);
+    sobol_fetch<F>(it, arena, n, gray, node, sf);
     sobol_finish<Real, F, kUnrollOwen>(it, node, sf, z);
   }
   correlate<Real, F>(it, z, y);
@@ -729,25 +681,23 @@ __global__ __launch_bounds__(256) void normal_table_kernel(const uint32_t* __res
   const uint32_t dim = blockIdx.y;
   if (path >= n_paths) return;
   // the prefix table covers the first owen_dims dimensions; the others hash every level
-  const OwenEntry* owen_tab = (int)dim < owen_dims ? owen_tab_in : nullptr;
+  const OwenEntry* owen_tab = (int)dim < owen_dims && nbits <= kOwenTableBits ? owen_tab_in : nullptr;
   const uint32_t n = (uint32_t)path + 1;
-  const uint32_t gray = n ^ (n >> 1);
-  const uint32_t x = sobol_x(gray, dirs + (size_t)dim * 32, nbits);
   const uint64_t key = seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
+  uint32_t xs;
+  if (owen_tab) {
+    xs = owen_tab[((size_t)dim << kOwenTableBits) + (n & ((1u << kOwenTableBits) - 1u))];
+  } else {
+    const uint32_t gray = n ^ (n >> 1);
+    const uint32_t x = sobol_x(gray, dirs + (size_t)dim * 32, nbits);
+    xs = sizeof(Real) == 8 ? owen_scramble(x, key) : owen_f32(x, key);
+  }
   Real z;
-  const bool whole = owen_entry_is_coordinate(nbits);
   if constexpr (sizeof(Real) == 8) {
-    uint32_t xs;
-    if (owen_tab) {
-      const uint32_t e = owen_tab[((size_t)dim << kOwenTableBits) + owen_slot_of(x)];
-      xs = whole ? e : owen_scramble_below<true>(x, key, e);
-    } else {
-      xs = owen_scramble(x, key);
-    }
     z = probit_f64(((double)xs + 0.5) / 4294967296.0);
   } else {
-    // the FP32 tier's 24-level scramble, as the inline ladder computes it
-    const uint32_t xs = owen_f32(x, key, owen_tab ? owen_tab + ((size_t)dim << kOwenTableBits) : nullptr, whole);
+    // the FP32 tier's 24-level rule, as the inline ladder applies it
+    if (owen_tab) xs = owen_f32_of_whole(xs, false);
     z = probit_f32(((double)xs + 0.5) / 4294967296.0);
   }
   out[(size_t)dim * n_paths + path] = z;
@@ -1022,7 +972,7 @@ void phoenix_kernel(const ItemDesc* __restrict__ items,
       if constexpr (kTab) {
         if (steps > 1) table_normals<Real, F>(it, bnodes[0].node, n, zn);
       } else {
-        if (sobol && steps > 1) sobol_fetch<F>(it, arena, gray, bnodes[0].node, sf);
+        if (sobol && steps > 1) sobol_fetch<F>(it, arena, n, gray, bnodes[0].node, sf);
       }
       int loaded_i0[F];
   #pragma unroll
@@ -1061,7 +1011,7 @@ void phoenix_kernel(const ItemDesc* __restrict__ items,
               Real z[F];
               sobol_finish<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, bn.node, sf, z);
               correlate<Real, F>(it, z, y);
-              if (node_ptr < steps - 1) sobol_fetch<F>(it, arena, gray, bnodes[node_ptr].node, sf);
+              if (node_ptr < steps - 1) sobol_fetch<F>(it, arena, n, gray, bnodes[node_ptr].node, sf);
             } else {
               node_normals<Real, F, !(sizeof(Real) == 8 && SMAX > 4)>(it, arena, n, gray, bn.node, y);
             }

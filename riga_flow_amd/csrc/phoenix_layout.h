@@ -28,15 +28,10 @@ constexpr int kMaxLvX = 256;        // log-spot points per local-vol row staged 
 #ifndef RIGA_OWEN_PREFIX_BITS
 #define RIGA_OWEN_PREFIX_BITS 22
 #endif
-constexpr int kOwenPrefixBits = RIGA_OWEN_PREFIX_BITS;  // Owen levels tabulated per dimension
+constexpr int kOwenPrefixBits = RIGA_OWEN_PREFIX_BITS;  // coordinate tables hold points below 2^bits
 static_assert(kOwenPrefixBits >= 8 && kOwenPrefixBits <= 24, "prefix table width out of range");
-// a 32-bit entry holds the whole scramble of its prefix followed by zero bits (whose top
-// kOwenPrefixBits bits are the scrambled prefix), a 16-bit one the scrambled prefix alone
-#if RIGA_OWEN_PREFIX_BITS <= 16
-typedef uint16_t OwenEntry;
-#else
+// an entry is the scrambled 32-bit Sobol coordinate of one point in one dimension
 typedef uint32_t OwenEntry;
-#endif
 constexpr int kBlock = 256;         // threads per workgroup (4 waves of 64)
 
 enum ItemFlags : uint32_t {
@@ -171,7 +166,7 @@ struct ItemDesc {
   int32_t n_blocks;
   int32_t paths_per_block;
   int32_t ladder_rows;   // ladder rows uploaded (min(ladder_dims, F*steps))
-  const OwenEntry* owen_tab; // device table [dims][2^kOwenPrefixBits]: entries per raw dimension
+  const OwenEntry* owen_tab; // device table [dims][2^kOwenPrefixBits]: scrambled coordinates [dim][point]
   // FP32 tier: constants pre-rounded on the host, so the path loop converts nothing
   float chol32[kMaxFactors * kMaxFactors];
   float neg_half_dt, ln_ko32, ln_cb32, root_sd32;
