@@ -133,9 +133,9 @@ typedef struct riga_launch_config {
   int32_t paths_per_thread; /* 1, 2, 4 or 8 paths per lane; 0: the autotuned choice (riga_autotune)
                                for the batch's shape, else 1 */
   int32_t use_graph;        /* 1: capture the launch sequence in a hipGraph at plan creation */
-  int32_t owen_table;       /* a seed's coordinates from a per-dimension table of its 22-bit prefixes
-                               (whole below 2^22 points, the top 22 Owen levels beyond) that the
-                               context keeps (16 MiB per dimension, covering a seed's first
+  int32_t owen_table;       /* a seed's scrambled Sobol coordinates of the points below 2^22 from a
+                               per-dimension table (items past those points, or whose dimensions
+                               wrap their direction rows, hash) that the context keeps (16 MiB per dimension, covering a seed's first
                                dimensions; the same integers as hashing them): 0 = a table the
                                context holds already, or a new one when the plan reads at least
                                2^18 points per dimension of the seed and memory allows; 1 =
@@ -388,7 +388,7 @@ int riga_debug_halton(riga_ctx *ctx, uint64_t seed, int32_t ladder_dims, uint64_
  * top 24 Owen levels with the low byte 0x80, or all 32 levels where the top 10
  * scrambled bits are all 0 or all 1 (u within 2^-10 of 0 or 1) -- and
  * out_normal[p][dim] its FP32 inverse normal.  owen_table bit 0 reads the context's
- * prefix table for the seed (whole coordinates below 2^22 points), as priced batches do;
+ * coordinate table for the seed (points below 2^22, dimensions within ladder_dims), as priced batches do;
  * bit 1 hashes all 32 levels of every coordinate (the fp32_full_ladder option).
  * No counterpart in the reference (a diagnostic for the parity tests). */
 int riga_debug_sobol_f32(riga_ctx *ctx, uint64_t seed, int32_t ladder_dims, uint64_t first_point,
