@@ -279,7 +279,7 @@ def result_to_json(res: PriceResult) -> dict:
 def _owen_mode(owen_table) -> int:
     """riga_launch_config.owen_table: None lets the library decide per seed (a table it
     holds already, or one worth building for the plan's points), True forces the
-    prefix table, an int k > 1 forces one covering at most the first k dimensions,
+    coordinate table, an int k > 1 forces one covering at most the first k dimensions,
     False hashes all 32 Owen levels (the same integers)."""
     if owen_table is None:
         return 0

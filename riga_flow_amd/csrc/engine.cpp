@@ -692,7 +692,7 @@ struct riga_plan {
     int dims = 0, nbits = 0;
     int64_t paths = 0;
     const OwenEntry* owen = nullptr;
-    int owen_dims = 0;  // dimensions the prefix table covers
+    int owen_dims = 0;  // dimensions the coordinate table covers
     DevBuf buf;
   };
   std::vector<std::unique_ptr<NormalTable>> ztables;
@@ -801,7 +801,7 @@ hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t st
   return hipSuccess;
 }
 
-// Which seeds of a plan read an Owen prefix table, and how many dimensions the
+// Which seeds of a plan read an Owen coordinate table, and how many dimensions the
 // table to build for each covers (a table covers the first dimensions of its seed;
 // higher ones, and the seeds left out, hash every level -- the same integers).
 // A table grows to at least twice its dimensions when it grows, and never past the
@@ -809,8 +809,8 @@ hipError_t owen_table_for(riga_ctx* ctx, uint64_t seed, int dims, hipStream_t st
 // every seed, at most k dimensions.  mode 0: a seed whose table the context holds
 // already, wide enough or as wide as it can be; otherwise a new table is built when
 // the plan reads at least 2^18 points per dimension of that seed (a table costs
-// 2^22 x 32 hashed levels per dimension; it saves 22 hashed levels per coordinate read
-// through it, and all 32 for points below 2^22, whose entries are the coordinates),
+// 2^22 coordinates of 32 hashed levels per dimension; each coordinate read through it
+// saves its gray-code XOR and 32 or 24 hashed levels),
 // largest users first, while the new tables stay within
 // ctx->owen_plan_cap and the context's tables that plans still hold plus the new
 // ones within ctx->owen_cap.  `dims` (seed -> dimensions wanted) comes back as
@@ -1045,7 +1045,7 @@ static int stack_pooled_levels(int precision, const LaunchPlan& lp, int levels, 
 // (no stream, no allocation): the serving loop admits requests with it.
 // With `scratch` set, every device buffer is taken from that fixed area and the
 // context's allocation-free stream, events and pinned block are used: nothing
-// is allocated (no shared normal tables, no Owen prefix tables, no graph).
+// is allocated (no shared normal tables, no Owen coordinate tables, no graph).
 int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, const riga_launch_config* cfg,
                 bool records, riga_plan** out, riga_error* err, int64_t* estimate = nullptr,
                 const ReplayOpts* replay = nullptr, Scratch* scratch = nullptr) {
@@ -1237,7 +1237,7 @@ int plan_create(riga_ctx* ctx, const riga_request* reqs, int n, int precision, c
   bool any_mc = false;
   for (Item& it : items) any_mc = any_mc || (it.d.flags & kMonteCarlo) != 0;
   const int64_t off_zig = any_mc ? arena.add(ziggurat_tables().data(), 2 * kZigLen) : 0;
-  // one Owen prefix table per seed (built on the device at creation), covering
+  // one Owen coordinate table per seed (built on the device at creation), covering
   // the largest raw dimension used -- for the seeds owen_table_seeds picks
   const int owen_mode = scratch ? -1 : (cfg ? cfg->owen_table : 0);
   const bool owen_table = owen_mode >= 0;

@@ -43,7 +43,7 @@ hipError_t launch_debug_sobol_f32(const uint32_t* dirs, int ladder_dims, const O
 hipError_t launch_phoenix_f32(const PhoenixLaunch& l, hipStream_t stream);
 
 // Shared normal table z[dim][path] (Real = double for FP64, float for FP32); the
-// Owen prefix table (may be null) covers dimensions [0, owen_dims).
+// Owen coordinate table (may be null) covers dimensions [0, owen_dims).
 hipError_t launch_normal_table_f64(const uint32_t* dirs, const OwenEntry* owen_tab, int owen_dims, uint64_t seed,
                                    int dims, int n_paths, int nbits, double* out, hipStream_t stream);
 hipError_t launch_normal_table_f32(const uint32_t* dirs, const OwenEntry* owen_tab, int owen_dims, uint64_t seed,
@@ -117,8 +117,8 @@ constexpr int kLvMaxKnots = 3072;
 hipError_t launch_worst_of(const float* a, const float* b, float* o, uint64_t n, hipStream_t stream);
 hipError_t launch_lv_propagate(const LvPropagateArgs& a, hipStream_t stream);
 
-// Owen prefix table: out[dim][owen_slot(v)] = scrambled top kOwenPrefixBits bits of
-// v << (32 - kOwenPrefixBits), dims [0, dims).
+// Owen coordinate table: out[dim][n] = the scrambled Sobol coordinate of point n < 2^kOwenPrefixBits
+// from the dimension's direction row dirs[dim][32], dims [0, dims).
 hipError_t launch_owen_table(uint64_t seed, int dims, const uint32_t* dirs, OwenEntry* out, hipStream_t stream);
 
 // Coupon strips [path][period] of a replayed item from its worst-of series.

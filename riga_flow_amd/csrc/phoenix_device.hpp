@@ -459,12 +459,12 @@ __device__ __forceinline__ void correlate(const ItemDesc& it, const Real (&z)[F]
 
 // The Sobol ladder of one bridge node in two halves, so a lane can issue the
 // next node's loads a step ahead: the fetch forms the unscrambled coordinates
-// (dims node*F + f) and gathers their Owen prefix-table entries; the finish
+// (dims node*F + f) or reads their scrambled values from the Owen coordinate table; the finish
 // hashes the remaining levels and applies the inverse normal.
 template <int F>
 struct SobolFetch {
   uint32_t x[F];
-  uint32_t top[F];  // prefix-table entries (unused without a table)
+  uint32_t top[F];  // coordinate-table entries (unused without a table)
 };
 
 // An item reads its scrambled coordinates from the seed's coordinate table when the
@@ -680,7 +680,7 @@ __global__ __launch_bounds__(256) void normal_table_kernel(const uint32_t* __res
   const int path = blockIdx.x * 256 + threadIdx.x;
   const uint32_t dim = blockIdx.y;
   if (path >= n_paths) return;
-  // the prefix table covers the first owen_dims dimensions; the others hash every level
+  // the coordinate table covers the first owen_dims dimensions; the others hash every level
   const OwenEntry* owen_tab = (int)dim < owen_dims && nbits <= kOwenTableBits ? owen_tab_in : nullptr;
   const uint32_t n = (uint32_t)path + 1;
   const uint64_t key = seed ^ ((uint64_t)dim * 0x9E3779B97F4A7C15ull);
@@ -960,7 +960,7 @@ void phoenix_kernel(const ItemDesc* __restrict__ items,
       int node_ptr = 0;
       // The next node's loads are issued one descent ahead, so their latency
       // overlaps the current step's scenario work: its normals from the shared
-      // table, or its Sobol coordinates and Owen prefix-table entries.
+      // table, or its Sobol coordinates or Owen coordinate-table entries.
       const bool sobol = kMode == kLadderDirect && (it.flags & (kNormalTable | kHalton)) == 0;
       Real zn[F];
       SobolFetch<F> sf;

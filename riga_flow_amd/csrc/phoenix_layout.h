@@ -23,13 +23,13 @@ constexpr int kMaxLvX = 256;        // log-spot points per local-vol row staged 
 // 22 levels: 16 MiB per raw dimension (u32 entries).  Measured on the 3-asset 756-step
 // batches: FP32 ten-scenario 38.96 / 37.06 / 35.55 ms, FP64 plain 14.26 / 13.74 / 13.01 ms
 // for 16 / 18 / 20 bits; with the gathers non-temporal, 34.00 / 12.77 ms at 20 bits,
-// 32.70 / 12.49 ms at 21 and 31.68 / 12.16 ms at 22; with whole-coordinate entries
+// 32.70 / 12.49 ms at 21 and 31.68 / 12.16 ms at 22; with whole-coordinate entries (indexed by prefix)
 // (no level hashed below 2^22 points), 28.65 / 8.73 ms.
 #ifndef RIGA_OWEN_PREFIX_BITS
 #define RIGA_OWEN_PREFIX_BITS 22
 #endif
 constexpr int kOwenPrefixBits = RIGA_OWEN_PREFIX_BITS;  // coordinate tables hold points below 2^bits
-static_assert(kOwenPrefixBits >= 8 && kOwenPrefixBits <= 24, "prefix table width out of range");
+static_assert(kOwenPrefixBits >= 8 && kOwenPrefixBits <= 24, "coordinate table width out of range");
 // an entry is the scrambled 32-bit Sobol coordinate of one point in one dimension
 typedef uint32_t OwenEntry;
 constexpr int kBlock = 256;         // threads per workgroup (4 waves of 64)
@@ -162,7 +162,7 @@ struct ItemDesc {
   int32_t stack_pooled;  // further stack positions (the bottom and top ones) in an HBM slot
   int64_t partial_base;  // first Partial slot: [scenario][block_in_item]
   int64_t record_base;   // first PathRecord slot: [scenario][path]
-  int32_t owen_dims;     // raw dimensions the Owen prefix table covers (all of the item's when kOwenTable)
+  int32_t owen_dims;     // raw dimensions the Owen coordinate table covers (all of the item's when kOwenTable)
   int32_t n_blocks;
   int32_t paths_per_block;
   int32_t ladder_rows;   // ladder rows uploaded (min(ladder_dims, F*steps))

@@ -118,7 +118,7 @@ def test_ctypes_mirrors_match_the_header_compiled_by_gcc(tmp_path):
 
 def test_bench_op_model_uses_the_built_owen_table_width():
     # the roofline's op model counts the Owen levels the kernels hash: those below
-    # the prefix table's width, which the library is built with
+    # the coordinate table's width, which the library is built with
     layout = open(os.path.join(ROOT, "riga_flow_amd", "csrc", "phoenix_layout.h")).read()
     built = int(re.search(r"#define RIGA_OWEN_PREFIX_BITS (\d+)", layout).group(1))
     model = int(re.search(r"^OWEN_TABLE_BITS = (\d+)", open(os.path.join(ROOT, "bench.py")).read(),

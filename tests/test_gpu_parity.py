@@ -66,7 +66,7 @@ def test_fp64_exp_and_log_on_the_device_equal_their_host_build_bit_for_bit():
 
 @pytest.mark.parametrize("factors,owen_table", [(3, True), (5, True), (3, False)])
 def test_fp32_tier_ladder_keeps_24_owen_levels_and_full_tails(factors, owen_table):
-    """The FP32 tier's integers, as its path kernels compute them (prefix table, 24
+    """The FP32 tier's integers, as its path kernels compute them (coordinate table, 24
     hashed levels, the wave's tail passes), against the rule applied to the full
     32-level scramble (whose integers the oracle pins): the top 24 bits with low byte
     0x80, and all 32 bits where the top 10 scrambled bits are all 0 or all 1.  The
@@ -128,7 +128,7 @@ def test_fp32_full_ladder_option_hashes_all_32_levels():
 
 @pytest.mark.parametrize("first_point", [(1 << 21) + 3, (1 << 24) + 11])
 def test_fp32_tier_ladder_beyond_21_bit_point_indices(first_point):
-    """The FP32 tier's ladder (prefix table, 24 hashed levels, tail passes) at point
+    """The FP32 tier's ladder (24 hashed levels, tail passes: the coordinate table holds points below 2^22) at point
     indices that take the 24-bit and the 32-bit gray-code sweeps, against the rule
     applied to the FP64 ladder's full 32-level integers at the same points."""
     seed, ladder, nodes, factors, n_pts = 4_242, 2268, 4, 3, 4096
@@ -312,7 +312,7 @@ def test_reference_dimension_cap_wraps_like_the_reference():
 
 @pytest.mark.parametrize("tier", ["fp64", "fp32"])
 def test_owen_table_and_full_hash_agree(tier):
-    # the tabulated top 22 levels and the hashed ones give identical integers, so
+    # the tabulated coordinates and the hashed ones are identical integers, so
     # identical prices: a flat 1y book and the 3y local-vol trade (2,268 dimensions)
     for req in (_flat(2048, steps=252), W.lv_book(paths=2048, steps=756)):
         md = W.max_dimension_for(req)
@@ -341,7 +341,7 @@ def test_owen_table_entries_past_22_bit_point_indices(tier):
 
 def test_owen_tables_are_built_for_plans_that_read_enough_points():
     # a seed no other test uses: a plan reading fewer than 2^18 points per dimension
-    # hashes every level; one reading 2^18 builds the seed's prefix table; a later
+    # hashes every level; one reading 2^18 builds the seed's coordinate table; a later
     # small plan of the seed reads that table; a forced table and none at all price
     # identically
     def req(paths):
